@@ -1,0 +1,7 @@
+"""MI355X implementation of baby_shark's implicit-modeling path (src/voxel).
+
+Public names mirror ``baby_shark::voxel::prelude``; see ``voxel.py``.
+"""
+from ._lib import BSError, load as load_library  # noqa: F401
+from .voxel import Context, MeshToVolume, Volume, VolumeExport, default_context  # noqa: F401
+from . import meshgen  # noqa: F401

@@ -1,0 +1,106 @@
+"""ctypes binding of libbs_voxel.so (the C ABI of include/bs_voxel.h).
+
+The library is built in-tree (``python __graft_entry__.py`` or
+``make -C baby_shark_amd/csrc``).  Loading fails loudly when it is missing:
+there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbs_voxel.so")
+
+BS_OK = 0
+BS_ERR_EMPTY = 1
+BS_ERR_INVALID_ARG = 2
+BS_ERR_HIP = 3
+BS_ERR_UNSUPPORTED_TILE = 4
+BS_ERR_INTERNAL = 5
+BS_ERR_NO_DEVICE = 6
+
+STATUS_NAMES = {0: "BS_OK", 1: "BS_ERR_EMPTY", 2: "BS_ERR_INVALID_ARG", 3: "BS_ERR_HIP",
+                4: "BS_ERR_UNSUPPORTED_TILE", 5: "BS_ERR_INTERNAL", 6: "BS_ERR_NO_DEVICE"}
+
+
+class M2VParams(C.Structure):
+    _fields_ = [("voxel_size", C.c_float), ("band", C.c_int64), ("wn_accuracy", C.c_float),
+                ("sign_threshold", C.c_float), ("fp64_band", C.c_float)]
+
+
+class VolumeInfo(C.Structure):
+    _fields_ = [("voxel_size", C.c_float), ("n_leaves", C.c_uint64), ("n_active", C.c_uint64),
+                ("n_tiles", C.c_uint64)]
+
+
+class M2VStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("n_tris", "n_sub", "n_pairs", "n_leaves", "n_active", "n_candidates",
+                                          "n_bvh_nodes", "wn_node_visits", "wn_leaf_tris", "wn_far_evals",
+                                          "n_fp64_rechecks")] + \
+              [(k, C.c_double) for k in ("ms_total", "ms_subdivide", "ms_leaves", "ms_distance", "ms_bvh",
+                                         "ms_winding")]
+
+
+class BSError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load():
+    """Load libbs_voxel.so; raises if it was not built."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"bs_voxel HIP library not built: {LIB_PATH} is missing "
+                               "(run `python __graft_entry__.py` or `make -C baby_shark_amd/csrc`)")
+        L = C.CDLL(LIB_PATH)
+        vp, pp = C.c_void_p, C.POINTER(C.c_void_p)
+        f32p, i32p, u64p = C.POINTER(C.c_float), C.POINTER(C.c_int32), C.POINTER(C.c_uint64)
+        sig = {
+            "bs_last_error": (C.c_char_p, []),
+            "bs_ctx_create": (C.c_int, [C.c_int, pp]),
+            "bs_ctx_destroy": (None, [vp]),
+            "bs_ctx_stream": (vp, [vp]),
+            "bs_ctx_set_profiling": (C.c_int, [vp, C.c_int]),
+            "bs_ctx_last_m2v_stats": (C.c_int, [vp, C.POINTER(M2VStats)]),
+            "bs_m2v_params_default": (None, [C.POINTER(M2VParams)]),
+            "bs_mesh_to_volume": (C.c_int, [vp, f32p, C.c_size_t, C.POINTER(M2VParams), pp]),
+            "bs_mesh_to_volume_device": (C.c_int, [vp, vp, C.c_size_t, C.POINTER(M2VParams), pp]),
+            "bs_volume_clone": (C.c_int, [vp, pp]),
+            "bs_volume_free": (None, [vp]),
+            "bs_volume_info_get": (C.c_int, [vp, C.POINTER(VolumeInfo)]),
+            "bs_volume_export": (C.c_int, [vp, i32p, u64p, f32p]),
+            "bs_volume_export_tiles": (C.c_int, [vp, i32p, i32p, f32p]),
+            "bs_volume_device_leaves": (C.c_int, [vp, C.POINTER(i32p), C.POINTER(u64p), C.POINTER(f32p),
+                                                  C.POINTER(C.c_uint64)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+        return L
+
+
+def check(status: int) -> None:
+    if status != BS_OK:
+        msg = load().bs_last_error()
+        raise BSError(status, msg.decode() if msg else "")
+
+
+def header_functions() -> list[str]:
+    """Names of the functions declared in include/bs_voxel.h."""
+    import re
+    hdr = os.path.join(os.path.dirname(HERE), "include", "bs_voxel.h")
+    src = open(hdr).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(bs_[a-z0-9_]+)\s*\(", src)))

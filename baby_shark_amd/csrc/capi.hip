@@ -1,0 +1,242 @@
+// C ABI entry points (include/bs_voxel.h): argument checks, error mapping and
+// the canonical host export of volumes.
+#include <cstring>
+#include <string>
+
+#include "common.hpp"
+
+namespace bs {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const bs_m2v_params& prm, bs_volume** out);
+
+template <class F>
+static bs_status guarded(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return BS_OK;
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host out of memory";
+        return BS_ERR_INTERNAL;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return BS_ERR_INTERNAL;
+    }
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        BS_HIP(hipGetDevice(&prev));
+        if (prev != dev) BS_HIP(hipSetDevice(dev));
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        if (hipGetDevice(&cur) == hipSuccess && prev >= 0 && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+static void free_volume_arrays(bs_volume* v) {
+    if (v->origins) (void)hipFree(v->origins);
+    if (v->masks) (void)hipFree(v->masks);
+    if (v->values) (void)hipFree(v->values);
+    v->origins = nullptr;
+    v->masks = nullptr;
+    v->values = nullptr;
+}
+
+}  // namespace bs
+
+using namespace bs;
+
+extern "C" {
+
+const char* bs_last_error(void) { return g_last_error.c_str(); }
+
+bs_status bs_ctx_create(int device, bs_ctx** out) {
+    if (!out) return BS_ERR_INVALID_ARG;
+    *out = nullptr;
+    return guarded([&] {
+        int count = 0;
+        hipError_t e = hipGetDeviceCount(&count);
+        if (e != hipSuccess || count == 0) throw Error(BS_ERR_NO_DEVICE, "no HIP device available");
+        BS_REQUIRE(device >= 0 && device < count, BS_ERR_INVALID_ARG, "device index out of range");
+        DeviceGuard g(device);
+        auto* c = new bs_ctx();
+        c->device = device;
+        hipError_t se = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (se != hipSuccess) {
+            delete c;
+            throw Error(BS_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(se));
+        }
+        *out = c;
+    });
+}
+
+void bs_ctx_destroy(bs_ctx* ctx) {
+    if (!ctx) return;
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->scratch.release();
+    (void)hipStreamDestroy(ctx->stream);
+    if (prev >= 0) (void)hipSetDevice(prev);
+    delete ctx;
+}
+
+void* bs_ctx_stream(bs_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+bs_status bs_ctx_set_profiling(bs_ctx* ctx, int enabled) {
+    if (!ctx) return BS_ERR_INVALID_ARG;
+    ctx->profiling = enabled != 0;
+    return BS_OK;
+}
+
+bs_status bs_ctx_last_m2v_stats(const bs_ctx* ctx, bs_m2v_stats* out) {
+    if (!ctx || !out) return BS_ERR_INVALID_ARG;
+    *out = ctx->last_m2v;
+    return BS_OK;
+}
+
+void bs_m2v_params_default(bs_m2v_params* p) {
+    if (!p) return;
+    p->voxel_size = 1.0f;     // mesh_to_volume.rs:226
+    p->band = 0;              // mesh_to_volume.rs:229
+    p->wn_accuracy = 2.0f;    // mesh_to_volume.rs:264
+    p->sign_threshold = 0.2f; // mesh_to_volume.rs:266
+    p->fp64_band = 0.1f;
+}
+
+bs_status bs_mesh_to_volume_device(bs_ctx* ctx, const float* tri_xyz_dev, size_t n, const bs_m2v_params* p,
+                                   bs_volume** out) {
+    if (!ctx || !out || !p || (n > 0 && !tri_xyz_dev)) {
+        g_last_error = "null argument";
+        return BS_ERR_INVALID_ARG;
+    }
+    *out = nullptr;
+    return guarded([&] {
+        DeviceGuard g(ctx->device);
+        mesh_to_volume_device(ctx, tri_xyz_dev, n, *p, out);
+    });
+}
+
+bs_status bs_mesh_to_volume(bs_ctx* ctx, const float* tri_xyz, size_t n, const bs_m2v_params* p, bs_volume** out) {
+    if (!ctx || !out || !p || (n > 0 && !tri_xyz)) {
+        g_last_error = "null argument";
+        return BS_ERR_INVALID_ARG;
+    }
+    *out = nullptr;
+    return guarded([&] {
+        DeviceGuard g(ctx->device);
+        if (n == 0) throw Error(BS_ERR_EMPTY, "empty mesh: the reference returns None");
+        float* d = ctx->scratch.as<float>(S_MISC2, 9 * (uint64_t)n);
+        BS_HIP(hipMemcpyAsync(d, tri_xyz, sizeof(float) * 9 * n, hipMemcpyHostToDevice, ctx->stream));
+        mesh_to_volume_device(ctx, d, n, *p, out);
+    });
+}
+
+bs_status bs_volume_clone(const bs_volume* v, bs_volume** out) {
+    if (!v || !out) return BS_ERR_INVALID_ARG;
+    *out = nullptr;
+    return guarded([&] {
+        DeviceGuard g(v->device);
+        auto* c = new bs_volume();
+        c->device = v->device;
+        c->voxel_size = v->voxel_size;
+        c->n_leaves = v->n_leaves;
+        c->tiles = v->tiles;
+        try {
+            if (v->n_leaves) {
+                BS_HIP(hipMalloc(&c->origins, sizeof(int32_t) * 4 * v->n_leaves));
+                BS_HIP(hipMalloc(&c->masks, sizeof(uint64_t) * 8 * v->n_leaves));
+                BS_HIP(hipMalloc(&c->values, sizeof(float) * 512 * v->n_leaves));
+                BS_HIP(hipMemcpy(c->origins, v->origins, sizeof(int32_t) * 4 * v->n_leaves, hipMemcpyDeviceToDevice));
+                BS_HIP(hipMemcpy(c->masks, v->masks, sizeof(uint64_t) * 8 * v->n_leaves, hipMemcpyDeviceToDevice));
+                BS_HIP(hipMemcpy(c->values, v->values, sizeof(float) * 512 * v->n_leaves, hipMemcpyDeviceToDevice));
+            }
+        } catch (...) {
+            free_volume_arrays(c);
+            delete c;
+            throw;
+        }
+        *out = c;
+    });
+}
+
+void bs_volume_free(bs_volume* v) {
+    if (!v) return;
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(v->device);
+    free_volume_arrays(v);
+    if (prev >= 0) (void)hipSetDevice(prev);
+    delete v;
+}
+
+bs_status bs_volume_info_get(const bs_volume* v, bs_volume_info* out) {
+    if (!v || !out) return BS_ERR_INVALID_ARG;
+    return guarded([&] {
+        DeviceGuard g(v->device);
+        out->voxel_size = v->voxel_size;
+        out->n_leaves = v->n_leaves;
+        out->n_tiles = v->tiles.size();
+        uint64_t act = 0;
+        if (v->n_leaves) {
+            std::vector<uint64_t> m(8 * v->n_leaves);
+            BS_HIP(hipMemcpy(m.data(), v->masks, sizeof(uint64_t) * m.size(), hipMemcpyDeviceToHost));
+            for (uint64_t w : m) act += (uint64_t)__builtin_popcountll(w);
+        }
+        out->n_active = act;
+    });
+}
+
+bs_status bs_volume_export(const bs_volume* v, int32_t* origins, uint64_t* masks, float* values) {
+    if (!v) return BS_ERR_INVALID_ARG;
+    return guarded([&] {
+        DeviceGuard g(v->device);
+        const uint64_t n = v->n_leaves;
+        if (!n) return;
+        if (origins) {
+            std::vector<int32_t> o(4 * n);
+            BS_HIP(hipMemcpy(o.data(), v->origins, sizeof(int32_t) * 4 * n, hipMemcpyDeviceToHost));
+            for (uint64_t i = 0; i < n; ++i)
+                for (int k = 0; k < 3; ++k) origins[3 * i + k] = o[4 * i + k];
+        }
+        if (masks) BS_HIP(hipMemcpy(masks, v->masks, sizeof(uint64_t) * 8 * n, hipMemcpyDeviceToHost));
+        if (values) BS_HIP(hipMemcpy(values, v->values, sizeof(float) * 512 * n, hipMemcpyDeviceToHost));
+    });
+}
+
+bs_status bs_volume_export_tiles(const bs_volume* v, int32_t* origins, int32_t* sizes, float* values) {
+    if (!v) return BS_ERR_INVALID_ARG;
+    for (size_t i = 0; i < v->tiles.size(); ++i) {
+        const bs_tile& t = v->tiles[i];
+        if (origins) {
+            origins[3 * i] = t.x;
+            origins[3 * i + 1] = t.y;
+            origins[3 * i + 2] = t.z;
+        }
+        if (sizes) sizes[i] = t.size;
+        if (values) values[i] = t.value;
+    }
+    return BS_OK;
+}
+
+bs_status bs_volume_device_leaves(const bs_volume* v, const int32_t** origins, const uint64_t** masks,
+                                  const float** values, uint64_t* n_leaves) {
+    if (!v) return BS_ERR_INVALID_ARG;
+    if (origins) *origins = v->origins;
+    if (masks) *masks = v->masks;
+    if (values) *values = v->values;
+    if (n_leaves) *n_leaves = v->n_leaves;
+    return BS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,156 @@
+// Shared host/device pieces of the bs_voxel library: error handling, the
+// per-context scratch arena, the device volume record and small device helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/bs_voxel.h"
+
+namespace bs {
+
+// ---------------------------------------------------------------- errors
+struct Error : std::runtime_error {
+    bs_status status;
+    Error(bs_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define BS_HIP(call)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            throw ::bs::Error(BS_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_) + " at " + \
+                                              __FILE__ + ":" + std::to_string(__LINE__));              \
+    } while (0)
+
+#define BS_REQUIRE(cond, status, msg)                          \
+    do {                                                       \
+        if (!(cond)) throw ::bs::Error((status), (msg));       \
+    } while (0)
+
+// ---------------------------------------------------------------- scratch arena
+// Named device buffers owned by a context that only ever grow, so a steady
+// stream of same-sized calls allocates nothing after the first.
+enum ScratchSlot : int {
+    S_ROWS, S_SUBS, S_ROW_OFF, S_SUB_OFF, S_SUBTRIS, S_RECS, S_LEAFCNT, S_LEAFOFF, S_HASH_KEYS, S_HASH_VALS,
+    S_KEYS, S_KEYS_SORTED, S_CAND_LEAF, S_CAND_CNT, S_CAND_OFF, S_CAND_LIST, S_CURSOR, S_ACT_CNT, S_ACT_OFF,
+    S_ACT_LIST, S_COUNTERS, S_CUB, S_MORTON, S_MORTON_SORTED, S_IDX, S_IDX_SORTED, S_BVH_TRIS, S_BVH_NODES,
+    S_BVH_LINKS, S_BVH_PARENT, S_BVH_FLAGS, S_BVH_INIT, S_BVH_BOX, S_RECHECK, S_ARGMIN, S_MISC0, S_MISC1,
+    S_MISC2, S_MISC3, S_COUNT
+};
+
+struct Scratch {
+    void* ptr[S_COUNT] = {};
+    size_t cap[S_COUNT] = {};
+    void* get(int slot, size_t bytes) {
+        if (bytes == 0) bytes = 256;
+        if (cap[slot] < bytes) {
+            if (ptr[slot]) BS_HIP(hipFree(ptr[slot]));
+            ptr[slot] = nullptr;
+            size_t b = bytes + bytes / 4 + 256;
+            BS_HIP(hipMalloc(&ptr[slot], b));
+            cap[slot] = b;
+        }
+        return ptr[slot];
+    }
+    template <class T> T* as(int slot, size_t count) { return static_cast<T*>(get(slot, count * sizeof(T))); }
+    void release() {
+        for (int i = 0; i < S_COUNT; ++i)
+            if (ptr[i]) {
+                (void)hipFree(ptr[i]);
+                ptr[i] = nullptr;
+                cap[i] = 0;
+            }
+    }
+};
+
+}  // namespace bs
+
+// ---------------------------------------------------------------- handles
+struct bs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool profiling = false;
+    bs::Scratch scratch;
+    bs_m2v_stats last_m2v = {};
+    uint64_t* host_counters = nullptr;  // pinned, for small readbacks
+};
+
+// An active tile of an internal node (origin, edge length in voxels, value).
+struct bs_tile {
+    int32_t x, y, z, size;
+    float value;
+};
+
+struct bs_volume {
+    int device = 0;
+    float voxel_size = 1.0f;
+    uint64_t n_leaves = 0;
+    int32_t* origins = nullptr;  // device, n_leaves*4 (x, y, z, 0)
+    uint64_t* masks = nullptr;   // device, n_leaves*8
+    float* values = nullptr;     // device, n_leaves*512
+    std::vector<bs_tile> tiles;  // active internal tiles (host, few)
+};
+
+namespace bs {
+
+// ---------------------------------------------------------------- device helpers
+constexpr int LEAF_DIM = 8;
+constexpr int LEAF_VOX = 512;
+constexpr int KEY_BIAS = 1 << 20;  // leaf coordinates are biased into 21 bits
+
+// Leaf key: biased leaf coordinates (origin >> 3) packed x|y|z into 63 bits so
+// that unsigned key order is lexicographic (x, y, z) order of origins.
+__host__ __device__ inline uint64_t leaf_key(int lx, int ly, int lz) {
+    return ((uint64_t)(uint32_t)(lx + KEY_BIAS) << 42) | ((uint64_t)(uint32_t)(ly + KEY_BIAS) << 21) |
+           (uint64_t)(uint32_t)(lz + KEY_BIAS);
+}
+__host__ __device__ inline void leaf_key_decode(uint64_t k, int& lx, int& ly, int& lz) {
+    lx = (int)((k >> 42) & 0x1FFFFF) - KEY_BIAS;
+    ly = (int)((k >> 21) & 0x1FFFFF) - KEY_BIAS;
+    lz = (int)(k & 0x1FFFFF) - KEY_BIAS;
+}
+constexpr uint64_t EMPTY_KEY = ~0ull;
+
+__device__ inline uint32_t hash_u64(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return (uint32_t)k;
+}
+
+// Open addressing, linear probing; capacity is a power of two.  Probing is
+// bounded by the capacity: a full table reports failure instead of spinning.
+__device__ inline bool hash_insert(uint64_t* keys, uint32_t cap_mask, uint64_t key) {
+    uint32_t h = hash_u64(key) & cap_mask;
+    for (uint32_t probe = 0; probe <= cap_mask; ++probe) {
+        unsigned long long prev = atomicCAS((unsigned long long*)&keys[h], (unsigned long long)EMPTY_KEY,
+                                            (unsigned long long)key);
+        if (prev == EMPTY_KEY || prev == key) return true;
+        h = (h + 1) & cap_mask;
+    }
+    return false;
+}
+constexpr uint32_t NOT_FOUND = 0xFFFFFFFFu;
+__device__ inline uint32_t hash_find(const uint64_t* keys, uint32_t cap_mask, uint64_t key) {
+    uint32_t h = hash_u64(key) & cap_mask;
+    for (uint32_t probe = 0; probe <= cap_mask; ++probe) {
+        uint64_t k = keys[h];
+        if (k == key) return h;
+        if (k == EMPTY_KEY) return NOT_FOUND;
+        h = (h + 1) & cap_mask;
+    }
+    return NOT_FOUND;
+}
+
+inline unsigned div_up(uint64_t a, unsigned b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace bs

@@ -1,0 +1,201 @@
+"""Host-side mirror of baby_shark's `voxel::prelude` (src/voxel/prelude.rs:1-4).
+
+Same names, argument meaning and error behaviour as the Rust API, backed by
+the HIP kernels of libbs_voxel.so through its C ABI:
+
+* ``MeshToVolume``  -- mesh_to_volume.rs:17-73 (``convert`` returns ``None``
+  for an empty mesh, like the reference);
+* ``Volume``        -- volume/mod.rs:10-123 (CSG and offset consume their
+  operands: the Rust methods take ``self``/``other`` by value).
+
+A mesh is a triangle soup: an ``(n, 3, 3)`` float32 array (triangle, vertex,
+xyz), or a float32 tensor of that shape on a GPU, which is used in place.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import BSError, check, load
+
+
+class Context:
+    """One HIP stream on one device (``bs_ctx``)."""
+
+    def __init__(self, device: int = 0):
+        L = load()
+        h = C.c_void_p()
+        check(L.bs_ctx_create(int(device), C.byref(h)))
+        self._h = h
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def stream(self) -> int:
+        return int(load().bs_ctx_stream(self._h) or 0)
+
+    def set_profiling(self, on: bool) -> None:
+        check(load().bs_ctx_set_profiling(self._h, 1 if on else 0))
+
+    def last_stats(self) -> dict:
+        s = _lib.M2VStats()
+        check(load().bs_ctx_last_m2v_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            load().bs_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx: dict[int, Context] = {}
+
+
+def default_context(device: int = 0) -> Context:
+    ctx = _default_ctx.get(device)
+    if ctx is None:
+        ctx = _default_ctx[device] = Context(device)
+    return ctx
+
+
+@dataclass
+class VolumeExport:
+    """Canonical dump: leaves sorted by origin (x, y, z); mask bit i of word w
+    is voxel offset w*64+i with offset = (x&7)<<6 | (y&7)<<3 | (z&7)."""
+    voxel_size: float
+    origins: np.ndarray       # (L, 3) int32
+    masks: np.ndarray         # (L, 8) uint64
+    values: np.ndarray        # (L, 512) float32
+    tile_origins: np.ndarray  # (T, 3) int32
+    tile_sizes: np.ndarray    # (T,) int32
+    tile_values: np.ndarray   # (T,) float32
+
+    def active(self) -> np.ndarray:
+        bits = np.unpackbits(self.masks.view(np.uint8).reshape(-1, 8, 8), axis=2, bitorder="little")
+        return bits.reshape(-1, 512).astype(bool)
+
+    @property
+    def n_active(self) -> int:
+        return int(self.active().sum())
+
+
+class Volume:
+    """A device-resident narrow-band volume (``bs_volume``)."""
+
+    def __init__(self, handle: C.c_void_p, ctx: Context):
+        self._h = handle
+        self._ctx = ctx
+
+    # ---- ownership ------------------------------------------------------
+    def _take(self) -> C.c_void_p:
+        if not self._h:
+            raise ValueError("volume was consumed by a previous operation")
+        h, self._h = self._h, None
+        return h
+
+    def free(self) -> None:
+        if getattr(self, "_h", None):
+            load().bs_volume_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    # ---- reference API --------------------------------------------------
+    def info(self) -> dict:
+        i = _lib.VolumeInfo()
+        check(load().bs_volume_info_get(self._h, C.byref(i)))
+        return {"voxel_size": i.voxel_size, "n_leaves": i.n_leaves, "n_active": i.n_active, "n_tiles": i.n_tiles}
+
+    def voxel_size(self) -> float:  # volume/mod.rs:32-34
+        return self.info()["voxel_size"]
+
+    def clone(self) -> "Volume":  # volume/mod.rs:116-123
+        h = C.c_void_p()
+        check(load().bs_volume_clone(self._h, C.byref(h)))
+        return Volume(h, self._ctx)
+
+    def export(self) -> VolumeExport:
+        L = load()
+        inf = self.info()
+        n, nt = inf["n_leaves"], inf["n_tiles"]
+        o = np.zeros((n, 3), np.int32)
+        m = np.zeros((n, 8), np.uint64)
+        v = np.zeros((n, 512), np.float32)
+        check(L.bs_volume_export(self._h, o.ctypes.data_as(C.POINTER(C.c_int32)),
+                                 m.ctypes.data_as(C.POINTER(C.c_uint64)), v.ctypes.data_as(C.POINTER(C.c_float))))
+        to = np.zeros((nt, 3), np.int32)
+        ts = np.zeros(nt, np.int32)
+        tv = np.zeros(nt, np.float32)
+        if nt:
+            check(L.bs_volume_export_tiles(self._h, to.ctypes.data_as(C.POINTER(C.c_int32)),
+                                           ts.ctypes.data_as(C.POINTER(C.c_int32)),
+                                           tv.ctypes.data_as(C.POINTER(C.c_float))))
+        return VolumeExport(inf["voxel_size"], o, m, v, to, ts, tv)
+
+
+class MeshToVolume:
+    """mesh_to_volume.rs:17-73.  Defaults: voxel size 1.0, band 0."""
+
+    def __init__(self, ctx: Optional[Context] = None):
+        self._ctx = ctx
+        self.params = _lib.M2VParams()
+        load().bs_m2v_params_default(C.byref(self.params))
+
+    @property
+    def ctx(self) -> Context:
+        if self._ctx is None:
+            self._ctx = default_context()
+        return self._ctx
+
+    def with_narrow_band_width(self, width: int) -> "MeshToVolume":
+        self.set_narrow_band_width(width)
+        return self
+
+    def set_narrow_band_width(self, width: int) -> "MeshToVolume":
+        self.params.band = int(width)
+        return self
+
+    def with_voxel_size(self, size: float) -> "MeshToVolume":
+        self.set_voxel_size(size)
+        return self
+
+    def set_voxel_size(self, size: float) -> "MeshToVolume":
+        self.params.voxel_size = float(size)
+        return self
+
+    def convert(self, mesh) -> Optional[Volume]:
+        """Narrow-band SDF of a triangle soup; ``None`` for an empty mesh."""
+        L = load()
+        h = C.c_void_p()
+        if hasattr(mesh, "is_cuda") and mesh.is_cuda:
+            import torch
+            t = mesh.detach().reshape(-1, 9)
+            if t.dtype != torch.float32 or not t.is_contiguous():
+                t = t.to(torch.float32).contiguous()
+            torch.cuda.current_stream(t.device).synchronize()
+            st = L.bs_mesh_to_volume_device(self.ctx.handle, C.c_void_p(t.data_ptr()), t.shape[0],
+                                            C.byref(self.params), C.byref(h))
+        else:
+            a = np.ascontiguousarray(np.asarray(mesh, dtype=np.float32).reshape(-1, 9))
+            st = L.bs_mesh_to_volume(self.ctx.handle, a.ctypes.data_as(C.POINTER(C.c_float)), a.shape[0],
+                                     C.byref(self.params), C.byref(h))
+        if st == _lib.BS_ERR_EMPTY:
+            return None
+        check(st)
+        return Volume(h, self.ctx)

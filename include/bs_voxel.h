@@ -1,0 +1,109 @@
+/*
+ * bs_voxel.h -- C ABI of the MI355X implicit-modeling path (mesh -> narrow-band
+ * SDF, CSG, offset, dual contouring) of baby_shark's `src/voxel` module.
+ *
+ * The reference exposes this path only as a Rust API (src/voxel/prelude.rs:1-4);
+ * each entry point below names the Rust item it replaces.  Conventions:
+ *   - plain pointers and sizes, opaque handles, one owner per handle;
+ *   - every call is synchronous on the calling thread and returns bs_status;
+ *     BS_ERR_EMPTY is the reference's `None`;
+ *   - a bs_ctx owns one HIP stream on one device; calls on one context must not
+ *     overlap (the Rust converters are `&mut self`, not re-entrant);
+ *   - the last error message of the calling thread is bs_last_error().
+ *
+ * Volume data layout (device resident, exported canonically):
+ *   leaves sorted by origin (x, then y, then z), origin = voxel index & ~7;
+ *   mask: 8 uint64 words per leaf, bit (i % 64) of word (i / 64) is voxel
+ *         offset i = (x & 7) << 6 | (y & 7) << 3 | (z & 7)  (leaf_node/mod.rs:30-36);
+ *   values: 512 float per leaf in offset order.
+ */
+#ifndef BS_VOXEL_H
+#define BS_VOXEL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    BS_OK = 0,
+    BS_ERR_EMPTY = 1,            /* reference returns None (empty mesh)      */
+    BS_ERR_INVALID_ARG = 2,      /* bad pointer/size/parameter               */
+    BS_ERR_HIP = 3,              /* HIP runtime error                        */
+    BS_ERR_UNSUPPORTED_TILE = 4, /* reference panics with todo!() on a tile  */
+    BS_ERR_INTERNAL = 5,         /* reference panics (unreachable!/unwrap)   */
+    BS_ERR_NO_DEVICE = 6         /* no HIP device / library built without GPU */
+} bs_status;
+
+typedef struct bs_ctx bs_ctx;
+typedef struct bs_volume bs_volume;
+
+/* Mesh-to-volume parameters (mesh_to_volume.rs:17-50, defaults :223-236). */
+typedef struct {
+    float voxel_size;     /* with_voxel_size; default 1.0                         */
+    int64_t band;         /* with_narrow_band_width; default 0                    */
+    float wn_accuracy;    /* accuracy scale of approximate(); reference: 2.0      */
+    float sign_threshold; /* wn < threshold => outside; reference: 0.2            */
+    float fp64_band;      /* |wn - threshold| < fp64_band is re-evaluated in FP64 */
+} bs_m2v_params;
+
+typedef struct {
+    float voxel_size;
+    uint64_t n_leaves;
+    uint64_t n_active;    /* active voxels in leaves                 */
+    uint64_t n_tiles;     /* active internal tiles                   */
+} bs_volume_info;
+
+/* Work counters and kernel timings of the last bs_mesh_to_volume* call. */
+typedef struct {
+    uint64_t n_tris, n_sub, n_pairs, n_leaves, n_active, n_candidates;
+    uint64_t n_bvh_nodes, wn_node_visits, wn_leaf_tris, wn_far_evals, n_fp64_rechecks;
+    double ms_total, ms_subdivide, ms_leaves, ms_distance, ms_bvh, ms_winding;
+} bs_m2v_stats;
+
+const char* bs_last_error(void);
+
+/* Context: binds `device`, creates one default-priority HIP stream. */
+bs_status bs_ctx_create(int device, bs_ctx** out);
+void bs_ctx_destroy(bs_ctx* ctx);
+/* The context's hipStream_t (as void*), for callers that time or order work. */
+void* bs_ctx_stream(bs_ctx* ctx);
+/* Per-kernel event timing inside bs_mesh_to_volume* (adds syncs; off by default). */
+bs_status bs_ctx_set_profiling(bs_ctx* ctx, int enabled);
+bs_status bs_ctx_last_m2v_stats(const bs_ctx* ctx, bs_m2v_stats* out);
+
+void bs_m2v_params_default(bs_m2v_params* p);
+
+/* MeshToVolume::convert (mesh_to_volume.rs:52-73).  `tri_xyz` is a host f32
+ * soup of n triangles (9 floats each, PolygonSoup order: mesh/polygon_soup/
+ * data_structure.rs:78-86).  *out is NULL and BS_ERR_EMPTY is returned where
+ * the reference returns None. */
+bs_status bs_mesh_to_volume(bs_ctx* ctx, const float* tri_xyz, size_t n, const bs_m2v_params* p, bs_volume** out);
+/* Same, with `tri_xyz_dev` already in device memory of the context's device. */
+bs_status bs_mesh_to_volume_device(bs_ctx* ctx, const float* tri_xyz_dev, size_t n, const bs_m2v_params* p,
+                                   bs_volume** out);
+
+/* Volume::clone (volume/mod.rs:116-123) and drop. */
+bs_status bs_volume_clone(const bs_volume* v, bs_volume** out);
+void bs_volume_free(bs_volume* v);
+
+/* Volume::voxel_size and sizes. */
+bs_status bs_volume_info_get(const bs_volume* v, bs_volume_info* out);
+
+/* Canonical host export (any pointer may be NULL to skip that array):
+ * origins n_leaves*3 int32, masks n_leaves*8 uint64, values n_leaves*512 float;
+ * tiles: origin n_tiles*3 int32, edge length n_tiles int32, value n_tiles float. */
+bs_status bs_volume_export(const bs_volume* v, int32_t* origins, uint64_t* masks, float* values);
+bs_status bs_volume_export_tiles(const bs_volume* v, int32_t* origins, int32_t* sizes, float* values);
+
+/* Device views of the leaf arrays (valid until the volume is freed/consumed). */
+bs_status bs_volume_device_leaves(const bs_volume* v, const int32_t** origins, const uint64_t** masks,
+                                  const float** values, uint64_t* n_leaves);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BS_VOXEL_H */

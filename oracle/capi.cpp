@@ -1,0 +1,364 @@
+// Test infrastructure only: the CPU oracle.  Nothing outside tests/, the smoke
+// check and bench.py's cpu_baseline leg may link or call this code.
+//
+// C entry points of the oracle (loaded by tests through ctypes).  Volumes are
+// exported canonically: leaves sorted by origin (x, then y, then z), an active
+// mask of 8 u64 words per leaf with bit (i % 64) of word (i / 64) standing for
+// voxel offset i = (x&7)<<6 | (y&7)<<3 | (z&7), 512 f32 values per leaf, and
+// active tiles as (origin, edge length, value).
+#include <algorithm>
+#include <array>
+#include <thread>
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <unordered_map>
+
+#include "mesh_to_volume.hpp"
+#include "parallel.hpp"
+
+using namespace oracle;
+
+struct or_volume {
+    std::unique_ptr<VolumeGrid> grid;
+    float voxel_size;
+    // canonical export cache
+    std::vector<const LeafNode*> leaves;
+    struct TileRec { Vec3i o; int size; float v; };
+    std::vector<TileRec> tiles;
+    void index() {
+        leaves.clear();
+        tiles.clear();
+        grid->visit_leafs([&](Vec3i o, int s, float v) { tiles.push_back({o, s, v}); },
+                          [&](const LeafNode& l) { leaves.push_back(&l); });
+        std::sort(leaves.begin(), leaves.end(), [](const LeafNode* a, const LeafNode* b) { return a->origin < b->origin; });
+        std::sort(tiles.begin(), tiles.end(), [](const TileRec& a, const TileRec& b) {
+            if (a.o != b.o) return a.o < b.o;
+            return a.size < b.size;
+        });
+    }
+};
+
+static std::vector<Tri> to_tris(const float* xyz, size_t n) {
+    std::vector<Tri> t(n);
+    for (size_t i = 0; i < n; ++i) {
+        const float* p = xyz + 9 * i;
+        t[i] = {{p[0], p[1], p[2]}, {p[3], p[4], p[5]}, {p[6], p[7], p[8]}};
+    }
+    return t;
+}
+
+extern "C" {
+
+// MeshToVolume::default().with_voxel_size(h).with_narrow_band_width(band).convert(mesh)
+// Returns NULL where the reference returns None.  `seconds` (nullable) receives
+// the wall time of convert() alone.
+or_volume* or_mesh_to_volume(const float* tris, size_t n, float voxel_size, int64_t band, int threads, double* seconds) {
+    MeshToVolume m;
+    m.set_voxel_size(voxel_size);
+    m.band_width = band;
+    if (threads > 0) m.threads = threads;
+    auto mesh = to_tris(tris, n);
+    auto t0 = std::chrono::steady_clock::now();
+    auto g = m.convert(mesh);
+    auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    if (!g) return nullptr;
+    auto* v = new or_volume{std::move(g), voxel_size, {}, {}};
+    v->index();
+    return v;
+}
+
+// Unsigned pass only (mesh_to_volume.rs:118-196), for stage-wise parity.
+or_volume* or_unsigned_distance(const float* tris, size_t n, float voxel_size, int64_t band, int threads) {
+    MeshToVolume m;
+    m.set_voxel_size(voxel_size);
+    m.band_width = band;
+    if (threads > 0) m.threads = threads;
+    auto mesh = to_tris(tris, n);
+    for (auto& t : mesh) subdivide_triangle(t, voxel_size, m.subdivided);
+    if (m.subdivided.empty()) return nullptr;
+    auto* v = new or_volume{m.unsigned_distance_field(), voxel_size, {}, {}};
+    v->index();
+    return v;
+}
+
+void or_volume_free(or_volume* v) { delete v; }
+
+size_t or_volume_leaf_count(const or_volume* v) { return v->leaves.size(); }
+size_t or_volume_tile_count(const or_volume* v) { return v->tiles.size(); }
+
+void or_volume_export(const or_volume* v, int32_t* origins, uint64_t* masks, float* values) {
+    for (size_t i = 0; i < v->leaves.size(); ++i) {
+        const LeafNode* l = v->leaves[i];
+        if (origins) {
+            origins[3 * i + 0] = (int32_t)l->origin.x;
+            origins[3 * i + 1] = (int32_t)l->origin.y;
+            origins[3 * i + 2] = (int32_t)l->origin.z;
+        }
+        if (masks)
+            for (int w = 0; w < 8; ++w) {
+                uint64_t m = 0;
+                for (int b = 0; b < 64; ++b)
+                    if (l->value_mask.at(w * 64 + b)) m |= uint64_t(1) << b;
+                masks[8 * i + w] = m;
+            }
+        if (values) std::memcpy(values + 512 * i, l->values, 512 * sizeof(float));
+    }
+}
+
+void or_volume_export_tiles(const or_volume* v, int32_t* origins, int32_t* sizes, float* values) {
+    for (size_t i = 0; i < v->tiles.size(); ++i) {
+        const auto& t = v->tiles[i];
+        origins[3 * i + 0] = (int32_t)t.o.x;
+        origins[3 * i + 1] = (int32_t)t.o.y;
+        origins[3 * i + 2] = (int32_t)t.o.z;
+        sizes[i] = t.size;
+        values[i] = t.v;
+    }
+}
+
+// Sub-triangles of mesh_to_volume.rs:75-116; returns the count, fills `out`
+// (n_sub*9 floats) when non-null.
+size_t or_subdivide(const float* tris, size_t n, float voxel_size, float* out) {
+    std::vector<Tri> sub;
+    for (size_t i = 0; i < n; ++i) {
+        const float* p = tris + 9 * i;
+        subdivide_triangle({{p[0], p[1], p[2]}, {p[3], p[4], p[5]}, {p[6], p[7], p[8]}}, voxel_size, sub);
+    }
+    if (out) std::memcpy(out, sub.data(), sub.size() * sizeof(Tri));
+    return sub.size();
+}
+
+// Σ over sub-triangles of the dilated-box voxel count (the (voxel, sub-triangle)
+// pair count of mesh_to_volume.rs:124-164).
+uint64_t or_pair_count(const float* tris, size_t n, float voxel_size, int64_t band) {
+    std::vector<Tri> sub;
+    for (size_t i = 0; i < n; ++i) {
+        const float* p = tris + 9 * i;
+        subdivide_triangle({{p[0], p[1], p[2]}, {p[3], p[4], p[5]}, {p[6], p[7], p[8]}}, voxel_size, sub);
+    }
+    float inv = 1.0f / voxel_size;
+    uint64_t total = 0;
+    for (auto& t : sub) total += (uint64_t)dilated_box(t, inv, band).volume();
+    return total;
+}
+
+// Winding numbers at query points: reference tree, accuracy scale `acc`
+// (f32 and FP64 re-evaluation), plus the brute-force f32 sum when `exact` is non-null.
+void or_winding_numbers(const float* tris, size_t n, const float* pts, size_t m, float acc, float* out_f32,
+                        double* out_f64, float* exact, int threads) {
+    auto mesh = to_tris(tris, n);
+    WindingNumbers w = WindingNumbers::from_mesh(mesh);
+    par_for(m, threads > 0 ? threads : default_threads(), [&](size_t i) {
+        Vec3f p{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+        if (out_f32) out_f32[i] = w.approximate(p, acc);
+        if (out_f64) out_f64[i] = w.approximate_f64(p, acc);
+        if (exact) exact[i] = winding_number_exact(mesh, p);
+    });
+}
+
+float or_solid_angle(const float* tri, const float* q) {
+    return solid_angle({{tri[0], tri[1], tri[2]}, {tri[3], tri[4], tri[5]}, {tri[6], tri[7], tri[8]}}, {q[0], q[1], q[2]});
+}
+
+void or_closest_point(const float* tri, const float* q, float* out) {
+    Vec3f c = closest_point({{tri[0], tri[1], tri[2]}, {tri[3], tri[4], tri[5]}, {tri[6], tri[7], tri[8]}}, {q[0], q[1], q[2]});
+    out[0] = c.x;
+    out[1] = c.y;
+    out[2] = c.z;
+}
+
+// FP64 unsigned distance of every active voxel of `v` (canonical leaf order,
+// 512 slots per leaf; inactive slots get +inf): min over exactly the
+// (voxel, sub-triangle) pairs of the reference pair set, evaluated in double.
+void or_unsigned_distance_f64(const or_volume* v, const float* tris, size_t n, int64_t band, double* out, int threads) {
+    std::vector<Tri> sub;
+    float h = v->voxel_size;
+    for (size_t i = 0; i < n; ++i) {
+        const float* p = tris + 9 * i;
+        subdivide_triangle({{p[0], p[1], p[2]}, {p[3], p[4], p[5]}, {p[6], p[7], p[8]}}, h, sub);
+    }
+    struct KeyHash {
+        size_t operator()(const Vec3i& k) const {
+            return (size_t)(k.x * 73856093) ^ (size_t)(k.y * 19349663) ^ (size_t)(k.z * 83492791);
+        }
+    };
+    std::unordered_map<Vec3i, size_t, KeyHash> leaf_index;
+    for (size_t i = 0; i < v->leaves.size(); ++i) leaf_index[v->leaves[i]->origin] = i;
+    size_t slots = v->leaves.size() * 512;
+    std::vector<std::atomic<uint64_t>> best(slots);
+    double inf = INFINITY;
+    uint64_t inf_bits;
+    std::memcpy(&inf_bits, &inf, 8);
+    for (auto& b : best) b.store(inf_bits, std::memory_order_relaxed);
+    float inv = 1.0f / h;
+    par_for(sub.size(), threads > 0 ? threads : default_threads(), [&](size_t k) {
+        const Tri& t = sub[k];
+        VoxelBox b = dilated_box(t, inv, band);
+        for (int64_t x = b.min.x; x <= b.max.x; ++x)
+            for (int64_t y = b.min.y; y <= b.max.y; ++y)
+                for (int64_t z = b.min.z; z <= b.max.z; ++z) {
+                    auto it = leaf_index.find({x & ~int64_t(7), y & ~int64_t(7), z & ~int64_t(7)});
+                    if (it == leaf_index.end()) continue;
+                    size_t slot = it->second * 512 + (size_t)(((x & 7) << 6) | ((y & 7) << 3) | (z & 7));
+                    double d = point_tri_distance_f64(t, {(float)x * h, (float)y * h, (float)z * h});
+                    uint64_t bits;
+                    std::memcpy(&bits, &d, 8);
+                    uint64_t cur = best[slot].load(std::memory_order_relaxed);
+                    while (bits < cur && !best[slot].compare_exchange_weak(cur, bits)) {
+                    }
+                }
+    }, 16);
+    for (size_t s = 0; s < slots; ++s) {
+        uint64_t bits = best[s].load();
+        std::memcpy(&out[s], &bits, 8);
+    }
+}
+
+// Active voxel set alone: the union over sub-triangles of the dilated boxes
+// (mesh_to_volume.rs:124-168), as canonical (origin, mask) pairs.  Returns
+// the leaf count; fills the arrays when non-null (call twice).
+struct ActiveSet {
+    std::vector<Vec3i> origins;
+    std::vector<std::array<uint64_t, 8>> masks;
+};
+static ActiveSet g_active;
+
+size_t or_active_set(const float* tris, size_t n, float voxel_size, int64_t band, int threads) {
+    std::vector<Tri> sub;
+    for (size_t i = 0; i < n; ++i) {
+        const float* p = tris + 9 * i;
+        subdivide_triangle({{p[0], p[1], p[2]}, {p[3], p[4], p[5]}, {p[6], p[7], p[8]}}, voxel_size, sub);
+    }
+    float inv = 1.0f / voxel_size;
+    struct KeyHash {
+        size_t operator()(const Vec3i& k) const {
+            return (size_t)(k.x * 73856093) ^ (size_t)(k.y * 19349663) ^ (size_t)(k.z * 83492791);
+        }
+    };
+    int T = threads > 0 ? threads : default_threads();
+    std::vector<std::unordered_map<Vec3i, std::array<uint64_t, 8>, KeyHash>> part(T);
+    std::atomic<size_t> next{0};
+    auto work = [&](int tid) {
+        auto& mp = part[tid];
+        for (;;) {
+            size_t s0 = next.fetch_add(4096);
+            if (s0 >= sub.size()) return;
+            size_t s1 = std::min(sub.size(), s0 + 4096);
+            for (size_t k = s0; k < s1; ++k) {
+                VoxelBox b = dilated_box(sub[k], inv, band);
+                for (int64_t lx = b.min.x >> 3; lx <= b.max.x >> 3; ++lx)
+                    for (int64_t ly = b.min.y >> 3; ly <= b.max.y >> 3; ++ly)
+                        for (int64_t lz = b.min.z >> 3; lz <= b.max.z >> 3; ++lz) {
+                            auto& m = mp[{lx * 8, ly * 8, lz * 8}];
+                            int64_t x0 = std::max(b.min.x, lx * 8), x1 = std::min(b.max.x, lx * 8 + 7);
+                            int64_t y0 = std::max(b.min.y, ly * 8), y1 = std::min(b.max.y, ly * 8 + 7);
+                            int64_t z0 = std::max(b.min.z, lz * 8) - lz * 8, z1 = std::min(b.max.z, lz * 8 + 7) - lz * 8;
+                            uint64_t zbits = ((uint64_t(1) << (z1 + 1)) - 1) & ~((uint64_t(1) << z0) - 1);
+                            for (int64_t x = x0; x <= x1; ++x)
+                                for (int64_t y = y0; y <= y1; ++y) {
+                                    int off = (int)(((x & 7) << 6) | ((y & 7) << 3));
+                                    m[off >> 6] |= zbits << (off & 63);
+                                }
+                        }
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < T; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (auto& th : pool) th.join();
+    for (int t = 1; t < T; ++t)
+        for (auto& kv : part[t]) {
+            auto& m = part[0][kv.first];
+            for (int w = 0; w < 8; ++w) m[w] |= kv.second[w];
+        }
+    g_active.origins.clear();
+    g_active.masks.clear();
+    std::vector<std::pair<Vec3i, std::array<uint64_t, 8>>> all(part[0].begin(), part[0].end());
+    std::sort(all.begin(), all.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    for (auto& kv : all) {
+        g_active.origins.push_back(kv.first);
+        g_active.masks.push_back(kv.second);
+    }
+    return g_active.origins.size();
+}
+
+void or_active_set_export(int32_t* origins, uint64_t* masks) {
+    for (size_t i = 0; i < g_active.origins.size(); ++i) {
+        for (int k = 0; k < 3; ++k) origins[3 * i + k] = (int32_t)g_active.origins[i][k];
+        for (int w = 0; w < 8; ++w) masks[8 * i + w] = g_active.masks[i][w];
+    }
+}
+
+// Unsigned distance at sample voxels (m x 3 int32 indices): the reference's
+// f32 value (min over the sub-triangles whose dilated box holds the voxel,
+// mesh_to_volume.rs:147-195) and the same min evaluated in FP64.  +inf when
+// no box holds the voxel.
+void or_sample_unsigned(const float* tris, size_t n, float voxel_size, int64_t band, const int32_t* vox, size_t m,
+                        float* out32, double* out64, int threads) {
+    std::vector<Tri> sub;
+    for (size_t i = 0; i < n; ++i) {
+        const float* p = tris + 9 * i;
+        subdivide_triangle({{p[0], p[1], p[2]}, {p[3], p[4], p[5]}, {p[6], p[7], p[8]}}, voxel_size, sub);
+    }
+    struct KeyHash {
+        size_t operator()(const Vec3i& k) const {
+            return (size_t)(k.x * 73856093) ^ (size_t)(k.y * 19349663) ^ (size_t)(k.z * 83492791);
+        }
+    };
+    std::unordered_map<Vec3i, std::vector<size_t>, KeyHash> by_leaf;
+    for (size_t i = 0; i < m; ++i)
+        by_leaf[{(int64_t)vox[3 * i] & ~int64_t(7), (int64_t)vox[3 * i + 1] & ~int64_t(7), (int64_t)vox[3 * i + 2] & ~int64_t(7)}]
+            .push_back(i);
+    std::vector<std::atomic<uint32_t>> b32(m);
+    std::vector<std::atomic<uint64_t>> b64(m);
+    float finf = INFINITY;
+    double dinf = INFINITY;
+    uint32_t fi;
+    uint64_t di;
+    std::memcpy(&fi, &finf, 4);
+    std::memcpy(&di, &dinf, 8);
+    for (size_t i = 0; i < m; ++i) {
+        b32[i].store(fi);
+        b64[i].store(di);
+    }
+    float inv = 1.0f / voxel_size;
+    float h = voxel_size;
+    par_for(sub.size(), threads > 0 ? threads : default_threads(), [&](size_t k) {
+        const Tri& t = sub[k];
+        VoxelBox b = dilated_box(t, inv, band);
+        for (int64_t lx = b.min.x >> 3; lx <= b.max.x >> 3; ++lx)
+            for (int64_t ly = b.min.y >> 3; ly <= b.max.y >> 3; ++ly)
+                for (int64_t lz = b.min.z >> 3; lz <= b.max.z >> 3; ++lz) {
+                    auto it = by_leaf.find({lx * 8, ly * 8, lz * 8});
+                    if (it == by_leaf.end()) continue;
+                    for (size_t s : it->second) {
+                        int64_t x = vox[3 * s], y = vox[3 * s + 1], z = vox[3 * s + 2];
+                        if (x < b.min.x || x > b.max.x || y < b.min.y || y > b.max.y || z < b.min.z || z > b.max.z) continue;
+                        Vec3f p{(float)x * h, (float)y * h, (float)z * h};
+                        float d = point_tri_distance(t, p);
+                        double dd = point_tri_distance_f64(t, p);
+                        uint32_t bits;
+                        std::memcpy(&bits, &d, 4);
+                        uint32_t cur = b32[s].load();
+                        while (bits < cur && !b32[s].compare_exchange_weak(cur, bits)) {
+                        }
+                        uint64_t bb;
+                        std::memcpy(&bb, &dd, 8);
+                        uint64_t c64 = b64[s].load();
+                        while (bb < c64 && !b64[s].compare_exchange_weak(c64, bb)) {
+                        }
+                    }
+                }
+    }, 256);
+    for (size_t i = 0; i < m; ++i) {
+        uint32_t a = b32[i].load();
+        uint64_t b = b64[i].load();
+        std::memcpy(&out32[i], &a, 4);
+        std::memcpy(&out64[i], &b, 8);
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,201 @@
+"""ctypes front end of the CPU oracle (test infrastructure).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg
+import this module, and only to check or time the reference algorithm; no
+product path calls it.  The library is the C++ restatement under this
+directory, built by ``make -C oracle``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
+        L = C.CDLL(LIB_PATH)
+        vp, f32p, f64p, i32p, u64p = C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_double), \
+            C.POINTER(C.c_int32), C.POINTER(C.c_uint64)
+        L.or_mesh_to_volume.restype = vp
+        L.or_mesh_to_volume.argtypes = [f32p, C.c_size_t, C.c_float, C.c_int64, C.c_int, f64p]
+        L.or_unsigned_distance.restype = vp
+        L.or_unsigned_distance.argtypes = [f32p, C.c_size_t, C.c_float, C.c_int64, C.c_int]
+        L.or_volume_free.argtypes = [vp]
+        L.or_volume_leaf_count.restype = C.c_size_t
+        L.or_volume_leaf_count.argtypes = [vp]
+        L.or_volume_tile_count.restype = C.c_size_t
+        L.or_volume_tile_count.argtypes = [vp]
+        L.or_volume_export.argtypes = [vp, i32p, u64p, f32p]
+        L.or_volume_export_tiles.argtypes = [vp, i32p, i32p, f32p]
+        L.or_subdivide.restype = C.c_size_t
+        L.or_subdivide.argtypes = [f32p, C.c_size_t, C.c_float, f32p]
+        L.or_pair_count.restype = C.c_uint64
+        L.or_pair_count.argtypes = [f32p, C.c_size_t, C.c_float, C.c_int64]
+        L.or_winding_numbers.argtypes = [f32p, C.c_size_t, f32p, C.c_size_t, C.c_float, f32p, f64p, f32p, C.c_int]
+        L.or_solid_angle.restype = C.c_float
+        L.or_solid_angle.argtypes = [f32p, f32p]
+        L.or_closest_point.argtypes = [f32p, f32p, f32p]
+        L.or_unsigned_distance_f64.argtypes = [vp, f32p, C.c_size_t, C.c_int64, f64p, C.c_int]
+        L.or_active_set.restype = C.c_size_t
+        L.or_active_set.argtypes = [f32p, C.c_size_t, C.c_float, C.c_int64, C.c_int]
+        L.or_active_set_export.argtypes = [i32p, u64p]
+        L.or_sample_unsigned.argtypes = [f32p, C.c_size_t, C.c_float, C.c_int64, i32p, C.c_size_t, f32p, f64p,
+                                         C.c_int]
+        _lib = L
+    return _lib
+
+
+def _f32(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+@dataclass
+class VolumeExport:
+    """Canonical dump: leaves sorted by origin; mask bit i of word w = voxel w*64+i."""
+    voxel_size: float
+    origins: np.ndarray  # (L,3) int32
+    masks: np.ndarray    # (L,8) uint64
+    values: np.ndarray   # (L,512) float32
+    tile_origins: np.ndarray  # (T,3) int32
+    tile_sizes: np.ndarray    # (T,) int32
+    tile_values: np.ndarray   # (T,) float32
+
+    def active(self) -> np.ndarray:
+        """(L,512) bool active flags."""
+        bits = np.unpackbits(self.masks.view(np.uint8).reshape(-1, 8, 8), axis=2, bitorder="little")
+        return bits.reshape(-1, 512).astype(bool)
+
+    @property
+    def n_active(self) -> int:
+        return int(self.active().sum())
+
+
+class Volume:
+    def __init__(self, handle, voxel_size):
+        self.h = handle
+        self.voxel_size = voxel_size
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_volume_free(self.h)
+            self.h = None
+
+    def export(self) -> VolumeExport:
+        L = lib()
+        n = L.or_volume_leaf_count(self.h)
+        o = np.zeros((n, 3), np.int32)
+        m = np.zeros((n, 8), np.uint64)
+        v = np.zeros((n, 512), np.float32)
+        L.or_volume_export(self.h, o.ctypes.data_as(C.POINTER(C.c_int32)), m.ctypes.data_as(C.POINTER(C.c_uint64)),
+                           v.ctypes.data_as(C.POINTER(C.c_float)))
+        nt = L.or_volume_tile_count(self.h)
+        to = np.zeros((nt, 3), np.int32)
+        ts = np.zeros(nt, np.int32)
+        tv = np.zeros(nt, np.float32)
+        if nt:
+            L.or_volume_export_tiles(self.h, to.ctypes.data_as(C.POINTER(C.c_int32)),
+                                     ts.ctypes.data_as(C.POINTER(C.c_int32)), tv.ctypes.data_as(C.POINTER(C.c_float)))
+        return VolumeExport(self.voxel_size, o, m, v, to, ts, tv)
+
+
+def mesh_to_volume(soup, voxel_size: float, band: int, threads: int = 0, timing: bool = False):
+    """MeshToVolume::default().with_voxel_size(h).with_narrow_band_width(band).convert(soup)."""
+    a, p = _f32(soup)
+    sec = C.c_double(0.0)
+    h = lib().or_mesh_to_volume(p, a.size // 9, voxel_size, band, threads, C.byref(sec))
+    vol = Volume(h, voxel_size) if h else None
+    return (vol, sec.value) if timing else vol
+
+
+def unsigned_distance(soup, voxel_size: float, band: int, threads: int = 0):
+    a, p = _f32(soup)
+    h = lib().or_unsigned_distance(p, a.size // 9, voxel_size, band, threads)
+    return Volume(h, voxel_size) if h else None
+
+
+def subdivide(soup, voxel_size: float) -> np.ndarray:
+    a, p = _f32(soup)
+    n = lib().or_subdivide(p, a.size // 9, voxel_size, None)
+    out = np.zeros((n, 3, 3), np.float32)
+    lib().or_subdivide(p, a.size // 9, voxel_size, out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
+
+
+def pair_count(soup, voxel_size: float, band: int) -> int:
+    a, p = _f32(soup)
+    return int(lib().or_pair_count(p, a.size // 9, voxel_size, band))
+
+
+def winding_numbers(soup, points, accuracy: float = 2.0, exact: bool = False, threads: int = 0):
+    a, p = _f32(soup)
+    q, qp = _f32(points)
+    m = q.size // 3
+    o32 = np.zeros(m, np.float32)
+    o64 = np.zeros(m, np.float64)
+    ex = np.zeros(m, np.float32) if exact else None
+    lib().or_winding_numbers(p, a.size // 9, qp, m, accuracy, o32.ctypes.data_as(C.POINTER(C.c_float)),
+                             o64.ctypes.data_as(C.POINTER(C.c_double)),
+                             ex.ctypes.data_as(C.POINTER(C.c_float)) if exact else None, threads)
+    return (o32, o64, ex) if exact else (o32, o64)
+
+
+def solid_angle(tri, q) -> float:
+    t, tp = _f32(tri)
+    qq, qp = _f32(q)
+    return float(lib().or_solid_angle(tp, qp))
+
+
+def closest_point(tri, q) -> np.ndarray:
+    t, tp = _f32(tri)
+    qq, qp = _f32(q)
+    out = np.zeros(3, np.float32)
+    lib().or_closest_point(tp, qp, out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
+
+
+def unsigned_distance_f64(vol: Volume, soup, band: int, threads: int = 0) -> np.ndarray:
+    """(L,512) float64 min over the reference pair set, canonical leaf order."""
+    a, p = _f32(soup)
+    n = lib().or_volume_leaf_count(vol.h)
+    out = np.zeros((n, 512), np.float64)
+    lib().or_unsigned_distance_f64(vol.h, p, a.size // 9, band, out.ctypes.data_as(C.POINTER(C.c_double)), threads)
+    return out
+
+
+def active_set(soup, voxel_size: float, band: int, threads: int = 0):
+    """(origins (L,3) int32, masks (L,8) uint64): union of dilated boxes, canonical order."""
+    a, p = _f32(soup)
+    n = lib().or_active_set(p, a.size // 9, voxel_size, band, threads)
+    o = np.zeros((n, 3), np.int32)
+    m = np.zeros((n, 8), np.uint64)
+    lib().or_active_set_export(o.ctypes.data_as(C.POINTER(C.c_int32)), m.ctypes.data_as(C.POINTER(C.c_uint64)))
+    return o, m
+
+
+def sample_unsigned(soup, voxel_size: float, band: int, voxels, threads: int = 0):
+    """Reference f32 unsigned distance and its FP64 counterpart at voxel indices (m,3)."""
+    a, p = _f32(soup)
+    v = np.ascontiguousarray(voxels, dtype=np.int32)
+    m = v.shape[0]
+    o32 = np.zeros(m, np.float32)
+    o64 = np.zeros(m, np.float64)
+    lib().or_sample_unsigned(p, a.size // 9, voxel_size, band, v.ctypes.data_as(C.POINTER(C.c_int32)), m,
+                             o32.ctypes.data_as(C.POINTER(C.c_float)), o64.ctypes.data_as(C.POINTER(C.c_double)),
+                             threads)
+    return o32, o64

@@ -1,0 +1,133 @@
+"""Mesh -> narrow-band SDF on the GPU against the CPU oracle (the reference's
+algorithm restated).  Leaf keys and active masks bit-exact; SDF within 1e-4
+voxel; signs bit-exact wherever |d| > 1e-6 voxel."""
+import numpy as np
+import pytest
+
+import baby_shark_amd as bs
+from baby_shark_amd import meshgen as mg
+from tests.helpers import compare_volumes
+
+pytestmark = pytest.mark.gpu
+TOL_VOX = 1e-4   # SDF tolerance, voxels
+RNG = np.random.default_rng(3)
+
+
+def run_gpu(ctx, soup, h, band):
+    return bs.MeshToVolume(ctx).with_voxel_size(h).with_narrow_band_width(band).convert(soup)
+
+
+def check_case(ctx, oracle_lib, soup, h, band, fp64=False):
+    g = run_gpu(ctx, soup, h, band)
+    r = oracle_lib.mesh_to_volume(soup, h, band)
+    if r is None:
+        assert g is None
+        return None
+    ge, re_ = g.export(), r.export()
+    res = compare_volumes(ge, re_, h)
+    if fp64:
+        d64 = oracle_lib.unsigned_distance_f64(r, soup, band)
+        act = re_.active()
+        res["max_err_vs_fp64_vox"] = float(np.max(np.abs(np.abs(ge.values[act]) - d64[act])) / h)
+        res["rms_err_vs_fp64_vox"] = float(np.sqrt(np.mean((np.abs(ge.values[act]) - d64[act]) ** 2)) / h)
+        res["ref_max_err_vs_fp64_vox"] = float(np.max(np.abs(np.abs(re_.values[act]) - d64[act])) / h)
+    print(f"case h={h} band={band} n={len(soup)}: {res}", flush=True)
+    assert res["max_err_vox"] <= TOL_VOX, res
+    assert res["sign_mismatches"] == 0, res
+    return res
+
+
+def test_cfg1_torus_parity(gpu_ctx, oracle_lib):
+    """Torus R=1, r=0.35, 128x64 segments, radial noise +-0.01 (seed 1), h=0.02, band 3."""
+    soup = mg.torus(1.0, 0.35, 128, 64, 0.01, seed=1)
+    res = check_case(gpu_ctx, oracle_lib, soup, 0.02, 3, fp64=True)
+    assert res["n_active"] > 300000
+    assert res["max_err_vs_fp64_vox"] <= TOL_VOX
+
+
+@pytest.mark.parametrize("band", [0, 1, 2, 5])
+def test_small_sphere_bands(gpu_ctx, oracle_lib, band):
+    check_case(gpu_ctx, oracle_lib, mg.displaced_icosphere(3, radius=0.7, amp=0.05, seed=9), 0.03, band)
+
+
+def test_single_triangle_and_axis_aligned(gpu_ctx, oracle_lib):
+    # one triangle; a triangle flat in z (zero box extent -> +-1 expansion,
+    # mesh_to_volume.rs:136-141); negative coordinates across leaf boundaries
+    tri = np.array([[[-0.31, 0.2, 0.05], [0.4, -0.27, 0.11], [0.05, 0.33, -0.2]]], np.float32)
+    check_case(gpu_ctx, oracle_lib, tri, 0.05, 2)
+    flat = np.array([[[-0.5, -0.5, 0.0], [0.5, -0.5, 0.0], [0.0, 0.5, 0.0]]], np.float32)
+    check_case(gpu_ctx, oracle_lib, flat, 0.1, 0)
+    check_case(gpu_ctx, oracle_lib, flat, 0.1, 2)
+
+
+def test_large_triangle_many_subdivisions(gpu_ctx, oracle_lib):
+    tri = np.array([[[-1.0, -0.8, 0.1], [1.1, -0.7, -0.05], [0.0, 1.2, 0.2]]], np.float32)
+    check_case(gpu_ctx, oracle_lib, tri, 0.02, 1)   # ~110 rows, ~1.2e4 sub-triangles
+
+
+def test_degenerate_triangles(gpu_ctx, oracle_lib):
+    soup = mg.sphere_soup(2, radius=0.6)
+    deg = np.array([[[0.1, 0.1, 0.1], [0.1, 0.1, 0.1], [0.1, 0.1, 0.1]],          # a point
+                    [[0.0, 0.0, 0.0], [0.3, 0.3, 0.3], [0.6, 0.6, 0.6]]], np.float32)  # a segment
+    check_case(gpu_ctx, oracle_lib, np.concatenate([soup, deg]), 0.05, 2)
+
+
+def test_empty_mesh_is_none(gpu_ctx):
+    assert run_gpu(gpu_ctx, np.zeros((0, 3, 3), np.float32), 0.1, 3) is None
+
+
+def test_device_input_matches_host_input(gpu_ctx):
+    torch = pytest.importorskip("torch")
+    soup = mg.torus(1.0, 0.35, 64, 32, 0.01, seed=4)
+    a = run_gpu(gpu_ctx, soup, 0.03, 2).export()
+    b = run_gpu(gpu_ctx, torch.from_numpy(soup).cuda(), 0.03, 2).export()
+    assert np.array_equal(a.origins, b.origins) and np.array_equal(a.masks, b.masks)
+    assert np.array_equal(a.values, b.values)
+
+
+def test_repeatable(gpu_ctx):
+    soup = mg.displaced_icosphere(4, seed=5)
+    a = run_gpu(gpu_ctx, soup, 0.01, 3).export()
+    b = run_gpu(gpu_ctx, soup, 0.01, 3).export()
+    assert np.array_equal(a.values, b.values) and np.array_equal(a.masks, b.masks)
+
+
+def test_cfg2_full_size(gpu_ctx, oracle_lib):
+    """Displaced icosphere L8 (1,310,720 tris), h=0.002, band 3: active set and
+    leaf keys bit-exact against the union of dilated boxes; values at 20000
+    sampled voxels against the reference arithmetic and FP64; signs at 4000
+    sampled voxels against the brute-force winding number."""
+    h, band = 0.002, 3
+    soup = mg.displaced_icosphere(8, 1.0, 0.05, seed=2)
+    ctx = gpu_ctx
+    ctx.set_profiling(True)
+    g = run_gpu(ctx, soup, h, band)
+    stats = ctx.last_stats()
+    ctx.set_profiling(False)
+    print("cfg2 stats:", stats, flush=True)
+    ge = g.export()
+    o, m = oracle_lib.active_set(soup, h, band)
+    assert np.array_equal(ge.origins, o), "leaf keys differ"
+    assert np.array_equal(ge.masks, m), "active masks differ"
+    act = ge.active()
+    li, off = np.nonzero(act)
+    pick = RNG.choice(li.size, 20000, replace=False)
+    vox = ge.origins[li[pick]] + np.stack([off[pick] >> 6, (off[pick] >> 3) & 7, off[pick] & 7], 1)
+    v = ge.values[li[pick], off[pick]]
+    r32, r64 = oracle_lib.sample_unsigned(soup, h, band, vox)
+    err = np.abs(np.abs(v).astype(np.float64) - r32) / h
+    err64 = np.abs(np.abs(v).astype(np.float64) - r64) / h
+    print(f"cfg2 sampled: max err {err.max():.3g} vox, rms {np.sqrt(np.mean(err**2)):.3g}, "
+          f"bit-exact {np.mean(np.abs(v) == r32):.4f}; vs FP64 max {err64.max():.3g} rms "
+          f"{np.sqrt(np.mean(err64**2)):.3g}", flush=True)
+    assert err.max() <= TOL_VOX
+    sp = pick[:4000]
+    pts = ((ge.origins[li[sp]] + np.stack([off[sp] >> 6, (off[sp] >> 3) & 7, off[sp] & 7], 1)).astype(np.float32)
+           * np.float32(h))
+    w_ref, _ = oracle_lib.winding_numbers(soup, pts, 2.0)
+    big = np.abs(ge.values[li[sp], off[sp]]) > 1e-6 * h
+    ref_neg = ~(w_ref < 0.2)
+    mism = (ge.values[li[sp], off[sp]] < 0) != ref_neg
+    print(f"cfg2 sign mismatches vs reference-tree winding number: {int(mism[big].sum())} / {int(big.sum())}",
+          flush=True)
+    assert int(mism[big].sum()) == 0
