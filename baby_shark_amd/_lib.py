@@ -59,6 +59,14 @@ def load():
     with _lock:
         if _lib is not None:
             return _lib
+        # One HIP runtime per process: torch ships its own libamdhip64.so.7, and
+        # a second copy (the system one this library links) initialised first
+        # leaves torch without devices.  Loading torch first makes the dynamic
+        # linker bind this library to torch's runtime (same SONAME).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"bs_voxel HIP library not built: {LIB_PATH} is missing "
                                "(run `python __graft_entry__.py` or `make -C baby_shark_amd/csrc`)")
