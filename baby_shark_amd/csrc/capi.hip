@@ -99,6 +99,12 @@ bs_status bs_ctx_set_profiling(bs_ctx* ctx, int enabled) {
     return BS_OK;
 }
 
+bs_status bs_ctx_set_force_hash(bs_ctx* ctx, int enabled) {
+    if (!ctx) return BS_ERR_INVALID_ARG;
+    ctx->force_hash = enabled != 0;
+    return BS_OK;
+}
+
 bs_status bs_ctx_last_m2v_stats(const bs_ctx* ctx, bs_m2v_stats* out) {
     if (!ctx || !out) return BS_ERR_INVALID_ARG;
     *out = ctx->last_m2v;
@@ -136,7 +142,7 @@ bs_status bs_mesh_to_volume(bs_ctx* ctx, const float* tri_xyz, size_t n, const b
     return guarded([&] {
         DeviceGuard g(ctx->device);
         if (n == 0) throw Error(BS_ERR_EMPTY, "empty mesh: the reference returns None");
-        float* d = ctx->scratch.as<float>(S_MISC2, 9 * (uint64_t)n);
+        float* d = ctx->scratch.as<float>(S_INPUT, 9 * (uint64_t)n);
         BS_HIP(hipMemcpyAsync(d, tri_xyz, sizeof(float) * 9 * n, hipMemcpyHostToDevice, ctx->stream));
         mesh_to_volume_device(ctx, d, n, *p, out);
     });

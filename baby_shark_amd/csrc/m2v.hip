@@ -2,8 +2,9 @@
 //
 // Replaces MeshToVolume::convert (src/voxel/mesh_to_volume.rs:52-73):
 //   subdivide     mesh_to_volume.rs:75-116   one thread per (triangle, row)
-//   leaves        :118-168 active set        dilated boxes -> GPU hash -> compact -> sort
-//   distance      :118-196 min over pairs    one workgroup per 8^3 leaf, 4^3 block per wave
+//   leaves        :118-168 active set        dilated boxes -> dense leaf grid or GPU hash -> compact
+//   distance      :118-196 min over pairs    one workgroup per 8^3 leaf, box/leaf regions scattered
+//                                            into an LDS arg-min, reference arithmetic for the winner
 //   winding       aabb_tree.rs:571-817       LBVH + order-2 dipoles, wave-coherent traversal
 //   sign          mesh_to_volume.rs:238-292  wn < threshold => +|d| else -|d|
 #include <hipcub/hipcub.hpp>
@@ -121,12 +122,85 @@ __global__ void k_subdiv_emit(const float* __restrict__ tris, uint64_t n, float 
 
 __device__ inline int floor_div8(int v) { return v >> 3; }
 
-__global__ void k_subtri_setup(const float* __restrict__ sub, uint64_t n_sub, float inv_h, int band,
-                               SubRec* __restrict__ recs, uint32_t* __restrict__ cov, unsigned long long* counters,
-                               unsigned* err) {
+// per-block partial sums/bounds, reduced by k_reduce_partials (no same-address
+// atomics in the bulk kernels)
+struct Partial {
+    unsigned long long pairs;
+    int lo[3], hi[3];
+};
+
+template <int BLOCK>
+__device__ inline void block_reduce_partial(Partial mine, Partial* out) {
+    __shared__ Partial sp[BLOCK / 64];
+    for (int s = 32; s >= 1; s >>= 1) {
+        mine.pairs += __shfl_xor(mine.pairs, s);
+        for (int k = 0; k < 3; ++k) {
+            mine.lo[k] = min(mine.lo[k], __shfl_xor(mine.lo[k], s));
+            mine.hi[k] = max(mine.hi[k], __shfl_xor(mine.hi[k], s));
+        }
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sp[w] = mine;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Partial r = sp[0];
+        for (int i = 1; i < BLOCK / 64; ++i) {
+            r.pairs += sp[i].pairs;
+            for (int k = 0; k < 3; ++k) {
+                r.lo[k] = min(r.lo[k], sp[i].lo[k]);
+                r.hi[k] = max(r.hi[k], sp[i].hi[k]);
+            }
+        }
+        out[blockIdx.x] = r;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_reduce_partials(const Partial* __restrict__ in, unsigned n,
+                                                          unsigned long long* counters) {
+    Partial mine{0, {INT32_MAX, INT32_MAX, INT32_MAX}, {INT32_MIN, INT32_MIN, INT32_MIN}};
+    for (unsigned i = threadIdx.x; i < n; i += blockDim.x) {
+        const Partial& p = in[i];
+        mine.pairs += p.pairs;
+        for (int k = 0; k < 3; ++k) {
+            mine.lo[k] = min(mine.lo[k], p.lo[k]);
+            mine.hi[k] = max(mine.hi[k], p.hi[k]);
+        }
+    }
+    __shared__ Partial sp[16];
+    for (int s = 32; s >= 1; s >>= 1) {
+        mine.pairs += __shfl_xor(mine.pairs, s);
+        for (int k = 0; k < 3; ++k) {
+            mine.lo[k] = min(mine.lo[k], __shfl_xor(mine.lo[k], s));
+            mine.hi[k] = max(mine.hi[k], __shfl_xor(mine.hi[k], s));
+        }
+    }
+    if ((threadIdx.x & 63) == 0) sp[threadIdx.x >> 6] = mine;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Partial r = sp[0];
+        for (unsigned i = 1; i < blockDim.x / 64; ++i) {
+            r.pairs += sp[i].pairs;
+            for (int k = 0; k < 3; ++k) {
+                r.lo[k] = min(r.lo[k], sp[i].lo[k]);
+                r.hi[k] = max(r.hi[k], sp[i].hi[k]);
+            }
+        }
+        counters[C_PAIRS] = r.pairs;
+        for (int k = 0; k < 3; ++k) {
+            counters[C_BOX_LO + k] = (unsigned long long)(long long)r.lo[k];
+            counters[C_BOX_HI + k] = (unsigned long long)(long long)r.hi[k];
+        }
+    }
+}
+
+constexpr int SETUP_BLOCK = 256;
+
+__global__ __launch_bounds__(SETUP_BLOCK) void k_subtri_setup(const float* __restrict__ sub, uint64_t n_sub, float inv_h,
+                                                              int band, DistRec* __restrict__ recs,
+                                                              uint32_t* __restrict__ cov, Partial* partials,
+                                                              unsigned* err) {
     uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long pairs = 0;
-    int blo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, bhi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    Partial mine{0, {INT32_MAX, INT32_MAX, INT32_MAX}, {INT32_MIN, INT32_MIN, INT32_MIN}};
     if (t < n_sub) {
         F3 a, b, c;
         load_tri(sub + 9 * t, a, b, c);
@@ -144,59 +218,77 @@ __global__ void k_subtri_setup(const float* __restrict__ sub, uint64_t n_sub, fl
         if (!ok) atomicOr(err, E_RANGE);
         if (hi[0] == lo[0] || hi[1] == lo[1] || hi[2] == lo[2])
             for (int k = 0; k < 3; ++k) { lo[k] -= 1; hi[k] += 1; }
-        SubRec r;
-        F3 v[3] = {a, b, c};
-        for (int k = 0; k < 3; ++k) {
-            r.v[k][0] = v[k].x; r.v[k][1] = v[k].y; r.v[k][2] = v[k].z;
-            F3 e = f3_sub(v[(k + 1) % 3], v[k]);
-            r.e[k][0] = e.x; r.e[k][1] = e.y; r.e[k][2] = e.z;
-            float l2 = e.x * e.x + e.y * e.y + e.z * e.z;
-            r.inv_l2[k] = l2 > 0.0f ? 1.0f / l2 : 0.0f;
+        DistRec r;
+        r.a[0] = a.x; r.a[1] = a.y; r.a[2] = a.z;
+        const F3 e0 = f3_sub(b, a), ec = f3_sub(c, a);
+        const F3 nn = {e0.y * ec.z - e0.z * ec.y, e0.z * ec.x - e0.x * ec.z, e0.x * ec.y - e0.y * ec.x};
+        const float nl2 = nn.x * nn.x + nn.y * nn.y + nn.z * nn.z;
+        const float l0sq = e0.x * e0.x + e0.y * e0.y + e0.z * e0.z;
+        const bool degenerate = !(nl2 > 0.0f) || !(l0sq > 0.0f);
+        if (!degenerate) {
+            const float l0 = sqrtf(l0sq), inl = 1.0f / sqrtf(nl2);
+            const F3 U = {e0.x / l0, e0.y / l0, e0.z / l0};
+            const F3 N = {nn.x * inl, nn.y * inl, nn.z * inl};
+            const F3 V = {N.y * U.z - N.z * U.y, N.z * U.x - N.x * U.z, N.x * U.y - N.y * U.x};
+            r.U[0] = U.x; r.U[1] = U.y; r.U[2] = U.z;
+            r.V[0] = V.x; r.V[1] = V.y; r.V[2] = V.z;
+            r.N[0] = N.x; r.N[1] = N.y; r.N[2] = N.z;
+            r.L0 = l0;
+            r.inv0 = 1.0f / l0;
+            r.cu = U.x * ec.x + U.y * ec.y + U.z * ec.z;
+            r.cv = V.x * ec.x + V.y * ec.y + V.z * ec.z;
+            r.e1x = r.cu - l0;
+            r.e1y = r.cv;
+            const float l1 = r.e1x * r.e1x + r.e1y * r.e1y;
+            const float l2 = r.cu * r.cu + r.cv * r.cv;
+            r.inv1 = l1 > 0.0f ? 1.0f / l1 : 0.0f;
+            r.inv2 = l2 > 0.0f ? 1.0f / l2 : 0.0f;
+            r.flags = 0;
+        } else {
+            for (int k = 0; k < 3; ++k) r.U[k] = r.V[k] = r.N[k] = 0.0f;
+            r.L0 = r.inv0 = r.cu = r.cv = r.e1x = r.e1y = r.inv1 = r.inv2 = 0.0f;
+            r.flags = 1;
         }
-        F3 e0 = f3_sub(b, a), e2 = f3_sub(c, a);
-        F3 nn = {e0.y * e2.z - e0.z * e2.y, e0.z * e2.x - e0.x * e2.z, e0.x * e2.y - e0.y * e2.x};
-        float nl2 = nn.x * nn.x + nn.y * nn.y + nn.z * nn.z;
-        bool degenerate = !(nl2 > 0.0f);
-        float inl = degenerate ? 0.0f : 1.0f / sqrtf(nl2);
-        r.n[0] = nn.x * inl; r.n[1] = nn.y * inl; r.n[2] = nn.z * inl;
-        for (int k = 0; k < 3; ++k) {
-            F3 e = {r.e[k][0], r.e[k][1], r.e[k][2]};
-            // inward normal of edge k: n x e_k
-            r.m[k][0] = degenerate ? 0.0f : nn.y * e.z - nn.z * e.y;
-            r.m[k][1] = degenerate ? 0.0f : nn.z * e.x - nn.x * e.z;
-            r.m[k][2] = degenerate ? 0.0f : nn.x * e.y - nn.y * e.x;
-        }
-        r.bias = degenerate ? -1.0f : 0.0f;
+        r.pad = 0;
         for (int k = 0; k < 3; ++k) { r.box[k] = lo[k]; r.box[3 + k] = hi[k]; }
         recs[t] = r;
         uint32_t nl = 1;
         for (int k = 0; k < 3; ++k) nl *= (uint32_t)(floor_div8(hi[k]) - floor_div8(lo[k]) + 1);
         cov[t] = nl;
-        pairs = (unsigned long long)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
-        for (int k = 0; k < 3; ++k) { blo[k] = lo[k]; bhi[k] = hi[k]; }
+        mine.pairs = (unsigned long long)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
+        for (int k = 0; k < 3; ++k) { mine.lo[k] = lo[k]; mine.hi[k] = hi[k]; }
     }
-    // wave reductions, then one atomic per wave
-    for (int s = 32; s >= 1; s >>= 1) {
-        pairs += __shfl_xor(pairs, s);
-        for (int k = 0; k < 3; ++k) {
-            blo[k] = min(blo[k], __shfl_xor(blo[k], s));
-            bhi[k] = max(bhi[k], __shfl_xor(bhi[k], s));
-        }
-    }
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&counters[C_PAIRS], pairs);
-        for (int k = 0; k < 3; ++k) {
-            atomicMin((long long*)&counters[C_BOX_LO + k], (long long)blo[k]);
-            atomicMax((long long*)&counters[C_BOX_HI + k], (long long)bhi[k]);
-        }
-    }
+    block_reduce_partial<SETUP_BLOCK>(mine, partials);
 }
 
 // ---------------------------------------------------------------------------
-// leaf hash-and-compact
+// leaf allocation: a dense leaf grid over the boxes' bounding box when that
+// grid is small, else an open-addressing hash; both end in the same sorted
+// leaf list (lexicographic origin order) and a key -> leaf id lookup.
+
+struct LeafIndex {
+    // dense grid
+    uint32_t* grid = nullptr;
+    int gx0 = 0, gy0 = 0, gz0 = 0, gny = 0, gnz = 0;
+    uint64_t gsize = 0;
+    // hash
+    uint64_t* hkeys = nullptr;
+    uint32_t* hvals = nullptr;
+    uint32_t cap_mask = 0;
+    bool dense = true;
+
+    __device__ uint64_t dense_index(int lx, int ly, int lz) const {
+        return ((uint64_t)(lx - gx0) * gny + (uint64_t)(ly - gy0)) * gnz + (uint64_t)(lz - gz0);
+    }
+    __device__ uint32_t find(int lx, int ly, int lz) const {
+        if (dense) return grid[dense_index(lx, ly, lz)];
+        uint32_t slot = hash_find(hkeys, cap_mask, leaf_key(lx, ly, lz));
+        return slot == NOT_FOUND ? NOT_FOUND : hvals[slot];
+    }
+};
 
 template <class F>
-__device__ inline void for_each_leaf(const SubRec& r, F&& f) {
+__device__ inline void for_each_leaf(const DistRec& r, F&& f) {
     int x0 = floor_div8(r.box[0]), y0 = floor_div8(r.box[1]), z0 = floor_div8(r.box[2]);
     int x1 = floor_div8(r.box[3]), y1 = floor_div8(r.box[4]), z1 = floor_div8(r.box[5]);
     uint32_t j = 0;
@@ -205,35 +297,51 @@ __device__ inline void for_each_leaf(const SubRec& r, F&& f) {
             for (int z = z0; z <= z1; ++z) f(j++, x, y, z);
 }
 
-__global__ void k_hash_insert(const SubRec* __restrict__ recs, uint64_t n_sub, uint64_t* keys, uint32_t cap_mask,
-                              unsigned* err) {
+__global__ void k_leaf_mark(const DistRec* __restrict__ recs, uint64_t n_sub, LeafIndex ix, unsigned* err) {
     uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_sub) return;
-    const SubRec& r = recs[t];
+    const DistRec& r = recs[t];
     bool ok = true;
-    for_each_leaf(r, [&](uint32_t, int x, int y, int z) { ok &= hash_insert(keys, cap_mask, leaf_key(x, y, z)); });
+    for_each_leaf(r, [&](uint32_t, int x, int y, int z) {
+        if (ix.dense) ix.grid[ix.dense_index(x, y, z)] = 1u;
+        else ok &= hash_insert(ix.hkeys, ix.cap_mask, leaf_key(x, y, z));
+    });
     if (!ok) atomicOr(err, E_HASH);
 }
 
-__global__ void k_hash_compact(const uint64_t* __restrict__ keys, uint32_t cap, uint64_t* out, unsigned long long* counters) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool occ = i < cap && keys[i] != EMPTY_KEY;
-    uint64_t m = __ballot(occ);
-    unsigned long long base = 0;
-    int lane = threadIdx.x & 63;
-    if (lane == 0 && m) base = atomicAdd(&counters[C_LEAVES], (unsigned long long)__popcll(m));
-    base = __shfl(base, 0);
-    if (occ) out[base + __popcll(m & ((1ull << lane) - 1))] = keys[i];
+struct DenseOccupied {
+    const uint32_t* grid;
+    __device__ bool operator()(uint32_t i) const { return grid[i] != 0u; }
+};
+struct HashOccupied {
+    __device__ bool operator()(uint64_t k) const { return k != EMPTY_KEY; }
+};
+
+// leaf ids and origins, from dense grid indices (already in lexicographic order)
+__global__ void k_leaf_assign_dense(const uint32_t* __restrict__ idx, uint64_t n_leaves, LeafIndex ix,
+                                    int32_t* origins) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_leaves) return;
+    uint64_t d = idx[i];
+    ix.grid[d] = (uint32_t)i;
+    const int lz = (int)(d % (uint64_t)ix.gnz) + ix.gz0;
+    d /= (uint64_t)ix.gnz;
+    const int ly = (int)(d % (uint64_t)ix.gny) + ix.gy0;
+    const int lx = (int)(d / (uint64_t)ix.gny) + ix.gx0;
+    origins[4 * i + 0] = lx * 8;
+    origins[4 * i + 1] = ly * 8;
+    origins[4 * i + 2] = lz * 8;
+    origins[4 * i + 3] = 0;
 }
 
-__global__ void k_hash_assign(const uint64_t* __restrict__ sorted, uint64_t n_leaves, const uint64_t* __restrict__ keys,
-                              uint32_t cap_mask, uint32_t* vals, int32_t* origins, unsigned* err) {
+__global__ void k_leaf_assign_hash(const uint64_t* __restrict__ sorted, uint64_t n_leaves, LeafIndex ix,
+                                   int32_t* origins, unsigned* err) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_leaves) return;
     uint64_t k = sorted[i];
-    uint32_t slot = hash_find(keys, cap_mask, k);
+    uint32_t slot = hash_find(ix.hkeys, ix.cap_mask, k);
     if (slot == NOT_FOUND) atomicOr(err, E_HASH);
-    else vals[slot] = (uint32_t)i;
+    else ix.hvals[slot] = (uint32_t)i;
     int lx, ly, lz;
     leaf_key_decode(k, lx, ly, lz);
     origins[4 * i + 0] = lx * 8;
@@ -242,92 +350,129 @@ __global__ void k_hash_assign(const uint64_t* __restrict__ sorted, uint64_t n_le
     origins[4 * i + 3] = 0;
 }
 
-// candidate lists: for every leaf, the sub-triangles whose dilated box touches it
-__global__ void k_cand_count(const SubRec* __restrict__ recs, uint64_t n_sub, const uint32_t* __restrict__ cov_off,
-                             const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals, uint32_t cap_mask,
-                             uint32_t* cand_leaf, uint32_t* cand_cnt, uint32_t n_leaves, unsigned* err) {
+// candidate lists: (leaf id, sub-triangle) for every box/leaf overlap,
+// radix-sorted by leaf id (stable, so sub-triangle order within a leaf)
+__global__ void k_cand_pairs(const DistRec* __restrict__ recs, uint64_t n_sub, const uint32_t* __restrict__ cov_off,
+                             LeafIndex ix, uint32_t n_leaves, uint32_t* keys, uint32_t* vals, unsigned* err) {
     uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_sub) return;
-    const SubRec& r = recs[t];
-    uint32_t base = cov_off[t];
+    const DistRec& r = recs[t];
+    const uint32_t base = cov_off[t];
+    bool ok = true;
     for_each_leaf(r, [&](uint32_t j, int x, int y, int z) {
-        uint32_t slot = hash_find(keys, cap_mask, leaf_key(x, y, z));
-        uint32_t lid = slot == NOT_FOUND ? NOT_FOUND : vals[slot];
+        uint32_t lid = ix.find(x, y, z);
         if (lid >= n_leaves) {
-            atomicOr(err, E_HASH);
-            lid = n_leaves;  // the spare counter slot: never read back as a leaf
+            ok = false;
+            lid = n_leaves;  // sorts past every real leaf; never read
         }
-        cand_leaf[base + j] = lid;
-        atomicAdd(&cand_cnt[lid], 1u);
+        keys[base + j] = lid;
+        vals[base + j] = (uint32_t)t;
     });
+    if (!ok) atomicOr(err, E_HASH);
 }
 
-__global__ void k_cand_fill(uint64_t n_sub, const uint32_t* __restrict__ cov_off, const uint32_t* __restrict__ cand_leaf,
-                            const uint32_t* __restrict__ cand_off, uint32_t* cursor, uint32_t* cand_list,
-                            uint32_t n_leaves) {
-    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_sub) return;
-    for (uint32_t e = cov_off[t]; e < cov_off[t + 1]; ++e) {
-        uint32_t lid = cand_leaf[e];
-        if (lid >= n_leaves) continue;
-        uint32_t pos = atomicAdd(&cursor[lid], 1u);
-        cand_list[cand_off[lid] + pos] = (uint32_t)t;
-    }
+__global__ void k_cand_offsets(const uint32_t* __restrict__ keys, uint32_t n, uint32_t n_leaves, uint32_t* off) {
+    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const uint32_t k = keys[e];
+    const uint32_t prev = e == 0 ? 0xFFFFFFFFu : keys[e - 1];
+    if (k != prev && k < n_leaves) off[k] = e;
+    if (k >= n_leaves && (prev < n_leaves || e == 0)) off[n_leaves] = e;  // first invalid entry ends the lists
+    if (e == n - 1 && k < n_leaves) off[n_leaves] = n;
 }
 
 // ---------------------------------------------------------------------------
-// distance: one workgroup per leaf, wave w owns the 4^3 block
-// (w>>2, (w>>1)&1, w&1) * 4, one voxel per lane.  A voxel's candidates are the
-// sub-triangles whose dilated box contains it (the reference pair set); the
-// wave skips a candidate whose box misses its block, each lane skips one whose
-// box misses its voxel.  The arg-min by squared distance (ties -> lower
-// sub-triangle index) is re-evaluated with the reference's closest-point
-// arithmetic to give the stored value.
-__global__ __launch_bounds__(512) void k_distance(const int32_t* __restrict__ origins, const uint32_t* __restrict__ cand_off,
-                                                  const uint32_t* __restrict__ cand_list, const SubRec* __restrict__ recs,
-                                                  float h, float* __restrict__ values, uint64_t* __restrict__ masks) {
-    __shared__ unsigned long long smask[8];
+// distance: one workgroup per leaf, LDS best[512] of (d^2 bits << 32 | t).
+// Wave w takes candidates w, w+4, ...; for a candidate it walks the box/leaf
+// intersection with lanes packed as (x-slice, y, z): the y,z part of the local
+// frame coordinates is computed once per candidate, a step in x costs one FMA
+// per coordinate.  Merging is a 64-bit LDS atomic min, so equal squared
+// distances resolve to the lower sub-triangle index whatever the order.
+// The arg-min is re-evaluated with the reference's closest-point arithmetic.
+
+// floor(65536 / d) + 1: (i * m) >> 16 == i / d for i < 1024, d <= 64
+__constant__ uint32_t kMagic[65] = {
+    0, 65537, 32769, 21846, 16385, 13108, 10923, 9363, 8193, 7282, 6554, 5958, 5462, 5042, 4682, 4370, 4097,
+    3856, 3641, 3450, 3277, 3121, 2979, 2850, 2731, 2622, 2521, 2428, 2341, 2260, 2185, 2115, 2049, 1986, 1928,
+    1873, 1821, 1772, 1725, 1681, 1639, 1599, 1561, 1525, 1490, 1457, 1425, 1395, 1366, 1338, 1311, 1286, 1261,
+    1237, 1214, 1192, 1171, 1150, 1130, 1111, 1093, 1075, 1058, 1041, 1025};
+
+constexpr int DIST_BLOCK = 256;
+
+__global__ __launch_bounds__(DIST_BLOCK) void k_distance(const int32_t* __restrict__ origins,
+                                                         const uint32_t* __restrict__ cand_off,
+                                                         const uint32_t* __restrict__ cand_list,
+                                                         const DistRec* __restrict__ recs,
+                                                         const float* __restrict__ sub, float h, uint32_t n_cand,
+                                                         float* __restrict__ values, uint64_t* __restrict__ masks) {
+    __shared__ unsigned long long best[LEAF_VOX];
     const int leaf = blockIdx.x;
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
-    if (tid < 8) smask[tid] = 0;
+    for (int i = tid; i < LEAF_VOX; i += DIST_BLOCK) best[i] = ~0ull;
     const int ox = origins[4 * leaf], oy = origins[4 * leaf + 1], oz = origins[4 * leaf + 2];
-    const int bx = (wave >> 2) * 4, by = ((wave >> 1) & 1) * 4, bz = (wave & 1) * 4;
-    const int lx = bx + (lane >> 4), ly = by + ((lane >> 2) & 3), lz = bz + (lane & 3);
-    const int vx = ox + lx, vy = oy + ly, vz = oz + lz;
-    const float px = (float)vx * h, py = (float)vy * h, pz = (float)vz * h;
-    const int X0 = ox + bx, Y0 = oy + by, Z0 = oz + bz;
-    float best = INFINITY;
-    uint32_t best_t = 0xFFFFFFFFu;
-    const uint32_t c0 = cand_off[leaf], c1 = cand_off[leaf + 1];
-    for (uint32_t c = c0; c < c1; ++c) {
+    const uint32_t c1 = min(cand_off[leaf + 1], n_cand), c0 = min(cand_off[leaf], c1);
+    __syncthreads();
+    for (uint32_t c = c0 + wave; c < c1; c += DIST_BLOCK / 64) {
         const uint32_t t = __builtin_amdgcn_readfirstlane(cand_list[c]);
-        const SubRec& r = recs[t];
-        const int b0 = r.box[0], b1 = r.box[1], b2 = r.box[2], b3 = r.box[3], b4 = r.box[4], b5 = r.box[5];
-        if (b0 > X0 + 3 || b3 < X0 || b1 > Y0 + 3 || b4 < Y0 || b2 > Z0 + 3 || b5 < Z0) continue;
-        const bool inb = (unsigned)(vx - b0) <= (unsigned)(b3 - b0) && (unsigned)(vy - b1) <= (unsigned)(b4 - b1) &&
-                         (unsigned)(vz - b2) <= (unsigned)(b5 - b2);
-        if (inb) {
-            float d2 = fast_dist2(r, px, py, pz);
-            if (d2 < best || (d2 == best && t < best_t)) {
-                best = d2;
-                best_t = t;
+        const DistRec& r = recs[t];
+        const int x0 = max(r.box[0], ox), x1 = min(r.box[3], ox + 7);
+        const int y0 = max(r.box[1], oy), y1 = min(r.box[4], oy + 7);
+        const int z0 = max(r.box[2], oz), z1 = min(r.box[5], oz + 7);
+        const int nz = z1 - z0 + 1, ny = y1 - y0 + 1, nyz = ny * nz;
+        if (x1 < x0 || ny <= 0 || nz <= 0) continue;  // box misses the leaf: never for a listed candidate
+        const int spp = 64 / nyz;  // x-slices per pass
+        const int s = (int)(((uint32_t)lane * kMagic[nyz]) >> 16);
+        const int rr = lane - s * nyz;
+        const int yq = (int)(((uint32_t)rr * kMagic[nz]) >> 16);
+        const int y = y0 + yq, z = z0 + (rr - yq * nz);
+        const int loff = ((y - oy) << 3) | (z - oz);
+        const float py = (float)y * h, pz = (float)z * h;
+        if (s < spp) {
+            if (r.flags == 0) {
+                const float dy = py - r.a[1], dz = pz - r.a[2];
+                const float bu = __builtin_fmaf(r.U[1], dy, r.U[2] * dz);
+                const float bv = __builtin_fmaf(r.V[1], dy, r.V[2] * dz);
+                const float bw = __builtin_fmaf(r.N[1], dy, r.N[2] * dz);
+                for (int x = x0 + s; x <= x1; x += spp) {
+                    const float dx = (float)x * h - r.a[0];
+                    const float pu = __builtin_fmaf(r.U[0], dx, bu);
+                    const float pv = __builtin_fmaf(r.V[0], dx, bv);
+                    const float pw = __builtin_fmaf(r.N[0], dx, bw);
+                    const float d2 = __builtin_fmaf(pw, pw, plane_dist2(r, pu, pv));
+                    const unsigned long long key = ((unsigned long long)__float_as_uint(d2) << 32) | t;
+                    atomicMin(&best[((x - ox) << 6) | loff], key);
+                }
+            } else {
+                const float* v = sub + 9 * (uint64_t)t;
+                for (int x = x0 + s; x <= x1; x += spp) {
+                    const float d = ericson_distance({v[0], v[1], v[2]}, {v[3], v[4], v[5]}, {v[6], v[7], v[8]},
+                                                     {(float)x * h, py, pz});
+                    if (d == d) {  // a NaN distance never enters the field (`dist < cur` is false)
+                        const unsigned long long key = ((unsigned long long)__float_as_uint(d * d) << 32) | t;
+                        atomicMin(&best[((x - ox) << 6) | loff], key);
+                    }
+                }
             }
         }
     }
-    const int off = (lx << 6) | (ly << 3) | lz;
-    const bool active = best_t != 0xFFFFFFFFu;
-    float v = 0.0f;
-    if (active) {
-        const SubRec& r = recs[best_t];
-        v = ericson_distance({r.v[0][0], r.v[0][1], r.v[0][2]}, {r.v[1][0], r.v[1][1], r.v[1][2]},
-                             {r.v[2][0], r.v[2][1], r.v[2][2]}, {px, py, pz});
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < LEAF_VOX / DIST_BLOCK; ++k) {
+        const int off = tid + DIST_BLOCK * k;
+        const unsigned long long key = best[off];
+        const bool active = key != ~0ull;
+        float v = 0.0f;
+        if (active) {
+            const float* tv = sub + 9 * (uint64_t)(uint32_t)key;
+            const float px = (float)(ox + (off >> 6)) * h, py = (float)(oy + ((off >> 3) & 7)) * h,
+                        pz = (float)(oz + (off & 7)) * h;
+            v = ericson_distance({tv[0], tv[1], tv[2]}, {tv[3], tv[4], tv[5]}, {tv[6], tv[7], tv[8]}, {px, py, pz});
+        }
+        values[(uint64_t)leaf * LEAF_VOX + off] = v;
+        const unsigned long long m = __ballot(active);
+        if (lane == 0) masks[(uint64_t)leaf * 8 + (off >> 6)] = m;
     }
-    values[(uint64_t)leaf * LEAF_VOX + off] = v;
-    __syncthreads();
-    if (active) atomicOr(&smask[off >> 6], 1ull << (off & 63));
-    __syncthreads();
-    if (tid < 8) masks[(uint64_t)leaf * 8 + tid] = smask[tid];
 }
 
 // active voxel list: slot = leaf * 512 + offset, leaves in order
@@ -360,7 +505,8 @@ __device__ inline float from_ordered_bits(unsigned u) {
     return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
 }
 
-__global__ void k_centroid_bounds(const float* __restrict__ tris, uint64_t n, unsigned* bounds) {
+// centroid bounds per block (ordered-int encoding), reduced by k_bounds_final
+__global__ __launch_bounds__(256) void k_centroid_bounds(const float* __restrict__ tris, uint64_t n, uint4* part) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned lo[3] = {~0u, ~0u, ~0u}, hi[3] = {0, 0, 0};
     if (i < n) {
@@ -375,11 +521,41 @@ __global__ void k_centroid_bounds(const float* __restrict__ tris, uint64_t n, un
             lo[k] = min(lo[k], (unsigned)__shfl_xor((int)lo[k], s));
             hi[k] = max(hi[k], (unsigned)__shfl_xor((int)hi[k], s));
         }
+    __shared__ unsigned sl[4][3], sh[4][3];
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 3; ++k) { sl[w][k] = lo[k]; sh[w][k] = hi[k]; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < 4; ++q)
+            for (int k = 0; k < 3; ++k) { lo[k] = min(lo[k], sl[q][k]); hi[k] = max(hi[k], sh[q][k]); }
+        part[2 * blockIdx.x] = make_uint4(lo[0], lo[1], lo[2], 0);
+        part[2 * blockIdx.x + 1] = make_uint4(hi[0], hi[1], hi[2], 0);
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_bounds_final(const uint4* __restrict__ part, unsigned nb, unsigned* bounds) {
+    unsigned lo[3] = {~0u, ~0u, ~0u}, hi[3] = {0, 0, 0};
+    for (unsigned b = threadIdx.x; b < nb; b += blockDim.x) {
+        uint4 l = part[2 * b], u = part[2 * b + 1];
+        lo[0] = min(lo[0], l.x); lo[1] = min(lo[1], l.y); lo[2] = min(lo[2], l.z);
+        hi[0] = max(hi[0], u.x); hi[1] = max(hi[1], u.y); hi[2] = max(hi[2], u.z);
+    }
+    for (int s = 32; s >= 1; s >>= 1)
         for (int k = 0; k < 3; ++k) {
-            atomicMin(&bounds[k], lo[k]);
-            atomicMax(&bounds[3 + k], hi[k]);
+            lo[k] = min(lo[k], (unsigned)__shfl_xor((int)lo[k], s));
+            hi[k] = max(hi[k], (unsigned)__shfl_xor((int)hi[k], s));
         }
+    __shared__ unsigned sl[16][3], sh[16][3];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 3; ++k) { sl[w][k] = lo[k]; sh[w][k] = hi[k]; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (unsigned q = 1; q < blockDim.x / 64; ++q)
+            for (int k = 0; k < 3; ++k) { lo[k] = min(lo[k], sl[q][k]); hi[k] = max(hi[k], sh[q][k]); }
+        for (int k = 0; k < 3; ++k) { bounds[k] = lo[k]; bounds[3 + k] = hi[k]; }
+    }
 }
 
 __device__ inline uint64_t spread21(uint64_t v) {
@@ -489,9 +665,14 @@ __device__ inline void acc_finalize(const NodeAcc& a, float acc, float4* node) {
     node[3] = make_float4(o2[2] + o2[6], o2[5] + o2[7], 0.0f, 0.0f);
 }
 
+// Leaf data, then node heights bottom-up with one 32-bit atomicMax per step:
+// the first child to arrive stores its height + 1 and stops, the second reads
+// it back from the same atomic and carries on.  No data other than the atomic
+// word crosses threads, so no memory fences are needed; node data is then
+// combined level by level (k_lbvh_level), one launch per height.
 __global__ void k_lbvh_leaves(const float* __restrict__ tris, int n, const uint32_t* __restrict__ order, float acc,
-                              const int4* __restrict__ links, const int* __restrict__ parent, unsigned* flags,
-                              NodeAcc* accs, float4* nodes, float* btris, int4* leaf_links, unsigned* err) {
+                              const int* __restrict__ parent, unsigned* hword, uint32_t* height, NodeAcc* accs,
+                              float4* nodes, float* btris, int4* leaf_links, unsigned* err) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     F3 a, b, c;
@@ -525,23 +706,43 @@ __global__ void k_lbvh_leaves(const float* __restrict__ tris, int n, const uint3
     accs[node] = d;
     acc_finalize(d, acc, nodes + 4 * (uint64_t)node);
     leaf_links[k] = make_int4(-1, -1, k, 1);
+    unsigned hgt = 0;
     int p = parent[node];
     for (int level = 0; p >= 0; ++level) {
-        if (level > 4096) {  // deeper than any tree of < 2^30 leaves: corrupt parent links
+        if (level > 250) {  // deeper than the 8-bit height sort allows: corrupt parent links
             atomicOr(err, E_TREE);
             return;
         }
-        __threadfence();
-        unsigned old = atomicAdd(&flags[p], 1u);
+        unsigned old = atomicMax(&hword[p], hgt + 1);
         if (old == 0) return;  // the sibling finishes this node
-        __threadfence();
-        int4 L = links[p];
-        NodeAcc l = accs[L.x], r = accs[L.y], s;
-        acc_add(s, l, r);
-        accs[p] = s;
-        acc_finalize(s, acc, nodes + 4 * (uint64_t)p);
+        hgt = max(old, hgt + 1);
+        height[p] = hgt;
         p = parent[p];
     }
+}
+
+__global__ void k_lbvh_level(const uint32_t* __restrict__ level_nodes, uint32_t count, float acc,
+                             const int4* __restrict__ links, NodeAcc* accs, float4* nodes) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int p = (int)level_nodes[i];
+    const int4 L = links[p];
+    NodeAcc s;
+    acc_add(s, accs[L.x], accs[L.y]);
+    accs[p] = s;
+    acc_finalize(s, acc, nodes + 4 * (uint64_t)p);
+}
+
+__global__ void k_iota(uint32_t* out, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = i;
+}
+
+__global__ void k_level_starts(const uint32_t* __restrict__ hs, uint32_t n, uint32_t* starts) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t h = hs[i];
+    if (h < 256 && (i == 0 || hs[i - 1] != h)) starts[h] = i;
 }
 
 // ---------------------------------------------------------------------------
@@ -554,13 +755,13 @@ __global__ void k_lbvh_leaves(const float* __restrict__ tris, int n, const uint3
 // coincides with the query or the triple product vanishes.
 __device__ inline float solid_angle_rel(float ax, float ay, float az, float bx, float by, float bz, float cx, float cy,
                                         float cz) {
-    float la = sqrtf(ax * ax + ay * ay + az * az);
-    float lb = sqrtf(bx * bx + by * by + bz * bz);
-    float lc = sqrtf(cx * cx + cy * cy + cz * cz);
-    float det = ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx);
-    float den = la * lb * lc + (ax * bx + ay * by + az * bz) * lc + (ax * cx + ay * cy + az * cz) * lb +
-                (bx * cx + by * cy + bz * cz) * la;
-    float w = 2.0f * atan2f(det, den);
+    const float la = fast_sqrt(ax * ax + ay * ay + az * az);
+    const float lb = fast_sqrt(bx * bx + by * by + bz * bz);
+    const float lc = fast_sqrt(cx * cx + cy * cy + cz * cz);
+    const float det = ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx);
+    const float den = la * lb * lc + (ax * bx + ay * by + az * bz) * lc + (ax * cx + ay * cy + az * cz) * lb +
+                      (bx * cx + by * cy + bz * cz) * la;
+    const float w = 2.0f * fast_atan2(det, den);
     return (la == 0.0f || lb == 0.0f || lc == 0.0f || det == 0.0f) ? 0.0f : w;
 }
 
@@ -576,20 +777,21 @@ __device__ inline double solid_angle_rel64(double ax, double ay, double az, doub
     return 2.0 * atan2(det, den);
 }
 
-// Neumaier compensated accumulation
+// Kahan-compensated accumulation
 __device__ inline void kb_add(float& s, float& c, float x) {
-    float t = s + x;
-    c += (fabsf(s) >= fabsf(x)) ? ((s - t) + x) : ((x - t) + s);
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
     s = t;
 }
 
 // order-2 far-field term of a node at query offset r = c - p
 __device__ inline float dipole_term(const float4* __restrict__ nd, float rx, float ry, float rz, float r2) {
-    float4 B = nd[1], C = nd[2], D = nd[3];
-    float inv_r = 1.0f / sqrtf(r2);
-    float inv_r2 = inv_r * inv_r;
-    float rsr = C.x * rx * rx + C.y * ry * ry + C.z * rz * rz + C.w * rx * ry + D.x * rx * rz + D.y * ry * rz;
-    float lin = B.x * rx + B.y * ry + B.z * rz + B.w;
+    const float4 B = nd[1], C = nd[2], D = nd[3];
+    const float inv_r = fast_rsq(r2);
+    const float inv_r2 = inv_r * inv_r;
+    const float rsr = C.x * rx * rx + C.y * ry * ry + C.z * rz * rz + C.w * rx * ry + D.x * rx * rz + D.y * ry * rz;
+    const float lin = B.x * rx + B.y * ry + B.z * rz + B.w;
     return INV_4PI * inv_r * inv_r2 * (lin - 3.0f * rsr * inv_r2);
 }
 
@@ -619,8 +821,10 @@ __global__ __launch_bounds__(WN_BLOCK) void k_winding(const uint32_t* __restrict
     float s = 0.0f, cmp = 0.0f;
     unsigned long long n_near_int = 0, n_far = 0, n_leaf = 0, n_tri = 0;
     int sp = 1;
-    st_node[wave][0] = 0;  // root: internal node 0 (or leaf 0 when n == 1)
-    st_mask[wave][0] = init;
+    if (lane == 0) {
+        st_node[wave][0] = 0;  // root: internal node 0 (or leaf 0 when n == 1)
+        st_mask[wave][0] = init;
+    }
     bool overflow = false;
     // every node is pushed at most once per wave in a tree: more pops than
     // nodes means corrupt links, and the walk stops
@@ -662,10 +866,12 @@ __global__ __launch_bounds__(WN_BLOCK) void k_winding(const uint32_t* __restrict
                 overflow = true;
                 break;
             }
-            st_node[wave][sp] = L.y;
-            st_mask[wave][sp] = nm;
-            st_node[wave][sp + 1] = L.x;
-            st_mask[wave][sp + 1] = nm;
+            if (lane == 0) {
+                st_node[wave][sp] = L.y;
+                st_mask[wave][sp] = nm;
+                st_node[wave][sp + 1] = L.x;
+                st_mask[wave][sp + 1] = nm;
+            }
             sp += 2;
         }
     }
@@ -677,7 +883,7 @@ __global__ __launch_bounds__(WN_BLOCK) void k_winding(const uint32_t* __restrict
         atomicAdd(&counters[C_LEAF_TRIS], n_tri);
     }
     if (!valid) return;
-    const float w = s + cmp;
+    const float w = s - cmp;
     float* vp = values + slot;
     const float d = *vp;
     *vp = (w < thr) ? copysignf(d, 1.0f) : copysignf(d, -1.0f);
@@ -758,10 +964,35 @@ T read_scalar(bs_ctx* ctx, const T* dptr) {
 
 template <class TIn, class TOut>
 void exclusive_scan(bs_ctx* ctx, const TIn* in, TOut* out, uint64_t n) {
+    BS_REQUIRE(n < (1ull << 31), BS_ERR_INVALID_ARG, "scan longer than 2^31 elements");
     size_t bytes = 0;
     BS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, ctx->stream));
     void* tmp = ctx->scratch.get(S_CUB, bytes);
     BS_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, (int)n, ctx->stream));
+}
+
+template <class K, class V>
+void sort_pairs(bs_ctx* ctx, const K* k, K* ks, const V* v, V* vs, uint64_t n, int end_bit) {
+    BS_REQUIRE(n < (1ull << 31), BS_ERR_INVALID_ARG, "sort longer than 2^31 elements");
+    size_t bytes = 0;
+    BS_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, ks, v, vs, (int)n, 0, end_bit, ctx->stream));
+    void* tmp = ctx->scratch.get(S_CUB, bytes);
+    BS_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, bytes, k, ks, v, vs, (int)n, 0, end_bit, ctx->stream));
+}
+
+template <class It, class Out, class Pred>
+void select_if(bs_ctx* ctx, It in, Out out, uint32_t* d_count, uint64_t n, Pred pred) {
+    BS_REQUIRE(n < (1ull << 31), BS_ERR_INVALID_ARG, "selection over more than 2^31 elements");
+    size_t bytes = 0;
+    BS_HIP(hipcub::DeviceSelect::If(nullptr, bytes, in, out, d_count, (int)n, pred, ctx->stream));
+    void* tmp = ctx->scratch.get(S_CUB, bytes);
+    BS_HIP(hipcub::DeviceSelect::If(tmp, bytes, in, out, d_count, (int)n, pred, ctx->stream));
+}
+
+int bits_for(uint64_t v) {
+    int b = 1;
+    while ((1ull << b) <= v) ++b;
+    return b;
 }
 
 struct PhaseTimer {
@@ -797,6 +1028,9 @@ uint32_t next_pow2(uint64_t v) {
 
 }  // namespace
 
+// Dense leaf grids up to this many cells (4 B each); larger extents hash.
+constexpr uint64_t DENSE_LEAF_GRID_MAX = 1ull << 28;
+
 void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const bs_m2v_params& prm, bs_volume** out) {
     *out = nullptr;
     BS_REQUIRE(prm.voxel_size > 0.0f && std::isfinite(prm.voxel_size), BS_ERR_INVALID_ARG, "voxel_size must be > 0");
@@ -814,15 +1048,8 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
 
     unsigned long long* counters = S.as<unsigned long long>(S_COUNTERS, C_COUNT);
     unsigned* err = reinterpret_cast<unsigned*>(S.get(S_MISC3, 64));
-    {
-        unsigned long long init[C_COUNT] = {};
-        for (int k = 0; k < 3; ++k) {
-            init[C_BOX_LO + k] = (unsigned long long)(long long)INT32_MAX;
-            init[C_BOX_HI + k] = (unsigned long long)(long long)INT32_MIN;
-        }
-        BS_HIP(hipMemcpyAsync(counters, init, sizeof(init), hipMemcpyHostToDevice, st));
-        BS_HIP(hipMemsetAsync(err, 0, 64, st));
-    }
+    BS_HIP(hipMemsetAsync(counters, 0, sizeof(unsigned long long) * C_COUNT, st));
+    BS_HIP(hipMemsetAsync(err, 0, 64, st));
     const unsigned TB = 256;
 
     // ---- 1. subdivision
@@ -840,86 +1067,105 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     if (read_scalar(ctx, err) & E_TOO_MANY_SUBS)
         throw Error(BS_ERR_INVALID_ARG, "a triangle is subdivided into more than 2^32 sub-triangles");
     if (n_sub == 0) throw Error(BS_ERR_EMPTY, "no sub-triangles: the reference returns None");
-    BS_REQUIRE(n_sub < (1ull << 32), BS_ERR_INVALID_ARG, "more than 2^32 sub-triangles");
+    BS_REQUIRE(n_sub < (1ull << 31) - 1, BS_ERR_INVALID_ARG, "more than 2^31 sub-triangles");
     float* sub = S.as<float>(S_SUBTRIS, 9 * n_sub);
     k_subdiv_emit<<<div_up(total_rows, TB), TB, 0, st>>>(d_tris, n, h, row_off, sub_off, total_rows, sub);
     tm.mark();
 
-    // ---- 2. boxes, records, leaf hash-and-compact
-    SubRec* recs = S.as<SubRec>(S_RECS, n_sub);
+    // ---- 2. boxes, records, leaf allocation, candidate lists
+    DistRec* recs = S.as<DistRec>(S_RECS, n_sub);
     uint32_t* cov = S.as<uint32_t>(S_LEAFCNT, n_sub + 1);
     uint32_t* cov_off = S.as<uint32_t>(S_LEAFOFF, n_sub + 1);
+    const unsigned setup_blocks = div_up(n_sub, SETUP_BLOCK);
+    Partial* partials = S.as<Partial>(S_MISC0, setup_blocks);
     BS_HIP(hipMemsetAsync(cov + n_sub, 0, sizeof(uint32_t), st));
-    k_subtri_setup<<<div_up(n_sub, TB), TB, 0, st>>>(sub, n_sub, inv_h, band, recs, cov, counters, err);
+    k_subtri_setup<<<setup_blocks, SETUP_BLOCK, 0, st>>>(sub, n_sub, inv_h, band, recs, cov, partials, err);
+    k_reduce_partials<<<1, 1024, 0, st>>>(partials, setup_blocks, counters);
     exclusive_scan(ctx, cov, cov_off, n_sub + 1);
     unsigned long long hc[C_COUNT];
     BS_HIP(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, st));
-    uint32_t n_cov = read_scalar(ctx, cov_off + n_sub);
+    const uint64_t n_cov = read_scalar(ctx, cov_off + n_sub);
     if (read_scalar(ctx, err) & E_RANGE)
         throw Error(BS_ERR_INVALID_ARG, "mesh extent exceeds 8e6 voxels from the origin");
+    BS_REQUIRE(n_cov < (1ull << 31), BS_ERR_INVALID_ARG, "more than 2^31 box/leaf overlaps");
     const uint64_t n_pairs = hc[C_PAIRS];
+    LeafIndex ix;
+    long long glo[3], ghi[3];
     uint64_t bbox_leaves = 1;
     for (int k = 0; k < 3; ++k) {
-        long long lo = (long long)hc[C_BOX_LO + k] >> 3, hi = (long long)hc[C_BOX_HI + k] >> 3;
-        bbox_leaves *= (uint64_t)(hi - lo + 1);
+        glo[k] = (long long)hc[C_BOX_LO + k] >> 3;
+        ghi[k] = (long long)hc[C_BOX_HI + k] >> 3;
+        bbox_leaves *= (uint64_t)(ghi[k] - glo[k] + 1);
         if (bbox_leaves > (1ull << 40)) bbox_leaves = 1ull << 40;
     }
-    uint32_t cap = next_pow2(2 * std::min<uint64_t>(n_cov, bbox_leaves));
-    uint64_t* hkeys = S.as<uint64_t>(S_HASH_KEYS, cap);
-    uint32_t* hvals = S.as<uint32_t>(S_HASH_VALS, cap);
-    BS_HIP(hipMemsetAsync(hkeys, 0xFF, sizeof(uint64_t) * cap, st));
-    k_hash_insert<<<div_up(n_sub, TB), TB, 0, st>>>(recs, n_sub, hkeys, cap - 1, err);
-    uint64_t* keys = S.as<uint64_t>(S_KEYS, std::min<uint64_t>(n_cov, cap));
-    k_hash_compact<<<div_up(cap, TB), TB, 0, st>>>(hkeys, cap, keys, counters);
-    const uint64_t n_leaves = read_scalar(ctx, counters + C_LEAVES);
-    uint64_t* keys_sorted = S.as<uint64_t>(S_KEYS_SORTED, n_leaves);
-    {
+    ix.dense = bbox_leaves <= DENSE_LEAF_GRID_MAX && !ctx->force_hash;
+    uint32_t* leaf_idx = nullptr;
+    uint64_t* hkeys_sorted = nullptr;
+    uint32_t* d_count = reinterpret_cast<uint32_t*>(S.get(S_MISC1, 64));
+    if (ix.dense) {
+        ix.gx0 = (int)glo[0];
+        ix.gy0 = (int)glo[1];
+        ix.gz0 = (int)glo[2];
+        ix.gny = (int)(ghi[1] - glo[1] + 1);
+        ix.gnz = (int)(ghi[2] - glo[2] + 1);
+        ix.gsize = bbox_leaves;
+        ix.grid = S.as<uint32_t>(S_HASH_VALS, bbox_leaves);
+        BS_HIP(hipMemsetAsync(ix.grid, 0, sizeof(uint32_t) * bbox_leaves, st));
+        k_leaf_mark<<<div_up(n_sub, TB), TB, 0, st>>>(recs, n_sub, ix, err);
+        leaf_idx = S.as<uint32_t>(S_KEYS, std::min<uint64_t>(n_cov, bbox_leaves));
+        select_if(ctx, hipcub::CountingInputIterator<uint32_t>(0), leaf_idx, d_count, bbox_leaves,
+                  DenseOccupied{ix.grid});
+    } else {
+        uint32_t cap = next_pow2(2 * std::min<uint64_t>(n_cov, bbox_leaves));
+        ix.cap_mask = cap - 1;
+        ix.hkeys = S.as<uint64_t>(S_HASH_KEYS, cap);
+        ix.hvals = S.as<uint32_t>(S_HASH_VALS, cap);
+        BS_HIP(hipMemsetAsync(ix.hkeys, 0xFF, sizeof(uint64_t) * cap, st));
+        k_leaf_mark<<<div_up(n_sub, TB), TB, 0, st>>>(recs, n_sub, ix, err);
+        uint64_t* keys = S.as<uint64_t>(S_KEYS, std::min<uint64_t>(n_cov, cap));
+        select_if(ctx, ix.hkeys, keys, d_count, cap, HashOccupied{});
+        const uint32_t nl = read_scalar(ctx, d_count);
+        hkeys_sorted = S.as<uint64_t>(S_KEYS_SORTED, nl);
         size_t bytes = 0;
-        BS_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, keys, keys_sorted, (int)n_leaves, 0, 63, st));
+        BS_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, keys, hkeys_sorted, (int)nl, 0, 63, st));
         void* tmp = S.get(S_CUB, bytes);
-        BS_HIP(hipcub::DeviceRadixSort::SortKeys(tmp, bytes, keys, keys_sorted, (int)n_leaves, 0, 63, st));
+        BS_HIP(hipcub::DeviceRadixSort::SortKeys(tmp, bytes, keys, hkeys_sorted, (int)nl, 0, 63, st));
     }
+    const uint64_t n_leaves = read_scalar(ctx, d_count);
+    if (read_scalar(ctx, err) & E_HASH) throw Error(BS_ERR_INTERNAL, "leaf hash table overflow");
     // output volume
     auto* vol = new bs_volume();
     vol->device = ctx->device;
     vol->voxel_size = h;
     vol->n_leaves = n_leaves;
-    try {
-        BS_HIP(hipMalloc(&vol->origins, sizeof(int32_t) * 4 * n_leaves));
-        BS_HIP(hipMalloc(&vol->masks, sizeof(uint64_t) * 8 * n_leaves));
-        BS_HIP(hipMalloc(&vol->values, sizeof(float) * 512 * n_leaves));
-    } catch (...) {
-        (void)hipFree(vol->origins);
-        (void)hipFree(vol->masks);
-        (void)hipFree(vol->values);
-        delete vol;
-        throw;
-    }
     std::unique_ptr<bs_volume, void (*)(bs_volume*)> guard(vol, [](bs_volume* v) {
         (void)hipFree(v->origins);
         (void)hipFree(v->masks);
         (void)hipFree(v->values);
         delete v;
     });
-    k_hash_assign<<<div_up(n_leaves, TB), TB, 0, st>>>(keys_sorted, n_leaves, hkeys, cap - 1, hvals, vol->origins, err);
+    BS_HIP(hipMalloc(&vol->origins, sizeof(int32_t) * 4 * n_leaves));
+    BS_HIP(hipMalloc(&vol->masks, sizeof(uint64_t) * 8 * n_leaves));
+    BS_HIP(hipMalloc(&vol->values, sizeof(float) * 512 * n_leaves));
+    if (ix.dense)
+        k_leaf_assign_dense<<<div_up(n_leaves, TB), TB, 0, st>>>(leaf_idx, n_leaves, ix, vol->origins);
+    else
+        k_leaf_assign_hash<<<div_up(n_leaves, TB), TB, 0, st>>>(hkeys_sorted, n_leaves, ix, vol->origins, err);
     // candidate lists
-    uint32_t* cand_leaf = S.as<uint32_t>(S_CAND_LEAF, n_cov);
-    uint32_t* cand_cnt = S.as<uint32_t>(S_CAND_CNT, n_leaves + 1);
-    uint32_t* cand_off = S.as<uint32_t>(S_CAND_OFF, n_leaves + 1);
-    uint32_t* cursor = S.as<uint32_t>(S_CURSOR, n_leaves);
+    uint32_t* ckeys = S.as<uint32_t>(S_CAND_LEAF, n_cov);
+    uint32_t* cvals = S.as<uint32_t>(S_CAND_CNT, n_cov);
+    uint32_t* ckeys_s = S.as<uint32_t>(S_CURSOR, n_cov);
     uint32_t* cand_list = S.as<uint32_t>(S_CAND_LIST, n_cov);
-    BS_HIP(hipMemsetAsync(cand_cnt, 0, sizeof(uint32_t) * (n_leaves + 1), st));
-    BS_HIP(hipMemsetAsync(cursor, 0, sizeof(uint32_t) * n_leaves, st));
-    k_cand_count<<<div_up(n_sub, TB), TB, 0, st>>>(recs, n_sub, cov_off, hkeys, hvals, cap - 1, cand_leaf, cand_cnt,
-                                                   (uint32_t)n_leaves, err);
-    exclusive_scan(ctx, cand_cnt, cand_off, n_leaves + 1);
-    k_cand_fill<<<div_up(n_sub, TB), TB, 0, st>>>(n_sub, cov_off, cand_leaf, cand_off, cursor, cand_list,
-                                                  (uint32_t)n_leaves);
-    if (read_scalar(ctx, err) & E_HASH) throw Error(BS_ERR_INTERNAL, "leaf hash table overflow");
+    uint32_t* cand_off = S.as<uint32_t>(S_CAND_OFF, n_leaves + 1);
+    k_cand_pairs<<<div_up(n_sub, TB), TB, 0, st>>>(recs, n_sub, cov_off, ix, (uint32_t)n_leaves, ckeys, cvals, err);
+    sort_pairs(ctx, ckeys, ckeys_s, cvals, cand_list, n_cov, bits_for(n_leaves));
+    k_cand_offsets<<<div_up(n_cov, TB), TB, 0, st>>>(ckeys_s, (uint32_t)n_cov, (uint32_t)n_leaves, cand_off);
+    if (read_scalar(ctx, err) & E_HASH) throw Error(BS_ERR_INTERNAL, "leaf lookup failed");
     tm.mark();
 
     // ---- 3. distance
-    k_distance<<<(unsigned)n_leaves, 512, 0, st>>>(vol->origins, cand_off, cand_list, recs, h, vol->values, vol->masks);
+    k_distance<<<(unsigned)n_leaves, DIST_BLOCK, 0, st>>>(vol->origins, cand_off, cand_list, recs, sub, h,
+                                                          (uint32_t)n_cov, vol->values, vol->masks);
     tm.mark();
 
     // ---- 4. active list
@@ -935,36 +1181,55 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     // ---- 5. LBVH + dipoles over the original mesh
     const int nt = (int)n;
     const int n_int = nt - 1;
-    unsigned* bounds = reinterpret_cast<unsigned*>(S.get(S_MISC0, 64));
-    {
-        unsigned init[6] = {~0u, ~0u, ~0u, 0u, 0u, 0u};
-        BS_HIP(hipMemcpyAsync(bounds, init, sizeof(init), hipMemcpyHostToDevice, st));
-    }
-    k_centroid_bounds<<<div_up(n, TB), TB, 0, st>>>(d_tris, n, bounds);
+    const int n_nodes = 2 * nt - 1;
+    unsigned* bounds = reinterpret_cast<unsigned*>(S.get(S_MISC2, 64));
+    const unsigned nb = div_up(n, 256);
+    uint4* bparts = S.as<uint4>(S_BVH_BOX, 2 * (uint64_t)nb);
+    k_centroid_bounds<<<nb, 256, 0, st>>>(d_tris, n, bparts);
+    k_bounds_final<<<1, 1024, 0, st>>>(bparts, nb, bounds);
     uint64_t* codes = S.as<uint64_t>(S_MORTON, n);
     uint64_t* codes_s = S.as<uint64_t>(S_MORTON_SORTED, n);
     uint32_t* idx = S.as<uint32_t>(S_IDX, n);
     uint32_t* idx_s = S.as<uint32_t>(S_IDX_SORTED, n);
     k_morton<<<div_up(n, TB), TB, 0, st>>>(d_tris, n, bounds, codes, idx);
-    {
-        size_t bytes = 0;
-        BS_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, codes, codes_s, idx, idx_s, nt, 0, 63, st));
-        void* tmp = S.get(S_CUB, bytes);
-        BS_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, bytes, codes, codes_s, idx, idx_s, nt, 0, 63, st));
-    }
-    const int n_nodes = 2 * nt - 1;
+    sort_pairs(ctx, codes, codes_s, idx, idx_s, n, 63);
     int4* links = S.as<int4>(S_BVH_LINKS, std::max(n_int, 1));
-    int4* leaf_links = S.as<int4>(S_MISC1, nt);
+    int4* leaf_links = S.as<int4>(S_ARGMIN, nt);
     int* parent = S.as<int>(S_BVH_PARENT, n_nodes);
-    unsigned* flags = S.as<unsigned>(S_BVH_FLAGS, std::max(n_int, 1));
+    unsigned* hword = S.as<unsigned>(S_BVH_FLAGS, std::max(n_int, 1));
+    uint32_t* height = S.as<uint32_t>(S_ROWS, std::max(n_int, 1));       // subdivision scratch is free now
+    uint32_t* height_s = S.as<uint32_t>(S_ROW_OFF, std::max(n_int, 1));
+    uint32_t* inode = S.as<uint32_t>(S_SUBS, std::max(n_int, 1));
+    uint32_t* inode_s = S.as<uint32_t>(S_SUB_OFF, std::max(n_int, 1));
     NodeAcc* accs = S.as<NodeAcc>(S_BVH_INIT, n_nodes);
     float4* nodes = S.as<float4>(S_BVH_NODES, 4 * (uint64_t)n_nodes);
     float* btris = S.as<float>(S_BVH_TRIS, 9 * n);
     BS_HIP(hipMemsetAsync(parent, 0xFF, sizeof(int) * n_nodes, st));
-    BS_HIP(hipMemsetAsync(flags, 0, sizeof(unsigned) * std::max(n_int, 1), st));
+    BS_HIP(hipMemsetAsync(hword, 0, sizeof(unsigned) * std::max(n_int, 1), st));
     if (n_int > 0) k_lbvh_build<<<div_up(n_int, TB), TB, 0, st>>>(codes_s, nt, links, parent);
-    k_lbvh_leaves<<<div_up(n, TB), TB, 0, st>>>(d_tris, nt, idx_s, prm.wn_accuracy, links, parent, flags, accs, nodes,
+    k_lbvh_leaves<<<div_up(n, TB), TB, 0, st>>>(d_tris, nt, idx_s, prm.wn_accuracy, parent, hword, height, accs, nodes,
                                                 btris, leaf_links, err);
+    if (n_int > 0) {
+        // internal nodes grouped by height, then combined level by level
+        k_iota<<<div_up(n_int, TB), TB, 0, st>>>(inode, (uint32_t)n_int);
+        sort_pairs(ctx, height, height_s, inode, inode_s, (uint64_t)n_int, 8);
+        uint32_t* starts = reinterpret_cast<uint32_t*>(S.get(S_MISC0, sizeof(uint32_t) * 258));
+        BS_HIP(hipMemsetAsync(starts, 0xFF, sizeof(uint32_t) * 258, st));
+        k_level_starts<<<div_up(n_int, TB), TB, 0, st>>>(height_s, (uint32_t)n_int, starts);
+        uint32_t hs[258];
+        BS_HIP(hipMemcpyAsync(hs, starts, sizeof(hs), hipMemcpyDeviceToHost, st));
+        BS_HIP(hipStreamSynchronize(st));
+        if (read_scalar(ctx, err) & E_TREE) throw Error(BS_ERR_INTERNAL, "BVH parent links are corrupt");
+        // heights run 1..H contiguously in the sorted order
+        int H = 0;
+        for (int k = 1; k < 256; ++k)
+            if (hs[k] != 0xFFFFFFFFu) H = k;
+        for (int k = 1; k <= H; ++k) {
+            BS_REQUIRE(hs[k] != 0xFFFFFFFFu, BS_ERR_INTERNAL, "BVH height levels are not contiguous");
+            const uint32_t b = hs[k], e = (k == H) ? (uint32_t)n_int : hs[k + 1];
+            k_lbvh_level<<<div_up(e - b, TB), TB, 0, st>>>(inode_s + b, e - b, prm.wn_accuracy, links, accs, nodes);
+        }
+    }
     tm.mark();
 
     // ---- 6. winding numbers and signs
@@ -996,8 +1261,10 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     stt.n_bvh_nodes = (uint64_t)n_nodes;
     stt.wn_node_visits = hc[C_NEAR_INTERNAL];
     stt.wn_far_evals = hc[C_FAR];
+    stt.wn_leaf_visits = hc[C_LEAF_VISITS];
     stt.wn_leaf_tris = hc[C_LEAF_TRIS];
     stt.n_fp64_rechecks = n_re;
+    stt.used_hash = ix.dense ? 0 : 1;
     stt.ms_total = std::chrono::duration<double, std::milli>(wall1 - wall0).count();
     stt.ms_subdivide = tm.ms(0, 1);
     stt.ms_leaves = tm.ms(1, 2);

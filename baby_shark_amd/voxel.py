@@ -43,6 +43,10 @@ class Context:
     def set_profiling(self, on: bool) -> None:
         check(load().bs_ctx_set_profiling(self._h, 1 if on else 0))
 
+    def set_force_hash(self, on: bool) -> None:
+        """Allocate leaves through the GPU hash even when a dense leaf grid fits."""
+        check(load().bs_ctx_set_force_hash(self._h, 1 if on else 0))
+
     def last_stats(self) -> dict:
         s = _lib.M2VStats()
         check(load().bs_ctx_last_m2v_stats(self._h, C.byref(s)))

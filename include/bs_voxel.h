@@ -60,6 +60,7 @@ typedef struct {
 typedef struct {
     uint64_t n_tris, n_sub, n_pairs, n_leaves, n_active, n_candidates;
     uint64_t n_bvh_nodes, wn_node_visits, wn_leaf_tris, wn_far_evals, n_fp64_rechecks;
+    uint64_t wn_leaf_visits, used_hash;
     double ms_total, ms_subdivide, ms_leaves, ms_distance, ms_bvh, ms_winding;
 } bs_m2v_stats;
 
@@ -73,6 +74,9 @@ void* bs_ctx_stream(bs_ctx* ctx);
 /* Per-kernel event timing inside bs_mesh_to_volume* (adds syncs; off by default). */
 bs_status bs_ctx_set_profiling(bs_ctx* ctx, int enabled);
 bs_status bs_ctx_last_m2v_stats(const bs_ctx* ctx, bs_m2v_stats* out);
+/* Leaf allocation: 0 = dense leaf grid over the band's bounding box when it
+ * has at most 2^28 cells, else hash (default); 1 = always hash. */
+bs_status bs_ctx_set_force_hash(bs_ctx* ctx, int enabled);
 
 void bs_m2v_params_default(bs_m2v_params* p);
 

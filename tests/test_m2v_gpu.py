@@ -154,3 +154,20 @@ def test_cfg2_full_size(gpu_ctx, oracle_lib):
     print(f"cfg2 sign mismatches vs reference-tree winding number: {int(mism[big].sum())} / {int(big.sum())}",
           flush=True)
     assert int(mism[big].sum()) == 0
+
+
+def test_hash_leaf_allocation_matches_dense(gpu_ctx, oracle_lib):
+    """The GPU hash (used for extents too large for a dense leaf grid) gives the
+    same volume as the dense leaf grid, and both match the oracle."""
+    soup = mg.displaced_icosphere(4, radius=0.8, seed=6)
+    gpu_ctx.set_force_hash(True)
+    try:
+        a = run_gpu(gpu_ctx, soup, 0.01, 3).export()
+        assert gpu_ctx.last_stats()["used_hash"] == 1
+    finally:
+        gpu_ctx.set_force_hash(False)
+    b = run_gpu(gpu_ctx, soup, 0.01, 3).export()
+    assert gpu_ctx.last_stats()["used_hash"] == 0
+    assert np.array_equal(a.origins, b.origins) and np.array_equal(a.masks, b.masks)
+    assert np.array_equal(a.values, b.values)
+    check_case(gpu_ctx, oracle_lib, soup, 0.01, 3)
