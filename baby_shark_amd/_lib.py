@@ -20,6 +20,8 @@ BS_ERR_HIP = 3
 BS_ERR_UNSUPPORTED_TILE = 4
 BS_ERR_INTERNAL = 5
 BS_ERR_NO_DEVICE = 6
+BS_PATH_FORCE_HASH = 1
+BS_PATH_FORCE_LDS_STACK = 2
 
 STATUS_NAMES = {0: "BS_OK", 1: "BS_ERR_EMPTY", 2: "BS_ERR_INVALID_ARG", 3: "BS_ERR_HIP",
                 4: "BS_ERR_UNSUPPORTED_TILE", 5: "BS_ERR_INTERNAL", 6: "BS_ERR_NO_DEVICE"}
@@ -38,7 +40,8 @@ class VolumeInfo(C.Structure):
 class M2VStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("n_tris", "n_sub", "n_pairs", "n_leaves", "n_active", "n_candidates",
                                           "n_bvh_nodes", "wn_node_visits", "wn_leaf_tris", "wn_far_evals",
-                                          "n_fp64_rechecks", "wn_leaf_visits", "used_hash")] + \
+                                          "n_fp64_rechecks", "wn_leaf_visits", "used_hash", "used_lds_stack",
+                                          "bvh_height")] + \
               [(k, C.c_double) for k in ("ms_total", "ms_subdivide", "ms_leaves", "ms_distance", "ms_bvh",
                                          "ms_winding")]
 
@@ -80,7 +83,7 @@ def load():
             "bs_ctx_stream": (vp, [vp]),
             "bs_ctx_set_profiling": (C.c_int, [vp, C.c_int]),
             "bs_ctx_last_m2v_stats": (C.c_int, [vp, C.POINTER(M2VStats)]),
-            "bs_ctx_set_force_hash": (C.c_int, [vp, C.c_int]),
+            "bs_ctx_set_paths": (C.c_int, [vp, C.c_int]),
             "bs_m2v_params_default": (None, [C.POINTER(M2VParams)]),
             "bs_mesh_to_volume": (C.c_int, [vp, f32p, C.c_size_t, C.POINTER(M2VParams), pp]),
             "bs_mesh_to_volume_device": (C.c_int, [vp, vp, C.c_size_t, C.POINTER(M2VParams), pp]),

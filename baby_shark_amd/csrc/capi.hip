@@ -99,9 +99,9 @@ bs_status bs_ctx_set_profiling(bs_ctx* ctx, int enabled) {
     return BS_OK;
 }
 
-bs_status bs_ctx_set_force_hash(bs_ctx* ctx, int enabled) {
-    if (!ctx) return BS_ERR_INVALID_ARG;
-    ctx->force_hash = enabled != 0;
+bs_status bs_ctx_set_paths(bs_ctx* ctx, int flags) {
+    if (!ctx || (flags & ~(BS_PATH_FORCE_HASH | BS_PATH_FORCE_LDS_STACK))) return BS_ERR_INVALID_ARG;
+    ctx->paths = flags;
     return BS_OK;
 }
 

@@ -42,7 +42,7 @@ enum ScratchSlot : int {
     S_KEYS, S_KEYS_SORTED, S_CAND_LEAF, S_CAND_CNT, S_CAND_OFF, S_CAND_LIST, S_CURSOR, S_ACT_CNT, S_ACT_OFF,
     S_ACT_LIST, S_COUNTERS, S_CUB, S_MORTON, S_MORTON_SORTED, S_IDX, S_IDX_SORTED, S_BVH_TRIS, S_BVH_NODES,
     S_BVH_LINKS, S_BVH_PARENT, S_BVH_FLAGS, S_BVH_INIT, S_BVH_BOX, S_RECHECK, S_ARGMIN, S_MISC0, S_MISC1,
-    S_MISC2, S_MISC3, S_INPUT, S_COUNT
+    S_MISC2, S_MISC3, S_INPUT, S_BVH_KIDS, S_COUNT
 };
 
 struct Scratch {
@@ -77,7 +77,7 @@ struct bs_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool profiling = false;
-    bool force_hash = false;  // leaf allocation through the hash even when a dense grid fits
+    int paths = 0;  // BS_PATH_* flags
     bs::Scratch scratch;
     bs_m2v_stats last_m2v = {};
     uint64_t* host_counters = nullptr;  // pinned, for small readbacks

@@ -13,6 +13,7 @@
 #include <chrono>
 #include <cmath>
 #include <memory>
+#include <type_traits>
 
 #include "common.hpp"
 #include "m2v_kernels.hpp"
@@ -484,14 +485,26 @@ __global__ void k_leaf_popcount(const uint64_t* __restrict__ masks, uint64_t n_l
     cnt[i] = c;
 }
 
-__global__ void k_active_list(const uint64_t* __restrict__ masks, const uint32_t* __restrict__ act_off, uint32_t* out) {
+// Active voxel list, slot = leaf * 512 + offset.  Within a leaf the voxels are
+// listed in Morton order, so 64 consecutive entries (one winding-number wave)
+// form compact 4x4x4 blocks rather than 8x8 slabs.
+__global__ __launch_bounds__(512) void k_active_list(const uint64_t* __restrict__ masks,
+                                                     const uint32_t* __restrict__ act_off, uint32_t* out) {
+    __shared__ uint32_t wcnt[8];
     const int leaf = blockIdx.x;
-    const int tid = threadIdx.x;  // 512 threads, one per voxel
-    const int w = tid >> 6, lane = tid & 63;
-    const uint64_t m = masks[(uint64_t)leaf * 8 + w];
+    const int m = threadIdx.x;  // Morton index of the voxel in the leaf
+    const int w = m >> 6, lane = m & 63;
+    const int x = ((m >> 2) & 1) | ((m >> 4) & 2) | ((m >> 6) & 4);
+    const int y = ((m >> 1) & 1) | ((m >> 3) & 2) | ((m >> 5) & 4);
+    const int z = (m & 1) | ((m >> 2) & 2) | ((m >> 4) & 4);
+    const int off = (x << 6) | (y << 3) | z;
+    const bool act = (masks[(uint64_t)leaf * 8 + (off >> 6)] >> (off & 63)) & 1ull;
+    const unsigned long long b = __ballot(act);
+    if (lane == 0) wcnt[w] = (uint32_t)__popcll(b);
+    __syncthreads();
     uint32_t before = 0;
-    for (int k = 0; k < w; ++k) before += __popcll(masks[(uint64_t)leaf * 8 + k]);
-    if ((m >> lane) & 1) out[act_off[leaf] + before + __popcll(m & ((1ull << lane) - 1))] = (uint32_t)leaf * 512u + tid;
+    for (int k = 0; k < w; ++k) before += wcnt[k];
+    if (act) out[act_off[leaf] + before + __popcll(b & ((1ull << lane) - 1))] = (uint32_t)leaf * 512u + off;
 }
 
 // ---------------------------------------------------------------------------
@@ -595,9 +608,21 @@ __device__ inline int lbvh_delta(const uint64_t* __restrict__ k, int n, int i, i
     return __clzll(a ^ b);
 }
 
+// Node record, 64 B = one s_load_dwordx16: A = (c.xyz, (radius*acc)^2),
+// B = (awn.xyz, trace O2), C = (S00, S11, S22, 2 S01), D = (2 S02, 2 S12) with
+// S the symmetric part of order2 = sum a c n^T - c_d awn^T (only S enters
+// r^T O2 r), and links L = (left, right) for an internal node or
+// (first triangle, -count) for a node evaluated as a leaf (count <= 4).
+struct alignas(16) NodeRec {
+    float4 A, B, C;
+    float2 D;
+    int2 L;
+};
+static_assert(sizeof(NodeRec) == 64, "NodeRec layout");
+
 // Karras 2012: internal node i of n-1; leaves are n-1+k.  links = (left, right,
 // first, count) over the sorted order.
-__global__ void k_lbvh_build(const uint64_t* __restrict__ keys, int n, int4* links, int* parent) {
+__global__ void k_lbvh_build(const uint64_t* __restrict__ keys, int n, NodeRec* nodes, int2* kids, int* parent) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     int d = (lbvh_delta(keys, n, i, i + 1) - lbvh_delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -619,7 +644,9 @@ __global__ void k_lbvh_build(const uint64_t* __restrict__ keys, int n, int4* lin
     int first = min(i, j), last = max(i, j);
     int left = (first == gamma) ? (n - 1 + gamma) : gamma;
     int right = (last == gamma + 1) ? (n - 1 + gamma + 1) : gamma + 1;
-    links[i] = make_int4(left, right, first, last - first + 1);
+    const int count = last - first + 1;
+    nodes[i].L = count <= BVH_LEAF_SIZE ? make_int2(first, -count) : make_int2(left, right);
+    kids[i] = make_int2(left, right);
     parent[left] = i;
     parent[right] = i;
 }
@@ -642,10 +669,8 @@ __device__ inline void acc_add(NodeAcc& r, const NodeAcc& a, const NodeAcc& b) {
     r.area = a.area + b.area;
 }
 
-// Node record (64 B): (c.xyz, (radius*acc)^2), (awn.xyz, trace O2),
-// (S00, S11, S22, 2 S01), (2 S02, 2 S12, 0, 0) with S the symmetric part of
-// order2 = sum a c n^T - c_d awn^T; only S enters r^T O2 r.
-__device__ inline void acc_finalize(const NodeAcc& a, float acc, float4* node) {
+__device__ inline void acc_finalize(const NodeAcc& a, float acc, NodeRec* rec) {
+    float4* node = &rec->A;
     float c[3];
     for (int k = 0; k < 3; ++k) c[k] = a.awc[k] / a.area;
     float o2[9];
@@ -659,10 +684,13 @@ __device__ inline void acc_finalize(const NodeAcc& a, float acc, float4* node) {
     float radius = sqrtf(fmaxf(dmin, dmax));
     float ra = radius * acc;
     float far2 = (a.area > 0.0f) ? ra * ra : INFINITY;  // all-degenerate: never far
+    // order-1 and trace terms scaled by 1/(4 pi), quadratic form by 3/(4 pi):
+    // w = (B.r + B.w - (r^T S r) / |r|^2) / |r|^3 (aabb_tree.rs:803-816, factored)
+    const float k1 = INV_4PI, k3 = 3.0f * INV_4PI;
     node[0] = make_float4(c[0], c[1], c[2], far2);
-    node[1] = make_float4(a.awn[0], a.awn[1], a.awn[2], o2[0] + o2[4] + o2[8]);
-    node[2] = make_float4(o2[0], o2[4], o2[8], o2[1] + o2[3]);
-    node[3] = make_float4(o2[2] + o2[6], o2[5] + o2[7], 0.0f, 0.0f);
+    node[1] = make_float4(k1 * a.awn[0], k1 * a.awn[1], k1 * a.awn[2], k1 * (o2[0] + o2[4] + o2[8]));
+    node[2] = make_float4(k3 * o2[0], k3 * o2[4], k3 * o2[8], k3 * (o2[1] + o2[3]));
+    rec->D = make_float2(k3 * (o2[2] + o2[6]), k3 * (o2[5] + o2[7]));
 }
 
 // Leaf data, then node heights bottom-up with one 32-bit atomicMax per step:
@@ -672,13 +700,16 @@ __device__ inline void acc_finalize(const NodeAcc& a, float acc, float4* node) {
 // combined level by level (k_lbvh_level), one launch per height.
 __global__ void k_lbvh_leaves(const float* __restrict__ tris, int n, const uint32_t* __restrict__ order, float acc,
                               const int* __restrict__ parent, unsigned* hword, uint32_t* height, NodeAcc* accs,
-                              float4* nodes, float* btris, int4* leaf_links, unsigned* err) {
+                              NodeRec* nodes, float* btris, unsigned* err) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     F3 a, b, c;
     load_tri(tris + 9 * (uint64_t)order[k], a, b, c);
-    float* bt = btris + 9 * (uint64_t)k;
-    bt[0] = a.x; bt[1] = a.y; bt[2] = a.z; bt[3] = b.x; bt[4] = b.y; bt[5] = b.z; bt[6] = c.x; bt[7] = c.y; bt[8] = c.z;
+    // traversal copy of the triangle in Morton order, 3 x float4 (48 B) per triangle
+    float4* bt = reinterpret_cast<float4*>(btris) + 3 * (uint64_t)k;
+    bt[0] = make_float4(a.x, a.y, a.z, b.x);
+    bt[1] = make_float4(b.y, b.z, c.x, c.y);
+    bt[2] = make_float4(c.z, 0.0f, 0.0f, 0.0f);
     NodeAcc d;
     F3 e0 = f3_sub(b, a), e1 = f3_sub(c, a);
     F3 cr = {e0.y * e1.z - e0.z * e1.y, e0.z * e1.x - e0.x * e1.z, e0.x * e1.y - e0.y * e1.x};
@@ -704,8 +735,8 @@ __global__ void k_lbvh_leaves(const float* __restrict__ tris, int n, const uint3
     }
     int node = n - 1 + k;
     accs[node] = d;
-    acc_finalize(d, acc, nodes + 4 * (uint64_t)node);
-    leaf_links[k] = make_int4(-1, -1, k, 1);
+    acc_finalize(d, acc, nodes + node);
+    nodes[node].L = make_int2(k, -1);
     unsigned hgt = 0;
     int p = parent[node];
     for (int level = 0; p >= 0; ++level) {
@@ -722,15 +753,15 @@ __global__ void k_lbvh_leaves(const float* __restrict__ tris, int n, const uint3
 }
 
 __global__ void k_lbvh_level(const uint32_t* __restrict__ level_nodes, uint32_t count, float acc,
-                             const int4* __restrict__ links, NodeAcc* accs, float4* nodes) {
+                             const int2* __restrict__ kids, NodeAcc* accs, NodeRec* nodes) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     const int p = (int)level_nodes[i];
-    const int4 L = links[p];
+    const int2 L = kids[p];
     NodeAcc s;
     acc_add(s, accs[L.x], accs[L.y]);
     accs[p] = s;
-    acc_finalize(s, acc, nodes + 4 * (uint64_t)p);
+    acc_finalize(s, acc, nodes + p);
 }
 
 __global__ void k_iota(uint32_t* out, uint32_t n) {
@@ -785,26 +816,133 @@ __device__ inline void kb_add(float& s, float& c, float x) {
     s = t;
 }
 
-// order-2 far-field term of a node at query offset r = c - p
-__device__ inline float dipole_term(const float4* __restrict__ nd, float rx, float ry, float rz, float r2) {
-    const float4 B = nd[1], C = nd[2], D = nd[3];
+// order-2 far-field term of a node at query offset r = c - p, with the node's
+// coefficients pre-scaled (acc_finalize): 17 VALU operations
+__device__ inline float dipole_term(const float4& B, const float4& C, const float2& D, float rx, float ry, float rz,
+                                   float r2) {
     const float inv_r = fast_rsq(r2);
     const float inv_r2 = inv_r * inv_r;
-    const float rsr = C.x * rx * rx + C.y * ry * ry + C.z * rz * rz + C.w * rx * ry + D.x * rx * rz + D.y * ry * rz;
-    const float lin = B.x * rx + B.y * ry + B.z * rz + B.w;
-    return INV_4PI * inv_r * inv_r2 * (lin - 3.0f * rsr * inv_r2);
+    const float u = __builtin_fmaf(C.x, rx, __builtin_fmaf(C.w, ry, D.x * rz));
+    const float v = __builtin_fmaf(C.y, ry, D.y * rz);
+    const float rsr = __builtin_fmaf(rx, u, __builtin_fmaf(ry, v, rz * (C.z * rz)));
+    const float lin = __builtin_fmaf(B.x, rx, __builtin_fmaf(B.y, ry, __builtin_fmaf(B.z, rz, B.w)));
+    return (inv_r * inv_r2) * __builtin_fmaf(-rsr, inv_r2, lin);
 }
 
+// Per-wave traversal stacks.  An entry is a pair of sibling nodes and the lanes
+// for which their parent was near; the stack never holds more entries than the
+// tree is high.  LaneStack keeps entry k in lane k of four VGPRs (written by a
+// per-lane select, read with v_readlane: no memory round trip) and serves trees
+// lower than 64; LdsStack serves any height up to 256 from LDS.
+struct LaneStack {
+    static constexpr int DEPTH = 64;
+    int a0 = 0, b0 = 0, l0 = 0, h0 = 0;
+    __device__ void push(int sp, int a, int b, unsigned long long m, int lane) {
+        const bool mine = lane == sp;
+        a0 = mine ? a : a0;
+        b0 = mine ? b : b0;
+        l0 = mine ? (int)(unsigned)m : l0;
+        h0 = mine ? (int)(unsigned)(m >> 32) : h0;
+    }
+    __device__ void pop(int sp, int& a, int& b, unsigned long long& m) const {
+        a = __builtin_amdgcn_readlane(a0, sp);
+        b = __builtin_amdgcn_readlane(b0, sp);
+        m = ((unsigned long long)(unsigned)__builtin_amdgcn_readlane(h0, sp) << 32) |
+            (unsigned)__builtin_amdgcn_readlane(l0, sp);
+    }
+};
+
+struct LdsStack {
+    static constexpr int DEPTH = 256;
+    int4* e;  // this wave's DEPTH entries: (a, b, mask lo, mask hi)
+    __device__ void push(int sp, int a, int b, unsigned long long m, int lane) {
+        if (lane == 0) e[sp] = make_int4(a, b, (int)(unsigned)m, (int)(unsigned)(m >> 32));
+    }
+    __device__ void pop(int sp, int& a, int& b, unsigned long long& m) const {
+        const int4 v = e[sp];
+        a = __builtin_amdgcn_readfirstlane(v.x);
+        b = __builtin_amdgcn_readfirstlane(v.y);
+        m = ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane(v.w) << 32) |
+            (unsigned)__builtin_amdgcn_readfirstlane(v.z);
+    }
+};
+
+
+// Empty asm that consumes every word of a node record as a scalar register:
+// the record's loads must all be issued before it, so the two siblings of a
+// pair cost one memory round trip instead of one per use site.
+__device__ inline void pin_record(const NodeRec& r) {
+    asm volatile("" ::"s"(r.A.x), "s"(r.A.y), "s"(r.A.z), "s"(r.A.w), "s"(r.B.x), "s"(r.B.y), "s"(r.B.z),
+                 "s"(r.B.w), "s"(r.C.x), "s"(r.C.y), "s"(r.C.z), "s"(r.C.w), "s"(r.D.x), "s"(r.D.y), "s"(r.L.x),
+                 "s"(r.L.y));
+}
+
+// A VGPR holding 0 that the compiler cannot prove uniform: adding it to an
+// address turns a wave-uniform read into a vector load (broadcast to every
+// lane, VGPR destination) instead of a scalar load competing for SGPRs.
+__device__ inline int opaque_zero() {
+    int z;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+    return z;
+}
+
+struct WnAcc {
+    float s = 0.0f, c = 0.0f;
+    unsigned long long near_int = 0, far = 0, leaf = 0, tri = 0;
+};
+
+// One node for the lanes in m.  Lane sets stay in scalar registers (ballot
+// results combined with SALU ops, turned back into per-lane conditions by
+// inverse_ballot): far lanes add the order-2 term, near lanes either sum the
+// exact solid angles of a leaf's triangles or ask for the node's children.
+// Returns the mask of lanes that need the children (0 for a leaf).
+__device__ inline unsigned long long wn_visit(const NodeRec& r, unsigned long long m, float px, float py, float pz,
+                                              const float* __restrict__ btris, WnAcc& a) {
+    const float rx = r.A.x - px, ry = r.A.y - py, rz = r.A.z - pz;
+    const float r2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
+    const unsigned long long far_all = __ballot(r2 > r.A.w);
+    const unsigned long long farm = m & far_all;
+    const unsigned long long nm = m & ~far_all;
+    a.far += __popcll(farm);
+    if (farm) {
+        if (__builtin_amdgcn_inverse_ballot_w64(farm)) kb_add(a.s, a.c, dipole_term(r.B, r.C, r.D, rx, ry, rz, r2));
+    }
+    if (nm == 0) return 0;
+    if (r.L.y < 0) {  // leaf: triangles [L.x, L.x - L.y)
+        const int cnt = -r.L.y;
+        a.leaf += __popcll(nm);
+        a.tri += (unsigned long long)__popcll(nm) * cnt;
+        if (__builtin_amdgcn_inverse_ballot_w64(nm)) {
+            // all BVH_LEAF_SIZE triangles requested at once (btris is padded)
+            const float4* tp = reinterpret_cast<const float4*>(btris) + 3 * (uint64_t)r.L.x + opaque_zero();
+            float4 q[3 * BVH_LEAF_SIZE];
+#pragma unroll
+            for (int k = 0; k < 3 * BVH_LEAF_SIZE; ++k) q[k] = tp[k];
+            const float* t = reinterpret_cast<const float*>(q);
+            float om = 0.0f;
+#pragma unroll
+            for (int k = 0; k < BVH_LEAF_SIZE; ++k) {
+                const float* tr = t + 12 * k;
+                const float sa = solid_angle_rel(tr[0] - px, tr[1] - py, tr[2] - pz, tr[3] - px, tr[4] - py,
+                                                 tr[5] - pz, tr[6] - px, tr[7] - py, tr[8] - pz);
+                om += (k < cnt) ? sa : 0.0f;
+            }
+            kb_add(a.s, a.c, om * INV_4PI);
+        }
+        return 0;
+    }
+    a.near_int += __popcll(nm);
+    return nm;
+}
+
+template <class Stack>
 __global__ __launch_bounds__(WN_BLOCK) void k_winding(const uint32_t* __restrict__ act, uint32_t n_act,
                                                       const int32_t* __restrict__ origins, float h,
-                                                      const float4* __restrict__ nodes, const int4* __restrict__ links,
-                                                      const int4* __restrict__ leaf_links, int n_int, int n_nodes,
+                                                      const NodeRec* __restrict__ nodes, int n_nodes,
                                                       const float* __restrict__ btris, float thr, float fp64_band,
                                                       float* __restrict__ values, uint32_t* __restrict__ recheck,
                                                       unsigned long long* counters, unsigned* err) {
-    __shared__ int st_node[WN_BLOCK / 64][WN_STACK];
-    __shared__ unsigned long long st_mask[WN_BLOCK / 64][WN_STACK];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63;
     const uint32_t gid = blockIdx.x * WN_BLOCK + threadIdx.x;
     const bool valid = gid < n_act;
     const uint32_t slot = valid ? act[gid] : 0u;
@@ -818,72 +956,56 @@ __global__ __launch_bounds__(WN_BLOCK) void k_winding(const uint32_t* __restrict
     const float px = (float)vx * h, py = (float)vy * h, pz = (float)vz * h;
     const unsigned long long init = __ballot(valid);
     if (init == 0) return;
-    float s = 0.0f, cmp = 0.0f;
-    unsigned long long n_near_int = 0, n_far = 0, n_leaf = 0, n_tri = 0;
-    int sp = 1;
-    if (lane == 0) {
-        st_node[wave][0] = 0;  // root: internal node 0 (or leaf 0 when n == 1)
-        st_mask[wave][0] = init;
+    WnAcc a;
+    Stack stk;
+    if constexpr (!std::is_same<Stack, LaneStack>::value) {
+        __shared__ int4 lds_stack[WN_BLOCK / 64][LdsStack::DEPTH];
+        stk.e = lds_stack[threadIdx.x >> 6];
     }
+    int sp = 0;
     bool overflow = false;
-    // every node is pushed at most once per wave in a tree: more pops than
-    // nodes means corrupt links, and the walk stops
+    {
+        const NodeRec root = nodes[0];  // internal node 0, or the only leaf when n == 1
+        pin_record(root);
+        const unsigned long long nm = wn_visit(root, init, px, py, pz, btris, a);
+        if (nm) {
+            stk.push(0, root.L.x, root.L.y, nm, lane);
+            sp = 1;
+        }
+    }
+    // every sibling pair is pushed at most once per wave in a tree: more pops
+    // than nodes means corrupt links, and the walk stops
     for (int pops = 0; sp > 0; ++pops) {
         if (pops > n_nodes) {
             overflow = true;
             break;
         }
         --sp;
-        const int node = __builtin_amdgcn_readfirstlane(st_node[wave][sp]);
-        const unsigned long long m = st_mask[wave][sp];
-        const bool act_l = (m >> lane) & 1ull;
-        const float4* nd = nodes + 4 * (uint64_t)node;
-        const float4 A = nd[0];
-        const float rx = A.x - px, ry = A.y - py, rz = A.z - pz;
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        const bool far = r2 > A.w;
-        if (act_l && far) kb_add(s, cmp, dipole_term(nd, rx, ry, rz, r2));
-        const bool near = act_l && !far;
-        const unsigned long long nm = __ballot(near);
-        n_far += __popcll(m & __ballot(far));
-        if (nm == 0) continue;
-        const int4 L = node < n_int ? links[node] : leaf_links[node - n_int];
-        if (L.w <= BVH_LEAF_SIZE) {
-            n_leaf += __popcll(nm);
-            n_tri += (unsigned long long)__popcll(nm) * L.w;
-            if (near) {
-                float om = 0.0f;
-                for (int k = 0; k < L.w; ++k) {
-                    const float* t = btris + 9 * (uint64_t)(L.z + k);
-                    om += solid_angle_rel(t[0] - px, t[1] - py, t[2] - pz, t[3] - px, t[4] - py, t[5] - pz, t[6] - px,
-                                          t[7] - py, t[8] - pz);
-                }
-                kb_add(s, cmp, om * INV_4PI);
-            }
-        } else {
-            n_near_int += __popcll(nm);
-            if (sp + 2 > WN_STACK) {
-                overflow = true;
-                break;
-            }
-            if (lane == 0) {
-                st_node[wave][sp] = L.y;
-                st_mask[wave][sp] = nm;
-                st_node[wave][sp + 1] = L.x;
-                st_mask[wave][sp + 1] = nm;
-            }
-            sp += 2;
+        int c0, c1;
+        unsigned long long m;
+        stk.pop(sp, c0, c1, m);
+        const NodeRec r0 = nodes[c0];
+        const NodeRec r1 = nodes[c1];
+        pin_record(r0);
+        pin_record(r1);
+        const unsigned long long n0 = wn_visit(r0, m, px, py, pz, btris, a);
+        const unsigned long long n1 = wn_visit(r1, m, px, py, pz, btris, a);
+        if (sp + 2 > Stack::DEPTH) {
+            overflow = true;
+            break;
         }
+        if (n1) stk.push(sp++, r1.L.x, r1.L.y, n1, lane);
+        if (n0) stk.push(sp++, r0.L.x, r0.L.y, n0, lane);
     }
     if (overflow && lane == 0) atomicOr(err, E_STACK);
     if (lane == 0) {
-        atomicAdd(&counters[C_NEAR_INTERNAL], n_near_int);
-        atomicAdd(&counters[C_FAR], n_far);
-        atomicAdd(&counters[C_LEAF_VISITS], n_leaf);
-        atomicAdd(&counters[C_LEAF_TRIS], n_tri);
+        atomicAdd(&counters[C_NEAR_INTERNAL], a.near_int);
+        atomicAdd(&counters[C_FAR], a.far);
+        atomicAdd(&counters[C_LEAF_VISITS], a.leaf);
+        atomicAdd(&counters[C_LEAF_TRIS], a.tri);
     }
     if (!valid) return;
-    const float w = s - cmp;
+    const float w = a.s - a.c;
     float* vp = values + slot;
     const float d = *vp;
     *vp = (w < thr) ? copysignf(d, 1.0f) : copysignf(d, -1.0f);
@@ -896,9 +1018,8 @@ __global__ __launch_bounds__(WN_BLOCK) void k_winding(const uint32_t* __restrict
 // FP64 re-evaluation of the voxels whose f32 winding number fell inside the
 // ambiguity band around the threshold: same tree, every term in double.
 __global__ void k_winding_fp64(const uint32_t* __restrict__ list, uint32_t n, const int32_t* __restrict__ origins, float h,
-                               const float4* __restrict__ nodes, const int4* __restrict__ links,
-                               const int4* __restrict__ leaf_links, int n_int, int n_nodes,
-                               const float* __restrict__ btris, float thr, float* __restrict__ values, unsigned* err) {
+                               const NodeRec* __restrict__ nodes, int n_nodes, const float* __restrict__ btris,
+                               float thr, float* __restrict__ values, unsigned* err) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t slot = list[i];
@@ -917,22 +1038,22 @@ __global__ void k_winding_fp64(const uint32_t* __restrict__ list, uint32_t n, co
             return;
         }
         const int node = stack[--sp];
-        const float4* nd = nodes + 4 * (uint64_t)node;
-        const float4 A = nd[0], B = nd[1], C = nd[2], D = nd[3];
+        const float4 A = nodes[node].A, B = nodes[node].B, C = nodes[node].C;
+        const float2 D = nodes[node].D;
         const double rx = A.x - px, ry = A.y - py, rz = A.z - pz;
         const double r2 = rx * rx + ry * ry + rz * rz;
-        if (r2 > (double)A.w) {
+        if (r2 > (double)A.w) {  // coefficients carry 1/(4 pi) and 3/(4 pi) (acc_finalize)
             double inv_r = 1.0 / sqrt(r2), inv_r2 = inv_r * inv_r;
             double rsr = C.x * rx * rx + C.y * ry * ry + C.z * rz * rz + C.w * rx * ry + D.x * rx * rz + D.y * ry * rz;
             double lin = B.x * rx + B.y * ry + B.z * rz + B.w;
-            w += (1.0 / (4.0 * M_PI)) * inv_r * inv_r2 * (lin - 3.0 * rsr * inv_r2);
+            w += inv_r * inv_r2 * (lin - rsr * inv_r2);
             continue;
         }
-        const int4 L = node < n_int ? links[node] : leaf_links[node - n_int];
-        if (L.w <= BVH_LEAF_SIZE) {
+        const int2 L = nodes[node].L;
+        if (L.y < 0) {
             double om = 0.0;
-            for (int k = 0; k < L.w; ++k) {
-                const float* t = btris + 9 * (uint64_t)(L.z + k);
+            for (int k = 0; k < -L.y; ++k) {
+                const float* t = btris + 12 * (uint64_t)(L.x + k);
                 om += solid_angle_rel64(t[0] - px, t[1] - py, t[2] - pz, t[3] - px, t[4] - py, t[5] - pz, t[6] - px,
                                         t[7] - py, t[8] - pz);
             }
@@ -1098,7 +1219,7 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
         bbox_leaves *= (uint64_t)(ghi[k] - glo[k] + 1);
         if (bbox_leaves > (1ull << 40)) bbox_leaves = 1ull << 40;
     }
-    ix.dense = bbox_leaves <= DENSE_LEAF_GRID_MAX && !ctx->force_hash;
+    ix.dense = bbox_leaves <= DENSE_LEAF_GRID_MAX && !(ctx->paths & BS_PATH_FORCE_HASH);
     uint32_t* leaf_idx = nullptr;
     uint64_t* hkeys_sorted = nullptr;
     uint32_t* d_count = reinterpret_cast<uint32_t*>(S.get(S_MISC1, 64));
@@ -1181,6 +1302,8 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     // ---- 5. LBVH + dipoles over the original mesh
     const int nt = (int)n;
     const int n_int = nt - 1;
+    int H = 0;  // tree height (root level)
+    bool use_lds = false;
     const int n_nodes = 2 * nt - 1;
     unsigned* bounds = reinterpret_cast<unsigned*>(S.get(S_MISC2, 64));
     const unsigned nb = div_up(n, 256);
@@ -1193,8 +1316,6 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     uint32_t* idx_s = S.as<uint32_t>(S_IDX_SORTED, n);
     k_morton<<<div_up(n, TB), TB, 0, st>>>(d_tris, n, bounds, codes, idx);
     sort_pairs(ctx, codes, codes_s, idx, idx_s, n, 63);
-    int4* links = S.as<int4>(S_BVH_LINKS, std::max(n_int, 1));
-    int4* leaf_links = S.as<int4>(S_ARGMIN, nt);
     int* parent = S.as<int>(S_BVH_PARENT, n_nodes);
     unsigned* hword = S.as<unsigned>(S_BVH_FLAGS, std::max(n_int, 1));
     uint32_t* height = S.as<uint32_t>(S_ROWS, std::max(n_int, 1));       // subdivision scratch is free now
@@ -1202,13 +1323,14 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     uint32_t* inode = S.as<uint32_t>(S_SUBS, std::max(n_int, 1));
     uint32_t* inode_s = S.as<uint32_t>(S_SUB_OFF, std::max(n_int, 1));
     NodeAcc* accs = S.as<NodeAcc>(S_BVH_INIT, n_nodes);
-    float4* nodes = S.as<float4>(S_BVH_NODES, 4 * (uint64_t)n_nodes);
-    float* btris = S.as<float>(S_BVH_TRIS, 9 * n);
+    NodeRec* nodes = S.as<NodeRec>(S_BVH_NODES, (uint64_t)n_nodes);  // internal nodes, then leaves
+    float* btris = S.as<float>(S_BVH_TRIS, 12 * (n + BVH_LEAF_SIZE));  // padded: leaves read 4 triangles
+    int2* kids = S.as<int2>(S_BVH_KIDS, std::max(n_int, 1));  // binary children of internal nodes
     BS_HIP(hipMemsetAsync(parent, 0xFF, sizeof(int) * n_nodes, st));
     BS_HIP(hipMemsetAsync(hword, 0, sizeof(unsigned) * std::max(n_int, 1), st));
-    if (n_int > 0) k_lbvh_build<<<div_up(n_int, TB), TB, 0, st>>>(codes_s, nt, links, parent);
+    if (n_int > 0) k_lbvh_build<<<div_up(n_int, TB), TB, 0, st>>>(codes_s, nt, nodes, kids, parent);
     k_lbvh_leaves<<<div_up(n, TB), TB, 0, st>>>(d_tris, nt, idx_s, prm.wn_accuracy, parent, hword, height, accs, nodes,
-                                                btris, leaf_links, err);
+                                                btris, err);
     if (n_int > 0) {
         // internal nodes grouped by height, then combined level by level
         k_iota<<<div_up(n_int, TB), TB, 0, st>>>(inode, (uint32_t)n_int);
@@ -1221,13 +1343,13 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
         BS_HIP(hipStreamSynchronize(st));
         if (read_scalar(ctx, err) & E_TREE) throw Error(BS_ERR_INTERNAL, "BVH parent links are corrupt");
         // heights run 1..H contiguously in the sorted order
-        int H = 0;
+        H = 0;
         for (int k = 1; k < 256; ++k)
             if (hs[k] != 0xFFFFFFFFu) H = k;
         for (int k = 1; k <= H; ++k) {
             BS_REQUIRE(hs[k] != 0xFFFFFFFFu, BS_ERR_INTERNAL, "BVH height levels are not contiguous");
             const uint32_t b = hs[k], e = (k == H) ? (uint32_t)n_int : hs[k + 1];
-            k_lbvh_level<<<div_up(e - b, TB), TB, 0, st>>>(inode_s + b, e - b, prm.wn_accuracy, links, accs, nodes);
+            k_lbvh_level<<<div_up(e - b, TB), TB, 0, st>>>(inode_s + b, e - b, prm.wn_accuracy, kids, accs, nodes);
         }
     }
     tm.mark();
@@ -1235,16 +1357,21 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     // ---- 6. winding numbers and signs
     uint32_t* recheck = S.as<uint32_t>(S_RECHECK, n_act);
     if (n_act > 0)
-        k_winding<<<div_up(n_act, WN_BLOCK), WN_BLOCK, 0, st>>>(act, n_act, vol->origins, h, nodes, links, leaf_links,
-                                                                n_int, n_nodes, btris, prm.sign_threshold, prm.fp64_band,
-                                                                vol->values, recheck, counters, err);
+    {
+        // a tree of height H needs at most H stack entries
+        use_lds = !(H + 2 < LaneStack::DEPTH) || (ctx->paths & BS_PATH_FORCE_LDS_STACK);
+        auto kern = use_lds ? k_winding<LdsStack> : k_winding<LaneStack>;
+        kern<<<div_up(n_act, WN_BLOCK), WN_BLOCK, 0, st>>>(act, n_act, vol->origins, h, nodes, n_nodes, btris,
+                                                           prm.sign_threshold, prm.fp64_band, vol->values, recheck,
+                                                           counters, err);
+    }
     tm.mark();
     BS_HIP(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, st));
     BS_HIP(hipStreamSynchronize(st));
     const uint32_t n_re = (uint32_t)hc[C_RECHECK];
     if (n_re > 0)
-        k_winding_fp64<<<div_up(n_re, 64), 64, 0, st>>>(recheck, n_re, vol->origins, h, nodes, links, leaf_links, n_int,
-                                                        n_nodes, btris, prm.sign_threshold, vol->values, err);
+        k_winding_fp64<<<div_up(n_re, 64), 64, 0, st>>>(recheck, n_re, vol->origins, h, nodes, n_nodes, btris,
+                                                        prm.sign_threshold, vol->values, err);
     unsigned e = read_scalar(ctx, err);
     if (e & E_STACK) throw Error(BS_ERR_INTERNAL, "winding-number traversal stack overflow");
     if (e & E_TREE) throw Error(BS_ERR_INTERNAL, "BVH parent links are corrupt");
@@ -1265,6 +1392,8 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     stt.wn_leaf_tris = hc[C_LEAF_TRIS];
     stt.n_fp64_rechecks = n_re;
     stt.used_hash = ix.dense ? 0 : 1;
+    stt.used_lds_stack = use_lds ? 1 : 0;
+    stt.bvh_height = (uint64_t)H;
     stt.ms_total = std::chrono::duration<double, std::milli>(wall1 - wall0).count();
     stt.ms_subdivide = tm.ms(0, 1);
     stt.ms_leaves = tm.ms(1, 2);

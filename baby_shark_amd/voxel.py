@@ -43,9 +43,9 @@ class Context:
     def set_profiling(self, on: bool) -> None:
         check(load().bs_ctx_set_profiling(self._h, 1 if on else 0))
 
-    def set_force_hash(self, on: bool) -> None:
-        """Allocate leaves through the GPU hash even when a dense leaf grid fits."""
-        check(load().bs_ctx_set_force_hash(self._h, 1 if on else 0))
+    def set_paths(self, flags: int) -> None:
+        """Force alternate code paths (BS_PATH_* flags of bs_voxel.h); 0 = by size."""
+        check(load().bs_ctx_set_paths(self._h, int(flags)))
 
     def last_stats(self) -> dict:
         s = _lib.M2VStats()

@@ -60,7 +60,7 @@ typedef struct {
 typedef struct {
     uint64_t n_tris, n_sub, n_pairs, n_leaves, n_active, n_candidates;
     uint64_t n_bvh_nodes, wn_node_visits, wn_leaf_tris, wn_far_evals, n_fp64_rechecks;
-    uint64_t wn_leaf_visits, used_hash;
+    uint64_t wn_leaf_visits, used_hash, used_lds_stack, bvh_height;
     double ms_total, ms_subdivide, ms_leaves, ms_distance, ms_bvh, ms_winding;
 } bs_m2v_stats;
 
@@ -74,9 +74,14 @@ void* bs_ctx_stream(bs_ctx* ctx);
 /* Per-kernel event timing inside bs_mesh_to_volume* (adds syncs; off by default). */
 bs_status bs_ctx_set_profiling(bs_ctx* ctx, int enabled);
 bs_status bs_ctx_last_m2v_stats(const bs_ctx* ctx, bs_m2v_stats* out);
-/* Leaf allocation: 0 = dense leaf grid over the band's bounding box when it
- * has at most 2^28 cells, else hash (default); 1 = always hash. */
-bs_status bs_ctx_set_force_hash(bs_ctx* ctx, int enabled);
+/* Alternate code paths, normally chosen by size (for tests and diagnosis):
+ * BS_PATH_FORCE_HASH      leaf allocation through the GPU hash even when a dense
+ *                         leaf grid over the band's bounding box (<= 2^28 cells) fits;
+ * BS_PATH_FORCE_LDS_STACK winding-number traversal with its LDS stack even for
+ *                         trees low enough for the register stack (height < 62). */
+#define BS_PATH_FORCE_HASH 1
+#define BS_PATH_FORCE_LDS_STACK 2
+bs_status bs_ctx_set_paths(bs_ctx* ctx, int flags);
 
 void bs_m2v_params_default(bs_m2v_params* p);
 

@@ -160,14 +160,31 @@ def test_hash_leaf_allocation_matches_dense(gpu_ctx, oracle_lib):
     """The GPU hash (used for extents too large for a dense leaf grid) gives the
     same volume as the dense leaf grid, and both match the oracle."""
     soup = mg.displaced_icosphere(4, radius=0.8, seed=6)
-    gpu_ctx.set_force_hash(True)
+    gpu_ctx.set_paths(bs._lib.BS_PATH_FORCE_HASH)
     try:
         a = run_gpu(gpu_ctx, soup, 0.01, 3).export()
         assert gpu_ctx.last_stats()["used_hash"] == 1
     finally:
-        gpu_ctx.set_force_hash(False)
+        gpu_ctx.set_paths(0)
     b = run_gpu(gpu_ctx, soup, 0.01, 3).export()
     assert gpu_ctx.last_stats()["used_hash"] == 0
     assert np.array_equal(a.origins, b.origins) and np.array_equal(a.masks, b.masks)
     assert np.array_equal(a.values, b.values)
     check_case(gpu_ctx, oracle_lib, soup, 0.01, 3)
+
+
+def test_lds_stack_traversal_matches_register_stack(gpu_ctx, oracle_lib):
+    """The LDS-stack traversal (trees of height >= 62) signs exactly like the
+    register-stack one used for lower trees."""
+    soup = mg.displaced_icosphere(5, radius=0.8, seed=8)
+    b = run_gpu(gpu_ctx, soup, 0.006, 3).export()
+    assert gpu_ctx.last_stats()["used_lds_stack"] == 0
+    gpu_ctx.set_paths(bs._lib.BS_PATH_FORCE_LDS_STACK)
+    try:
+        a = run_gpu(gpu_ctx, soup, 0.006, 3).export()
+        assert gpu_ctx.last_stats()["used_lds_stack"] == 1
+    finally:
+        gpu_ctx.set_paths(0)
+    assert np.array_equal(a.masks, b.masks)
+    assert np.array_equal(np.signbit(a.values), np.signbit(b.values))
+    assert np.max(np.abs(np.abs(a.values) - np.abs(b.values))) == 0.0
