@@ -92,6 +92,13 @@ def load():
             "bs_volume_info_get": (C.c_int, [vp, C.POINTER(VolumeInfo)]),
             "bs_volume_export": (C.c_int, [vp, i32p, u64p, f32p]),
             "bs_volume_export_tiles": (C.c_int, [vp, i32p, i32p, f32p]),
+            "bs_volume_sphere": (C.c_int, [vp, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, pp]),
+            "bs_volume_from_voxels": (C.c_int, [vp, i32p, f32p, C.c_size_t, C.c_float, pp]),
+            "bs_volume_union": (C.c_int, [vp, vp, vp]),
+            "bs_volume_subtract": (C.c_int, [vp, vp, vp]),
+            "bs_volume_intersect": (C.c_int, [vp, vp, vp]),
+            "bs_volume_flood_fill": (C.c_int, [vp, vp]),
+            "bs_volume_signs_at": (C.c_int, [vp, vp, i32p, C.c_size_t, C.POINTER(C.c_uint8)]),
             "bs_volume_device_leaves": (C.c_int, [vp, C.POINTER(i32p), C.POINTER(u64p), C.POINTER(f32p),
                                                   C.POINTER(C.c_uint64)]),
         }

@@ -11,6 +11,11 @@ static thread_local std::string g_last_error;
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
 void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const bs_m2v_params& prm, bs_volume** out);
+void volume_sphere(bs_ctx* ctx, float radius, float cx, float cy, float cz, float h, bs_volume** out);
+void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op);
+void volume_flood_fill(bs_ctx* ctx, bs_volume* v);
+void volume_from_voxels(bs_ctx* ctx, const int32_t* idx, const float* values, uint64_t n, float h, bs_volume** out);
+void volume_signs_at(bs_ctx* ctx, bs_volume* v, const int32_t* idx, uint64_t n, uint8_t* out_neg);
 
 template <class F>
 static bs_status guarded(F&& f) {
@@ -158,6 +163,7 @@ bs_status bs_volume_clone(const bs_volume* v, bs_volume** out) {
         c->voxel_size = v->voxel_size;
         c->n_leaves = v->n_leaves;
         c->tiles = v->tiles;
+        if (v->topo) c->topo = v->topo->clone();
         try {
             if (v->n_leaves) {
                 BS_HIP(hipMalloc(&c->origins, sizeof(int32_t) * 4 * v->n_leaves));
@@ -223,7 +229,7 @@ bs_status bs_volume_export(const bs_volume* v, int32_t* origins, uint64_t* masks
 bs_status bs_volume_export_tiles(const bs_volume* v, int32_t* origins, int32_t* sizes, float* values) {
     if (!v) return BS_ERR_INVALID_ARG;
     for (size_t i = 0; i < v->tiles.size(); ++i) {
-        const bs_tile& t = v->tiles[i];
+        const bs::TileRec& t = v->tiles[i];
         if (origins) {
             origins[3 * i] = t.x;
             origins[3 * i + 1] = t.y;
@@ -233,6 +239,79 @@ bs_status bs_volume_export_tiles(const bs_volume* v, int32_t* origins, int32_t* 
         if (values) values[i] = t.value;
     }
     return BS_OK;
+}
+
+bs_status bs_volume_sphere(bs_ctx* ctx, float radius, float cx, float cy, float cz, float voxel_size,
+                           bs_volume** out) {
+    if (!ctx || !out) {
+        g_last_error = "null argument";
+        return BS_ERR_INVALID_ARG;
+    }
+    *out = nullptr;
+    return guarded([&] {
+        DeviceGuard g(ctx->device);
+        volume_sphere(ctx, radius, cx, cy, cz, voxel_size, out);
+    });
+}
+
+bs_status bs_volume_from_voxels(bs_ctx* ctx, const int32_t* idx, const float* values, size_t n, float voxel_size,
+                                bs_volume** out) {
+    if (!ctx || !out || (n > 0 && (!idx || !values))) {
+        g_last_error = "null argument";
+        return BS_ERR_INVALID_ARG;
+    }
+    *out = nullptr;
+    return guarded([&] {
+        DeviceGuard g(ctx->device);
+        volume_from_voxels(ctx, idx, values, n, voxel_size, out);
+    });
+}
+
+static bs_status csg_entry(bs_ctx* ctx, bs_volume* self, bs_volume* other, CsgOp op) {
+    if (!ctx || !self || !other) {
+        g_last_error = "null argument";
+        return BS_ERR_INVALID_ARG;
+    }
+    const bs_status st = guarded([&] {
+        DeviceGuard g(ctx->device);
+        volume_csg(ctx, self, other, op);
+    });
+    if (st == BS_OK) bs_volume_free(other);
+    return st;
+}
+
+bs_status bs_volume_union(bs_ctx* ctx, bs_volume* self, bs_volume* other) {
+    return csg_entry(ctx, self, other, CSG_UNION);
+}
+bs_status bs_volume_subtract(bs_ctx* ctx, bs_volume* self, bs_volume* other) {
+    return csg_entry(ctx, self, other, CSG_SUBTRACT);
+}
+bs_status bs_volume_intersect(bs_ctx* ctx, bs_volume* self, bs_volume* other) {
+    return csg_entry(ctx, self, other, CSG_INTERSECT);
+}
+
+bs_status bs_volume_flood_fill(bs_ctx* ctx, bs_volume* v) {
+    if (!ctx || !v) {
+        g_last_error = "null argument";
+        return BS_ERR_INVALID_ARG;
+    }
+    return guarded([&] {
+        DeviceGuard g(ctx->device);
+        BS_REQUIRE(v->device == ctx->device, BS_ERR_INVALID_ARG, "volume lives on another device");
+        volume_flood_fill(ctx, v);
+    });
+}
+
+bs_status bs_volume_signs_at(bs_ctx* ctx, bs_volume* v, const int32_t* idx, size_t n, uint8_t* out_negative) {
+    if (!ctx || !v || (n > 0 && (!idx || !out_negative))) {
+        g_last_error = "null argument";
+        return BS_ERR_INVALID_ARG;
+    }
+    return guarded([&] {
+        DeviceGuard g(ctx->device);
+        BS_REQUIRE(v->device == ctx->device, BS_ERR_INVALID_ARG, "volume lives on another device");
+        volume_signs_at(ctx, v, idx, n, out_negative);
+    });
 }
 
 bs_status bs_volume_device_leaves(const bs_volume* v, const int32_t** origins, const uint64_t** masks,

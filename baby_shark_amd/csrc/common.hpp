@@ -7,9 +7,11 @@
 #include <cstdio>
 #include <stdexcept>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "../../include/bs_voxel.h"
+#include "topology.hpp"
 
 namespace bs {
 
@@ -42,7 +44,7 @@ enum ScratchSlot : int {
     S_KEYS, S_KEYS_SORTED, S_CAND_LEAF, S_CAND_CNT, S_CAND_OFF, S_CAND_LIST, S_CURSOR, S_ACT_CNT, S_ACT_OFF,
     S_ACT_LIST, S_COUNTERS, S_CUB, S_MORTON, S_MORTON_SORTED, S_IDX, S_IDX_SORTED, S_BVH_TRIS, S_BVH_NODES,
     S_BVH_LINKS, S_BVH_PARENT, S_BVH_FLAGS, S_BVH_INIT, S_BVH_BOX, S_RECHECK, S_ARGMIN, S_MISC0, S_MISC1,
-    S_MISC2, S_MISC3, S_INPUT, S_BVH_KIDS, S_COUNT
+    S_MISC2, S_MISC3, S_INPUT, S_BVH_KIDS, S_FLAGS_A, S_FLAGS_B, S_CSG_PLAN, S_COUNT
 };
 
 struct Scratch {
@@ -83,12 +85,6 @@ struct bs_ctx {
     uint64_t* host_counters = nullptr;  // pinned, for small readbacks
 };
 
-// An active tile of an internal node (origin, edge length in voxels, value).
-struct bs_tile {
-    int32_t x, y, z, size;
-    float value;
-};
-
 struct bs_volume {
     int device = 0;
     float voxel_size = 1.0f;
@@ -96,7 +92,8 @@ struct bs_volume {
     int32_t* origins = nullptr;  // device, n_leaves*4 (x, y, z, 0)
     uint64_t* masks = nullptr;   // device, n_leaves*8
     float* values = nullptr;     // device, n_leaves*512
-    std::vector<bs_tile> tiles;  // active internal tiles (host, few)
+    std::vector<bs::TileRec> tiles;           // active internal tiles (host, few)
+    std::unique_ptr<bs::Topology> topo;       // upper levels; built from the leaves when absent
 };
 
 namespace bs {

@@ -1,0 +1,418 @@
+// Host-side topology: slot rules of flood fill and CSG for the two internal
+// levels and the root of the volume grid.  See topology.hpp.
+#include "topology.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace bs {
+
+namespace {
+
+constexpr int32_t L1_SPAN = 4096;  // voxels covered by a Node1
+constexpr int32_t L2_SPAN = 128;   // voxels covered by a Node2
+
+inline bool neg(float v) { return std::signbit(v); }
+inline float far_value(bool negative) { return negative ? -F32_MAX_V : F32_MAX_V; }
+
+// internal_node/mod.rs:130-136 with BRANCHING 5 (children of 128) and 4 (children of 8)
+inline int l1_slot(int32_t x, int32_t y, int32_t z) {
+    return (((x & (L1_SPAN - 1)) >> 7) << 10) | (((y & (L1_SPAN - 1)) >> 7) << 5) | ((z & (L1_SPAN - 1)) >> 7);
+}
+inline int l2_slot(int32_t x, int32_t y, int32_t z) {
+    return (((x & (L2_SPAN - 1)) >> 3) << 8) | (((y & (L2_SPAN - 1)) >> 3) << 4) | ((z & (L2_SPAN - 1)) >> 3);
+}
+
+template <int LOG2>
+inline void slot_origin(const SlotNode<LOG2>& n, int o, int child_span, int32_t& x, int32_t& y, int32_t& z) {
+    constexpr int D = SlotNode<LOG2>::DIM;
+    x = n.ox + (o / (D * D)) * child_span;
+    y = n.oy + ((o / D) % D) * child_span;
+    z = n.oz + (o % D) * child_span;
+}
+
+// first/last value sign of a node (internal_node/flood_fill.rs:94-108): the
+// "last" sign of a branch in the last slot is that branch's *first* sign.
+template <int LOG2, class ChildFirst>
+bool first_neg(const SlotNode<LOG2>& n, ChildFirst child_first) {
+    return n.kind[0] == SLOT_BRANCH ? child_first(n.child[0]) : neg(n.tile[0]);
+}
+template <int LOG2, class ChildFirst>
+bool last_neg(const SlotNode<LOG2>& n, ChildFirst child_first) {
+    constexpr int L = SlotNode<LOG2>::SIZE - 1;
+    return n.kind[L] == SLOT_BRANCH ? child_first(n.child[L]) : neg(n.tile[L]);
+}
+
+// Scan-line sign propagation over the slots of one node
+// (internal_node/flood_fill.rs:17-75).  child_first / child_last give the
+// first / last value sign of a branch child.
+template <int LOG2, class ChildFirst, class ChildLast>
+void fill_node(SlotNode<LOG2>& n, ChildFirst child_first, ChildLast child_last) {
+    constexpr int D = SlotNode<LOG2>::DIM, N = SlotNode<LOG2>::SIZE;
+    int fv = -1, fb = -1;
+    bool full = true;
+    for (int o = 0; o < N; ++o) {
+        const uint8_t k = n.kind[o];
+        full &= k == SLOT_TILE;
+        if (k == SLOT_TILE && fv < 0) fv = o;
+        if (k == SLOT_BRANCH && fb < 0) fb = o;
+    }
+    if (full) return;
+    bool s;
+    if (fv >= 0 && fb >= 0) s = (fv <= fb) ? neg(n.tile[fv]) : child_first(n.child[fb]);
+    else if (fv >= 0) s = neg(n.tile[fv]);
+    else if (fb >= 0) s = child_first(n.child[fb]);
+    else return;
+    auto present = [&](int o, bool& sign) {
+        if (n.kind[o] == SLOT_BRANCH) {
+            sign = child_last(n.child[o]);
+            return true;
+        }
+        if (n.kind[o] == SLOT_TILE) {
+            sign = neg(n.tile[o]);
+            return true;
+        }
+        return false;
+    };
+    bool i = s;
+    for (int x = 0; x < D; ++x) {
+        const int x00 = x * D * D;
+        present(x00, i);
+        bool j = i;
+        for (int y = 0; y < D; ++y) {
+            const int xy0 = x00 + y * D;
+            present(xy0, j);
+            bool k = j;
+            for (int z = 0; z < D; ++z) {
+                const int o = xy0 + z;
+                if (!present(o, k)) n.tile[o] = far_value(k);
+            }
+        }
+    }
+}
+
+}  // namespace
+
+std::unique_ptr<Topology> Topology::from_leaves(const std::vector<std::array<int32_t, 3>>& origins,
+                                                const std::vector<TileRec>& tiles) {
+    auto t = std::make_unique<Topology>();
+    auto node1_for = [&](int32_t x, int32_t y, int32_t z) -> Node1& {
+        Key k{x & ~(L1_SPAN - 1), y & ~(L1_SPAN - 1), z & ~(L1_SPAN - 1)};
+        auto it = t->root.find(k);
+        if (it == t->root.end()) {
+            t->n1.emplace_back(k[0], k[1], k[2]);
+            it = t->root.emplace(k, (int32_t)t->n1.size() - 1).first;
+        }
+        return t->n1[it->second];
+    };
+    auto node2_for = [&](int32_t x, int32_t y, int32_t z) -> Node2& {
+        Node1& a = node1_for(x, y, z);
+        const int o = l1_slot(x, y, z);
+        if (a.kind[o] != SLOT_BRANCH) {
+            int32_t nx, ny, nz;
+            slot_origin(a, o, L2_SPAN, nx, ny, nz);
+            const int32_t idx = (int32_t)t->n2.size();
+            t->n2.emplace_back(nx, ny, nz);
+            Node1& a2 = node1_for(x, y, z);  // n1 may not move, but re-fetch for clarity
+            a2.kind[o] = SLOT_BRANCH;
+            a2.child[o] = idx;
+            return t->n2.back();
+        }
+        return t->n2[a.child[o]];
+    };
+    t->leaves.resize(origins.size());
+    for (size_t i = 0; i < origins.size(); ++i) {
+        const auto& g = origins[i];
+        Node2& b = node2_for(g[0], g[1], g[2]);
+        const int o = l2_slot(g[0], g[1], g[2]);
+        b.kind[o] = SLOT_BRANCH;
+        b.child[o] = (int32_t)i;
+        LeafRef& r = t->leaves[i];
+        r.x = g[0];
+        r.y = g[1];
+        r.z = g[2];
+        r.a = (int32_t)i;
+        r.op = LEAF_COPY_A;
+    }
+    for (const auto& tl : tiles) {
+        if (tl.size == L2_SPAN) {
+            Node1& a = node1_for(tl.x, tl.y, tl.z);
+            const int o = l1_slot(tl.x, tl.y, tl.z);
+            a.kind[o] = SLOT_TILE;
+            a.tile[o] = tl.value;
+        } else {
+            Node2& b = node2_for(tl.x, tl.y, tl.z);
+            const int o = l2_slot(tl.x, tl.y, tl.z);
+            b.kind[o] = SLOT_TILE;
+            b.tile[o] = tl.value;
+        }
+    }
+    return t;
+}
+
+void Topology::flood_fill_upper() {
+    auto leaf_first = [&](int32_t li) { return leaves[li].first_neg != 0; };
+    auto leaf_last = [&](int32_t li) { return leaves[li].last_neg != 0; };
+    for (auto& n : n2) fill_node(n, leaf_first, leaf_last);
+    auto n2_first = [&](int32_t k) { return first_neg(n2[k], leaf_first); };
+    auto n2_last = [&](int32_t k) { return last_neg(n2[k], leaf_first); };
+    for (auto& n : n1) fill_node(n, n2_first, n2_last);
+    // root_node/flood_fill.rs:14-41: negative tiles between two negative
+    // children on one z line
+    std::vector<Key> keys;
+    for (auto& kv : root) keys.push_back(kv.first);
+    for (size_t i = 0; i + 1 < keys.size(); ++i) {
+        const Key& a = keys[i];
+        const Key& b = keys[i + 1];
+        if (a[0] != b[0] || a[1] != b[1] || b[2] == a[2] + L1_SPAN) continue;
+        const bool a_neg = last_neg(n1[root[a]], n2_first);
+        const bool b_neg = first_neg(n1[root[b]], n2_first);
+        if (!a_neg || !b_neg) continue;
+        for (int32_t z = a[2] + L1_SPAN; z < b[2]; z += L1_SPAN) {
+            n1.emplace_back(a[0], a[1], z);
+            Node1& fresh = n1.back();
+            std::fill(fresh.tile.begin(), fresh.tile.end(), -F32_MAX_V);  // fill_with_sign(Negative)
+            root[Key{a[0], a[1], z}] = (int32_t)n1.size() - 1;
+        }
+    }
+}
+
+namespace {
+
+struct Csg {
+    Topology& t;
+    CsgOp op;
+
+    template <int LOG2>
+    static bool inside(const SlotNode<LOG2>& n, int o) { return n.kind[o] != SLOT_BRANCH && neg(n.tile[o]); }
+    template <int LOG2>
+    static bool outside(const SlotNode<LOG2>& n, int o) { return n.kind[o] != SLOT_BRANCH && !neg(n.tile[o]); }
+    // internal_node/mod.rs:81-93
+    template <int LOG2>
+    static void remove_child(SlotNode<LOG2>& n, int o) {
+        if (n.kind[o] != SLOT_EMPTY) n.kind[o] = SLOT_EMPTY;
+    }
+    // internal_node/csg.rs:42-48
+    template <int LOG2>
+    static void make_inside(SlotNode<LOG2>& n, int o) {
+        n.kind[o] = SLOT_TILE;
+        n.tile[o] = -F32_MAX_V;
+    }
+    // internal_node/csg.rs:31-39: move other's branch into self
+    template <int LOG2>
+    static void take(SlotNode<LOG2>& s, SlotNode<LOG2>& other, int o) {
+        if (other.kind[o] != SLOT_BRANCH) return;
+        std::swap(s.child[o], other.child[o]);
+        std::swap(s.tile[o], other.tile[o]);
+        other.kind[o] = s.kind[o];
+        s.kind[o] = SLOT_BRANCH;
+    }
+
+    void flip_leaf(int32_t li) { t.leaves[li].neg ^= 1; }
+    // internal_node/csg.rs:154-162 on a Node2
+    void flip_node2(int32_t k) {
+        Node2& n = t.n2[k];
+        for (int o = 0; o < Node2::SIZE; ++o) {
+            if (n.kind[o] == SLOT_BRANCH) flip_leaf(n.child[o]);
+            else if (n.kind[o] == SLOT_TILE) n.tile[o] = -n.tile[o];
+        }
+    }
+
+    // leaf_node/csg.rs:17-45: the combination is recorded, the device does it
+    void leaf_pair(int32_t sa, int32_t ob) {
+        LeafRef& r = t.leaves[sa];
+        const LeafRef& b = t.leaves[ob];
+        r.b = b.b;
+        r.op = op == CSG_UNION ? LEAF_UNION : (op == CSG_SUBTRACT ? LEAF_SUBTRACT : LEAF_INTERSECT);
+        r.empty = r.empty && b.empty;
+    }
+    bool node2_empty(int32_t k) const {
+        const Node2& n = t.n2[k];
+        for (int o = 0; o < Node2::SIZE; ++o)
+            if (n.kind[o] != SLOT_EMPTY) return false;
+        return true;
+    }
+
+    // internal_node/csg.rs:68-152 over the slots of one node; `pair` combines
+    // two branch children and returns true when the result is empty.
+    template <int LOG2, class Pair, class Flip>
+    void slots(SlotNode<LOG2>& s, SlotNode<LOG2>& o_, Pair pair, Flip flip) {
+        for (int o = 0; o < SlotNode<LOG2>::SIZE; ++o) {
+            if (op == CSG_UNION) {
+                if (inside(s, o)) continue;
+                if (inside(o_, o)) {
+                    make_inside(s, o);
+                    continue;
+                }
+                if (outside(s, o)) {
+                    take(s, o_, o);
+                    continue;
+                }
+                if (s.kind[o] == SLOT_BRANCH && o_.kind[o] == SLOT_BRANCH)
+                    if (pair(s.child[o], o_.child[o])) make_inside(s, o);
+            } else if (op == CSG_SUBTRACT) {
+                if (outside(s, o) || outside(o_, o)) continue;
+                if (inside(o_, o)) {
+                    remove_child(s, o);
+                    continue;
+                }
+                if (inside(s, o)) {
+                    take(s, o_, o);
+                    if (s.kind[o] == SLOT_BRANCH) flip(s.child[o]);
+                    else if (s.kind[o] == SLOT_TILE) s.tile[o] = -s.tile[o];
+                    continue;
+                }
+                if (s.kind[o] == SLOT_BRANCH && o_.kind[o] == SLOT_BRANCH) pair(s.child[o], o_.child[o]);
+            } else {
+                if (outside(s, o) || inside(o_, o)) continue;
+                if (outside(o_, o)) {
+                    remove_child(s, o);
+                    continue;
+                }
+                if (inside(s, o)) {
+                    take(s, o_, o);
+                    continue;
+                }
+                if (s.kind[o] == SLOT_BRANCH && o_.kind[o] == SLOT_BRANCH) pair(s.child[o], o_.child[o]);
+            }
+        }
+    }
+
+    bool node2_pair(int32_t a, int32_t b) {
+        slots(t.n2[a], t.n2[b],
+              [&](int32_t la, int32_t lb) {
+                  leaf_pair(la, lb);
+                  return t.leaves[la].empty != 0;
+              },
+              [&](int32_t li) { flip_leaf(li); });
+        return node2_empty(a);
+    }
+
+    void node1_pair(int32_t a, int32_t b) {
+        slots(t.n1[a], t.n1[b], [&](int32_t ka, int32_t kb) { return node2_pair(ka, kb); },
+              [&](int32_t k) { flip_node2(k); });
+    }
+};
+
+}  // namespace
+
+void Topology::csg(Topology&& other, CsgOp op) {
+    // other's pools are appended; its leaves become B-side references
+    const int32_t n1_base = (int32_t)n1.size(), n2_base = (int32_t)n2.size(), leaf_base = (int32_t)leaves.size();
+    for (auto r : other.leaves) {
+        r.b = r.a;
+        r.a = -1;
+        r.op = LEAF_COPY_B;
+        leaves.push_back(r);
+    }
+    for (auto& n : other.n2) {
+        for (int o = 0; o < Node2::SIZE; ++o)
+            if (n.kind[o] == SLOT_BRANCH) n.child[o] += leaf_base;
+        n2.push_back(std::move(n));
+    }
+    for (auto& n : other.n1) {
+        for (int o = 0; o < Node1::SIZE; ++o)
+            if (n.kind[o] == SLOT_BRANCH) n.child[o] += n2_base;
+        n1.push_back(std::move(n));
+    }
+    std::map<Key, int32_t> oroot;
+    for (auto& kv : other.root) oroot[kv.first] = kv.second + n1_base;
+    Csg c{*this, op};
+    if (op == CSG_UNION) {  // root_node/csg.rs:9-26
+        for (auto& kv : oroot) {
+            auto it = root.find(kv.first);
+            if (it != root.end()) c.node1_pair(it->second, kv.second);
+            else root[kv.first] = kv.second;
+        }
+    } else if (op == CSG_SUBTRACT) {  // root_node/csg.rs:28-37
+        for (auto& kv : oroot) {
+            auto it = root.find(kv.first);
+            if (it != root.end()) c.node1_pair(it->second, kv.second);
+        }
+    } else {  // root_node/csg.rs:39-54
+        for (auto it = root.begin(); it != root.end();) {
+            if (!oroot.count(it->first)) it = root.erase(it);
+            else ++it;
+        }
+        for (auto& kv : root) c.node1_pair(kv.second, oroot[kv.first]);
+    }
+}
+
+void Topology::collect(std::vector<int32_t>& leaf_refs, std::vector<TileRec>& tiles) const {
+    for (const auto& kv : root) {
+        const Node1& a = n1[kv.second];
+        for (int o1 = 0; o1 < Node1::SIZE; ++o1) {
+            if (a.kind[o1] == SLOT_TILE) {
+                int32_t x, y, z;
+                slot_origin(a, o1, L2_SPAN, x, y, z);
+                tiles.push_back({x, y, z, L2_SPAN, a.tile[o1]});
+            } else if (a.kind[o1] == SLOT_BRANCH) {
+                const Node2& b = n2[a.child[o1]];
+                for (int o2 = 0; o2 < Node2::SIZE; ++o2) {
+                    if (b.kind[o2] == SLOT_BRANCH) {
+                        leaf_refs.push_back(b.child[o2]);
+                    } else if (b.kind[o2] == SLOT_TILE) {
+                        int32_t x, y, z;
+                        slot_origin(b, o2, 8, x, y, z);
+                        tiles.push_back({x, y, z, 8, b.tile[o2]});
+                    }
+                }
+            }
+        }
+    }
+}
+
+int32_t Topology::locate(int32_t x, int32_t y, int32_t z, bool* neg_out) const {
+    auto it = root.find(Key{x & ~(L1_SPAN - 1), y & ~(L1_SPAN - 1), z & ~(L1_SPAN - 1)});
+    if (it == root.end()) {
+        *neg_out = false;
+        return -1;
+    }
+    const Node1& a = n1[it->second];
+    const int o1 = l1_slot(x, y, z);
+    if (a.kind[o1] != SLOT_BRANCH) {
+        *neg_out = neg(a.tile[o1]);
+        return -1;
+    }
+    const Node2& b = n2[a.child[o1]];
+    const int o2 = l2_slot(x, y, z);
+    if (b.kind[o2] != SLOT_BRANCH) {
+        *neg_out = neg(b.tile[o2]);
+        return -1;
+    }
+    return b.child[o2];
+}
+
+std::unique_ptr<Topology> Topology::compact(const std::vector<int32_t>& new_index) const {
+    auto t = std::make_unique<Topology>();
+    int32_t max_leaf = -1;
+    for (int32_t v : new_index) max_leaf = std::max(max_leaf, v);
+    t->leaves.resize((size_t)(max_leaf + 1));
+    for (const auto& kv : root) {
+        const Node1& a = n1[kv.second];
+        Node1 na = a;
+        for (int o1 = 0; o1 < Node1::SIZE; ++o1) {
+            if (a.kind[o1] != SLOT_BRANCH) continue;
+            Node2 nb = n2[a.child[o1]];
+            for (int o2 = 0; o2 < Node2::SIZE; ++o2) {
+                if (nb.kind[o2] != SLOT_BRANCH) continue;
+                const int32_t r = nb.child[o2];
+                const int32_t ni = new_index[r];
+                LeafRef& lr = t->leaves[ni];
+                lr = leaves[r];
+                lr.a = ni;
+                lr.b = -1;
+                lr.op = LEAF_COPY_A;
+                lr.neg = 0;
+                nb.child[o2] = ni;
+            }
+            t->n2.push_back(std::move(nb));
+            na.child[o1] = (int32_t)t->n2.size() - 1;
+        }
+        t->n1.push_back(std::move(na));
+        t->root[kv.first] = (int32_t)t->n1.size() - 1;
+    }
+    return t;
+}
+
+}  // namespace bs

@@ -1,0 +1,89 @@
+// Host-side topology of a volume: the upper levels of the reference's grid
+// `dynamic_vdb!(f32, par 5, 4, 3)` (src/voxel/volume/mod.rs:8) -- an ordered
+// root map of 4096^3 blocks, 32^3-slot and 16^3-slot internal nodes -- kept as
+// flat slot arrays (kind, tile value, child index).  Leaves (8^3 values and a
+// mask) live in device memory; a slot of a 16^3 node refers to a leaf by index.
+//
+// The slot rules of flood fill and CSG follow internal_node/{flood_fill,csg}.rs
+// and root_node/{flood_fill,csg}.rs; each is cited where implemented.
+#pragma once
+#include <array>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <vector>
+
+namespace bs {
+
+enum : uint8_t { SLOT_EMPTY = 0, SLOT_TILE = 1, SLOT_BRANCH = 2 };
+enum : uint8_t { LEAF_COPY_A = 0, LEAF_COPY_B = 1, LEAF_UNION = 2, LEAF_SUBTRACT = 3, LEAF_INTERSECT = 4 };
+enum CsgOp : int { CSG_UNION = 0, CSG_SUBTRACT = 1, CSG_INTERSECT = 2 };
+
+constexpr float F32_MAX_V = 3.40282347e+38f;
+
+// An active tile of an internal node: origin, edge length (8 or 128), value.
+struct TileRec {
+    int32_t x, y, z, size;
+    float value;
+};
+
+// A leaf of the volume being built: where its 512 values come from.
+struct LeafRef {
+    int32_t x, y, z;       // origin
+    int32_t a = -1, b = -1;  // source leaf in operand A / operand B
+    uint8_t op = LEAF_COPY_A;
+    uint8_t neg = 0;        // negate every value (flip_signs)
+    uint8_t first_neg = 0, last_neg = 0, empty = 0;  // after the leaf flood fill
+};
+
+template <int LOG2>
+struct SlotNode {
+    static constexpr int DIM = 1 << LOG2;
+    static constexpr int SIZE = 1 << (3 * LOG2);
+    int32_t ox, oy, oz;
+    std::vector<uint8_t> kind;
+    std::vector<float> tile;
+    std::vector<int32_t> child;
+    SlotNode(int32_t x, int32_t y, int32_t z) : ox(x), oy(y), oz(z), kind(SIZE, SLOT_EMPTY), tile(SIZE, 0.0f), child(SIZE, -1) {}
+};
+using Node2 = SlotNode<4>;  // 16^3 slots of 8^3 leaves: covers 128^3
+using Node1 = SlotNode<5>;  // 32^3 slots of Node2: covers 4096^3
+
+struct Topology {
+    using Key = std::array<int32_t, 3>;  // ordered (x, y, z), root_node/mod.rs:55-67
+    std::map<Key, int32_t> root;         // -> Node1 index
+    std::vector<Node1> n1;
+    std::vector<Node2> n2;
+    std::vector<LeafRef> leaves;
+
+    // Builds the tree of a leaf set sorted or not, plus active tiles
+    // (origin, edge 8 or 128, value).
+    static std::unique_ptr<Topology> from_leaves(const std::vector<std::array<int32_t, 3>>& origins,
+                                                 const std::vector<TileRec>& tiles);
+    std::unique_ptr<Topology> clone() const { return std::make_unique<Topology>(*this); }
+
+    // Flood fill of the internal levels and the root (leaves already filled,
+    // their first/last signs in `leaves`): internal_node/flood_fill.rs:17-75,
+    // root_node/flood_fill.rs:8-42.
+    void flood_fill_upper();
+
+    // Merges `other` into this topology (its leaves become B-side references,
+    // its nodes are appended) and applies the CSG of root_node/csg.rs:9-58 /
+    // internal_node/csg.rs:68-162 with this tree as `self`.
+    void csg(Topology&& other, CsgOp op);
+
+    // Leaves (in tree order) and active tiles of the result.
+    void collect(std::vector<int32_t>& leaf_refs, std::vector<TileRec>& tiles) const;
+
+    // sign_at down to the leaf level (root_node/flood_fill.rs:56-63,
+    // internal_node/flood_fill.rs:110-117): the index of the leaf holding
+    // voxel (x, y, z), or -1 with *neg set from the slot's tile value (active
+    // or not) or, for a missing root entry, positive.
+    int32_t locate(int32_t x, int32_t y, int32_t z, bool* neg) const;
+
+    // The reachable part of the tree with leaf reference r renumbered to
+    // new_index[r] (a plain copy of the device leaf at that index).
+    std::unique_ptr<Topology> compact(const std::vector<int32_t>& new_index) const;
+};
+
+}  // namespace bs

@@ -1,0 +1,559 @@
+// Volume operations on the device: sampling a sphere (Volume::from_fn for
+// VolumeBuilder::sphere), the leaf flood fill, and CSG.
+//
+// Division of labour.  The leaf payloads (8^3 f32 values + a 512-bit mask per
+// leaf) live in device memory and every pass over them is a streaming kernel
+// (one wavefront per leaf, 8 values per lane).  The upper tree levels are a
+// few hundred slot nodes per volume; their flood fill and CSG slot decisions
+// run on the host (topology.cpp) and produce, for each output leaf, a
+// reference to its source leaves and the combination to apply.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <tuple>
+
+#include "common.hpp"
+
+namespace bs {
+
+namespace {
+
+constexpr int WAVES_PER_BLOCK = 4;  // one leaf per wavefront, 256-thread blocks
+constexpr float FAR = 3.40282347e+38f;
+
+__device__ inline uint64_t shfl_xor_u64(uint64_t v, int m) {
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Mask word x of a leaf (bits y*8+z) from the 8-bit z-rows held by lanes
+// (x, 0..7); valid in every lane of the group.
+__device__ inline uint64_t gather_word(uint32_t row, int y) {
+    uint64_t w = (uint64_t)row << (8 * y);
+    w |= shfl_xor_u64(w, 1);
+    w |= shfl_xor_u64(w, 2);
+    w |= shfl_xor_u64(w, 4);
+    return w;
+}
+
+// Highest set bit of m at or below position `top` (-1 when none).
+__device__ inline int last_at_or_below(uint64_t m, int top) {
+    const uint64_t keep = (top >= 63) ? ~0ull : ((2ull << top) - 1);
+    m &= keep;
+    return m ? 63 - __builtin_clzll(m) : -1;
+}
+
+// ------------------------------------------------------------------ flood fill
+// leaf_node/flood_fill.rs:12-50.  The scan line keeps three running signs:
+// i over x (entries (x,0,0)), j over y (entries (x,y,0)), k over z.  The value
+// of each is the sign of the last active entry of its line at or before the
+// position, else the running sign one level up; the seed is the sign of the
+// first active entry in offset order.  Lane (x, y) owns z-row (x, y, 0..7), so
+// i and j come from two ballots and k is a serial walk along the row.
+// Inactive entries become +-FLT_MAX with the running sign and stay inactive.
+// flags[leaf]: bit 0 first value negative, bit 1 last value negative (after
+// the fill, leaf_node/flood_fill.rs:62-70), bit 2 mask empty.
+__global__ __launch_bounds__(256) void k_leaf_flood_fill(uint64_t n_leaves, const uint64_t* __restrict__ masks,
+                                                          float* __restrict__ values, uint8_t* __restrict__ flags) {
+    const uint64_t leaf = (uint64_t)blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x >> 6);
+    if (leaf >= n_leaves) return;  // uniform per wavefront
+    const int lane = threadIdx.x & 63;
+    const int x = lane >> 3, y = lane & 7;
+    const uint32_t row = (uint32_t)(masks[leaf * 8 + x] >> (8 * y)) & 0xFFu;
+    float4* vp = reinterpret_cast<float4*>(values + leaf * 512 + (uint64_t)lane * 8);
+    float4 q0 = vp[0], q1 = vp[1];
+    float v[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+
+    const uint64_t any_rows = __ballot(row != 0);
+    const uint64_t full_rows = __ballot(row == 0xFFu);
+    if (any_rows != 0 && full_rows != ~0ull) {
+        // seed: first active entry in offset order = lowest lane with a row, lowest z in it
+        const int seed_lane = __builtin_ctzll(any_rows);
+        const int my_first = row ? (int)(__float_as_uint(v[__builtin_ctz(row)]) >> 31) : 0;
+        const int seed = __shfl(my_first, seed_lane);
+        const int z0_neg = (int)(__float_as_uint(v[0]) >> 31);
+        const uint64_t x00 = __ballot(y == 0 && (row & 1u));  // active (x, 0, 0)
+        const uint64_t xy0 = __ballot((row & 1u) != 0);       // active (x, y, 0)
+        const int src_i = last_at_or_below(x00, x * 8);
+        int src_j = last_at_or_below(xy0, lane);
+        if (src_j < x * 8) src_j = -1;  // the j line restarts at every x
+        const int from_i = __shfl(z0_neg, src_i < 0 ? lane : src_i);
+        const int from_j = __shfl(z0_neg, src_j < 0 ? lane : src_j);
+        const int i_sign = src_i < 0 ? seed : from_i;
+        int k = src_j < 0 ? i_sign : from_j;
+#pragma unroll
+        for (int z = 0; z < 8; ++z) {
+            if (row & (1u << z)) k = (int)(__float_as_uint(v[z]) >> 31);
+            else v[z] = k ? -FAR : FAR;
+        }
+        vp[0] = make_float4(v[0], v[1], v[2], v[3]);
+        vp[1] = make_float4(v[4], v[5], v[6], v[7]);
+    }
+    const int first_neg = __shfl((int)(__float_as_uint(v[0]) >> 31), 0);
+    const int last_neg = __shfl((int)(__float_as_uint(v[7]) >> 31), 63);
+    if (lane == 0) flags[leaf] = (uint8_t)(first_neg | (last_neg << 1) | ((any_rows == 0) << 2));
+}
+
+// ------------------------------------------------------------------ CSG assembly
+// One output leaf per wavefront.  ref = (a, b, op, negate): leaf a of operand
+// A, leaf b of operand B, and the combination of leaf_node/csg.rs:17-45 over
+// all 512 entries, active or not -- union min, subtract max(a, -b), intersect
+// max, each with utils.rs partial_min / partial_max (ties and NaN take the
+// second argument) -- and the OR of the masks.  A copied leaf keeps its mask.
+// `negate` is flip_signs (internal_node/csg.rs:103-111) of a leaf moved from B.
+__global__ __launch_bounds__(256) void k_csg_assemble(uint64_t n_out, const int4* __restrict__ refs,
+                                                      const int4* __restrict__ origins_in,
+                                                      const uint64_t* __restrict__ ma, const float* __restrict__ va,
+                                                      const uint64_t* __restrict__ mb, const float* __restrict__ vb,
+                                                      int4* __restrict__ origins, uint64_t* __restrict__ masks,
+                                                      float* __restrict__ values) {
+    const uint64_t leaf = (uint64_t)blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x >> 6);
+    if (leaf >= n_out) return;
+    const int lane = threadIdx.x & 63;
+    const int4 r = refs[leaf];
+    const int op = r.z & 0xFF;
+    const bool negate = (r.z >> 8) & 1;
+    const float4* pa = reinterpret_cast<const float4*>(va + (uint64_t)(uint32_t)r.x * 512 + lane * 8);
+    const float4* pb = reinterpret_cast<const float4*>(vb + (uint64_t)(uint32_t)r.y * 512 + lane * 8);
+    float o[8];
+    uint64_t m = 0;
+    if (op == LEAF_COPY_A || op == LEAF_COPY_B) {
+        const float4* p = op == LEAF_COPY_A ? pa : pb;
+        const float4 q0 = p[0], q1 = p[1];
+        o[0] = q0.x, o[1] = q0.y, o[2] = q0.z, o[3] = q0.w, o[4] = q1.x, o[5] = q1.y, o[6] = q1.z, o[7] = q1.w;
+        if (lane < 8) m = (op == LEAF_COPY_A ? ma + (uint64_t)(uint32_t)r.x * 8 : mb + (uint64_t)(uint32_t)r.y * 8)[lane];
+    } else {
+        const float4 a0 = pa[0], a1 = pa[1], b0 = pb[0], b1 = pb[1];
+        const float a[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+        const float b[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+        for (int z = 0; z < 8; ++z) {
+            if (op == LEAF_UNION) o[z] = a[z] < b[z] ? a[z] : b[z];
+            else if (op == LEAF_SUBTRACT) o[z] = a[z] > -b[z] ? a[z] : -b[z];
+            else o[z] = a[z] > b[z] ? a[z] : b[z];
+        }
+        if (lane < 8) m = ma[(uint64_t)(uint32_t)r.x * 8 + lane] | mb[(uint64_t)(uint32_t)r.y * 8 + lane];
+    }
+    if (negate) {
+#pragma unroll
+        for (int z = 0; z < 8; ++z) o[z] = -o[z];
+    }
+    float4* po = reinterpret_cast<float4*>(values + leaf * 512 + (uint64_t)lane * 8);
+    po[0] = make_float4(o[0], o[1], o[2], o[3]);
+    po[1] = make_float4(o[4], o[5], o[6], o[7]);
+    if (lane < 8) masks[leaf * 8 + lane] = m;
+    if (lane == 0) origins[leaf] = origins_in[leaf];
+}
+
+// ------------------------------------------------------------------ sphere
+// Volume::from_fn (volume/mod.rs:40-72) with f(p) = |p - o| - r, the function
+// of VolumeBuilder::sphere (builder.rs:21-30).  The reference visits every
+// index of the inclusive box [lo, hi]^3, evaluates f at p = idx * h and
+// inserts the value when |f| <= (1 + 1) * h; a leaf exists when it received a
+// value and its other entries keep the default 0.0.  Here one wavefront takes
+// one 8^3 cell of the box; cells whose distance bounds put every entry outside
+// the band (with a margin far above f32 rounding) are skipped without
+// evaluation.  Pass 1 flags cells that hold a value, pass 2 writes them at
+// their exclusive-scan position: cells are numbered in (x, y, z) order, so the
+// leaves come out in lexicographic origin order.
+struct SphereGrid {
+    float r, ox, oy, oz, h, band;
+    double margin;
+    int32_t lo[3], hi[3];  // inclusive voxel index box
+    int32_t c0[3];         // first cell (in units of 8 voxels)
+    uint32_t nc[3];        // cells per axis
+};
+
+__device__ inline bool sphere_cell_culled(const SphereGrid& g, int32_t cx, int32_t cy, int32_t cz) {
+    const int32_t c[3] = {cx, cy, cz};
+    const double o[3] = {g.ox, g.oy, g.oz};
+    double dmin2 = 0.0, dmax2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        const int32_t i0 = max(c[k] * 8, g.lo[k]), i1 = min(c[k] * 8 + 7, g.hi[k]);
+        const double p0 = (double)i0 * (double)g.h - o[k], p1 = (double)i1 * (double)g.h - o[k];
+        const double near = (p0 > 0.0) ? p0 : (p1 < 0.0 ? -p1 : 0.0);
+        const double far = fmax(fabs(p0), fabs(p1));
+        dmin2 += near * near;
+        dmax2 += far * far;
+    }
+    const double lim = (double)g.band + g.margin;
+    return (sqrt(dmin2) - (double)g.r > lim) || ((double)g.r - sqrt(dmax2) > lim);
+}
+
+// f32 evaluation in the reference's operation order: p = idx as f32 * h,
+// d = p - o, |d| = sqrt((dx*dx + dy*dy) + dz*dz), f = |d| - r.
+__device__ inline float sphere_value(const SphereGrid& g, int32_t xi, int32_t yi, int32_t zi) {
+    const float dx = __fsub_rn(__fmul_rn((float)xi, g.h), g.ox);
+    const float dy = __fsub_rn(__fmul_rn((float)yi, g.h), g.oy);
+    const float dz = __fsub_rn(__fmul_rn((float)zi, g.h), g.oz);
+    const float n2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+    return __fsub_rn(__fsqrt_rn(n2), g.r);
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_sphere(SphereGrid g, uint64_t n_cells, uint32_t* __restrict__ cell_flag,
+                                                const uint32_t* __restrict__ cell_off, int4* __restrict__ origins,
+                                                uint64_t* __restrict__ masks, float* __restrict__ values) {
+    const uint64_t cell = (uint64_t)blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x >> 6);
+    if (cell >= n_cells) return;
+    if (WRITE && !cell_flag[cell]) return;
+    const uint32_t iz = (uint32_t)(cell % g.nc[2]);
+    const uint32_t iy = (uint32_t)((cell / g.nc[2]) % g.nc[1]);
+    const uint32_t ix = (uint32_t)(cell / ((uint64_t)g.nc[2] * g.nc[1]));
+    const int32_t cx = g.c0[0] + (int32_t)ix, cy = g.c0[1] + (int32_t)iy, cz = g.c0[2] + (int32_t)iz;
+    const int lane = threadIdx.x & 63;
+    if (!WRITE && sphere_cell_culled(g, cx, cy, cz)) {
+        if (lane == 0) cell_flag[cell] = 0;
+        return;
+    }
+    const int x = lane >> 3, y = lane & 7;
+    const int32_t xi = cx * 8 + x, yi = cy * 8 + y;
+    const bool xy_in = xi >= g.lo[0] && xi <= g.hi[0] && yi >= g.lo[1] && yi <= g.hi[1];
+    uint32_t row = 0;
+    float v[8];
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+        const int32_t zi = cz * 8 + z;
+        v[z] = 0.0f;
+        if (xy_in && zi >= g.lo[2] && zi <= g.hi[2]) {
+            const float f = sphere_value(g, xi, yi, zi);
+            if (!(fabsf(f) > g.band)) {
+                row |= 1u << z;
+                v[z] = f;
+            }
+        }
+    }
+    if (!WRITE) {
+        const uint64_t any = __ballot(row != 0);
+        if (lane == 0) cell_flag[cell] = any != 0;
+        return;
+    }
+    const uint64_t out = cell_off[cell];
+    float4* po = reinterpret_cast<float4*>(values + out * 512 + (uint64_t)lane * 8);
+    po[0] = make_float4(v[0], v[1], v[2], v[3]);
+    po[1] = make_float4(v[4], v[5], v[6], v[7]);
+    const uint64_t w = gather_word(row, y);
+    if (y == 0) masks[out * 8 + x] = w;
+    if (lane == 0) origins[out] = make_int4(cx * 8, cy * 8, cz * 8, 0);
+}
+
+// Sign bit of entry (leaf, offset) for each query q = leaf * 512 + offset.
+__global__ void k_gather_signs(uint64_t n, const uint64_t* __restrict__ q, const float* __restrict__ values,
+                               uint8_t* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (uint8_t)(__float_as_uint(values[q[i]]) >> 31);
+}
+
+inline int32_t floor_div8(int64_t v) { return (int32_t)(v >= 0 ? v / 8 : -((-v + 7) / 8)); }
+
+void alloc_leaves(bs_volume* v, uint64_t n) {
+    v->n_leaves = n;
+    if (!n) return;
+    BS_HIP(hipMalloc(&v->origins, sizeof(int32_t) * 4 * n));
+    BS_HIP(hipMalloc(&v->masks, sizeof(uint64_t) * 8 * n));
+    BS_HIP(hipMalloc(&v->values, sizeof(float) * 512 * n));
+}
+
+void free_leaves(bs_volume* v) {
+    if (v->origins) BS_HIP(hipFree(v->origins));
+    if (v->masks) BS_HIP(hipFree(v->masks));
+    if (v->values) BS_HIP(hipFree(v->values));
+    v->origins = nullptr;
+    v->masks = nullptr;
+    v->values = nullptr;
+    v->n_leaves = 0;
+}
+
+// The host topology of a volume, built from its leaf origins and active tiles
+// the first time an operation needs it.
+Topology& ensure_topology(bs_ctx* ctx, bs_volume* v) {
+    if (v->topo) return *v->topo;
+    std::vector<int32_t> o4(4 * v->n_leaves);
+    if (v->n_leaves) {
+        BS_HIP(hipMemcpyAsync(o4.data(), v->origins, sizeof(int32_t) * o4.size(), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        BS_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    std::vector<std::array<int32_t, 3>> origins(v->n_leaves);
+    for (uint64_t i = 0; i < v->n_leaves; ++i) origins[i] = {o4[4 * i], o4[4 * i + 1], o4[4 * i + 2]};
+    v->topo = Topology::from_leaves(origins, v->tiles);
+    BS_REQUIRE(v->topo->leaves.size() == v->n_leaves, BS_ERR_INTERNAL, "volume has two leaves at one origin");
+    return *v->topo;
+}
+
+// Runs the leaf flood fill of one volume on the device (values in place) and
+// records each leaf's first/last sign and emptiness in its topology.  The
+// caller synchronises before reading `flags_host`.
+void launch_leaf_flood_fill(bs_ctx* ctx, bs_volume* v, int slot, std::vector<uint8_t>& flags_host) {
+    flags_host.assign(v->n_leaves, 0);
+    if (!v->n_leaves) return;
+    uint8_t* d_flags = ctx->scratch.as<uint8_t>(slot, v->n_leaves);
+    k_leaf_flood_fill<<<div_up(v->n_leaves, WAVES_PER_BLOCK), 256, 0, ctx->stream>>>(v->n_leaves, v->masks, v->values,
+                                                                                   d_flags);
+    BS_HIP(hipGetLastError());
+    BS_HIP(hipMemcpyAsync(flags_host.data(), d_flags, v->n_leaves, hipMemcpyDeviceToHost, ctx->stream));
+}
+
+void apply_flags(Topology& t, const std::vector<uint8_t>& flags) {
+    for (size_t i = 0; i < flags.size(); ++i) {
+        LeafRef& r = t.leaves[i];
+        r.first_neg = flags[i] & 1;
+        r.last_neg = (flags[i] >> 1) & 1;
+        r.empty = (flags[i] >> 2) & 1;
+    }
+}
+
+}  // namespace
+
+// FloodFill::flood_fill of a whole grid (root_node/flood_fill.rs:8-42).
+void volume_flood_fill(bs_ctx* ctx, bs_volume* v) {
+    Topology& t = ensure_topology(ctx, v);
+    std::vector<uint8_t> flags;
+    launch_leaf_flood_fill(ctx, v, S_FLAGS_A, flags);
+    BS_HIP(hipStreamSynchronize(ctx->stream));
+    apply_flags(t, flags);
+    t.flood_fill_upper();
+}
+
+// sign_at for a batch of voxel indices (root_node/flood_fill.rs:56-63): tile
+// and root answers come from the host topology, leaf answers from the sign
+// bits of the device values.
+void volume_signs_at(bs_ctx* ctx, bs_volume* v, const int32_t* idx, uint64_t n, uint8_t* out_neg) {
+    const Topology& t = ensure_topology(ctx, v);
+    std::vector<uint64_t> q;
+    std::vector<uint64_t> where;
+    for (uint64_t i = 0; i < n; ++i) {
+        const int32_t x = idx[3 * i], y = idx[3 * i + 1], z = idx[3 * i + 2];
+        bool neg = false;
+        const int32_t leaf = t.locate(x, y, z, &neg);
+        if (leaf < 0) {
+            out_neg[i] = neg;
+            continue;
+        }
+        BS_REQUIRE((uint64_t)leaf < v->n_leaves, BS_ERR_INTERNAL, "topology refers to a missing leaf");
+        q.push_back((uint64_t)leaf * 512 + (uint64_t)(((x & 7) << 6) | ((y & 7) << 3) | (z & 7)));
+        where.push_back(i);
+    }
+    if (q.empty()) return;
+    uint64_t* d_q = ctx->scratch.as<uint64_t>(S_MISC0, q.size());
+    uint8_t* d_o = ctx->scratch.as<uint8_t>(S_MISC1, q.size());
+    std::vector<uint8_t> got(q.size());
+    BS_HIP(hipMemcpyAsync(d_q, q.data(), sizeof(uint64_t) * q.size(), hipMemcpyHostToDevice, ctx->stream));
+    k_gather_signs<<<div_up(q.size(), 256), 256, 0, ctx->stream>>>(q.size(), d_q, v->values, d_o);
+    BS_HIP(hipGetLastError());
+    BS_HIP(hipMemcpyAsync(got.data(), d_o, q.size(), hipMemcpyDeviceToHost, ctx->stream));
+    BS_HIP(hipStreamSynchronize(ctx->stream));
+    for (size_t k = 0; k < q.size(); ++k) out_neg[where[k]] = got[k];
+}
+
+// Volume::union / subtract / intersect (volume/mod.rs:74-93): both operands
+// are flood filled, then the tree CSG runs with `a` as self.  `a` receives the
+// result; `b`'s leaves are consumed (the caller frees its handle).
+void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op) {
+    BS_REQUIRE(a != b, BS_ERR_INVALID_ARG, "CSG operands must be two distinct volumes");
+    BS_REQUIRE(a->device == ctx->device && b->device == ctx->device, BS_ERR_INVALID_ARG,
+               "CSG operands must live on the context's device");
+    Topology& ta = ensure_topology(ctx, a);
+    Topology& tb = ensure_topology(ctx, b);
+    std::vector<uint8_t> fa, fb;
+    launch_leaf_flood_fill(ctx, a, S_FLAGS_A, fa);
+    launch_leaf_flood_fill(ctx, b, S_FLAGS_B, fb);
+    BS_HIP(hipStreamSynchronize(ctx->stream));
+    apply_flags(ta, fa);
+    apply_flags(tb, fb);
+    ta.flood_fill_upper();
+    tb.flood_fill_upper();
+
+    std::unique_ptr<Topology> tb_owned = std::move(b->topo);
+    ta.csg(std::move(*tb_owned), op);
+    tb_owned.reset();
+
+    std::vector<int32_t> refs;
+    std::vector<TileRec> tiles;
+    ta.collect(refs, tiles);
+    // output leaves in lexicographic origin order, as every volume here keeps them
+    std::sort(refs.begin(), refs.end(), [&](int32_t p, int32_t q) {
+        const LeafRef &u = ta.leaves[p], &w = ta.leaves[q];
+        return std::tie(u.x, u.y, u.z) < std::tie(w.x, w.y, w.z);
+    });
+    const uint64_t n_out = refs.size();
+    std::vector<int32_t> new_index(ta.leaves.size(), -1);
+    std::vector<int4> host(2 * n_out);
+    for (uint64_t i = 0; i < n_out; ++i) {
+        const LeafRef& r = ta.leaves[refs[i]];
+        new_index[refs[i]] = (int32_t)i;
+        BS_REQUIRE(r.op == LEAF_COPY_B || (r.a >= 0 && (uint64_t)r.a < a->n_leaves), BS_ERR_INTERNAL,
+                   "CSG plan refers to a missing leaf of the first operand");
+        BS_REQUIRE(r.op == LEAF_COPY_A || (r.b >= 0 && (uint64_t)r.b < b->n_leaves), BS_ERR_INTERNAL,
+                   "CSG plan refers to a missing leaf of the second operand");
+        host[i] = make_int4(std::max(r.a, 0), std::max(r.b, 0), (int)r.op | ((int)r.neg << 8), 0);
+        host[n_out + i] = make_int4(r.x, r.y, r.z, 0);
+    }
+
+    bs_volume out;
+    out.device = a->device;
+    try {
+        alloc_leaves(&out, n_out);
+        if (n_out) {
+            int4* d_host = ctx->scratch.as<int4>(S_CSG_PLAN, 2 * n_out);
+            BS_HIP(hipMemcpyAsync(d_host, host.data(), sizeof(int4) * host.size(), hipMemcpyHostToDevice, ctx->stream));
+            k_csg_assemble<<<div_up(n_out, WAVES_PER_BLOCK), 256, 0, ctx->stream>>>(
+                n_out, d_host, d_host + n_out, a->masks, a->values, b->masks, b->values,
+                reinterpret_cast<int4*>(out.origins), out.masks, out.values);
+            BS_HIP(hipGetLastError());
+        }
+        BS_HIP(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+        free_leaves(&out);
+        throw;
+    }
+    free_leaves(a);
+    free_leaves(b);
+    a->n_leaves = out.n_leaves;
+    a->origins = out.origins;
+    a->masks = out.masks;
+    a->values = out.values;
+    out.origins = nullptr;
+    out.masks = nullptr;
+    out.values = nullptr;
+    a->tiles = std::move(tiles);
+    a->topo = ta.compact(new_index);
+}
+
+// An empty grid receiving insert(idx[i], values[i]) for i = 0..n-1 in order
+// (TreeNode::insert, leaf_node/tree_node.rs:46-50): a leaf exists once any of
+// its voxels was inserted, its other entries hold the default 0.0, and a later
+// insert of the same voxel overwrites an earlier one.  The leaves are packed
+// on the host (this is an import of caller data) and uploaded once.
+void volume_from_voxels(bs_ctx* ctx, const int32_t* idx, const float* values, uint64_t n, float h, bs_volume** out) {
+    BS_REQUIRE(std::isfinite(h) && h > 0.0f, BS_ERR_INVALID_ARG, "voxel size must be finite and positive");
+    std::vector<uint64_t> keys(n);
+    for (uint64_t i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k)
+            BS_REQUIRE(idx[3 * i + k] >= -(1 << 23) && idx[3 * i + k] < (1 << 23), BS_ERR_INVALID_ARG,
+                       "voxel index outside +-2^23");
+        keys[i] = leaf_key(idx[3 * i] >> 3, idx[3 * i + 1] >> 3, idx[3 * i + 2] >> 3);
+    }
+    std::vector<uint64_t> uniq(keys);
+    std::sort(uniq.begin(), uniq.end());
+    uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+    const uint64_t nl = uniq.size();
+    std::vector<int32_t> o4(4 * nl);
+    std::vector<uint64_t> m(8 * nl, 0);
+    std::vector<float> v(512 * nl, 0.0f);
+    for (uint64_t l = 0; l < nl; ++l) {
+        int lx, ly, lz;
+        leaf_key_decode(uniq[l], lx, ly, lz);
+        o4[4 * l] = lx * 8;
+        o4[4 * l + 1] = ly * 8;
+        o4[4 * l + 2] = lz * 8;
+    }
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint64_t l = (uint64_t)(std::lower_bound(uniq.begin(), uniq.end(), keys[i]) - uniq.begin());
+        const int off = ((idx[3 * i] & 7) << 6) | ((idx[3 * i + 1] & 7) << 3) | (idx[3 * i + 2] & 7);
+        m[8 * l + (off >> 6)] |= 1ull << (off & 63);
+        v[512 * l + off] = values[i];
+    }
+    auto* vol = new bs_volume();
+    vol->device = ctx->device;
+    vol->voxel_size = h;
+    try {
+        alloc_leaves(vol, nl);
+        if (nl) {
+            BS_HIP(hipMemcpyAsync(vol->origins, o4.data(), sizeof(int32_t) * o4.size(), hipMemcpyHostToDevice,
+                                  ctx->stream));
+            BS_HIP(hipMemcpyAsync(vol->masks, m.data(), sizeof(uint64_t) * m.size(), hipMemcpyHostToDevice,
+                                  ctx->stream));
+            BS_HIP(hipMemcpyAsync(vol->values, v.data(), sizeof(float) * v.size(), hipMemcpyHostToDevice,
+                                  ctx->stream));
+        }
+        BS_HIP(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+        free_leaves(vol);
+        delete vol;
+        throw;
+    }
+    *out = vol;
+}
+
+// VolumeBuilder::sphere (builder.rs:21-30) through Volume::from_fn.
+void volume_sphere(bs_ctx* ctx, float radius, float cx, float cy, float cz, float h, bs_volume** out) {
+    BS_REQUIRE(std::isfinite(radius) && std::isfinite(cx) && std::isfinite(cy) && std::isfinite(cz),
+               BS_ERR_INVALID_ARG, "sphere radius and centre must be finite");
+    BS_REQUIRE(std::isfinite(h) && h > 0.0f, BS_ERR_INVALID_ARG, "voxel size must be finite and positive");
+    // builder.rs:22-25: offset = r + band_width * h with band_width 1; min/max = o -+ offset
+    const float offset = radius + 1.0f * h;
+    const float c[3] = {cx, cy, cz};
+    // volume/mod.rs:49-51: band (1 + 1) * h; index box floor(min / h) .. ceil(max / h)
+    SphereGrid g{};
+    g.r = radius;
+    g.ox = cx;
+    g.oy = cy;
+    g.oz = cz;
+    g.h = h;
+    g.band = 2.0f * h;
+    bool empty = false;
+    double pmax = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        const float mn = c[k] + (-offset), mx = c[k] + offset;
+        const double lo = std::floor((double)(mn / h)), hi = std::ceil((double)(mx / h));
+        BS_REQUIRE(std::fabs(lo) < (double)(1 << 22) && std::fabs(hi) < (double)(1 << 22), BS_ERR_INVALID_ARG,
+                   "sphere index box exceeds +-2^22 voxels");
+        g.lo[k] = (int32_t)lo;
+        g.hi[k] = (int32_t)hi;
+        empty |= g.lo[k] > g.hi[k];
+        pmax = std::max(pmax, std::max(std::fabs(lo), std::fabs(hi)) * (double)h);
+    }
+    // f32 rounding of idx*h, p - o, the norm and the subtraction stays below
+    // 8 ulps of the largest magnitude involved; the margin is far above it.
+    g.margin = 2.0 * (double)h + 1e-5 * (pmax + std::fabs((double)cx) + std::fabs((double)cy) +
+                                         std::fabs((double)cz) + std::fabs((double)radius));
+    uint64_t n_cells = 1;
+    for (int k = 0; k < 3; ++k) {
+        g.c0[k] = empty ? 0 : floor_div8(g.lo[k]);
+        const int32_t c1 = empty ? -1 : floor_div8(g.hi[k]);
+        g.nc[k] = (uint32_t)(c1 - g.c0[k] + 1);
+        n_cells *= g.nc[k];
+    }
+    BS_REQUIRE(n_cells < (1ull << 31), BS_ERR_INVALID_ARG, "sphere spans more than 2^31 leaf cells");
+
+    auto* v = new bs_volume();
+    v->device = ctx->device;
+    v->voxel_size = h;
+    try {
+        if (n_cells) {
+            uint32_t* flag = ctx->scratch.as<uint32_t>(S_MISC0, n_cells);
+            uint32_t* off = ctx->scratch.as<uint32_t>(S_MISC1, n_cells);
+            const unsigned grid = div_up(n_cells, WAVES_PER_BLOCK);
+            k_sphere<false><<<grid, 256, 0, ctx->stream>>>(g, n_cells, flag, nullptr, nullptr, nullptr, nullptr);
+            BS_HIP(hipGetLastError());
+            size_t bytes = 0;
+            BS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, flag, off, (int)n_cells, ctx->stream));
+            void* tmp = ctx->scratch.get(S_CUB, bytes);
+            BS_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, flag, off, (int)n_cells, ctx->stream));
+            uint32_t tail[2] = {0, 0};
+            BS_HIP(hipMemcpyAsync(&tail[0], off + n_cells - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+            BS_HIP(hipMemcpyAsync(&tail[1], flag + n_cells - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+            BS_HIP(hipStreamSynchronize(ctx->stream));
+            const uint64_t n_leaves = (uint64_t)tail[0] + tail[1];
+            alloc_leaves(v, n_leaves);
+            if (n_leaves) {
+                k_sphere<true><<<grid, 256, 0, ctx->stream>>>(g, n_cells, flag, off,
+                                                               reinterpret_cast<int4*>(v->origins), v->masks,
+                                                               v->values);
+                BS_HIP(hipGetLastError());
+            }
+            BS_HIP(hipStreamSynchronize(ctx->stream));
+        }
+    } catch (...) {
+        free_leaves(v);
+        delete v;
+        throw;
+    }
+    *out = v;
+}
+
+}  // namespace bs

@@ -134,6 +134,38 @@ class Volume:
         check(load().bs_volume_clone(self._h, C.byref(h)))
         return Volume(h, self._ctx)
 
+    # ---- CSG (volume/mod.rs:74-93): both operands are consumed ----------
+    def _csg(self, other: "Volume", fn: str) -> "Volume":
+        if other is self:
+            raise ValueError("CSG operands must be two distinct volumes")
+        if not self._h or not other._h:
+            raise ValueError("volume was consumed by a previous operation")
+        check(getattr(load(), fn)(self._ctx.handle, self._h, other._h))
+        other._h = None                      # freed by the library
+        return Volume(self._take(), self._ctx)
+
+    def union(self, other: "Volume") -> "Volume":
+        return self._csg(other, "bs_volume_union")
+
+    def subtract(self, other: "Volume") -> "Volume":
+        return self._csg(other, "bs_volume_subtract")
+
+    def intersect(self, other: "Volume") -> "Volume":
+        return self._csg(other, "bs_volume_intersect")
+
+    def flood_fill(self) -> "Volume":
+        """FloodFill of the grid in place (root_node/flood_fill.rs:8-42)."""
+        check(load().bs_volume_flood_fill(self._ctx.handle, self._h))
+        return self
+
+    def signs_at(self, idx) -> np.ndarray:
+        """FloodFill::sign_at for (n, 3) voxel indices: True where negative."""
+        a = np.ascontiguousarray(np.asarray(idx, dtype=np.int32).reshape(-1, 3))
+        out = np.zeros(a.shape[0], np.uint8)
+        check(load().bs_volume_signs_at(self._ctx.handle, self._h, a.ctypes.data_as(C.POINTER(C.c_int32)),
+                                        a.shape[0], out.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return out.astype(bool)
+
     def export(self) -> VolumeExport:
         L = load()
         inf = self.info()
@@ -151,6 +183,49 @@ class Volume:
                                            ts.ctypes.data_as(C.POINTER(C.c_int32)),
                                            tv.ctypes.data_as(C.POINTER(C.c_float))))
         return VolumeExport(inf["voxel_size"], o, m, v, to, ts, tv)
+
+
+class VolumeBuilder:
+    """builder.rs:5-30: primitives sampled on the grid of one voxel size."""
+
+    def __init__(self, ctx: Optional[Context] = None, voxel_size: float = 1.0):
+        self._ctx = ctx
+        self.voxel_size = float(voxel_size)
+
+    @property
+    def ctx(self) -> Context:
+        if self._ctx is None:
+            self._ctx = default_context()
+        return self._ctx
+
+    def with_voxel_size(self, size: float) -> "VolumeBuilder":
+        self.set_voxel_size(size)
+        return self
+
+    def set_voxel_size(self, size: float) -> None:
+        self.voxel_size = float(size)
+
+    def sphere(self, radius: float, origin) -> Volume:
+        """|p - origin| - radius, kept within 2 voxels (builder.rs:21-30)."""
+        o = [float(c) for c in origin]
+        h = C.c_void_p()
+        check(load().bs_volume_sphere(self.ctx.handle, float(radius), o[0], o[1], o[2], self.voxel_size,
+                                      C.byref(h)))
+        return Volume(h, self.ctx)
+
+
+def volume_from_voxels(idx, values, voxel_size: float, ctx: Optional[Context] = None) -> Volume:
+    """A grid built by inserting values[i] at voxel idx[i] in order."""
+    ctx = ctx or default_context()
+    a = np.ascontiguousarray(np.asarray(idx, dtype=np.int32).reshape(-1, 3))
+    v = np.ascontiguousarray(np.asarray(values, dtype=np.float32).reshape(-1))
+    if a.shape[0] != v.shape[0]:
+        raise ValueError("one value per voxel index")
+    h = C.c_void_p()
+    check(load().bs_volume_from_voxels(ctx.handle, a.ctypes.data_as(C.POINTER(C.c_int32)),
+                                       v.ctypes.data_as(C.POINTER(C.c_float)), a.shape[0], float(voxel_size),
+                                       C.byref(h)))
+    return Volume(h, ctx)
 
 
 class MeshToVolume:

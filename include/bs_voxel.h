@@ -107,6 +107,47 @@ bs_status bs_volume_info_get(const bs_volume* v, bs_volume_info* out);
 bs_status bs_volume_export(const bs_volume* v, int32_t* origins, uint64_t* masks, float* values);
 bs_status bs_volume_export_tiles(const bs_volume* v, int32_t* origins, int32_t* sizes, float* values);
 
+/* VolumeBuilder::sphere (builder.rs:21-30) through Volume::from_fn
+ * (volume/mod.rs:40-72): f(p) = |p - (cx,cy,cz)| - radius sampled at p = idx*h
+ * over floor((c - r - h)/h) .. ceil((c + r + h)/h), kept where |f| <= 2h.
+ * The result may have no leaves (e.g. a negative radius). */
+bs_status bs_volume_sphere(bs_ctx* ctx, float radius, float cx, float cy, float cz, float voxel_size,
+                           bs_volume** out);
+
+/* A grid built by inserting voxel values one by one (TreeNode::insert on an
+ * empty VolumeGrid: root_node/tree_node.rs, internal_node/tree_node.rs:54-79,
+ * leaf_node/tree_node.rs:46-50): idx n*3 int32 voxel indices within +-2^23,
+ * values n floats, both host.  Each inserted voxel becomes active; other
+ * entries of a touched leaf hold 0.0; a repeated index keeps the last value. */
+bs_status bs_volume_from_voxels(bs_ctx* ctx, const int32_t* idx, const float* values, size_t n, float voxel_size,
+                                bs_volume** out);
+
+/* Volume::union / subtract / intersect (volume/mod.rs:74-93).  Both operands
+ * are flood filled (FloodFill, root_node/flood_fill.rs:8-42) and the tree CSG
+ * runs with `self` as the receiver (root_node/csg.rs:9-58,
+ * internal_node/csg.rs:68-162, leaf_node/csg.rs:17-45).  The Rust methods take
+ * both volumes by value: on BS_OK `self` holds the result and `other` has been
+ * freed (its handle is dangling).  On an error both handles stay owned by the
+ * caller and their contents are unspecified.  Like the reference, voxel sizes
+ * are not checked; the result keeps self's. */
+bs_status bs_volume_union(bs_ctx* ctx, bs_volume* self, bs_volume* other);
+bs_status bs_volume_subtract(bs_ctx* ctx, bs_volume* self, bs_volume* other);
+bs_status bs_volume_intersect(bs_ctx* ctx, bs_volume* self, bs_volume* other);
+
+/* FloodFill::flood_fill of the whole grid (root_node/flood_fill.rs:8-42,
+ * internal_node/flood_fill.rs:17-75, leaf_node/flood_fill.rs:12-50): inactive
+ * leaf entries become +-FLT_MAX with the scan-line sign and stay inactive. */
+bs_status bs_volume_flood_fill(bs_ctx* ctx, bs_volume* v);
+
+/* FloodFill::sign_at (root_node/flood_fill.rs:56-63, internal_node/
+ * flood_fill.rs:110-117, leaf_node/flood_fill.rs:72-75) for n voxel indices
+ * (idx: n*3 int32, host): out_negative[i] = 1 when the sign is negative.  A
+ * missing root entry is positive; a tile slot answers with its value's sign,
+ * active or not.  The tile of a slot whose branch a CSG removed is undefined
+ * in the reference (it reads pointer bits); here it keeps the slot's earlier
+ * tile value. */
+bs_status bs_volume_signs_at(bs_ctx* ctx, bs_volume* v, const int32_t* idx, size_t n, uint8_t* out_negative);
+
 /* Device views of the leaf arrays (valid until the volume is freed/consumed). */
 bs_status bs_volume_device_leaves(const bs_volume* v, const int32_t** origins, const uint64_t** masks,
                                   const float** values, uint64_t* n_leaves);

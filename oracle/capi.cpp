@@ -361,4 +361,159 @@ void or_sample_unsigned(const float* tris, size_t n, float voxel_size, int64_t b
     }
 }
 
+// ---------------------------------------------------------------- volume ops
+// Volume::from_fn (volume/mod.rs:40-72) for VolumeBuilder::sphere (builder.rs:21-30)
+or_volume* or_volume_sphere(float radius, float cx, float cy, float cz, float h) {
+    auto g = std::make_unique<VolumeGrid>();
+    const float band_offset = 1.0f * h;  // band_width = 1, `band_width as f32 * voxel_size`
+    const float offset = radius + band_offset;
+    const Vec3f o{cx, cy, cz};
+    const Vec3f mn{cx + (-offset), cy + (-offset), cz + (-offset)};
+    const Vec3f mx{cx + offset, cy + offset, cz + offset};
+    const float nbw = (float)(1 + 1) * h;
+    int64_t lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = f2i(std::floor(mn[k] / h));
+        hi[k] = f2i(std::ceil(mx[k] / h));
+    }
+    for (int64_t x = lo[0]; x <= hi[0]; ++x)
+        for (int64_t y = lo[1]; y <= hi[1]; ++y)
+            for (int64_t z = lo[2]; z <= hi[2]; ++z) {
+                Vec3f p{(float)x * h, (float)y * h, (float)z * h};
+                float v = norm(sub(p, o)) - radius;
+                if (std::fabs(v) > nbw) continue;
+                g->insert({x, y, z}, v);
+            }
+    auto* r = new or_volume{std::move(g), h, {}, {}};
+    r->index();
+    return r;
+}
+
+// An empty VolumeGrid receiving `insert(idx, value)` for each voxel in order
+// (TreeNode::insert: root_node/tree_node.rs, internal_node/tree_node.rs:54-79,
+// leaf_node/tree_node.rs:46-50); later inserts overwrite earlier ones.
+or_volume* or_volume_from_voxels(const int32_t* idx, const float* values, size_t n, float h) {
+    auto g = std::make_unique<VolumeGrid>();
+    for (size_t i = 0; i < n; ++i) g->insert({idx[3 * i], idx[3 * i + 1], idx[3 * i + 2]}, values[i]);
+    auto* r = new or_volume{std::move(g), h, {}, {}};
+    r->index();
+    return r;
+}
+
+// Volume::union / intersect / subtract (volume/mod.rs:74-93): both operands
+// flood filled, then the tree CSG.  `b` is consumed (freed) as in Rust.
+// op: 0 union, 1 subtract, 2 intersect.
+void or_volume_csg(or_volume* a, or_volume* b, int op) {
+    a->grid->flood_fill();
+    b->grid->flood_fill();
+    if (op == 0) a->grid->csg_union(std::move(b->grid));
+    else if (op == 1) a->grid->csg_subtract(std::move(b->grid));
+    else a->grid->csg_intersect(std::move(b->grid));
+    delete b;
+    a->index();
+}
+
+// FloodFill::flood_fill of the whole grid (root_node/flood_fill.rs:8-42)
+void or_volume_flood_fill(or_volume* a) {
+    a->grid->flood_fill();
+    a->index();
+}
+
+or_volume* or_volume_clone(const or_volume* a) {
+    auto* r = new or_volume{a->grid->clone(), a->voxel_size, {}, {}};
+    r->index();
+    return r;
+}
+
+// Sign of the flood-filled value at voxel indices (root_node/flood_fill.rs:56-63):
+// 1 negative, 0 positive.
+void or_volume_signs_at(const or_volume* a, const int32_t* idx, size_t m, uint8_t* out) {
+    for (size_t i = 0; i < m; ++i)
+        out[i] = a->grid->sign_at({idx[3 * i], idx[3 * i + 1], idx[3 * i + 2]}) == Sign::Negative;
+}
+
+// ---------------------------------------------------------------- reference unit tests
+// Restatements of the trees used by the reference's own unit tests, so that
+// tests can compare the oracle with the answers those tests assert.
+
+// leaf_node/csg.rs:53-132: static_vdb!(f32, 1) leaves with full masks.
+void or_reftest_leaf_csg(int op, const float* a, const float* b, float* out, int* mask_full) {
+    using L = Leaf<1>;
+    auto n1 = std::make_unique<L>(Vec3i{0, 0, 0});
+    auto n2 = std::make_unique<L>(Vec3i{0, 0, 0});
+    for (int i = 0; i < 8; ++i) {
+        n1->values[i] = a[i];
+        n2->values[i] = b[i];
+    }
+    n1->value_mask.on_all();
+    n2->value_mask.on_all();
+    if (op == 0) n1->csg_union(std::move(n2));
+    else if (op == 1) n1->csg_subtract(std::move(n2));
+    else n1->csg_intersect(std::move(n2));
+    for (int i = 0; i < 8; ++i) out[i] = n1->values[i];
+    *mask_full = n1->value_mask.is_full();
+}
+
+// leaf_node/flood_fill.rs:82-113 on static_vdb!(f32, 2): returns the 64 signs.
+void or_reftest_leaf_flood_fill(int which, uint8_t* neg_out) {
+    using L = Leaf<2>;
+    L n(Vec3i{0, 0, 0});
+    if (which == 0) n.insert({2, 2, 2}, 1.0f);
+    else if (which == 1) n.insert({2, 2, 2}, -1.0f);
+    else
+        for (int y = 0; y < 4; ++y)
+            for (int z = 0; z < 4; ++z) {
+                n.insert({1, y, z}, -1.0f);
+                n.insert({2, y, z}, 1.0f);
+            }
+    n.flood_fill();
+    for (int i = 0; i < 64; ++i) neg_out[i] = std::signbit(n.values[i]);
+}
+
+// internal_node/flood_fill.rs:128-193 on static_vdb!(f32, 2, 1): signs over the
+// node's 8^3 voxels, x-major.
+void or_reftest_internal_flood_fill(int which, uint8_t* neg_out) {
+    using I = Internal<Leaf<1>, 2>;
+    auto n = std::make_unique<I>(Vec3i{0, 0, 0});
+    const int R = 8;
+    if (which == 0) n->insert({3, 2, 1}, 1.0f);
+    else if (which == 1) n->insert({3, 3, 3}, -1.0f);
+    else if (which == 2) {
+        for (int y = 0; y < R; ++y)
+            for (int z = 0; z < R; ++z) {
+                n->insert({4, y, z}, -1.0f);
+                n->insert({5, y, z}, 1.0f);
+            }
+    } else {
+        for (int x = 2; x < 2 + 2; ++x)
+            for (int y = 0; y < R; ++y)
+                for (int z = 0; z < R; ++z) n->insert({x, y, z}, -1.0f);
+        for (int x = 4; x < 4 + 2; ++x)
+            for (int y = 0; y < R; ++y)
+                for (int z = 0; z < R; ++z) n->insert({x, y, z}, 1.0f);
+    }
+    n->flood_fill();
+    int k = 0;
+    for (int x = 0; x < R; ++x)
+        for (int y = 0; y < R; ++y)
+            for (int z = 0; z < R; ++z) neg_out[k++] = n->sign_at({x, y, z}) == Sign::Negative;
+}
+
+// root_node/flood_fill.rs:74-95 on dynamic_vdb!(f32, 1): signs over the
+// region (0,4,4)..(2,6,12), x-major.
+void or_reftest_root_flood_fill(uint8_t* neg_out) {
+    Root<Leaf<1>> t;
+    for (int x = 0; x < 2; ++x)
+        for (int y = 4; y < 6; ++y)
+            for (int z = 4; z < 6; ++z) t.insert({x, y, z}, -1.0f);
+    for (int x = 0; x < 2; ++x)
+        for (int y = 4; y < 6; ++y)
+            for (int z = 10; z < 12; ++z) t.insert({x, y, z}, -1.0f);
+    t.flood_fill();
+    int k = 0;
+    for (int x = 0; x < 2; ++x)
+        for (int y = 4; y < 6; ++y)
+            for (int z = 4; z < 12; ++z) neg_out[k++] = t.sign_at({x, y, z}) == Sign::Negative;
+}
+
 }  // extern "C"

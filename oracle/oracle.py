@@ -56,6 +56,20 @@ def lib():
         L.or_active_set_export.argtypes = [i32p, u64p]
         L.or_sample_unsigned.argtypes = [f32p, C.c_size_t, C.c_float, C.c_int64, i32p, C.c_size_t, f32p, f64p,
                                          C.c_int]
+        u8p = C.POINTER(C.c_uint8)
+        L.or_volume_sphere.restype = vp
+        L.or_volume_sphere.argtypes = [C.c_float] * 5
+        L.or_volume_from_voxels.restype = vp
+        L.or_volume_from_voxels.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_float), C.c_size_t, C.c_float]
+        L.or_volume_csg.argtypes = [vp, vp, C.c_int]
+        L.or_volume_flood_fill.argtypes = [vp]
+        L.or_volume_clone.restype = vp
+        L.or_volume_clone.argtypes = [vp]
+        L.or_volume_signs_at.argtypes = [vp, i32p, C.c_size_t, u8p]
+        L.or_reftest_leaf_csg.argtypes = [C.c_int, f32p, f32p, f32p, C.POINTER(C.c_int)]
+        L.or_reftest_leaf_flood_fill.argtypes = [C.c_int, u8p]
+        L.or_reftest_internal_flood_fill.argtypes = [C.c_int, u8p]
+        L.or_reftest_root_flood_fill.argtypes = [u8p]
         _lib = L
     return _lib
 
@@ -90,6 +104,37 @@ class Volume:
     def __init__(self, handle, voxel_size):
         self.h = handle
         self.voxel_size = voxel_size
+
+    def _take(self):
+        h, self.h = self.h, None
+        return h
+
+    def clone(self) -> "Volume":
+        return Volume(lib().or_volume_clone(self.h), self.voxel_size)
+
+    def _csg(self, other: "Volume", op: int) -> "Volume":
+        lib().or_volume_csg(self.h, other._take(), op)
+        return self
+
+    def union(self, other):      # volume/mod.rs:74-79
+        return self._csg(other, 0)
+
+    def subtract(self, other):   # volume/mod.rs:88-93
+        return self._csg(other, 1)
+
+    def intersect(self, other):  # volume/mod.rs:81-86
+        return self._csg(other, 2)
+
+    def flood_fill(self) -> "Volume":
+        lib().or_volume_flood_fill(self.h)
+        return self
+
+    def signs_at(self, idx) -> np.ndarray:
+        a = np.ascontiguousarray(idx, dtype=np.int32)
+        out = np.zeros(a.shape[0], np.uint8)
+        lib().or_volume_signs_at(self.h, a.ctypes.data_as(C.POINTER(C.c_int32)), a.shape[0],
+                                 out.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return out.astype(bool)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -199,3 +244,46 @@ def sample_unsigned(soup, voxel_size: float, band: int, voxels, threads: int = 0
                              o32.ctypes.data_as(C.POINTER(C.c_float)), o64.ctypes.data_as(C.POINTER(C.c_double)),
                              threads)
     return o32, o64
+
+
+def sphere(radius: float, center, voxel_size: float) -> Volume:
+    """VolumeBuilder::default().with_voxel_size(h).sphere(radius, center)."""
+    return Volume(lib().or_volume_sphere(radius, center[0], center[1], center[2], voxel_size), voxel_size)
+
+
+def from_voxels(idx, values, voxel_size: float) -> Volume:
+    """An empty grid receiving insert(idx[i], values[i]) in order."""
+    a = np.ascontiguousarray(np.asarray(idx, np.int32).reshape(-1, 3))
+    v = np.ascontiguousarray(np.asarray(values, np.float32).reshape(-1))
+    assert a.shape[0] == v.shape[0]
+    return Volume(lib().or_volume_from_voxels(a.ctypes.data_as(C.POINTER(C.c_int32)),
+                                              v.ctypes.data_as(C.POINTER(C.c_float)), a.shape[0], voxel_size),
+                  voxel_size)
+
+
+def reftest_leaf_csg(op: int, a, b):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    out = np.zeros(8, np.float32)
+    full = C.c_int(0)
+    lib().or_reftest_leaf_csg(op, a.ctypes.data_as(C.POINTER(C.c_float)), b.ctypes.data_as(C.POINTER(C.c_float)),
+                              out.ctypes.data_as(C.POINTER(C.c_float)), C.byref(full))
+    return out, bool(full.value)
+
+
+def _signs(fn, n, *args):
+    out = np.zeros(n, np.uint8)
+    fn(*args, out.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return out.astype(bool)
+
+
+def reftest_leaf_flood_fill(which: int):
+    return _signs(lib().or_reftest_leaf_flood_fill, 64, which)
+
+
+def reftest_internal_flood_fill(which: int):
+    return _signs(lib().or_reftest_internal_flood_fill, 512, which)
+
+
+def reftest_root_flood_fill():
+    return _signs(lib().or_reftest_root_flood_fill, 2 * 2 * 8)

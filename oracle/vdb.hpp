@@ -149,6 +149,7 @@ struct Leaf {
     }
     Sign first_value_sign() const { return sign_of(values[0]); }
     Sign last_value_sign() const { return sign_of(values[SIZE - 1]); }
+    Sign sign_at(const Vec3i& i) const { return sign_of(values[offset(i)]); }
 
     // ---- CSG (leaf_node/csg.rs:17-45), over every entry including inactive ones
     void csg_union(std::unique_ptr<Leaf> o) {
@@ -407,6 +408,11 @@ struct Internal {
         if (child_mask.at(0)) return branch[0]->first_value_sign();
         return sign_of(tile[0]);
     }
+    Sign sign_at(const Vec3i& i) const {  // internal_node/flood_fill.rs:110-117
+        int o = offset(i);
+        if (child_mask.at(o)) return branch[o]->sign_at(i);
+        return sign_of(tile[o]);
+    }
     // internal_node/flood_fill.rs:103-107 returns the branch's *first* sign.
     Sign last_value_sign() const {
         if (child_mask.at(SIZE - 1)) return branch[SIZE - 1]->first_value_sign();
@@ -562,6 +568,10 @@ struct Root {
             if (it->second->is_empty()) it = root.erase(it);
             else ++it;
         }
+    }
+    Sign sign_at(const Vec3i& i) const {  // root_node/flood_fill.rs:56-63
+        auto it = root.find(root_key(i));
+        return it == root.end() ? Sign::Positive : it->second->sign_at(i);
     }
     // root_node/flood_fill.rs:8-42
     void flood_fill() {
