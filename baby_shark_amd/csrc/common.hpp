@@ -149,6 +149,13 @@ __device__ inline uint32_t hash_find(const uint64_t* keys, uint32_t cap_mask, ui
     return NOT_FOUND;
 }
 
+// Correctly rounded f32 square root and division.  HIP's __fsqrt_rn is the
+// native (approximate) v_sqrt_f32 unless OCML's rounded operations are
+// enabled; __builtin_sqrtf and `/` are correctly rounded under
+// -fhip-fp32-correctly-rounded-divide-sqrt, which the build passes.
+__device__ inline float sqrt_rn(float x) { return __builtin_sqrtf(x); }
+__device__ inline float div_rn(float a, float b) { return a / b; }
+
 inline unsigned div_up(uint64_t a, unsigned b) { return (unsigned)((a + b - 1) / b); }
 
 }  // namespace bs

@@ -48,7 +48,7 @@ __global__ void k_subdiv_count(const float* __restrict__ tris, uint64_t n, float
     if (i >= n) return;
     F3 a, b, c;
     load_tri(tris + 9 * i, a, b, c);
-    float num_subs = floorf(__fdiv_rn(tri_max_side(a, b, c), h));
+    float num_subs = floorf(div_rn(tri_max_side(a, b, c), h));
     if (num_subs < 2.0f) {
         rows[i] = 1;
         subs[i] = 1;
@@ -92,12 +92,12 @@ __global__ void k_subdiv_emit(const float* __restrict__ tris, uint64_t n, float 
         q[3] = b.x; q[4] = b.y; q[5] = b.z;
         q[6] = c.x; q[7] = c.y; q[8] = c.z;
     };
-    float num_subs = floorf(__fdiv_rn(tri_max_side(p1, p2, p3), h));
+    float num_subs = floorf(div_rn(tri_max_side(p1, p2, p3), h));
     if (num_subs < 2.0f) {
         put(0, p1, p2, p3);
         return;
     }
-    float inv = __fdiv_rn(1.0f, num_subs);
+    float inv = div_rn(1.0f, num_subs);
     F3 s1 = f3_muls(f3_sub(p2, p1), inv);
     F3 s2 = f3_muls(f3_sub(p3, p2), inv);
     F3 a = p1;

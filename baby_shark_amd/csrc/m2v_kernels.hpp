@@ -35,7 +35,7 @@ __device__ inline float rust_max(float a, float b) {
 __device__ inline float tri_max_side(F3 a, F3 b, F3 c) {
     F3 ab = f3_sub(b, a), ac = f3_sub(c, a), bc = f3_sub(c, b);
     float m = rust_max(rust_max(f3_dot_exact(ab, ab), f3_dot_exact(ac, ac)), f3_dot_exact(bc, bc));
-    return __fsqrt_rn(m);
+    return sqrt_rn(m);
 }
 
 // Triangle3::closest_point (triangle3.rs:317-382) followed by
@@ -54,7 +54,7 @@ __device__ inline float ericson_distance(F3 A, F3 B, F3 C, F3 p) {
         if (d3 >= 0.0f && d4 <= d3) {
             q = B;
         } else if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
-            float v = __fdiv_rn(d1, d1 - d3);
+            float v = div_rn(d1, d1 - d3);
             q = f3_add(A, f3_muls(ab, v));
         } else {
             F3 cp = f3_sub(p, C);
@@ -63,15 +63,15 @@ __device__ inline float ericson_distance(F3 A, F3 B, F3 C, F3 p) {
             if (d6 >= 0.0f && d5 <= d6) {
                 q = C;
             } else if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
-                float w = __fdiv_rn(d2, d2 - d6);
+                float w = div_rn(d2, d2 - d6);
                 q = f3_add(A, f3_muls(ac, w));
             } else {
                 float va = d3 * d6 - d5 * d4;
                 if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
-                    float w = __fdiv_rn(d4 - d3, (d4 - d3) + (d5 - d6));
+                    float w = div_rn(d4 - d3, (d4 - d3) + (d5 - d6));
                     q = f3_add(B, f3_muls(f3_sub(C, B), w));
                 } else {
-                    float denom = __fdiv_rn(1.0f, (va + vb) + vc);
+                    float denom = div_rn(1.0f, (va + vb) + vc);
                     float v = vb * denom, w = vc * denom;
                     q = f3_add(f3_add(A, f3_muls(ab, v)), f3_muls(ac, w));
                 }
@@ -79,7 +79,7 @@ __device__ inline float ericson_distance(F3 A, F3 B, F3 C, F3 p) {
         }
     }
     F3 d = f3_sub(q, p);
-    return __fsqrt_rn(f3_dot_exact(d, d));
+    return sqrt_rn(f3_dot_exact(d, d));
 }
 
 // Per sub-triangle record for the distance kernel (112 B, read wave-uniformly

@@ -190,7 +190,7 @@ __device__ inline float sphere_value(const SphereGrid& g, int32_t xi, int32_t yi
     const float dy = __fsub_rn(__fmul_rn((float)yi, g.h), g.oy);
     const float dz = __fsub_rn(__fmul_rn((float)zi, g.h), g.oz);
     const float n2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
-    return __fsub_rn(__fsqrt_rn(n2), g.r);
+    return __fsub_rn(sqrt_rn(n2), g.r);
 }
 
 template <bool WRITE>
