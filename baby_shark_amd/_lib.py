@@ -46,6 +46,11 @@ class M2VStats(C.Structure):
                                          "ms_winding")]
 
 
+class OffsetStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("n_kept", "n_pool", "n_out", "n_planes", "n_launches", "n_capacity",
+                                          "used_hash")]
+
+
 class BSError(RuntimeError):
     def __init__(self, status: int, msg: str):
         super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
@@ -98,6 +103,8 @@ def load():
             "bs_volume_subtract": (C.c_int, [vp, vp, vp]),
             "bs_volume_intersect": (C.c_int, [vp, vp, vp]),
             "bs_volume_flood_fill": (C.c_int, [vp, vp]),
+            "bs_volume_offset": (C.c_int, [vp, vp, C.c_float]),
+            "bs_ctx_last_offset_stats": (C.c_int, [vp, C.POINTER(OffsetStats)]),
             "bs_volume_signs_at": (C.c_int, [vp, vp, i32p, C.c_size_t, C.POINTER(C.c_uint8)]),
             "bs_volume_device_leaves": (C.c_int, [vp, C.POINTER(i32p), C.POINTER(u64p), C.POINTER(f32p),
                                                   C.POINTER(C.c_uint64)]),

@@ -14,6 +14,7 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
 void volume_sphere(bs_ctx* ctx, float radius, float cx, float cy, float cz, float h, bs_volume** out);
 void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op);
 void volume_flood_fill(bs_ctx* ctx, bs_volume* v);
+void volume_offset(bs_ctx* ctx, bs_volume* vol, float distance, bs_offset_stats* stats);
 void volume_from_voxels(bs_ctx* ctx, const int32_t* idx, const float* values, uint64_t n, float h, bs_volume** out);
 void volume_signs_at(bs_ctx* ctx, bs_volume* v, const int32_t* idx, uint64_t n, uint8_t* out_neg);
 
@@ -300,6 +301,23 @@ bs_status bs_volume_flood_fill(bs_ctx* ctx, bs_volume* v) {
         BS_REQUIRE(v->device == ctx->device, BS_ERR_INVALID_ARG, "volume lives on another device");
         volume_flood_fill(ctx, v);
     });
+}
+
+bs_status bs_volume_offset(bs_ctx* ctx, bs_volume* v, float distance) {
+    if (!ctx || !v) {
+        g_last_error = "null argument";
+        return BS_ERR_INVALID_ARG;
+    }
+    return guarded([&] {
+        DeviceGuard g(ctx->device);
+        volume_offset(ctx, v, distance, &ctx->last_offset);
+    });
+}
+
+bs_status bs_ctx_last_offset_stats(const bs_ctx* ctx, bs_offset_stats* out) {
+    if (!ctx || !out) return BS_ERR_INVALID_ARG;
+    *out = ctx->last_offset;
+    return BS_OK;
 }
 
 bs_status bs_volume_signs_at(bs_ctx* ctx, bs_volume* v, const int32_t* idx, size_t n, uint8_t* out_negative) {

@@ -82,6 +82,7 @@ struct bs_ctx {
     int paths = 0;  // BS_PATH_* flags
     bs::Scratch scratch;
     bs_m2v_stats last_m2v = {};
+    bs_offset_stats last_offset = {};
     uint64_t* host_counters = nullptr;  // pinned, for small readbacks
 };
 
@@ -155,6 +156,12 @@ __device__ inline uint32_t hash_find(const uint64_t* keys, uint32_t cap_mask, ui
 // -fhip-fp32-correctly-rounded-divide-sqrt, which the build passes.
 __device__ inline float sqrt_rn(float x) { return __builtin_sqrtf(x); }
 __device__ inline float div_rn(float a, float b) { return a / b; }
+
+__device__ inline uint64_t shfl_xor_u64(uint64_t v, int m) {
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m);
+    return ((uint64_t)hi << 32) | lo;
+}
 
 inline unsigned div_up(uint64_t a, unsigned b) { return (unsigned)((a + b - 1) / b); }
 

@@ -1,0 +1,756 @@
+// Volume::offset on the device (src/voxel/volume/mod.rs:95-108): remove_if
+// |v| > 2h, fast sweeping (src/voxel/fast_sweep.rs), removal of empty leaves,
+// v -= D on the active values.
+//
+// Parallel schedule with the reference's exact result.  A sweep in direction
+// (sx, sy, sz) pops leaves in lexicographic order with each axis ascending or
+// descending by its sign (fast_sweep.rs:612-684), and a leaf reads only its
+// six face neighbours: the three upwind ones are already swept, the three
+// downwind ones are not.  With K = sx*lx + sy*ly + sz*lz over leaf
+// coordinates, face neighbours sit on planes K - 1 and K + 1, so every leaf on
+// one plane can be swept at once once plane K - 1 is done.  Inside a leaf the
+// x-y-z loop order has the same structure over voxel planes u + v + w
+// (u, v, w the voxel coordinates counted in the sweep direction): a voxel
+// reads updated values on plane - 1 and old ones on plane + 1.  So a sweep is
+// a sequence of plane launches, and inside each leaf a wavefront walks the 22
+// voxel planes.
+//
+// The leaf queue.  Every leaf in the grid at the start of a sweep is in its
+// queue (sweep 1 collects every leaf; sweep s + 1 takes the leaves popped in
+// sweep s), and a leaf joins when a swept upwind neighbour pushes it
+// (fast_sweep.rs:186-273).  The reference also creates empty leaves at the
+// other faces of every stencil; they stay empty unless pushed and are removed
+// at the end, so only pushed leaves are created here.  A pushed leaf lies on
+// the next plane and is swept there.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+
+#include "common.hpp"
+
+namespace bs {
+
+namespace {
+
+constexpr float FS_FAR = 3.40282347e+38f;
+
+// Voxels of an 8^3 leaf grouped by diagonal plane u + v + w = p (p = 0..21),
+// entry = u << 6 | v << 3 | w.
+struct VoxelPlanes {
+    uint16_t start[23];
+    uint16_t entry[512];
+};
+constexpr VoxelPlanes make_voxel_planes() {
+    VoxelPlanes t{};
+    int n = 0;
+    for (int p = 0; p < 22; ++p) {
+        t.start[p] = (uint16_t)n;
+        for (int u = 0; u < 8; ++u)
+            for (int v = 0; v < 8; ++v) {
+                const int w = p - u - v;
+                if (w >= 0 && w < 8) t.entry[n++] = (uint16_t)((u << 6) | (v << 3) | w);
+            }
+    }
+    t.start[22] = (uint16_t)n;
+    return t;
+}
+__constant__ VoxelPlanes kVoxelPlanes = make_voxel_planes();
+
+// Leaf lookup by leaf coordinate (origin >> 3): a dense cell grid over the
+// reachable box, or an open-addressing hash when that box is too large.  A
+// cell holds the pool slot, -1 when empty, -2 while a leaf is being created
+// (read as empty: a new leaf has no active voxel yet).
+struct LeafMap {
+    int dense;
+    int32_t* cells;
+    int32_t x0, y0, z0;
+    uint32_t nx, ny, nz;
+    uint64_t* keys;  // hash only
+    uint32_t cap_mask;
+
+    __device__ int32_t find(int32_t lx, int32_t ly, int32_t lz) const {
+        if (dense) {
+            const uint32_t ix = (uint32_t)(lx - x0), iy = (uint32_t)(ly - y0), iz = (uint32_t)(lz - z0);
+            if (ix >= nx || iy >= ny || iz >= nz) return -1;
+            const int32_t s = cells[((uint64_t)ix * ny + iy) * nz + iz];
+            return s >= 0 ? s : -1;
+        }
+        const uint64_t key = leaf_key(lx, ly, lz);
+        uint32_t h = hash_u64(key) & cap_mask;
+        for (uint32_t probe = 0; probe <= cap_mask; ++probe) {
+            const uint64_t k = keys[h];
+            if (k == key) {
+                const int32_t s = cells[h];
+                return s >= 0 ? s : -1;
+            }
+            if (k == EMPTY_KEY) return -1;
+            h = (h + 1) & cap_mask;
+        }
+        return -1;
+    }
+
+    // Claims the cell of a leaf that does not exist yet: the cell address, or
+    // nullptr when the leaf exists (or is being created) already.  *fail is
+    // set when the coordinate is outside the map.
+    __device__ int32_t* claim(int32_t lx, int32_t ly, int32_t lz, uint32_t* fail) const {
+        if (dense) {
+            const uint32_t ix = (uint32_t)(lx - x0), iy = (uint32_t)(ly - y0), iz = (uint32_t)(lz - z0);
+            if (ix >= nx || iy >= ny || iz >= nz) {
+                atomicOr(fail, 1u);
+                return nullptr;
+            }
+            int32_t* c = &cells[((uint64_t)ix * ny + iy) * nz + iz];
+            return atomicCAS(c, -1, -2) == -1 ? c : nullptr;
+        }
+        const uint64_t key = leaf_key(lx, ly, lz);
+        uint32_t h = hash_u64(key) & cap_mask;
+        for (uint32_t probe = 0; probe <= cap_mask; ++probe) {
+            const unsigned long long prev =
+                atomicCAS((unsigned long long*)&keys[h], (unsigned long long)EMPTY_KEY, (unsigned long long)key);
+            if (prev == EMPTY_KEY) return &cells[h];
+            if (prev == key) return nullptr;
+            h = (h + 1) & cap_mask;
+        }
+        atomicOr(fail, 2u);
+        return nullptr;
+    }
+
+    // Registers a leaf before any sweep (single writer per key).
+    __device__ bool put(int32_t lx, int32_t ly, int32_t lz, int32_t slot) const {
+        uint32_t fail = 0;
+        int32_t* c = claim(lx, ly, lz, &fail);
+        if (!c) return false;
+        *c = slot;
+        return true;
+    }
+};
+
+struct OffsetState {
+    uint32_t pool_count;  // leaves in the pool
+    uint32_t new_lo, new_hi;  // pool range created while sweeping the previous plane
+    uint32_t fail;        // bit 0: a leaf outside the reachable box, bit 1: pool or hash full
+};
+
+struct SweepArgs {
+    LeafMap map;
+    int4* origins;
+    uint64_t* masks;
+    uint64_t* frozen;
+    float* values;
+    uint32_t capacity;
+    int sx, sy, sz;
+    float h, limit_abs;
+    int neg;  // sweep sign
+};
+
+// ---------------------------------------------------------------- remove_if
+// leaf_node/tree_node.rs:151-160: active entries with |v| > 2h become
+// inactive.  keep[leaf] = the leaf still has an active entry.
+__global__ __launch_bounds__(256) void k_remove_if(uint64_t n, const uint64_t* __restrict__ masks,
+                                                    const float* __restrict__ values, float two_h,
+                                                    uint64_t* __restrict__ out_masks, uint32_t* __restrict__ keep) {
+    const uint64_t leaf = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (leaf >= n) return;
+    const int lane = threadIdx.x & 63;
+    const int w = lane >> 3;
+    const uint64_t word = masks[leaf * 8 + w];
+    const float4* vp = reinterpret_cast<const float4*>(values + leaf * 512 + (uint64_t)lane * 8);
+    const float4 q0 = vp[0], q1 = vp[1];
+    const float v[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+    uint32_t drop = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) drop |= (uint32_t)(fabsf(v[k]) > two_h) << k;
+    const uint64_t bits = (word >> ((lane & 7) * 8)) & 0xFFull;
+    uint64_t part = (bits & ~(uint64_t)drop) << ((lane & 7) * 8);
+    part |= shfl_xor_u64(part, 1);
+    part |= shfl_xor_u64(part, 2);
+    part |= shfl_xor_u64(part, 4);
+    if ((lane & 7) == 0) out_masks[leaf * 8 + w] = part;
+    const uint64_t any = __ballot(part != 0);
+    if (lane == 0) keep[leaf] = any != 0;
+}
+
+// Copies the kept leaves into the pool (frozen = mask after remove_if) and
+// registers them in the leaf map.
+__global__ __launch_bounds__(256) void k_pool_init(uint64_t n, const uint32_t* __restrict__ keep,
+                                                    const uint32_t* __restrict__ dst, const int32_t* __restrict__ origins,
+                                                    const uint64_t* __restrict__ rm_masks,
+                                                    const float* __restrict__ values, SweepArgs a, uint32_t* fail) {
+    const uint64_t leaf = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (leaf >= n || !keep[leaf]) return;
+    const int lane = threadIdx.x & 63;
+    const uint32_t s = dst[leaf];
+    const float4* src = reinterpret_cast<const float4*>(values + leaf * 512 + (uint64_t)lane * 8);
+    float4* out = reinterpret_cast<float4*>(a.values + (uint64_t)s * 512 + (uint64_t)lane * 8);
+    out[0] = src[0];
+    out[1] = src[1];
+    if (lane < 8) {
+        const uint64_t m = rm_masks[leaf * 8 + lane];
+        a.masks[(uint64_t)s * 8 + lane] = m;
+        a.frozen[(uint64_t)s * 8 + lane] = m;
+    }
+    if (lane == 0) {
+        const int4 o = make_int4(origins[leaf * 4], origins[leaf * 4 + 1], origins[leaf * 4 + 2], 0);
+        a.origins[s] = o;
+        if (!a.map.put(o.x >> 3, o.y >> 3, o.z >> 3, (int32_t)s)) atomicOr(fail, 1u);
+    }
+}
+
+// ---------------------------------------------------------------- plane buckets
+__device__ inline int32_t leaf_plane(const int4 o, int sx, int sy, int sz) {
+    return sx * (o.x >> 3) + sy * (o.y >> 3) + sz * (o.z >> 3);
+}
+
+__global__ void k_plane_hist(uint32_t n, const int4* __restrict__ origins, int sx, int sy, int sz, int32_t kmin,
+                             uint32_t n_planes, uint32_t* __restrict__ hist, uint32_t* fail) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t k = leaf_plane(origins[i], sx, sy, sz) - kmin;
+    if (k < 0 || (uint32_t)k >= n_planes) {
+        atomicOr(fail, 1u);
+        return;
+    }
+    atomicAdd(&hist[k], 1u);
+}
+
+__global__ void k_plane_scatter(uint32_t n, const int4* __restrict__ origins, int sx, int sy, int sz, int32_t kmin,
+                                uint32_t n_planes, uint32_t* __restrict__ cursor, uint32_t* __restrict__ order) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t k = leaf_plane(origins[i], sx, sy, sz) - kmin;
+    if (k < 0 || (uint32_t)k >= n_planes) return;
+    order[atomicAdd(&cursor[k], 1u)] = i;
+}
+
+// Opens plane p: the leaves created while plane p - 1 was swept are the
+// pool range [new_lo, new_hi) (clamped to the pool; an overflow is retried).
+__global__ void k_plane_begin(OffsetState* st, uint32_t capacity) {
+    st->new_lo = st->new_hi;
+    st->new_hi = min(st->pool_count, capacity);
+}
+
+// ---------------------------------------------------------------- sweep
+// utils.rs:64-73 option_min_by under fast_sweep.rs:137-140's cmp_abs: the
+// first value unless the second has the strictly smaller magnitude.
+__device__ inline bool min_abs(bool ha, float a, bool hb, float b, float& out) {
+    if (ha && hb) {
+        out = (fabsf(a) > fabsf(b)) ? b : a;
+        return true;
+    }
+    if (ha) {
+        out = a;
+        return true;
+    }
+    if (hb) {
+        out = b;
+        return true;
+    }
+    return false;
+}
+
+// fast_sweep.rs:509-537 with sort3 (helpers/utils.rs:4-16), f32 throughout
+__device__ inline float compute_distance(float a1, float a2, float a3, float h) {
+    float t;
+    if (a1 > a3) t = a1, a1 = a3, a3 = t;
+    if (a1 > a2) t = a1, a1 = a2, a2 = t;
+    if (a2 > a3) t = a2, a2 = a3, a3 = t;
+    const float s1 = a1 + h;
+    if (fabsf(s1) <= a2) return s1;
+    const float a1a2_sum = a1 + a2;
+    const float h_sq = h * h;
+    const float two_h_sq = h_sq + h_sq;
+    const float a1a2_diff = a1 - a2;
+    const float a1a2_diff_sq = a1a2_diff * a1a2_diff;
+    const float s2 = (a1a2_sum + sqrt_rn(two_h_sq - a1a2_diff_sq)) * 0.5f;
+    if (fabsf(s2) <= a3) return s2;
+    const float a1a2a3_sum = a1a2_sum + a3;
+    const float three_h_sq = two_h_sq + h_sq;
+    const float a1a3_diff = a1 - a3;
+    const float a1a3_diff_sq = a1a3_diff * a1a3_diff;
+    const float a2a3_diff = a2 - a3;
+    const float a2a3_diff_sq = a2a3_diff * a2a3_diff;
+    return (a1a2a3_sum + sqrt_rn(three_h_sq - a1a2_diff_sq - a1a3_diff_sq - a2a3_diff_sq)) * (1.0f / 3.0f);
+}
+
+struct LeafLds {
+    float val[512];
+    uint8_t act[512];
+    uint8_t frz[512];
+    int32_t nb[6];  // -x, +x, -y, +y, -z, +z neighbour slots
+};
+
+// Value of voxel (x, y, z) in leaf-local coordinates, one step outside the
+// leaf at most (fast_sweep.rs:359-387: the stencil's face neighbours).
+__device__ inline bool stencil_at(const LeafLds& L, const SweepArgs& a, int x, int y, int z, float& v) {
+    int k = -1;
+    if (z < 0) k = 4;
+    else if (z > 7) k = 5;
+    else if (y < 0) k = 2;
+    else if (y > 7) k = 3;
+    else if (x < 0) k = 0;
+    else if (x > 7) k = 1;
+    const int o = ((x & 7) << 6) | ((y & 7) << 3) | (z & 7);
+    if (k < 0) {
+        if (!L.act[o]) return false;
+        v = L.val[o];
+        return true;
+    }
+    const int32_t n = L.nb[k];
+    if (n < 0) return false;
+    if (!((a.masks[(uint64_t)n * 8 + (o >> 6)] >> (o & 63)) & 1ull)) return false;
+    v = a.values[(uint64_t)n * 512 + o];
+    return true;
+}
+
+// fast_sweep.rs:120-184
+__device__ inline void sweep_voxel(LeafLds& L, const SweepArgs& a, int x, int y, int z) {
+    const int o = (x << 6) | (y << 3) | z;
+    if (L.frz[o]) return;
+    float p, m, dx = 0.f, dy = 0.f, dz = 0.f;
+    bool hp = stencil_at(L, a, x + 1, y, z, p), hm = stencil_at(L, a, x - 1, y, z, m);
+    const bool hx = min_abs(hp, p, hm, m, dx);
+    hp = stencil_at(L, a, x, y + 1, z, p), hm = stencil_at(L, a, x, y - 1, z, m);
+    const bool hy = min_abs(hp, p, hm, m, dy);
+    hp = stencil_at(L, a, x, y, z + 1, p), hm = stencil_at(L, a, x, y, z - 1, m);
+    const bool hz = min_abs(hp, p, hm, m, dz);
+    if (!hx && !hy && !hz) return;
+    const float first = hx ? dx : (hy ? dy : dz);
+    if ((int)(__float_as_uint(first) >> 31) != a.neg) return;
+    const float d1 = hx ? dx : FS_FAR, d2 = hy ? dy : FS_FAR, d3 = hz ? dz : FS_FAR;
+    const float d_abs = compute_distance(fabsf(d1), fabsf(d2), fabsf(d3), a.h);
+    if (d_abs > a.limit_abs) return;
+    const float d_old = L.act[o] ? L.val[o] : FS_FAR;
+    if (d_abs < fabsf(d_old)) {
+        L.val[o] = a.neg ? -fabsf(d_abs) : fabsf(d_abs);  // copysign(d_abs, sweep sign)
+        L.act[o] = 1;
+    }
+}
+
+// One leaf plane of one sweep: the leaves of bucket [b0, b1) of `order` plus
+// the leaves created while the previous plane was swept.  One 64-lane block
+// per leaf, looping over the plane's leaves.
+__global__ __launch_bounds__(64) void k_sweep_plane(SweepArgs a, const uint32_t* __restrict__ order,
+                                                     const uint32_t* __restrict__ bucket_off, uint32_t plane,
+                                                     uint32_t n_planes, OffsetState* st) {
+    __shared__ LeafLds L;
+    const uint32_t b0 = bucket_off[plane], b1 = bucket_off[plane + 1];
+    const uint32_t n_old = b1 - b0;
+    const uint32_t new_lo = st->new_lo, n_new = st->new_hi - new_lo;
+    const int lane = threadIdx.x;
+    for (uint32_t item = blockIdx.x; item < n_old + n_new; item += gridDim.x) {
+        const uint32_t slot = item < n_old ? order[b0 + item] : new_lo + (item - n_old);
+        const int4 org = a.origins[slot];
+        {
+            const float4* vp = reinterpret_cast<const float4*>(a.values + (uint64_t)slot * 512 + lane * 8);
+            const float4 q0 = vp[0], q1 = vp[1];
+            const float v[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+            const uint64_t mw = a.masks[(uint64_t)slot * 8 + (lane >> 3)] >> ((lane & 7) * 8);
+            const uint64_t fw = a.frozen[(uint64_t)slot * 8 + (lane >> 3)] >> ((lane & 7) * 8);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                L.val[lane * 8 + k] = v[k];
+                L.act[lane * 8 + k] = (mw >> k) & 1;
+                L.frz[lane * 8 + k] = (fw >> k) & 1;
+            }
+            if (lane < 6) {
+                const int ax = lane >> 1, dir = (lane & 1) ? 1 : -1;
+                L.nb[lane] = a.map.find((org.x >> 3) + (ax == 0 ? dir : 0), (org.y >> 3) + (ax == 1 ? dir : 0),
+                                        (org.z >> 3) + (ax == 2 ? dir : 0));
+            }
+        }
+        __syncthreads();
+        for (int p = 0; p < 22; ++p) {
+            const int s0 = kVoxelPlanes.start[p], cnt = kVoxelPlanes.start[p + 1] - s0;
+            if (lane < cnt) {
+                const int e = kVoxelPlanes.entry[s0 + lane];
+                const int u = e >> 6, v = (e >> 3) & 7, w = e & 7;
+                sweep_voxel(L, a, a.sx > 0 ? u : 7 - u, a.sy > 0 ? v : 7 - v, a.sz > 0 ? w : 7 - w);
+            }
+            __syncthreads();
+        }
+        // write back
+        {
+            float4* vp = reinterpret_cast<float4*>(a.values + (uint64_t)slot * 512 + lane * 8);
+            vp[0] = make_float4(L.val[lane * 8 + 0], L.val[lane * 8 + 1], L.val[lane * 8 + 2], L.val[lane * 8 + 3]);
+            vp[1] = make_float4(L.val[lane * 8 + 4], L.val[lane * 8 + 5], L.val[lane * 8 + 6], L.val[lane * 8 + 7]);
+#pragma unroll
+            for (int w = 0; w < 8; ++w) {
+                const uint64_t word = __ballot(L.act[w * 64 + lane] != 0);
+                if (lane == 0) a.masks[(uint64_t)slot * 8 + w] = word;
+            }
+        }
+        // fast_sweep.rs:186-273: the +direction face of each axis
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            const int dir = ax == 0 ? a.sx : (ax == 1 ? a.sy : a.sz);
+            const int f = dir > 0 ? 7 : 0;
+            const int i = lane >> 3, j = lane & 7;
+            const int o = ax == 0 ? ((f << 6) | (i << 3) | j) : (ax == 1 ? ((i << 6) | (f << 3) | j) : ((i << 6) | (j << 3) | f));
+            const float v = L.val[o];
+            const bool q = L.act[o] && (int)(__float_as_uint(v) >> 31) == a.neg && fabsf(v) < a.limit_abs;
+            if (__ballot(q) != 0 && lane == 0) {
+                if (plane + 1 >= n_planes) atomicOr(&st->fail, 1u);
+                const int32_t nx = (org.x >> 3) + (ax == 0 ? dir : 0), ny = (org.y >> 3) + (ax == 1 ? dir : 0),
+                              nz = (org.z >> 3) + (ax == 2 ? dir : 0);
+                int32_t* cell = a.map.claim(nx, ny, nz, &st->fail);
+                if (cell) {
+                    const uint32_t s = atomicAdd(&st->pool_count, 1u);
+                    if (s < a.capacity) {
+                        a.origins[s] = make_int4(nx * 8, ny * 8, nz * 8, 0);
+                        __threadfence();
+                        atomicExch(cell, (int32_t)s);
+                    } else {
+                        atomicOr(&st->fail, 2u);  // pool overflow: retried with a larger pool
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- output
+// Sort keys of the final pool: empty leaves (fast_sweep.rs:61-64
+// remove_empty_branches) sort last.
+__global__ __launch_bounds__(256) void k_final_keys(uint32_t n, const int4* __restrict__ origins,
+                                                     const uint64_t* __restrict__ masks, uint64_t* __restrict__ keys,
+                                                     uint32_t* __restrict__ idx, uint32_t* __restrict__ n_live) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t any = 0;
+#pragma unroll
+    for (int w = 0; w < 8; ++w) any |= masks[(uint64_t)i * 8 + w];
+    const int4 o = origins[i];
+    keys[i] = any ? leaf_key(o.x >> 3, o.y >> 3, o.z >> 3) : ~0ull;
+    idx[i] = i;
+    if (any) atomicAdd(n_live, 1u);
+}
+
+// Gathers the live leaves in key order and applies v -= D to active values
+// (visitors/value_mut_visitor.rs:4-32 visits active values only).
+__global__ __launch_bounds__(256) void k_final_gather(uint64_t n_out, const uint32_t* __restrict__ src_idx,
+                                                       const int4* __restrict__ porigins,
+                                                       const uint64_t* __restrict__ pmasks,
+                                                       const float* __restrict__ pvalues, float distance,
+                                                       int4* __restrict__ origins, uint64_t* __restrict__ masks,
+                                                       float* __restrict__ values) {
+    const uint64_t leaf = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (leaf >= n_out) return;
+    const int lane = threadIdx.x & 63;
+    const uint32_t s = src_idx[leaf];
+    const uint64_t word = pmasks[(uint64_t)s * 8 + (lane >> 3)] >> ((lane & 7) * 8);
+    const float4* vp = reinterpret_cast<const float4*>(pvalues + (uint64_t)s * 512 + lane * 8);
+    const float4 q0 = vp[0], q1 = vp[1];
+    float v[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if ((word >> k) & 1) v[k] = v[k] - distance;
+    float4* op = reinterpret_cast<float4*>(values + leaf * 512 + (uint64_t)lane * 8);
+    op[0] = make_float4(v[0], v[1], v[2], v[3]);
+    op[1] = make_float4(v[4], v[5], v[6], v[7]);
+    if (lane < 8) masks[leaf * 8 + lane] = pmasks[(uint64_t)s * 8 + lane];
+    if (lane == 0) origins[leaf] = porigins[s];
+}
+
+constexpr int kDirs[8][3] = {{1, 1, 1},  {-1, 1, 1},  {1, -1, 1},  {-1, -1, 1},
+                             {1, 1, -1}, {-1, 1, -1}, {1, -1, -1}, {-1, -1, -1}};  // fast_sweep.rs:36-57
+
+template <class K, class V>
+void sort_pairs_u64(bs_ctx* ctx, const K* k, K* ks, const V* v, V* vs, uint64_t n) {
+    size_t bytes = 0;
+    BS_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, ks, v, vs, (int)n, 0, 64, ctx->stream));
+    void* tmp = ctx->scratch.get(S_CUB, bytes);
+    BS_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, bytes, k, ks, v, vs, (int)n, 0, 64, ctx->stream));
+}
+
+void exclusive_scan_u32(bs_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n) {
+    size_t bytes = 0;
+    BS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, ctx->stream));
+    void* tmp = ctx->scratch.get(S_CUB, bytes);
+    BS_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, (int)n, ctx->stream));
+}
+
+template <class T>
+T read_back(bs_ctx* ctx, const T* d) {
+    T h{};
+    BS_HIP(hipMemcpyAsync(&h, d, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+    BS_HIP(hipStreamSynchronize(ctx->stream));
+    return h;
+}
+
+// Device buffers of one offset attempt (released on every exit path).
+struct OffsetBuffers {
+    std::vector<void*> ptrs;
+    template <class T> T* alloc(uint64_t count) {
+        void* p = nullptr;
+        BS_HIP(hipMalloc(&p, std::max<uint64_t>(count, 1) * sizeof(T)));
+        ptrs.push_back(p);
+        return static_cast<T*>(p);
+    }
+    ~OffsetBuffers() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
+void alloc_leaves(bs_volume* v, uint64_t n);
+void free_leaves(bs_volume* v);
+Topology& ensure_topology(bs_ctx* ctx, bs_volume* v);
+
+namespace {
+
+struct OffsetResult {
+    uint64_t n_out = 0;
+    std::vector<uint32_t> keep;  // per input leaf
+    bool retry = false;
+};
+
+// One attempt with a pool of `capacity` leaves; retry = the pool or the leaf
+// map was too small (nothing of `vol` was modified).
+OffsetResult offset_attempt(bs_ctx* ctx, bs_volume* vol, float distance, uint64_t capacity, bool force_hash,
+                            bs_volume& out, bs_offset_stats* stats) {
+    hipStream_t st = ctx->stream;
+    OffsetResult res;
+    const uint64_t n = vol->n_leaves;
+    const float h = vol->voxel_size;
+    const float two_h = h * 2.0f;  // volume/mod.rs:96
+    float ext = fabsf(distance) + h + h;  // volume/mod.rs:98-99
+    ext = std::copysign(ext, std::signbit(distance) ? -1.0f : 1.0f);
+    const float limit_abs = fabsf(ext);
+    const int neg = std::signbit(ext) ? 1 : 0;
+
+    OffsetBuffers B;
+    // remove_if and compaction of the surviving leaves
+    uint64_t* rm_masks = B.alloc<uint64_t>(8 * n);
+    uint32_t* keep = B.alloc<uint32_t>(n);
+    uint32_t* dst = B.alloc<uint32_t>(n);
+    if (n) {
+        k_remove_if<<<div_up(n, 4), 256, 0, st>>>(n, vol->masks, vol->values, two_h, rm_masks, keep);
+        BS_HIP(hipGetLastError());
+        exclusive_scan_u32(ctx, keep, dst, n);
+    }
+    res.keep.assign(n, 0);
+    std::vector<int32_t> o4(4 * n);
+    if (n) {
+        BS_HIP(hipMemcpyAsync(res.keep.data(), keep, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, st));
+        BS_HIP(hipMemcpyAsync(o4.data(), vol->origins, sizeof(int32_t) * 4 * n, hipMemcpyDeviceToHost, st));
+        BS_HIP(hipStreamSynchronize(st));
+    }
+    uint64_t n_keep = 0;
+    int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    for (uint64_t i = 0; i < n; ++i) {
+        if (!res.keep[i]) continue;
+        ++n_keep;
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], o4[4 * i + k] >> 3);
+            hi[k] = std::max(hi[k], o4[4 * i + k] >> 3);
+        }
+    }
+    if (stats) stats->n_kept = n_keep;
+    if (n_keep == 0) {
+        out.n_leaves = 0;
+        return res;
+    }
+    BS_REQUIRE(capacity >= n_keep && capacity < (1ull << 31), BS_ERR_INVALID_ARG, "offset leaf pool too large");
+
+    // Reach of the extension: a value grows by at least h/sqrt(3) per voxel
+    // step (the three-term Godunov update is the smallest increment), so no
+    // leaf beyond limit*sqrt(3)/h + 2 voxels of the band is ever pushed.
+    const int32_t margin = (int32_t)std::ceil(((double)limit_abs / (double)h * 1.7320508075688772 + 4.0) / 8.0) + 1;
+    LeafMap map{};
+    uint64_t cells = 1;
+    for (int k = 0; k < 3; ++k) {
+        lo[k] -= margin;
+        hi[k] += margin;
+        cells *= (uint64_t)(hi[k] - lo[k] + 1);
+    }
+    map.dense = !force_hash && cells <= (1ull << 28);
+    if (map.dense) {
+        map.x0 = lo[0], map.y0 = lo[1], map.z0 = lo[2];
+        map.nx = hi[0] - lo[0] + 1, map.ny = hi[1] - lo[1] + 1, map.nz = hi[2] - lo[2] + 1;
+        map.cells = B.alloc<int32_t>(cells);
+        BS_HIP(hipMemsetAsync(map.cells, 0xFF, sizeof(int32_t) * cells, st));
+    } else {
+        uint64_t cap = 1024;
+        while (cap < 2 * capacity) cap <<= 1;
+        map.cap_mask = (uint32_t)(cap - 1);
+        map.keys = B.alloc<uint64_t>(cap);
+        map.cells = B.alloc<int32_t>(cap);
+        BS_HIP(hipMemsetAsync(map.keys, 0xFF, sizeof(uint64_t) * cap, st));
+        BS_HIP(hipMemsetAsync(map.cells, 0xFF, sizeof(int32_t) * cap, st));
+    }
+    if (stats) stats->used_hash = !map.dense;
+    if (stats) stats->n_capacity = capacity;
+
+    SweepArgs a{};
+    a.map = map;
+    a.capacity = (uint32_t)capacity;
+    a.origins = B.alloc<int4>(capacity);
+    a.masks = B.alloc<uint64_t>(8 * capacity);
+    a.frozen = B.alloc<uint64_t>(8 * capacity);
+    a.values = B.alloc<float>(512 * capacity);
+    a.h = h;
+    a.limit_abs = limit_abs;
+    a.neg = neg;
+    // leaves created by the sweep start empty, with 0.0 values (Leaf::empty)
+    BS_HIP(hipMemsetAsync(a.masks, 0, sizeof(uint64_t) * 8 * capacity, st));
+    BS_HIP(hipMemsetAsync(a.frozen, 0, sizeof(uint64_t) * 8 * capacity, st));
+    BS_HIP(hipMemsetAsync(a.values + 512 * n_keep, 0, sizeof(float) * 512 * (capacity - n_keep), st));
+    OffsetState* dstate = B.alloc<OffsetState>(1);
+    OffsetState init{(uint32_t)n_keep, (uint32_t)n_keep, (uint32_t)n_keep, 0};
+    BS_HIP(hipMemcpyAsync(dstate, &init, sizeof(init), hipMemcpyHostToDevice, st));
+    k_pool_init<<<div_up(n, 4), 256, 0, st>>>(n, keep, dst, vol->origins, rm_masks, vol->values, a, &dstate->fail);
+    BS_HIP(hipGetLastError());
+
+    // plane range over the whole map box
+    uint32_t* order = B.alloc<uint32_t>(capacity);
+    uint64_t planes_total = 0, launches = 0;
+    for (int s = 0; s < 8; ++s) {
+        a.sx = kDirs[s][0], a.sy = kDirs[s][1], a.sz = kDirs[s][2];
+        const OffsetState cur = read_back(ctx, dstate);
+        BS_REQUIRE(!(cur.fail & 1u), BS_ERR_INTERNAL, "offset: a pushed leaf left the reachable box");
+        if (cur.fail) {
+            res.retry = true;
+            return res;
+        }
+        const uint32_t n_pool = cur.pool_count;
+        // nothing was created before the first plane: new_lo = new_hi = pool_count
+        const uint32_t reset[3] = {n_pool, n_pool, n_pool};
+        BS_HIP(hipMemcpyAsync(dstate, reset, sizeof(reset), hipMemcpyHostToDevice, st));
+        // planes of the map box in this direction
+        // planes of the reachable box (the leaves plus the extension margin)
+        int32_t kmin = 0, kmax = 0;
+        const int sgn[3] = {a.sx, a.sy, a.sz};
+        for (int k = 0; k < 3; ++k) {
+            kmin += sgn[k] > 0 ? lo[k] : -hi[k];
+            kmax += sgn[k] > 0 ? hi[k] : -lo[k];
+        }
+        const uint32_t n_planes = (uint32_t)(kmax - kmin + 1);
+        uint32_t* hist = B.alloc<uint32_t>(n_planes + 1);
+        uint32_t* off = B.alloc<uint32_t>(n_planes + 1);
+        BS_HIP(hipMemsetAsync(hist, 0, sizeof(uint32_t) * (n_planes + 1), st));
+        k_plane_hist<<<div_up(n_pool, 256), 256, 0, st>>>(n_pool, a.origins, a.sx, a.sy, a.sz, kmin, n_planes, hist,
+                                                         &dstate->fail);
+        BS_HIP(hipGetLastError());
+        exclusive_scan_u32(ctx, hist, off, n_planes + 1);
+        BS_HIP(hipMemcpyAsync(hist, off, sizeof(uint32_t) * (n_planes + 1), hipMemcpyDeviceToDevice, st));
+        k_plane_scatter<<<div_up(n_pool, 256), 256, 0, st>>>(n_pool, a.origins, a.sx, a.sy, a.sz, kmin, n_planes, hist,
+                                                            order);
+        BS_HIP(hipGetLastError());
+        // host view of the bucket sizes: the first and last plane holding a leaf
+        std::vector<uint32_t> hoff(n_planes + 1);
+        BS_HIP(hipMemcpyAsync(hoff.data(), off, sizeof(uint32_t) * (n_planes + 1), hipMemcpyDeviceToHost, st));
+        BS_HIP(hipStreamSynchronize(st));
+        uint32_t first = 0, last = 0, max_bucket = 0;
+        bool any = false;
+        for (uint32_t p = 0; p < n_planes; ++p) {
+            const uint32_t c = hoff[p + 1] - hoff[p];
+            if (!c) continue;
+            if (!any) first = p;
+            any = true;
+            last = p;
+            max_bucket = std::max(max_bucket, c);
+        }
+        if (!any) break;
+        const unsigned grid = (unsigned)std::min<uint64_t>(std::max<uint32_t>(max_bucket, 64) * 2, 4096);
+        for (uint32_t p = first; p < n_planes; ++p) {
+            if (p > last) {
+                // past the last bucket: continue only while the previous plane created leaves
+                const OffsetState t = read_back(ctx, dstate);
+                if (t.pool_count == t.new_hi) break;
+            }
+            k_plane_begin<<<1, 1, 0, st>>>(dstate, a.capacity);
+            k_sweep_plane<<<grid, 64, 0, st>>>(a, order, off, p, n_planes, dstate);
+            BS_HIP(hipGetLastError());
+            ++planes_total;
+            launches += 2;
+        }
+    }
+    const OffsetState fin = read_back(ctx, dstate);
+    BS_REQUIRE(!(fin.fail & 1u), BS_ERR_INTERNAL, "offset: a pushed leaf left the reachable box");
+    if (fin.fail) {
+        res.retry = true;
+        return res;
+    }
+    const uint32_t n_pool = fin.pool_count;
+    if (stats) {
+        stats->n_pool = n_pool;
+        stats->n_planes = planes_total;
+        stats->n_launches = launches;
+    }
+
+    // empty leaves go, the rest sorted by origin, shifted by -D
+    uint64_t* keys = B.alloc<uint64_t>(n_pool);
+    uint64_t* keys_s = B.alloc<uint64_t>(n_pool);
+    uint32_t* idx = B.alloc<uint32_t>(n_pool);
+    uint32_t* idx_s = B.alloc<uint32_t>(n_pool);
+    uint32_t* n_live = B.alloc<uint32_t>(1);
+    BS_HIP(hipMemsetAsync(n_live, 0, sizeof(uint32_t), st));
+    k_final_keys<<<div_up(n_pool, 256), 256, 0, st>>>(n_pool, a.origins, a.masks, keys, idx, n_live);
+    BS_HIP(hipGetLastError());
+    sort_pairs_u64(ctx, keys, keys_s, idx, idx_s, n_pool);
+    const uint32_t n_out = read_back(ctx, n_live);
+    alloc_leaves(&out, n_out);
+    if (n_out) {
+        k_final_gather<<<div_up(n_out, 4), 256, 0, st>>>(n_out, idx_s, a.origins, a.masks, a.values, distance,
+                                                         reinterpret_cast<int4*>(out.origins), out.masks, out.values);
+        BS_HIP(hipGetLastError());
+    }
+    BS_HIP(hipStreamSynchronize(st));
+    res.n_out = n_out;
+    return res;
+}
+
+}  // namespace
+
+// Volume::offset (volume/mod.rs:95-108) of `vol` in place.
+void volume_offset(bs_ctx* ctx, bs_volume* vol, float distance, bs_offset_stats* stats) {
+    BS_REQUIRE(std::isfinite(distance), BS_ERR_INVALID_ARG, "offset distance must be finite");
+    BS_REQUIRE(vol->device == ctx->device, BS_ERR_INVALID_ARG, "volume lives on another device");
+    Topology& topo = ensure_topology(ctx, vol);
+    const float two_h = vol->voxel_size * 2.0f;
+    // fast_sweep.rs:100 unwraps the leaf at every queued origin, and the queue
+    // holds every active tile that survives remove_if: the reference panics.
+    for (const TileRec& t : vol->tiles)
+        BS_REQUIRE(fabsf(t.value) > two_h, BS_ERR_INTERNAL,
+                   "offset: an active tile lies within the band; the reference panics (fast_sweep.rs:100)");
+    if (stats) *stats = bs_offset_stats{};
+    uint64_t capacity = std::max<uint64_t>(4 * vol->n_leaves + 4096, 8192);
+    bs_volume out;
+    out.device = vol->device;
+    OffsetResult r;
+    for (int attempt = 0;; ++attempt) {
+        try {
+            r = offset_attempt(ctx, vol, distance, capacity, (ctx->paths & BS_PATH_FORCE_HASH) != 0, out, stats);
+        } catch (...) {
+            free_leaves(&out);
+            throw;
+        }
+        if (!r.retry) break;
+        BS_REQUIRE(attempt < 6, BS_ERR_INTERNAL, "offset: leaf pool kept overflowing");
+        capacity *= 4;
+    }
+    // topology: remove_if (tiles, emptied leaves, empty nodes), the new leaves, empty nodes again
+    std::vector<std::array<int32_t, 3>> out_origins(out.n_leaves);
+    if (out.n_leaves) {
+        std::vector<int32_t> o4(4 * out.n_leaves);
+        BS_HIP(hipMemcpy(o4.data(), out.origins, sizeof(int32_t) * o4.size(), hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < out.n_leaves; ++i) out_origins[i] = {o4[4 * i], o4[4 * i + 1], o4[4 * i + 2]};
+    }
+    std::unique_ptr<Topology> t2 = topo.after_offset(r.keep, out_origins, two_h);
+    free_leaves(vol);
+    vol->n_leaves = out.n_leaves;
+    vol->origins = out.origins;
+    vol->masks = out.masks;
+    vol->values = out.values;
+    out.origins = nullptr;
+    out.masks = nullptr;
+    out.values = nullptr;
+    vol->tiles.clear();  // remove_if deactivated every tile (checked above)
+    vol->topo = std::move(t2);
+    if (stats) stats->n_out = out.n_leaves;
+}
+
+}  // namespace bs

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <numeric>
 
 namespace bs {
 
@@ -381,6 +382,76 @@ int32_t Topology::locate(int32_t x, int32_t y, int32_t z, bool* neg_out) const {
         return -1;
     }
     return b.child[o2];
+}
+
+std::unique_ptr<Topology> Topology::after_offset(const std::vector<uint32_t>& keep,
+                                                const std::vector<std::array<int32_t, 3>>& out_origins,
+                                                float two_h) const {
+    auto t = std::make_unique<Topology>(*this);
+    // remove_if: tiles beyond the band go inactive, emptied leaves go
+    for (auto& n : t->n1)
+        for (int o = 0; o < Node1::SIZE; ++o)
+            if (n.kind[o] == SLOT_TILE && std::fabs(n.tile[o]) > two_h) n.kind[o] = SLOT_EMPTY;
+    for (auto& n : t->n2)
+        for (int o = 0; o < Node2::SIZE; ++o) {
+            if (n.kind[o] == SLOT_TILE && std::fabs(n.tile[o]) > two_h) n.kind[o] = SLOT_EMPTY;
+            if (n.kind[o] == SLOT_BRANCH) {
+                const int32_t li = n.child[o];
+                if (li < 0 || (size_t)li >= keep.size() || !keep[li]) n.kind[o] = SLOT_EMPTY;
+                n.child[o] = -1;  // renumbered below
+            }
+        }
+    auto prune = [&]() {
+        for (auto it = t->root.begin(); it != t->root.end();) {
+            Node1& a = t->n1[it->second];
+            bool a_empty = true;
+            for (int o1 = 0; o1 < Node1::SIZE; ++o1) {
+                if (a.kind[o1] == SLOT_BRANCH) {
+                    const Node2& b = t->n2[a.child[o1]];
+                    bool b_empty = true;
+                    for (int o2 = 0; o2 < Node2::SIZE && b_empty; ++o2) b_empty = b.kind[o2] == SLOT_EMPTY;
+                    if (b_empty) a.kind[o1] = SLOT_EMPTY;
+                }
+                a_empty &= a.kind[o1] == SLOT_EMPTY;
+            }
+            if (a_empty) it = t->root.erase(it);
+            else ++it;
+        }
+    };
+    prune();
+    t->leaves.assign(out_origins.size(), LeafRef{});
+    for (size_t i = 0; i < out_origins.size(); ++i) {
+        const int32_t x = out_origins[i][0], y = out_origins[i][1], z = out_origins[i][2];
+        const Key k{x & ~(L1_SPAN - 1), y & ~(L1_SPAN - 1), z & ~(L1_SPAN - 1)};
+        auto it = t->root.find(k);
+        if (it == t->root.end()) {
+            t->n1.emplace_back(k[0], k[1], k[2]);
+            it = t->root.emplace(k, (int32_t)t->n1.size() - 1).first;
+        }
+        const int32_t ai = it->second;
+        const int o1 = l1_slot(x, y, z);
+        if (t->n1[ai].kind[o1] != SLOT_BRANCH) {
+            int32_t nx, ny, nz;
+            slot_origin(t->n1[ai], o1, L2_SPAN, nx, ny, nz);
+            t->n2.emplace_back(nx, ny, nz);
+            t->n1[ai].kind[o1] = SLOT_BRANCH;
+            t->n1[ai].child[o1] = (int32_t)t->n2.size() - 1;
+        }
+        Node2& b = t->n2[t->n1[ai].child[o1]];
+        const int o2 = l2_slot(x, y, z);
+        b.kind[o2] = SLOT_BRANCH;
+        b.child[o2] = (int32_t)i;
+        LeafRef& r = t->leaves[i];
+        r.x = x;
+        r.y = y;
+        r.z = z;
+        r.a = (int32_t)i;
+    }
+    prune();
+    // drop the node records no slot refers to any more
+    std::vector<int32_t> keep_leaf(out_origins.size());
+    std::iota(keep_leaf.begin(), keep_leaf.end(), 0);
+    return t->compact(keep_leaf);
 }
 
 std::unique_ptr<Topology> Topology::compact(const std::vector<int32_t>& new_index) const {

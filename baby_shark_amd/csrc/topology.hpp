@@ -81,6 +81,16 @@ struct Topology {
     // or not) or, for a missing root entry, positive.
     int32_t locate(int32_t x, int32_t y, int32_t z, bool* neg) const;
 
+    // The tree after Volume::offset (volume/mod.rs:95-108): remove_if turns
+    // active tiles with |v| > two_h inactive and drops the leaves whose index
+    // has keep[i] == 0, empty nodes are pruned (internal_node/tree_node.rs:
+    // 312-337, root_node/tree_node.rs:150-158), then the result's leaves
+    // (origins in output order; leaf r is device leaf r) are inserted, creating
+    // fresh nodes where needed, and empty nodes are pruned again
+    // (remove_empty_branches, fast_sweep.rs:61-64).
+    std::unique_ptr<Topology> after_offset(const std::vector<uint32_t>& keep,
+                                           const std::vector<std::array<int32_t, 3>>& out_origins, float two_h) const;
+
     // The reachable part of the tree with leaf reference r renumbered to
     // new_index[r] (a plain copy of the device leaf at that index).
     std::unique_ptr<Topology> compact(const std::vector<int32_t>& new_index) const;

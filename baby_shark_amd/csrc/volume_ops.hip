@@ -23,12 +23,6 @@ namespace {
 constexpr int WAVES_PER_BLOCK = 4;  // one leaf per wavefront, 256-thread blocks
 constexpr float FAR = 3.40282347e+38f;
 
-__device__ inline uint64_t shfl_xor_u64(uint64_t v, int m) {
-    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m);
-    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m);
-    return ((uint64_t)hi << 32) | lo;
-}
-
 // Mask word x of a leaf (bits y*8+z) from the 8-bit z-rows held by lanes
 // (x, 0..7); valid in every lane of the group.
 __device__ inline uint64_t gather_word(uint32_t row, int y) {
@@ -45,6 +39,45 @@ __device__ inline int last_at_or_below(uint64_t m, int top) {
     m &= keep;
     return m ? 63 - __builtin_clzll(m) : -1;
 }
+
+}  // namespace
+
+void alloc_leaves(bs_volume* v, uint64_t n) {
+    v->n_leaves = n;
+    if (!n) return;
+    BS_HIP(hipMalloc(&v->origins, sizeof(int32_t) * 4 * n));
+    BS_HIP(hipMalloc(&v->masks, sizeof(uint64_t) * 8 * n));
+    BS_HIP(hipMalloc(&v->values, sizeof(float) * 512 * n));
+}
+
+void free_leaves(bs_volume* v) {
+    if (v->origins) BS_HIP(hipFree(v->origins));
+    if (v->masks) BS_HIP(hipFree(v->masks));
+    if (v->values) BS_HIP(hipFree(v->values));
+    v->origins = nullptr;
+    v->masks = nullptr;
+    v->values = nullptr;
+    v->n_leaves = 0;
+}
+
+// The host topology of a volume, built from its leaf origins and active tiles
+// the first time an operation needs it.
+Topology& ensure_topology(bs_ctx* ctx, bs_volume* v) {
+    if (v->topo) return *v->topo;
+    std::vector<int32_t> o4(4 * v->n_leaves);
+    if (v->n_leaves) {
+        BS_HIP(hipMemcpyAsync(o4.data(), v->origins, sizeof(int32_t) * o4.size(), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        BS_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    std::vector<std::array<int32_t, 3>> origins(v->n_leaves);
+    for (uint64_t i = 0; i < v->n_leaves; ++i) origins[i] = {o4[4 * i], o4[4 * i + 1], o4[4 * i + 2]};
+    v->topo = Topology::from_leaves(origins, v->tiles);
+    BS_REQUIRE(v->topo->leaves.size() == v->n_leaves, BS_ERR_INTERNAL, "volume has two leaves at one origin");
+    return *v->topo;
+}
+
+namespace {
 
 // ------------------------------------------------------------------ flood fill
 // leaf_node/flood_fill.rs:12-50.  The scan line keeps three running signs:
@@ -248,41 +281,6 @@ __global__ void k_gather_signs(uint64_t n, const uint64_t* __restrict__ q, const
 }
 
 inline int32_t floor_div8(int64_t v) { return (int32_t)(v >= 0 ? v / 8 : -((-v + 7) / 8)); }
-
-void alloc_leaves(bs_volume* v, uint64_t n) {
-    v->n_leaves = n;
-    if (!n) return;
-    BS_HIP(hipMalloc(&v->origins, sizeof(int32_t) * 4 * n));
-    BS_HIP(hipMalloc(&v->masks, sizeof(uint64_t) * 8 * n));
-    BS_HIP(hipMalloc(&v->values, sizeof(float) * 512 * n));
-}
-
-void free_leaves(bs_volume* v) {
-    if (v->origins) BS_HIP(hipFree(v->origins));
-    if (v->masks) BS_HIP(hipFree(v->masks));
-    if (v->values) BS_HIP(hipFree(v->values));
-    v->origins = nullptr;
-    v->masks = nullptr;
-    v->values = nullptr;
-    v->n_leaves = 0;
-}
-
-// The host topology of a volume, built from its leaf origins and active tiles
-// the first time an operation needs it.
-Topology& ensure_topology(bs_ctx* ctx, bs_volume* v) {
-    if (v->topo) return *v->topo;
-    std::vector<int32_t> o4(4 * v->n_leaves);
-    if (v->n_leaves) {
-        BS_HIP(hipMemcpyAsync(o4.data(), v->origins, sizeof(int32_t) * o4.size(), hipMemcpyDeviceToHost,
-                              ctx->stream));
-        BS_HIP(hipStreamSynchronize(ctx->stream));
-    }
-    std::vector<std::array<int32_t, 3>> origins(v->n_leaves);
-    for (uint64_t i = 0; i < v->n_leaves; ++i) origins[i] = {o4[4 * i], o4[4 * i + 1], o4[4 * i + 2]};
-    v->topo = Topology::from_leaves(origins, v->tiles);
-    BS_REQUIRE(v->topo->leaves.size() == v->n_leaves, BS_ERR_INTERNAL, "volume has two leaves at one origin");
-    return *v->topo;
-}
 
 // Runs the leaf flood fill of one volume on the device (values in place) and
 // records each leaf's first/last sign and emptiness in its topology.  The

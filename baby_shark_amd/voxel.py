@@ -47,6 +47,11 @@ class Context:
         """Force alternate code paths (BS_PATH_* flags of bs_voxel.h); 0 = by size."""
         check(load().bs_ctx_set_paths(self._h, int(flags)))
 
+    def last_offset_stats(self) -> dict:
+        s = _lib.OffsetStats()
+        check(load().bs_ctx_last_offset_stats(self.handle, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
     def last_stats(self) -> dict:
         s = _lib.M2VStats()
         check(load().bs_ctx_last_m2v_stats(self._h, C.byref(s)))
@@ -152,6 +157,13 @@ class Volume:
 
     def intersect(self, other: "Volume") -> "Volume":
         return self._csg(other, "bs_volume_intersect")
+
+    def offset(self, distance: float) -> "Volume":
+        """Volume::offset (volume/mod.rs:95-108); consumes this handle."""
+        if not self._h:
+            raise ValueError("volume was consumed by a previous operation")
+        check(load().bs_volume_offset(self._ctx.handle, self._h, float(distance)))
+        return Volume(self._take(), self._ctx)
 
     def flood_fill(self) -> "Volume":
         """FloodFill of the grid in place (root_node/flood_fill.rs:8-42)."""

@@ -64,6 +64,17 @@ typedef struct {
     double ms_total, ms_subdivide, ms_leaves, ms_distance, ms_bvh, ms_winding;
 } bs_m2v_stats;
 
+/* Work counters of the last bs_volume_offset call. */
+typedef struct {
+    uint64_t n_kept;      /* leaves left by remove_if                          */
+    uint64_t n_pool;      /* leaves after the eight sweeps (empty ones included) */
+    uint64_t n_out;       /* leaves of the result                               */
+    uint64_t n_planes;    /* leaf planes swept over all sweeps                  */
+    uint64_t n_launches;  /* kernel launches of the sweeps                      */
+    uint64_t n_capacity;  /* leaf pool capacity of the successful attempt       */
+    uint64_t used_hash;   /* leaf map was a hash (box too large for a grid)     */
+} bs_offset_stats;
+
 const char* bs_last_error(void);
 
 /* Context: binds `device`, creates one default-priority HIP stream. */
@@ -138,6 +149,15 @@ bs_status bs_volume_intersect(bs_ctx* ctx, bs_volume* self, bs_volume* other);
  * internal_node/flood_fill.rs:17-75, leaf_node/flood_fill.rs:12-50): inactive
  * leaf entries become +-FLT_MAX with the scan-line sign and stay inactive. */
 bs_status bs_volume_flood_fill(bs_ctx* ctx, bs_volume* v);
+
+/* Volume::offset (volume/mod.rs:95-108) in place: entries with |v| > 2h are
+ * deactivated (remove_if), the band is extended by fast sweeping
+ * (fast_sweep.rs:10-287) up to |D| + 2h on the side of D's sign, empty leaves
+ * are removed and D is subtracted from every active value.  A volume whose
+ * active tiles are not all beyond 2h gives BS_ERR_INTERNAL (the reference
+ * panics at fast_sweep.rs:100); the volume is then unchanged. */
+bs_status bs_volume_offset(bs_ctx* ctx, bs_volume* v, float distance);
+bs_status bs_ctx_last_offset_stats(const bs_ctx* ctx, bs_offset_stats* out);
 
 /* FloodFill::sign_at (root_node/flood_fill.rs:56-63, internal_node/
  * flood_fill.rs:110-117, leaf_node/flood_fill.rs:72-75) for n voxel indices

@@ -14,6 +14,7 @@
 #include <cstring>
 #include <unordered_map>
 
+#include "fast_sweep.hpp"
 #include "mesh_to_volume.hpp"
 #include "parallel.hpp"
 
@@ -398,6 +399,22 @@ or_volume* or_volume_from_voxels(const int32_t* idx, const float* values, size_t
     auto* r = new or_volume{std::move(g), h, {}, {}};
     r->index();
     return r;
+}
+
+// fast_sweep.rs:509-537 (exposed for its closed-form checks)
+float or_fs_compute_distance(float a1, float a2, float a3, float h) { return fs_compute_distance(a1, a2, a3, h); }
+
+// Volume::offset (volume/mod.rs:95-108).  Returns 0, or 1 where the reference
+// panics (an active tile inside the band reaches the leaf queue).
+int or_volume_offset(or_volume* a, float distance) {
+    try {
+        volume_offset(*a->grid, a->voxel_size, distance);
+    } catch (const FastSweepError&) {
+        a->index();
+        return 1;
+    }
+    a->index();
+    return 0;
 }
 
 // Volume::union / intersect / subtract (volume/mod.rs:74-93): both operands

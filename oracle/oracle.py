@@ -59,6 +59,10 @@ def lib():
         u8p = C.POINTER(C.c_uint8)
         L.or_volume_sphere.restype = vp
         L.or_volume_sphere.argtypes = [C.c_float] * 5
+        L.or_fs_compute_distance.restype = C.c_float
+        L.or_fs_compute_distance.argtypes = [C.c_float] * 4
+        L.or_volume_offset.restype = C.c_int
+        L.or_volume_offset.argtypes = [vp, C.c_float]
         L.or_volume_from_voxels.restype = vp
         L.or_volume_from_voxels.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_float), C.c_size_t, C.c_float]
         L.or_volume_csg.argtypes = [vp, vp, C.c_int]
@@ -124,6 +128,11 @@ class Volume:
 
     def intersect(self, other):  # volume/mod.rs:81-86
         return self._csg(other, 2)
+
+    def offset(self, distance: float) -> "Volume":   # volume/mod.rs:95-108
+        if lib().or_volume_offset(self.h, distance) != 0:
+            raise RuntimeError("offset: the reference panics (tile origin in the leaf queue)")
+        return self
 
     def flood_fill(self) -> "Volume":
         lib().or_volume_flood_fill(self.h)
@@ -249,6 +258,10 @@ def sample_unsigned(soup, voxel_size: float, band: int, voxels, threads: int = 0
 def sphere(radius: float, center, voxel_size: float) -> Volume:
     """VolumeBuilder::default().with_voxel_size(h).sphere(radius, center)."""
     return Volume(lib().or_volume_sphere(radius, center[0], center[1], center[2], voxel_size), voxel_size)
+
+
+def fs_compute_distance(a1, a2, a3, h) -> float:
+    return lib().or_fs_compute_distance(a1, a2, a3, h)
 
 
 def from_voxels(idx, values, voxel_size: float) -> Volume:
