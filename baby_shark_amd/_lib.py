@@ -51,6 +51,11 @@ class OffsetStats(C.Structure):
                                           "used_hash")]
 
 
+class DcStats(C.Structure):
+    _fields_ = [("n_leaves", C.c_uint64), ("n_tris", C.c_uint64), ("used_hash", C.c_uint64),
+                ("ms_total", C.c_double)]
+
+
 class BSError(RuntimeError):
     def __init__(self, status: int, msg: str):
         super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
@@ -104,6 +109,11 @@ def load():
             "bs_volume_intersect": (C.c_int, [vp, vp, vp]),
             "bs_volume_flood_fill": (C.c_int, [vp, vp]),
             "bs_volume_offset": (C.c_int, [vp, vp, C.c_float]),
+            "bs_dual_contour": (C.c_int, [vp, vp, C.c_float, C.POINTER(f32p), C.POINTER(C.c_size_t)]),
+            "bs_dual_contour_device": (C.c_int, [vp, vp, C.c_float, C.POINTER(vp), C.POINTER(C.c_size_t)]),
+            "bs_ctx_last_dc_stats": (C.c_int, [vp, C.POINTER(DcStats)]),
+            "bs_free": (None, [vp]),
+            "bs_device_free": (None, [vp]),
             "bs_ctx_last_offset_stats": (C.c_int, [vp, C.POINTER(OffsetStats)]),
             "bs_volume_signs_at": (C.c_int, [vp, vp, i32p, C.c_size_t, C.POINTER(C.c_uint8)]),
             "bs_volume_device_leaves": (C.c_int, [vp, C.POINTER(i32p), C.POINTER(u64p), C.POINTER(f32p),

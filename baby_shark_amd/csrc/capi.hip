@@ -1,5 +1,6 @@
 // C ABI entry points (include/bs_voxel.h): argument checks, error mapping and
 // the canonical host export of volumes.
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -15,6 +16,8 @@ void volume_sphere(bs_ctx* ctx, float radius, float cx, float cy, float cz, floa
 void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op);
 void volume_flood_fill(bs_ctx* ctx, bs_volume* v);
 void volume_offset(bs_ctx* ctx, bs_volume* vol, float distance, bs_offset_stats* stats);
+void dual_contour_device(bs_ctx* ctx, bs_volume* vol, float voxel_size, float** out_dev, uint64_t* n_tris,
+                         bs_dc_stats* stats);
 void volume_from_voxels(bs_ctx* ctx, const int32_t* idx, const float* values, uint64_t n, float h, bs_volume** out);
 void volume_signs_at(bs_ctx* ctx, bs_volume* v, const int32_t* idx, uint64_t n, uint8_t* out_neg);
 
@@ -318,6 +321,63 @@ bs_status bs_ctx_last_offset_stats(const bs_ctx* ctx, bs_offset_stats* out) {
     if (!ctx || !out) return BS_ERR_INVALID_ARG;
     *out = ctx->last_offset;
     return BS_OK;
+}
+
+bs_status bs_dual_contour_device(bs_ctx* ctx, const bs_volume* v, float voxel_size, float** out_xyz_dev,
+                                 size_t* n_tris) {
+    if (!ctx || !v || !out_xyz_dev || !n_tris) {
+        g_last_error = "null argument";
+        return BS_ERR_INVALID_ARG;
+    }
+    *out_xyz_dev = nullptr;
+    *n_tris = 0;
+    return guarded([&] {
+        DeviceGuard g(ctx->device);
+        uint64_t n = 0;
+        dual_contour_device(ctx, const_cast<bs_volume*>(v), voxel_size, out_xyz_dev, &n, &ctx->last_dc);
+        *n_tris = (size_t)n;
+    });
+}
+
+bs_status bs_dual_contour(bs_ctx* ctx, const bs_volume* v, float voxel_size, float** out_xyz, size_t* n_tris) {
+    if (!ctx || !v || !out_xyz || !n_tris) {
+        g_last_error = "null argument";
+        return BS_ERR_INVALID_ARG;
+    }
+    *out_xyz = nullptr;
+    *n_tris = 0;
+    return guarded([&] {
+        DeviceGuard g(ctx->device);
+        float* d = nullptr;
+        uint64_t n = 0;
+        dual_contour_device(ctx, const_cast<bs_volume*>(v), voxel_size, &d, &n, &ctx->last_dc);
+        if (!n) return;
+        float* h = static_cast<float*>(std::malloc(sizeof(float) * 9 * n));
+        if (!h) {
+            (void)hipFree(d);
+            throw Error(BS_ERR_INTERNAL, "host out of memory");
+        }
+        hipError_t e = hipMemcpy(h, d, sizeof(float) * 9 * n, hipMemcpyDeviceToHost);
+        (void)hipFree(d);
+        if (e != hipSuccess) {
+            std::free(h);
+            BS_HIP(e);
+        }
+        *out_xyz = h;
+        *n_tris = (size_t)n;
+    });
+}
+
+bs_status bs_ctx_last_dc_stats(const bs_ctx* ctx, bs_dc_stats* out) {
+    if (!ctx || !out) return BS_ERR_INVALID_ARG;
+    *out = ctx->last_dc;
+    return BS_OK;
+}
+
+void bs_free(void* p) { std::free(p); }
+
+void bs_device_free(void* p) {
+    if (p) (void)hipFree(p);
 }
 
 bs_status bs_volume_signs_at(bs_ctx* ctx, bs_volume* v, const int32_t* idx, size_t n, uint8_t* out_negative) {

@@ -44,7 +44,8 @@ enum ScratchSlot : int {
     S_KEYS, S_KEYS_SORTED, S_CAND_LEAF, S_CAND_CNT, S_CAND_OFF, S_CAND_LIST, S_CURSOR, S_ACT_CNT, S_ACT_OFF,
     S_ACT_LIST, S_COUNTERS, S_CUB, S_MORTON, S_MORTON_SORTED, S_IDX, S_IDX_SORTED, S_BVH_TRIS, S_BVH_NODES,
     S_BVH_LINKS, S_BVH_PARENT, S_BVH_FLAGS, S_BVH_INIT, S_BVH_BOX, S_RECHECK, S_ARGMIN, S_MISC0, S_MISC1,
-    S_MISC2, S_MISC3, S_INPUT, S_BVH_KIDS, S_FLAGS_A, S_FLAGS_B, S_CSG_PLAN, S_COUNT
+    S_MISC2, S_MISC3, S_INPUT, S_BVH_KIDS, S_FLAGS_A, S_FLAGS_B, S_CSG_PLAN, S_DC_MAP, S_DC_KEYS, S_DC_FAIL, S_DC_CELLS,
+    S_DC_CELLMASK, S_DC_TRICNT, S_DC_TRIOFF, S_COUNT
 };
 
 struct Scratch {
@@ -83,6 +84,7 @@ struct bs_ctx {
     bs::Scratch scratch;
     bs_m2v_stats last_m2v = {};
     bs_offset_stats last_offset = {};
+    bs_dc_stats last_dc = {};
     uint64_t* host_counters = nullptr;  // pinned, for small readbacks
 };
 
@@ -161,6 +163,16 @@ __device__ inline uint64_t shfl_xor_u64(uint64_t v, int m) {
     const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m);
     const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m);
     return ((uint64_t)hi << 32) | lo;
+}
+
+// Mask word x of a leaf (bits y*8+z) from the 8-bit z-rows held by lanes
+// (x, 0..7); valid in every lane of the group.
+__device__ inline uint64_t gather_word(uint32_t row, int y) {
+    uint64_t w = (uint64_t)row << (8 * y);
+    w |= shfl_xor_u64(w, 1);
+    w |= shfl_xor_u64(w, 2);
+    w |= shfl_xor_u64(w, 4);
+    return w;
 }
 
 inline unsigned div_up(uint64_t a, unsigned b) { return (unsigned)((a + b - 1) / b); }

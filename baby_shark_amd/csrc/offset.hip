@@ -28,6 +28,7 @@
 #include <cmath>
 
 #include "common.hpp"
+#include "leafmap.hpp"
 
 namespace bs {
 
@@ -56,75 +57,6 @@ constexpr VoxelPlanes make_voxel_planes() {
     return t;
 }
 __constant__ VoxelPlanes kVoxelPlanes = make_voxel_planes();
-
-// Leaf lookup by leaf coordinate (origin >> 3): a dense cell grid over the
-// reachable box, or an open-addressing hash when that box is too large.  A
-// cell holds the pool slot, -1 when empty, -2 while a leaf is being created
-// (read as empty: a new leaf has no active voxel yet).
-struct LeafMap {
-    int dense;
-    int32_t* cells;
-    int32_t x0, y0, z0;
-    uint32_t nx, ny, nz;
-    uint64_t* keys;  // hash only
-    uint32_t cap_mask;
-
-    __device__ int32_t find(int32_t lx, int32_t ly, int32_t lz) const {
-        if (dense) {
-            const uint32_t ix = (uint32_t)(lx - x0), iy = (uint32_t)(ly - y0), iz = (uint32_t)(lz - z0);
-            if (ix >= nx || iy >= ny || iz >= nz) return -1;
-            const int32_t s = cells[((uint64_t)ix * ny + iy) * nz + iz];
-            return s >= 0 ? s : -1;
-        }
-        const uint64_t key = leaf_key(lx, ly, lz);
-        uint32_t h = hash_u64(key) & cap_mask;
-        for (uint32_t probe = 0; probe <= cap_mask; ++probe) {
-            const uint64_t k = keys[h];
-            if (k == key) {
-                const int32_t s = cells[h];
-                return s >= 0 ? s : -1;
-            }
-            if (k == EMPTY_KEY) return -1;
-            h = (h + 1) & cap_mask;
-        }
-        return -1;
-    }
-
-    // Claims the cell of a leaf that does not exist yet: the cell address, or
-    // nullptr when the leaf exists (or is being created) already.  *fail is
-    // set when the coordinate is outside the map.
-    __device__ int32_t* claim(int32_t lx, int32_t ly, int32_t lz, uint32_t* fail) const {
-        if (dense) {
-            const uint32_t ix = (uint32_t)(lx - x0), iy = (uint32_t)(ly - y0), iz = (uint32_t)(lz - z0);
-            if (ix >= nx || iy >= ny || iz >= nz) {
-                atomicOr(fail, 1u);
-                return nullptr;
-            }
-            int32_t* c = &cells[((uint64_t)ix * ny + iy) * nz + iz];
-            return atomicCAS(c, -1, -2) == -1 ? c : nullptr;
-        }
-        const uint64_t key = leaf_key(lx, ly, lz);
-        uint32_t h = hash_u64(key) & cap_mask;
-        for (uint32_t probe = 0; probe <= cap_mask; ++probe) {
-            const unsigned long long prev =
-                atomicCAS((unsigned long long*)&keys[h], (unsigned long long)EMPTY_KEY, (unsigned long long)key);
-            if (prev == EMPTY_KEY) return &cells[h];
-            if (prev == key) return nullptr;
-            h = (h + 1) & cap_mask;
-        }
-        atomicOr(fail, 2u);
-        return nullptr;
-    }
-
-    // Registers a leaf before any sweep (single writer per key).
-    __device__ bool put(int32_t lx, int32_t ly, int32_t lz, int32_t slot) const {
-        uint32_t fail = 0;
-        int32_t* c = claim(lx, ly, lz, &fail);
-        if (!c) return false;
-        *c = slot;
-        return true;
-    }
-};
 
 struct OffsetState {
     uint32_t pool_count;  // leaves in the pool

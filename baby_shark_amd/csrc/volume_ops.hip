@@ -23,16 +23,6 @@ namespace {
 constexpr int WAVES_PER_BLOCK = 4;  // one leaf per wavefront, 256-thread blocks
 constexpr float FAR = 3.40282347e+38f;
 
-// Mask word x of a leaf (bits y*8+z) from the 8-bit z-rows held by lanes
-// (x, 0..7); valid in every lane of the group.
-__device__ inline uint64_t gather_word(uint32_t row, int y) {
-    uint64_t w = (uint64_t)row << (8 * y);
-    w |= shfl_xor_u64(w, 1);
-    w |= shfl_xor_u64(w, 2);
-    w |= shfl_xor_u64(w, 4);
-    return w;
-}
-
 // Highest set bit of m at or below position `top` (-1 when none).
 __device__ inline int last_at_or_below(uint64_t m, int top) {
     const uint64_t keep = (top >= 63) ? ~0ull : ((2ull << top) - 1);
@@ -431,8 +421,8 @@ void volume_from_voxels(bs_ctx* ctx, const int32_t* idx, const float* values, ui
     std::vector<uint64_t> keys(n);
     for (uint64_t i = 0; i < n; ++i) {
         for (int k = 0; k < 3; ++k)
-            BS_REQUIRE(idx[3 * i + k] >= -(1 << 23) && idx[3 * i + k] < (1 << 23), BS_ERR_INVALID_ARG,
-                       "voxel index outside +-2^23");
+            BS_REQUIRE(idx[3 * i + k] >= -(1 << 23) + 16 && idx[3 * i + k] < (1 << 23) - 16, BS_ERR_INVALID_ARG,
+                       "voxel index outside +-(2^23 - 16)");
         keys[i] = leaf_key(idx[3 * i] >> 3, idx[3 * i + 1] >> 3, idx[3 * i + 2] >> 3);
     }
     std::vector<uint64_t> uniq(keys);

@@ -52,6 +52,11 @@ class Context:
         check(load().bs_ctx_last_offset_stats(self.handle, C.byref(s)))
         return {k: getattr(s, k) for k, _ in s._fields_}
 
+    def last_dc_stats(self) -> dict:
+        s = _lib.DcStats()
+        check(load().bs_ctx_last_dc_stats(self.handle, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
     def last_stats(self) -> dict:
         s = _lib.M2VStats()
         check(load().bs_ctx_last_m2v_stats(self._h, C.byref(s)))
@@ -238,6 +243,72 @@ def volume_from_voxels(idx, values, voxel_size: float, ctx: Optional[Context] = 
                                        v.ctypes.data_as(C.POINTER(C.c_float)), a.shape[0], float(voxel_size),
                                        C.byref(h)))
     return Volume(h, ctx)
+
+
+class DualContouringMesher:
+    """meshing/dual_contouring.rs:12-83.  Default voxel size 1.0."""
+
+    def __init__(self, ctx: Optional[Context] = None):
+        self._ctx = ctx
+        self.voxel_size = 1.0
+
+    @property
+    def ctx(self) -> Context:
+        if self._ctx is None:
+            self._ctx = default_context()
+        return self._ctx
+
+    def with_voxel_size(self, size: float) -> "DualContouringMesher":
+        self.voxel_size = float(size)
+        return self
+
+    def mesh(self, volume: "Volume") -> np.ndarray:
+        """Triangle soup (n, 3, 3) float32 on the host."""
+        L = load()
+        out = C.POINTER(C.c_float)()
+        n = C.c_size_t(0)
+        check(L.bs_dual_contour(self.ctx.handle, volume._h, self.voxel_size, C.byref(out), C.byref(n)))
+        if n.value == 0:
+            return np.zeros((0, 3, 3), np.float32)
+        try:
+            return np.ctypeslib.as_array(out, shape=(n.value * 9,)).copy().reshape(-1, 3, 3)
+        finally:
+            L.bs_free(C.cast(out, C.c_void_p))
+
+    def mesh_device(self, volume: "Volume") -> "DeviceTriangles":
+        """Triangle soup left in device memory (see DeviceTriangles)."""
+        ptr = C.c_void_p()
+        n = C.c_size_t(0)
+        check(load().bs_dual_contour_device(self.ctx.handle, volume._h, self.voxel_size, C.byref(ptr), C.byref(n)))
+        return DeviceTriangles(ptr.value or 0, n.value)
+
+
+class DeviceTriangles:
+    """A device-resident (n, 3, 3) float32 soup owned by the library's
+    allocator; exposes __cuda_array_interface__ so torch.as_tensor can view it
+    (clone before the object is dropped)."""
+
+    def __init__(self, ptr: int, n: int):
+        self.ptr, self.n = ptr, n
+
+    @property
+    def __cuda_array_interface__(self):
+        return {"shape": (self.n, 3, 3), "typestr": "<f4", "data": (self.ptr, False), "version": 2,
+                "strides": None}
+
+    def __len__(self):
+        return self.n
+
+    def free(self) -> None:
+        if self.ptr:
+            load().bs_device_free(C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class MeshToVolume:

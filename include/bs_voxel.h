@@ -75,6 +75,12 @@ typedef struct {
     uint64_t used_hash;   /* leaf map was a hash (box too large for a grid)     */
 } bs_offset_stats;
 
+/* Counters of the last bs_dual_contour* call. */
+typedef struct {
+    uint64_t n_leaves, n_tris, used_hash;
+    double ms_total;  /* device time from the first kernel to the last (stream events) */
+} bs_dc_stats;
+
 const char* bs_last_error(void);
 
 /* Context: binds `device`, creates one default-priority HIP stream. */
@@ -127,9 +133,10 @@ bs_status bs_volume_sphere(bs_ctx* ctx, float radius, float cx, float cy, float 
 
 /* A grid built by inserting voxel values one by one (TreeNode::insert on an
  * empty VolumeGrid: root_node/tree_node.rs, internal_node/tree_node.rs:54-79,
- * leaf_node/tree_node.rs:46-50): idx n*3 int32 voxel indices within +-2^23,
+ * leaf_node/tree_node.rs:46-50): idx n*3 int32 voxel indices,
  * values n floats, both host.  Each inserted voxel becomes active; other
- * entries of a touched leaf hold 0.0; a repeated index keeps the last value. */
+ * entries of a touched leaf hold 0.0; a repeated index keeps the last value.
+ * Indices must lie within +-(2^23 - 16) on every axis. */
 bs_status bs_volume_from_voxels(bs_ctx* ctx, const int32_t* idx, const float* values, size_t n, float voxel_size,
                                 bs_volume** out);
 
@@ -158,6 +165,22 @@ bs_status bs_volume_flood_fill(bs_ctx* ctx, bs_volume* v);
  * panics at fast_sweep.rs:100); the volume is then unchanged. */
 bs_status bs_volume_offset(bs_ctx* ctx, bs_volume* v, float distance);
 bs_status bs_ctx_last_offset_stats(const bs_ctx* ctx, bs_offset_stats* out);
+
+/* DualContouringMesher::default().with_voxel_size(voxel_size).mesh(volume)
+ * (meshing/dual_contouring.rs:23-83): a triangle soup of *n_tris triangles,
+ * 9 floats each, in the sequential order (leaves by origin, voxels x-y-z,
+ * edges X-Y-Z; the reference's parallel order is unspecified).  *out_xyz is
+ * malloc'd host memory released with bs_free (NULL when there is no
+ * triangle).  A volume with active tiles gives BS_ERR_UNSUPPORTED_TILE (the
+ * reference stops at todo!()); a crossing voxel with no neighbour along an axis
+ * gives BS_ERR_INTERNAL (unreachable!() at :347). */
+bs_status bs_dual_contour(bs_ctx* ctx, const bs_volume* v, float voxel_size, float** out_xyz, size_t* n_tris);
+/* Same, with the soup left in device memory (release with bs_device_free). */
+bs_status bs_dual_contour_device(bs_ctx* ctx, const bs_volume* v, float voxel_size, float** out_xyz_dev,
+                                 size_t* n_tris);
+bs_status bs_ctx_last_dc_stats(const bs_ctx* ctx, bs_dc_stats* out);
+void bs_free(void* p);
+void bs_device_free(void* p_dev);
 
 /* FloodFill::sign_at (root_node/flood_fill.rs:56-63, internal_node/
  * flood_fill.rs:110-117, leaf_node/flood_fill.rs:72-75) for n voxel indices

@@ -12,8 +12,10 @@
 #include <atomic>
 #include <chrono>
 #include <cstring>
+#include <cstdlib>
 #include <unordered_map>
 
+#include "dual_contouring.hpp"
 #include "fast_sweep.hpp"
 #include "mesh_to_volume.hpp"
 #include "parallel.hpp"
@@ -400,6 +402,31 @@ or_volume* or_volume_from_voxels(const int32_t* idx, const float* values, size_t
     r->index();
     return r;
 }
+
+// DualContouringMesher::default().with_voxel_size(h).mesh(volume)
+// (dual_contouring.rs:23-83).  *out receives n_tris*9 floats (free with
+// or_free).  Returns 0, 1 where the reference reaches todo!() (a tile) or 2
+// where it reaches unreachable!() (a crossing voxel without neighbours).
+int or_dual_contour(const or_volume* a, float h, float** out, size_t* n_tris) {
+    *out = nullptr;
+    *n_tris = 0;
+    try {
+        DualContouring dc(*a->grid);
+        std::vector<Vec3f> v = dc.mesh(h);
+        *n_tris = v.size() / 3;
+        *out = (float*)std::malloc(sizeof(float) * 3 * (v.size() ? v.size() : 1));
+        for (size_t i = 0; i < v.size(); ++i) {
+            (*out)[3 * i] = v[i].x;
+            (*out)[3 * i + 1] = v[i].y;
+            (*out)[3 * i + 2] = v[i].z;
+        }
+    } catch (const DcError& e) {
+        return e.code;
+    }
+    return 0;
+}
+
+void or_free(void* p) { std::free(p); }
 
 // fast_sweep.rs:509-537 (exposed for its closed-form checks)
 float or_fs_compute_distance(float a1, float a2, float a3, float h) { return fs_compute_distance(a1, a2, a3, h); }

@@ -59,6 +59,9 @@ def lib():
         u8p = C.POINTER(C.c_uint8)
         L.or_volume_sphere.restype = vp
         L.or_volume_sphere.argtypes = [C.c_float] * 5
+        L.or_dual_contour.restype = C.c_int
+        L.or_dual_contour.argtypes = [vp, C.c_float, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_size_t)]
+        L.or_free.argtypes = [vp]
         L.or_fs_compute_distance.restype = C.c_float
         L.or_fs_compute_distance.argtypes = [C.c_float] * 4
         L.or_volume_offset.restype = C.c_int
@@ -133,6 +136,20 @@ class Volume:
         if lib().or_volume_offset(self.h, distance) != 0:
             raise RuntimeError("offset: the reference panics (tile origin in the leaf queue)")
         return self
+
+    def dual_contour(self, voxel_size: float) -> np.ndarray:
+        """DualContouringMesher (dual_contouring.rs:23-83): (n_tris, 3, 3) float32."""
+        out = C.POINTER(C.c_float)()
+        n = C.c_size_t(0)
+        rc = lib().or_dual_contour(self.h, voxel_size, C.byref(out), C.byref(n))
+        if rc != 0:
+            raise RuntimeError({1: "dual contouring: tile (reference todo!)",
+                                2: "dual contouring: gradient without neighbours (reference unreachable!)"}[rc])
+        try:
+            return np.ctypeslib.as_array(out, shape=(n.value * 9,)).copy().reshape(-1, 3, 3) if n.value else \
+                np.zeros((0, 3, 3), np.float32)
+        finally:
+            lib().or_free(C.cast(out, C.c_void_p))
 
     def flood_fill(self) -> "Volume":
         lib().or_volume_flood_fill(self.h)
