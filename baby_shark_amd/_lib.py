@@ -48,7 +48,13 @@ class M2VStats(C.Structure):
 
 class OffsetStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("n_kept", "n_pool", "n_out", "n_planes", "n_launches", "n_capacity",
-                                          "used_hash")]
+                                          "used_hash", "n_swept")] + \
+              [("ms_device", C.c_double), ("ms_wall", C.c_double)]
+
+
+class CsgStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("n_a", "n_b", "n_out", "n_paired")] + \
+              [(k, C.c_double) for k in ("ms_flood", "ms_assemble", "ms_wall")]
 
 
 class DcStats(C.Structure):
@@ -107,6 +113,7 @@ def load():
             "bs_volume_union": (C.c_int, [vp, vp, vp]),
             "bs_volume_subtract": (C.c_int, [vp, vp, vp]),
             "bs_volume_intersect": (C.c_int, [vp, vp, vp]),
+            "bs_ctx_last_csg_stats": (C.c_int, [vp, C.POINTER(CsgStats)]),
             "bs_volume_flood_fill": (C.c_int, [vp, vp]),
             "bs_volume_offset": (C.c_int, [vp, vp, C.c_float]),
             "bs_dual_contour": (C.c_int, [vp, vp, C.c_float, C.POINTER(f32p), C.POINTER(C.c_size_t)]),

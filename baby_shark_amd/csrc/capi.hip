@@ -294,6 +294,12 @@ bs_status bs_volume_intersect(bs_ctx* ctx, bs_volume* self, bs_volume* other) {
     return csg_entry(ctx, self, other, CSG_INTERSECT);
 }
 
+bs_status bs_ctx_last_csg_stats(const bs_ctx* ctx, bs_csg_stats* out) {
+    if (!ctx || !out) return BS_ERR_INVALID_ARG;
+    *out = ctx->last_csg;
+    return BS_OK;
+}
+
 bs_status bs_volume_flood_fill(bs_ctx* ctx, bs_volume* v) {
     if (!ctx || !v) {
         g_last_error = "null argument";

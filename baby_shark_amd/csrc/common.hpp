@@ -85,6 +85,7 @@ struct bs_ctx {
     bs_m2v_stats last_m2v = {};
     bs_offset_stats last_offset = {};
     bs_dc_stats last_dc = {};
+    bs_csg_stats last_csg = {};
     uint64_t* host_counters = nullptr;  // pinned, for small readbacks
 };
 
@@ -174,6 +175,30 @@ __device__ inline uint64_t gather_word(uint32_t row, int y) {
     w |= shfl_xor_u64(w, 4);
     return w;
 }
+
+// Two HIP events around a stretch of work on a stream.
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    EventPair() {
+        BS_HIP(hipEventCreate(&a));
+        if (hipEventCreate(&b) != hipSuccess) {
+            (void)hipEventDestroy(a);
+            throw Error(BS_ERR_HIP, "hipEventCreate failed");
+        }
+    }
+    ~EventPair() {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+    void start(hipStream_t s) { BS_HIP(hipEventRecord(a, s)); }
+    void stop(hipStream_t s) { BS_HIP(hipEventRecord(b, s)); }
+    double ms() {
+        BS_HIP(hipEventSynchronize(b));
+        float f = 0.0f;
+        BS_HIP(hipEventElapsedTime(&f, a, b));
+        return f;
+    }
+};
 
 inline unsigned div_up(uint64_t a, unsigned b) { return (unsigned)((a + b - 1) / b); }
 

@@ -25,6 +25,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 
 #include "common.hpp"
@@ -538,7 +539,7 @@ OffsetResult offset_attempt(bs_ctx* ctx, bs_volume* vol, float distance, uint64_
 
     // plane range over the whole map box
     uint32_t* order = B.alloc<uint32_t>(capacity);
-    uint64_t planes_total = 0, launches = 0;
+    uint64_t planes_total = 0, launches = 0, swept = 0;
     for (int s = 0; s < 8; ++s) {
         a.sx = kDirs[s][0], a.sy = kDirs[s][1], a.sz = kDirs[s][2];
         const OffsetState cur = read_back(ctx, dstate);
@@ -548,6 +549,7 @@ OffsetResult offset_attempt(bs_ctx* ctx, bs_volume* vol, float distance, uint64_
             return res;
         }
         const uint32_t n_pool = cur.pool_count;
+        if (s > 0) swept += n_pool;  // every leaf in the pool at the end of a sweep was swept in it
         // nothing was created before the first plane: new_lo = new_hi = pool_count
         const uint32_t reset[3] = {n_pool, n_pool, n_pool};
         BS_HIP(hipMemcpyAsync(dstate, reset, sizeof(reset), hipMemcpyHostToDevice, st));
@@ -607,7 +609,9 @@ OffsetResult offset_attempt(bs_ctx* ctx, bs_volume* vol, float distance, uint64_
         return res;
     }
     const uint32_t n_pool = fin.pool_count;
+    swept += n_pool;
     if (stats) {
+        stats->n_swept = swept;
         stats->n_pool = n_pool;
         stats->n_planes = planes_total;
         stats->n_launches = launches;
@@ -640,6 +644,9 @@ OffsetResult offset_attempt(bs_ctx* ctx, bs_volume* vol, float distance, uint64_
 // Volume::offset (volume/mod.rs:95-108) of `vol` in place.
 void volume_offset(bs_ctx* ctx, bs_volume* vol, float distance, bs_offset_stats* stats) {
     BS_REQUIRE(std::isfinite(distance), BS_ERR_INVALID_ARG, "offset distance must be finite");
+    const auto t_wall = std::chrono::steady_clock::now();
+    EventPair ev;
+    ev.start(ctx->stream);
     BS_REQUIRE(vol->device == ctx->device, BS_ERR_INVALID_ARG, "volume lives on another device");
     Topology& topo = ensure_topology(ctx, vol);
     const float two_h = vol->voxel_size * 2.0f;
@@ -682,7 +689,12 @@ void volume_offset(bs_ctx* ctx, bs_volume* vol, float distance, bs_offset_stats*
     out.values = nullptr;
     vol->tiles.clear();  // remove_if deactivated every tile (checked above)
     vol->topo = std::move(t2);
-    if (stats) stats->n_out = out.n_leaves;
+    ev.stop(ctx->stream);
+    if (stats) {
+        stats->n_out = out.n_leaves;
+        stats->ms_device = ev.ms();
+        stats->ms_wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wall).count();
+    }
 }
 
 }  // namespace bs

@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <chrono>
 #include <numeric>
 #include <tuple>
 
@@ -344,12 +345,20 @@ void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op) {
     BS_REQUIRE(a != b, BS_ERR_INVALID_ARG, "CSG operands must be two distinct volumes");
     BS_REQUIRE(a->device == ctx->device && b->device == ctx->device, BS_ERR_INVALID_ARG,
                "CSG operands must live on the context's device");
+    const auto t_wall = std::chrono::steady_clock::now();
+    bs_csg_stats cs{};
+    cs.n_a = a->n_leaves;
+    cs.n_b = b->n_leaves;
     Topology& ta = ensure_topology(ctx, a);
     Topology& tb = ensure_topology(ctx, b);
     std::vector<uint8_t> fa, fb;
+    EventPair ev_flood, ev_asm;
+    ev_flood.start(ctx->stream);
     launch_leaf_flood_fill(ctx, a, S_FLAGS_A, fa);
     launch_leaf_flood_fill(ctx, b, S_FLAGS_B, fb);
+    ev_flood.stop(ctx->stream);
     BS_HIP(hipStreamSynchronize(ctx->stream));
+    cs.ms_flood = ev_flood.ms();
     apply_flags(ta, fa);
     apply_flags(tb, fb);
     ta.flood_fill_upper();
@@ -378,6 +387,7 @@ void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op) {
         BS_REQUIRE(r.op == LEAF_COPY_A || (r.b >= 0 && (uint64_t)r.b < b->n_leaves), BS_ERR_INTERNAL,
                    "CSG plan refers to a missing leaf of the second operand");
         host[i] = make_int4(std::max(r.a, 0), std::max(r.b, 0), (int)r.op | ((int)r.neg << 8), 0);
+        cs.n_paired += r.op == LEAF_UNION || r.op == LEAF_SUBTRACT || r.op == LEAF_INTERSECT;
         host[n_out + i] = make_int4(r.x, r.y, r.z, 0);
     }
 
@@ -385,6 +395,7 @@ void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op) {
     out.device = a->device;
     try {
         alloc_leaves(&out, n_out);
+        ev_asm.start(ctx->stream);
         if (n_out) {
             int4* d_host = ctx->scratch.as<int4>(S_CSG_PLAN, 2 * n_out);
             BS_HIP(hipMemcpyAsync(d_host, host.data(), sizeof(int4) * host.size(), hipMemcpyHostToDevice, ctx->stream));
@@ -393,7 +404,9 @@ void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op) {
                 reinterpret_cast<int4*>(out.origins), out.masks, out.values);
             BS_HIP(hipGetLastError());
         }
+        ev_asm.stop(ctx->stream);
         BS_HIP(hipStreamSynchronize(ctx->stream));
+        cs.ms_assemble = ev_asm.ms();
     } catch (...) {
         free_leaves(&out);
         throw;
@@ -409,6 +422,9 @@ void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op) {
     out.values = nullptr;
     a->tiles = std::move(tiles);
     a->topo = ta.compact(new_index);
+    cs.n_out = n_out;
+    cs.ms_wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wall).count();
+    ctx->last_csg = cs;
 }
 
 // An empty grid receiving insert(idx[i], values[i]) for i = 0..n-1 in order

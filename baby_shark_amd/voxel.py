@@ -52,6 +52,11 @@ class Context:
         check(load().bs_ctx_last_offset_stats(self.handle, C.byref(s)))
         return {k: getattr(s, k) for k, _ in s._fields_}
 
+    def last_csg_stats(self) -> dict:
+        s = _lib.CsgStats()
+        check(load().bs_ctx_last_csg_stats(self.handle, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
     def last_dc_stats(self) -> dict:
         s = _lib.DcStats()
         check(load().bs_ctx_last_dc_stats(self.handle, C.byref(s)))

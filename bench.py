@@ -38,21 +38,92 @@ def winding_flop(stats):
 
 
 def cpu_baseline(soup, h, band, max_threads=16):
-    """The oracle on a polar cap of the same mesh (triangles with centroid z > 0.8,
-    about a tenth of the surface)."""
+    """The oracle on a polar cap of the same mesh (triangles with centroid z > 0.4,
+    about 30% of the surface: 10-30 s of CPU work)."""
     import numpy as np
     from oracle import oracle as O
     if not os.path.exists(O.LIB_PATH):
         O.build()
     cent = soup.mean(axis=1)
-    cap = np.ascontiguousarray(soup[cent[:, 2] > 0.8])
+    cap = np.ascontiguousarray(soup[cent[:, 2] > 0.4])
     threads = max(1, min(max_threads, os.cpu_count() or 1))
     vol, sec = O.mesh_to_volume(cap, h, band, threads=threads, timing=True)
     n_act = vol.export().n_active
     return {"value": n_act / sec, "unit": "voxels/s", "cores": threads, "kind": "port",
-            "sample": f"polar cap z>0.8 of the same mesh: {cap.shape[0]} triangles, {n_act} active voxels, "
+            "sample": f"polar cap z>0.4 of the same mesh: {cap.shape[0]} triangles, {n_act} active voxels, "
                       f"{sec:.2f} s",
             "seconds": round(sec, 3)}
+
+
+def pipeline(ctx, level=7, h=0.001, band=3, reps=2):
+    """CSG, offset and dual contouring on the two-icosphere workload: displaced
+    icospheres of `level` subdivisions (seeds 3 and 4, the second moved by
+    (0.3, 0, 0)) voxelized at h; S = A - B, I = A n B, S.offset(+0.02),
+    I.offset(-0.01), dual contouring of S.  Each operation is timed on
+    fresh clones (clone time excluded); the last of `reps` rounds is kept.
+    Algorithmic bytes: flood fill 4160 B per operand leaf (read 2048 + 64,
+    write 2048), assembly 6336 B per paired result leaf (two leaves read, one
+    written) and 4224 B per copied one; offset 2176 B per input leaf
+    (remove_if), 4224 B per leaf sweep (read + write), 4096 B per output leaf
+    (shift)."""
+    import torch
+    import baby_shark_amd as bs
+    from baby_shark_amd import meshgen as mg
+    a = torch.from_numpy(mg.displaced_icosphere(level, 1.0, 0.05, seed=3)).cuda()
+    b = torch.from_numpy(mg.displaced_icosphere(level, 1.0, 0.05, seed=4, center=(0.3, 0.0, 0.0))).cuda()
+    torch.cuda.synchronize()
+    m2v = bs.MeshToVolume(ctx).with_voxel_size(h).with_narrow_band_width(band)
+    A, B = m2v.convert(a), m2v.convert(b)
+    res = {}
+    for _ in range(reps):
+        r = {}
+        for name, op in (("subtract", "subtract"), ("intersect", "intersect")):
+            x, y = A.clone(), B.clone()
+            t0 = time.perf_counter()
+            out = getattr(x, op)(y)
+            wall = (time.perf_counter() - t0) * 1e3
+            st = ctx.last_csg_stats()
+            flood_b = 4160.0 * (st["n_a"] + st["n_b"])
+            asm_b = 6336.0 * st["n_paired"] + 4224.0 * (st["n_out"] - st["n_paired"])
+            dev_ms = st["ms_flood"] + st["ms_assemble"]
+            r[name] = {"wall_ms": round(wall, 3), "flood_ms": round(st["ms_flood"], 4),
+                       "assemble_ms": round(st["ms_assemble"], 4), "leaves_a": st["n_a"], "leaves_b": st["n_b"],
+                       "leaves_out": st["n_out"], "paired": st["n_paired"],
+                       "flood_gbs": round(flood_b / (st["ms_flood"] * 1e6), 1) if st["ms_flood"] > 0 else None,
+                       "assemble_gbs": round(asm_b / (st["ms_assemble"] * 1e6), 1) if st["ms_assemble"] > 0 else None,
+                       "kernels_gbs": round((flood_b + asm_b) / (dev_ms * 1e6), 1) if dev_ms > 0 else None,
+                       "wall_gbs": round((flood_b + asm_b) / (wall * 1e6), 1)}
+            r["_" + name] = out
+        for name, src, dist in (("offset_S_+0.02", "_subtract", 0.02), ("offset_I_-0.01", "_intersect", -0.01)):
+            x = r[src].clone()
+            n_in = x.info()["n_leaves"]
+            t0 = time.perf_counter()
+            out = x.offset(dist)
+            wall = (time.perf_counter() - t0) * 1e3
+            st = ctx.last_offset_stats()
+            byts = 2176.0 * n_in + 4224.0 * st["n_swept"] + 4096.0 * st["n_out"]
+            r[name] = {"wall_ms": round(wall, 3), "device_ms": round(st["ms_device"], 3), "leaves_in": n_in,
+                       "leaves_swept": st["n_swept"], "leaves_out": st["n_out"], "planes": st["n_planes"],
+                       "launches": st["n_launches"],
+                       "device_gbs": round(byts / (st["ms_device"] * 1e6), 1) if st["ms_device"] > 0 else None}
+            r["_" + name] = out
+        t0 = time.perf_counter()
+        dm = bs.DualContouringMesher(ctx).with_voxel_size(h).mesh_device(r["_offset_S_+0.02"])
+        wall = (time.perf_counter() - t0) * 1e3
+        st = ctx.last_dc_stats()
+        r["dual_contour_S"] = {"wall_ms": round(wall, 3), "device_ms": round(st["ms_total"], 3),
+                               "leaves": st["n_leaves"], "triangles": st["n_tris"],
+                               "tris_per_s": round(st["n_tris"] / (st["ms_total"] * 1e-3)) if st["ms_total"] else None}
+        dm.free()
+        for k in [k for k in r if k.startswith("_")]:
+            r[k].free()
+        res = r
+    A.free()
+    B.free()
+    res = {k: v for k, v in res.items() if not k.startswith("_")}
+    res["workload"] = (f"two displaced icospheres L{level} ({a.shape[0]} tris each, second moved by 0.3 in x), "
+                       f"voxel {h}, band {band}")
+    return res
 
 
 def main():
@@ -65,6 +136,7 @@ def main():
     ap.add_argument("--band", type=int, default=3)
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-pipeline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU layout is a design "
@@ -144,6 +216,8 @@ def main():
         "phases_ms": {k: round(float(np.mean([p[k] for p in prof])), 4)
                       for k in ("ms_subdivide", "ms_leaves", "ms_distance", "ms_bvh", "ms_winding")},
     }
+    if not args.no_pipeline:
+        out["pipeline"] = pipeline(ctx)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(soup, args.voxel, args.band)
     print(json.dumps(out), flush=True)

@@ -73,7 +73,20 @@ typedef struct {
     uint64_t n_launches;  /* kernel launches of the sweeps                      */
     uint64_t n_capacity;  /* leaf pool capacity of the successful attempt       */
     uint64_t used_hash;   /* leaf map was a hash (box too large for a grid)     */
+    uint64_t n_swept;     /* leaf sweeps summed over the eight directions       */
+    double ms_device;     /* first to last kernel of the call (stream events)   */
+    double ms_wall;       /* the whole call on the host clock                    */
 } bs_offset_stats;
+
+/* Counters of the last CSG call (bs_volume_union/subtract/intersect). */
+typedef struct {
+    uint64_t n_a, n_b;        /* leaves of the two operands                   */
+    uint64_t n_out;           /* leaves of the result                         */
+    uint64_t n_paired;        /* result leaves combined from both operands    */
+    double ms_flood;          /* leaf flood fill of both operands (events)    */
+    double ms_assemble;       /* result assembly kernel (events)              */
+    double ms_wall;           /* the whole call on the host clock             */
+} bs_csg_stats;
 
 /* Counters of the last bs_dual_contour* call. */
 typedef struct {
@@ -151,6 +164,7 @@ bs_status bs_volume_from_voxels(bs_ctx* ctx, const int32_t* idx, const float* va
 bs_status bs_volume_union(bs_ctx* ctx, bs_volume* self, bs_volume* other);
 bs_status bs_volume_subtract(bs_ctx* ctx, bs_volume* self, bs_volume* other);
 bs_status bs_volume_intersect(bs_ctx* ctx, bs_volume* self, bs_volume* other);
+bs_status bs_ctx_last_csg_stats(const bs_ctx* ctx, bs_csg_stats* out);
 
 /* FloodFill::flood_fill of the whole grid (root_node/flood_fill.rs:8-42,
  * internal_node/flood_fill.rs:17-75, leaf_node/flood_fill.rs:12-50): inactive
