@@ -60,9 +60,9 @@ constexpr VoxelPlanes make_voxel_planes() {
 __constant__ VoxelPlanes kVoxelPlanes = make_voxel_planes();
 
 struct OffsetState {
-    uint32_t pool_count;  // leaves in the pool
-    uint32_t new_lo, new_hi;  // pool range created while sweeping the previous plane
-    uint32_t fail;        // bit 0: a leaf outside the reachable box, bit 1: pool or hash full
+    uint32_t pool_count;  // leaves in the pool before the sweeps
+    uint32_t fail;        // bit 0: a leaf outside the reachable box (bit 2: past the last plane,
+                          // bit 3: a pool leaf off the planes, else off the map), bit 1: pool or hash full
 };
 
 struct SweepArgs {
@@ -141,7 +141,7 @@ __global__ void k_plane_hist(uint32_t n, const int4* __restrict__ origins, int s
     if (i >= n) return;
     const int32_t k = leaf_plane(origins[i], sx, sy, sz) - kmin;
     if (k < 0 || (uint32_t)k >= n_planes) {
-        atomicOr(fail, 1u);
+        atomicOr(fail, 1u | 8u);
         return;
     }
     atomicAdd(&hist[k], 1u);
@@ -156,121 +156,91 @@ __global__ void k_plane_scatter(uint32_t n, const int4* __restrict__ origins, in
     order[atomicAdd(&cursor[k], 1u)] = i;
 }
 
-// Opens plane p: the leaves created while plane p - 1 was swept are the
-// pool range [new_lo, new_hi) (clamped to the pool; an overflow is retried).
-__global__ void k_plane_begin(OffsetState* st, uint32_t capacity) {
-    st->new_lo = st->new_hi;
-    st->new_hi = min(st->pool_count, capacity);
-}
-
 // ---------------------------------------------------------------- sweep
-// utils.rs:64-73 option_min_by under fast_sweep.rs:137-140's cmp_abs: the
-// first value unless the second has the strictly smaller magnitude.
-__device__ inline bool min_abs(bool ha, float a, bool hb, float b, float& out) {
-    if (ha && hb) {
-        out = (fabsf(a) > fabsf(b)) ? b : a;
-        return true;
-    }
-    if (ha) {
-        out = a;
-        return true;
-    }
-    if (hb) {
-        out = b;
-        return true;
-    }
-    return false;
+// The leaf and the six neighbour layers touching it in one padded 10^3 block
+// (leaf-local -1..8 per axis; edge and corner cells are never read), so a
+// stencil read is one unconditional LDS load.  st bit 0: active, bit 1: frozen.
+constexpr int PB = 10;
+__device__ inline int pidx(int x, int y, int z) { return ((x + 1) * PB + (y + 1)) * PB + (z + 1); }
+
+struct LeafLds {
+    float val[PB * PB * PB];
+    uint8_t st[PB * PB * PB];
+    int32_t nb[6];  // -x, +x, -y, +y, -z, +z neighbour slots
+};
+
+// option_min_by(at(+), at(-), cmp_abs) (utils.rs:64-73, fast_sweep.rs:137-140):
+// the + value unless the - value has the strictly smaller magnitude.
+__device__ inline float pick(bool hp, float p, bool hm, float m) {
+    return (hp && hm) ? (fabsf(p) > fabsf(m) ? m : p) : (hp ? p : m);
 }
 
-// fast_sweep.rs:509-537 with sort3 (helpers/utils.rs:4-16), f32 throughout
+// fast_sweep.rs:509-537 with sort3 (helpers/utils.rs:4-16), f32 throughout;
+// the three candidate solutions are all computed and one is selected, which
+// yields the same value as the reference's early returns.
 __device__ inline float compute_distance(float a1, float a2, float a3, float h) {
     float t;
     if (a1 > a3) t = a1, a1 = a3, a3 = t;
     if (a1 > a2) t = a1, a1 = a2, a2 = t;
     if (a2 > a3) t = a2, a2 = a3, a3 = t;
     const float s1 = a1 + h;
-    if (fabsf(s1) <= a2) return s1;
     const float a1a2_sum = a1 + a2;
     const float h_sq = h * h;
     const float two_h_sq = h_sq + h_sq;
     const float a1a2_diff = a1 - a2;
     const float a1a2_diff_sq = a1a2_diff * a1a2_diff;
     const float s2 = (a1a2_sum + sqrt_rn(two_h_sq - a1a2_diff_sq)) * 0.5f;
-    if (fabsf(s2) <= a3) return s2;
     const float a1a2a3_sum = a1a2_sum + a3;
     const float three_h_sq = two_h_sq + h_sq;
     const float a1a3_diff = a1 - a3;
     const float a1a3_diff_sq = a1a3_diff * a1a3_diff;
     const float a2a3_diff = a2 - a3;
     const float a2a3_diff_sq = a2a3_diff * a2a3_diff;
-    return (a1a2a3_sum + sqrt_rn(three_h_sq - a1a2_diff_sq - a1a3_diff_sq - a2a3_diff_sq)) * (1.0f / 3.0f);
+    const float s3 = (a1a2a3_sum + sqrt_rn(three_h_sq - a1a2_diff_sq - a1a3_diff_sq - a2a3_diff_sq)) * (1.0f / 3.0f);
+    return fabsf(s1) <= a2 ? s1 : (fabsf(s2) <= a3 ? s2 : s3);
 }
 
-struct LeafLds {
-    float val[512];
-    uint8_t act[512];
-    uint8_t frz[512];
-    int32_t nb[6];  // -x, +x, -y, +y, -z, +z neighbour slots
-};
-
-// Value of voxel (x, y, z) in leaf-local coordinates, one step outside the
-// leaf at most (fast_sweep.rs:359-387: the stencil's face neighbours).
-__device__ inline bool stencil_at(const LeafLds& L, const SweepArgs& a, int x, int y, int z, float& v) {
-    int k = -1;
-    if (z < 0) k = 4;
-    else if (z > 7) k = 5;
-    else if (y < 0) k = 2;
-    else if (y > 7) k = 3;
-    else if (x < 0) k = 0;
-    else if (x > 7) k = 1;
-    const int o = ((x & 7) << 6) | ((y & 7) << 3) | (z & 7);
-    if (k < 0) {
-        if (!L.act[o]) return false;
-        v = L.val[o];
-        return true;
-    }
-    const int32_t n = L.nb[k];
-    if (n < 0) return false;
-    if (!((a.masks[(uint64_t)n * 8 + (o >> 6)] >> (o & 63)) & 1ull)) return false;
-    v = a.values[(uint64_t)n * 512 + o];
-    return true;
-}
-
-// fast_sweep.rs:120-184
-__device__ inline void sweep_voxel(LeafLds& L, const SweepArgs& a, int x, int y, int z) {
-    const int o = (x << 6) | (y << 3) | z;
-    if (L.frz[o]) return;
-    float p, m, dx = 0.f, dy = 0.f, dz = 0.f;
-    bool hp = stencil_at(L, a, x + 1, y, z, p), hm = stencil_at(L, a, x - 1, y, z, m);
-    const bool hx = min_abs(hp, p, hm, m, dx);
-    hp = stencil_at(L, a, x, y + 1, z, p), hm = stencil_at(L, a, x, y - 1, z, m);
-    const bool hy = min_abs(hp, p, hm, m, dy);
-    hp = stencil_at(L, a, x, y, z + 1, p), hm = stencil_at(L, a, x, y, z - 1, m);
-    const bool hz = min_abs(hp, p, hm, m, dz);
-    if (!hx && !hy && !hz) return;
+// fast_sweep.rs:120-184 for the voxel at padded index o
+__device__ inline void sweep_voxel(LeafLds& L, const SweepArgs& a, int o) {
+    const uint8_t so = L.st[o];
+    const uint8_t sxp = L.st[o + PB * PB], sxm = L.st[o - PB * PB], syp = L.st[o + PB], sym = L.st[o - PB],
+                  szp = L.st[o + 1], szm = L.st[o - 1];
+    const float vxp = L.val[o + PB * PB], vxm = L.val[o - PB * PB], vyp = L.val[o + PB], vym = L.val[o - PB],
+                vzp = L.val[o + 1], vzm = L.val[o - 1], vo = L.val[o];
+    const bool hx = (sxp | sxm) & 1, hy = (syp | sym) & 1, hz = (szp | szm) & 1;
+    const float dx = pick(sxp & 1, vxp, sxm & 1, vxm);
+    const float dy = pick(syp & 1, vyp, sym & 1, vym);
+    const float dz = pick(szp & 1, vzp, szm & 1, vzm);
     const float first = hx ? dx : (hy ? dy : dz);
-    if ((int)(__float_as_uint(first) >> 31) != a.neg) return;
-    const float d1 = hx ? dx : FS_FAR, d2 = hy ? dy : FS_FAR, d3 = hz ? dz : FS_FAR;
-    const float d_abs = compute_distance(fabsf(d1), fabsf(d2), fabsf(d3), a.h);
-    if (d_abs > a.limit_abs) return;
-    const float d_old = L.act[o] ? L.val[o] : FS_FAR;
-    if (d_abs < fabsf(d_old)) {
+    const bool go = !(so & 2) && (hx || hy || hz) && (int)(__float_as_uint(first) >> 31) == a.neg;
+    const float d_abs = compute_distance(fabsf(hx ? dx : FS_FAR), fabsf(hy ? dy : FS_FAR), fabsf(hz ? dz : FS_FAR), a.h);
+    const float d_old = (so & 1) ? vo : FS_FAR;
+    if (go && !(d_abs > a.limit_abs) && d_abs < fabsf(d_old)) {
         L.val[o] = a.neg ? -fabsf(d_abs) : fabsf(d_abs);  // copysign(d_abs, sweep sign)
-        L.act[o] = 1;
+        L.st[o] = so | 1;
     }
 }
 
 // One leaf plane of one sweep: the leaves of bucket [b0, b1) of `order` plus
-// the leaves created while the previous plane was swept.  One 64-lane block
-// per leaf, looping over the plane's leaves.
+// the leaves created while the previous plane was swept.  starts[p + 1] is
+// the pool size when plane p begins and created[p + 1] the leaves plane p
+// creates (all on plane p + 1), so this plane sweeps the pool range
+// [starts[plane], starts[plane] + created[plane]) after its bucket and
+// allocates its own new leaves from starts[plane] + created[plane] on.
+// One 64-lane block per leaf, looping over the plane's leaves.
 __global__ __launch_bounds__(64) void k_sweep_plane(SweepArgs a, const uint32_t* __restrict__ order,
                                                      const uint32_t* __restrict__ bucket_off, uint32_t plane,
-                                                     uint32_t n_planes, OffsetState* st) {
+                                                     uint32_t n_planes, uint32_t* starts, uint32_t* created,
+                                                     OffsetState* st) {
     __shared__ LeafLds L;
     const uint32_t b0 = bucket_off[plane], b1 = bucket_off[plane + 1];
     const uint32_t n_old = b1 - b0;
-    const uint32_t new_lo = st->new_lo, n_new = st->new_hi - new_lo;
+    const uint32_t new_lo = starts[plane];
+    const uint32_t start = min(new_lo + created[plane], a.capacity);
+    const uint32_t n_new = start - new_lo;
     const int lane = threadIdx.x;
+    const int lx = lane >> 3, ly = lane & 7;  // this lane's z-row of the leaf
+    if (blockIdx.x == 0 && lane == 0) starts[plane + 1] = start;
     for (uint32_t item = blockIdx.x; item < n_old + n_new; item += gridDim.x) {
         const uint32_t slot = item < n_old ? order[b0 + item] : new_lo + (item - n_old);
         const int4 org = a.origins[slot];
@@ -278,13 +248,13 @@ __global__ __launch_bounds__(64) void k_sweep_plane(SweepArgs a, const uint32_t*
             const float4* vp = reinterpret_cast<const float4*>(a.values + (uint64_t)slot * 512 + lane * 8);
             const float4 q0 = vp[0], q1 = vp[1];
             const float v[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
-            const uint64_t mw = a.masks[(uint64_t)slot * 8 + (lane >> 3)] >> ((lane & 7) * 8);
-            const uint64_t fw = a.frozen[(uint64_t)slot * 8 + (lane >> 3)] >> ((lane & 7) * 8);
+            const uint64_t mw = a.masks[(uint64_t)slot * 8 + lx] >> (ly * 8);
+            const uint64_t fw = a.frozen[(uint64_t)slot * 8 + lx] >> (ly * 8);
+            const int base = pidx(lx, ly, 0);
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                L.val[lane * 8 + k] = v[k];
-                L.act[lane * 8 + k] = (mw >> k) & 1;
-                L.frz[lane * 8 + k] = (fw >> k) & 1;
+                L.val[base + k] = v[k];
+                L.st[base + k] = (uint8_t)(((mw >> k) & 1) | (((fw >> k) & 1) << 1));
             }
             if (lane < 6) {
                 const int ax = lane >> 1, dir = (lane & 1) ? 1 : -1;
@@ -293,42 +263,62 @@ __global__ __launch_bounds__(64) void k_sweep_plane(SweepArgs a, const uint32_t*
             }
         }
         __syncthreads();
+        // the six neighbour layers next to this leaf, into the padding
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int32_t n = L.nb[k];
+            const int i = lx, j = ly;
+            const int f = (k & 1) ? 0 : 7;  // -side neighbour: its far layer 7; +side: layer 0
+            const int o = k < 2 ? ((f << 6) | (i << 3) | j) : (k < 4 ? ((i << 6) | (f << 3) | j) : ((i << 6) | (j << 3) | f));
+            const int g = (k & 1) ? 8 : -1;  // the padding layer this maps to
+            const int po = k < 2 ? pidx(g, i, j) : (k < 4 ? pidx(i, g, j) : pidx(i, j, g));
+            uint8_t on = 0;
+            float v = 0.0f;
+            if (n >= 0) {
+                on = (a.masks[(uint64_t)n * 8 + (o >> 6)] >> (o & 63)) & 1ull;
+                if (on) v = a.values[(uint64_t)n * 512 + o];
+            }
+            L.st[po] = on;
+            L.val[po] = v;
+        }
+        __syncthreads();
         for (int p = 0; p < 22; ++p) {
             const int s0 = kVoxelPlanes.start[p], cnt = kVoxelPlanes.start[p + 1] - s0;
             if (lane < cnt) {
                 const int e = kVoxelPlanes.entry[s0 + lane];
                 const int u = e >> 6, v = (e >> 3) & 7, w = e & 7;
-                sweep_voxel(L, a, a.sx > 0 ? u : 7 - u, a.sy > 0 ? v : 7 - v, a.sz > 0 ? w : 7 - w);
+                sweep_voxel(L, a, pidx(a.sx > 0 ? u : 7 - u, a.sy > 0 ? v : 7 - v, a.sz > 0 ? w : 7 - w));
             }
             __syncthreads();
         }
         // write back
         {
+            const int base = pidx(lx, ly, 0);
             float4* vp = reinterpret_cast<float4*>(a.values + (uint64_t)slot * 512 + lane * 8);
-            vp[0] = make_float4(L.val[lane * 8 + 0], L.val[lane * 8 + 1], L.val[lane * 8 + 2], L.val[lane * 8 + 3]);
-            vp[1] = make_float4(L.val[lane * 8 + 4], L.val[lane * 8 + 5], L.val[lane * 8 + 6], L.val[lane * 8 + 7]);
+            vp[0] = make_float4(L.val[base + 0], L.val[base + 1], L.val[base + 2], L.val[base + 3]);
+            vp[1] = make_float4(L.val[base + 4], L.val[base + 5], L.val[base + 6], L.val[base + 7]);
+            uint32_t row = 0;
 #pragma unroll
-            for (int w = 0; w < 8; ++w) {
-                const uint64_t word = __ballot(L.act[w * 64 + lane] != 0);
-                if (lane == 0) a.masks[(uint64_t)slot * 8 + w] = word;
-            }
+            for (int k = 0; k < 8; ++k) row |= (uint32_t)(L.st[base + k] & 1) << k;
+            const uint64_t w = gather_word(row, ly);
+            if (ly == 0) a.masks[(uint64_t)slot * 8 + lx] = w;
         }
         // fast_sweep.rs:186-273: the +direction face of each axis
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
             const int dir = ax == 0 ? a.sx : (ax == 1 ? a.sy : a.sz);
             const int f = dir > 0 ? 7 : 0;
-            const int i = lane >> 3, j = lane & 7;
-            const int o = ax == 0 ? ((f << 6) | (i << 3) | j) : (ax == 1 ? ((i << 6) | (f << 3) | j) : ((i << 6) | (j << 3) | f));
+            const int i = lx, j = ly;
+            const int o = ax == 0 ? pidx(f, i, j) : (ax == 1 ? pidx(i, f, j) : pidx(i, j, f));
             const float v = L.val[o];
-            const bool q = L.act[o] && (int)(__float_as_uint(v) >> 31) == a.neg && fabsf(v) < a.limit_abs;
+            const bool q = (L.st[o] & 1) && (int)(__float_as_uint(v) >> 31) == a.neg && fabsf(v) < a.limit_abs;
             if (__ballot(q) != 0 && lane == 0) {
-                if (plane + 1 >= n_planes) atomicOr(&st->fail, 1u);
+                if (plane + 1 >= n_planes) atomicOr(&st->fail, 1u | 4u);
                 const int32_t nx = (org.x >> 3) + (ax == 0 ? dir : 0), ny = (org.y >> 3) + (ax == 1 ? dir : 0),
                               nz = (org.z >> 3) + (ax == 2 ? dir : 0);
                 int32_t* cell = a.map.claim(nx, ny, nz, &st->fail);
                 if (cell) {
-                    const uint32_t s = atomicAdd(&st->pool_count, 1u);
+                    const uint32_t s = start + atomicAdd(&created[plane + 1], 1u);
                     if (s < a.capacity) {
                         a.origins[s] = make_int4(nx * 8, ny * 8, nz * 8, 0);
                         __threadfence();
@@ -532,7 +522,7 @@ OffsetResult offset_attempt(bs_ctx* ctx, bs_volume* vol, float distance, uint64_
     BS_HIP(hipMemsetAsync(a.frozen, 0, sizeof(uint64_t) * 8 * capacity, st));
     BS_HIP(hipMemsetAsync(a.values + 512 * n_keep, 0, sizeof(float) * 512 * (capacity - n_keep), st));
     OffsetState* dstate = B.alloc<OffsetState>(1);
-    OffsetState init{(uint32_t)n_keep, (uint32_t)n_keep, (uint32_t)n_keep, 0};
+    OffsetState init{(uint32_t)n_keep, 0};
     BS_HIP(hipMemcpyAsync(dstate, &init, sizeof(init), hipMemcpyHostToDevice, st));
     k_pool_init<<<div_up(n, 4), 256, 0, st>>>(n, keep, dst, vol->origins, rm_masks, vol->values, a, &dstate->fail);
     BS_HIP(hipGetLastError());
@@ -540,19 +530,19 @@ OffsetResult offset_attempt(bs_ctx* ctx, bs_volume* vol, float distance, uint64_
     // plane range over the whole map box
     uint32_t* order = B.alloc<uint32_t>(capacity);
     uint64_t planes_total = 0, launches = 0, swept = 0;
+    uint32_t pool = (uint32_t)n_keep;  // pool size at the start of the sweep
     for (int s = 0; s < 8; ++s) {
         a.sx = kDirs[s][0], a.sy = kDirs[s][1], a.sz = kDirs[s][2];
         const OffsetState cur = read_back(ctx, dstate);
-        BS_REQUIRE(!(cur.fail & 1u), BS_ERR_INTERNAL, "offset: a pushed leaf left the reachable box");
+        BS_REQUIRE(!(cur.fail & 1u), BS_ERR_INTERNAL,
+                   "offset: a pushed leaf left the reachable box (flags " + std::to_string(cur.fail) + ", sweep " +
+                       std::to_string(s) + ", pool " + std::to_string(pool) + ")");
         if (cur.fail) {
             res.retry = true;
             return res;
         }
-        const uint32_t n_pool = cur.pool_count;
+        const uint32_t n_pool = pool;
         if (s > 0) swept += n_pool;  // every leaf in the pool at the end of a sweep was swept in it
-        // nothing was created before the first plane: new_lo = new_hi = pool_count
-        const uint32_t reset[3] = {n_pool, n_pool, n_pool};
-        BS_HIP(hipMemcpyAsync(dstate, reset, sizeof(reset), hipMemcpyHostToDevice, st));
         // planes of the map box in this direction
         // planes of the reachable box (the leaves plus the extension margin)
         int32_t kmin = 0, kmax = 0;
@@ -588,27 +578,50 @@ OffsetResult offset_attempt(bs_ctx* ctx, bs_volume* vol, float distance, uint64_
             max_bucket = std::max(max_bucket, c);
         }
         if (!any) break;
-        const unsigned grid = (unsigned)std::min<uint64_t>(std::max<uint32_t>(max_bucket, 64) * 2, 4096);
+        (void)max_bucket;
+        // starts[p + 1], created[p + 1]: pool size when plane p begins, leaves it creates
+        uint32_t* starts = B.alloc<uint32_t>(n_planes + 2);
+        uint32_t* created = B.alloc<uint32_t>(n_planes + 2);
+        BS_HIP(hipMemsetAsync(created, 0, sizeof(uint32_t) * (n_planes + 2), st));
+        BS_HIP(hipMemcpyAsync(starts + first, &pool, sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        uint32_t p_end = first;
         for (uint32_t p = first; p < n_planes; ++p) {
+            const uint32_t bucket = hoff[p + 1] - hoff[p];
             if (p > last) {
                 // past the last bucket: continue only while the previous plane created leaves
-                const OffsetState t = read_back(ctx, dstate);
-                if (t.pool_count == t.new_hi) break;
+                uint32_t c = 0;
+                BS_HIP(hipMemcpyAsync(&c, created + p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+                BS_HIP(hipStreamSynchronize(st));
+                if (c == 0) break;
             }
-            k_plane_begin<<<1, 1, 0, st>>>(dstate, a.capacity);
-            k_sweep_plane<<<grid, 64, 0, st>>>(a, order, off, p, n_planes, dstate);
+            // enough blocks for the bucket and the leaves the previous plane may have created
+            const uint32_t prev = p > first ? hoff[p] - hoff[p - 1] : 0;
+            const unsigned grid = (unsigned)std::min<uint64_t>(std::max<uint64_t>((uint64_t)bucket + prev + 64, 64), 4096);
+            k_sweep_plane<<<grid, 64, 0, st>>>(a, order, off, p, n_planes, starts, created, dstate);
             BS_HIP(hipGetLastError());
             ++planes_total;
-            launches += 2;
+            ++launches;
+            p_end = p;
+        }
+        // pool size after the last plane swept
+        uint32_t tail[2] = {0, 0};
+        BS_HIP(hipMemcpyAsync(&tail[0], starts + p_end + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        BS_HIP(hipMemcpyAsync(&tail[1], created + p_end + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        BS_HIP(hipStreamSynchronize(st));
+        pool = std::min<uint32_t>(tail[0] + tail[1], (uint32_t)capacity);
+        if ((uint64_t)tail[0] + tail[1] > capacity) {
+            res.retry = true;
+            return res;
         }
     }
     const OffsetState fin = read_back(ctx, dstate);
-    BS_REQUIRE(!(fin.fail & 1u), BS_ERR_INTERNAL, "offset: a pushed leaf left the reachable box");
+    BS_REQUIRE(!(fin.fail & 1u), BS_ERR_INTERNAL,
+               "offset: a pushed leaf left the reachable box (flags " + std::to_string(fin.fail) + ", after the sweeps)");
     if (fin.fail) {
         res.retry = true;
         return res;
     }
-    const uint32_t n_pool = fin.pool_count;
+    const uint32_t n_pool = pool;
     swept += n_pool;
     if (stats) {
         stats->n_swept = swept;
