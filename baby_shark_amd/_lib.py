@@ -103,6 +103,9 @@ def load():
             "bs_m2v_params_default": (None, [C.POINTER(M2VParams)]),
             "bs_mesh_to_volume": (C.c_int, [vp, f32p, C.c_size_t, C.POINTER(M2VParams), pp]),
             "bs_mesh_to_volume_device": (C.c_int, [vp, vp, C.c_size_t, C.POINTER(M2VParams), pp]),
+            "bs_batch_mesh_to_volume": (C.c_int, [vp, C.c_size_t, C.POINTER(vp), C.POINTER(C.c_size_t),
+                                                  C.POINTER(M2VParams), C.c_int, C.c_int, pp,
+                                                  C.POINTER(C.c_int)]),
             "bs_volume_clone": (C.c_int, [vp, pp]),
             "bs_volume_free": (None, [vp]),
             "bs_volume_info_get": (C.c_int, [vp, C.POINTER(VolumeInfo)]),

@@ -2,6 +2,9 @@
 // the canonical host export of volumes.
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
+#include <atomic>
+#include <thread>
 #include <string>
 
 #include "common.hpp"
@@ -95,6 +98,12 @@ void bs_ctx_destroy(bs_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     ctx->scratch.release();
+    for (auto& w : ctx->workers) {
+        (void)hipStreamSynchronize(w->stream);
+        w->scratch.release();
+        (void)hipStreamDestroy(w->stream);
+    }
+    ctx->workers.clear();
     (void)hipStreamDestroy(ctx->stream);
     if (prev >= 0) (void)hipSetDevice(prev);
     delete ctx;
@@ -155,6 +164,74 @@ bs_status bs_mesh_to_volume(bs_ctx* ctx, const float* tri_xyz, size_t n, const b
         BS_HIP(hipMemcpyAsync(d, tri_xyz, sizeof(float) * 9 * n, hipMemcpyHostToDevice, ctx->stream));
         mesh_to_volume_device(ctx, d, n, *p, out);
     });
+}
+
+bs_status bs_batch_mesh_to_volume(bs_ctx* ctx, size_t n_meshes, const float* const* tri_xyz, const size_t* n_tris,
+                                  const bs_m2v_params* params, int host_input, int n_workers, bs_volume** out,
+                                  bs_status* status) {
+    if (!ctx || (n_meshes > 0 && (!tri_xyz || !n_tris || !params || !out))) {
+        g_last_error = "null argument";
+        return BS_ERR_INVALID_ARG;
+    }
+    for (size_t i = 0; i < n_meshes; ++i) {
+        out[i] = nullptr;
+        if (status) status[i] = BS_OK;
+    }
+    if (n_workers <= 0) n_workers = 4;
+    n_workers = (int)std::min<size_t>((size_t)std::min(n_workers, 16), std::max<size_t>(n_meshes, 1));
+    std::vector<bs_status> st(n_meshes, BS_OK);
+    std::vector<std::string> msg(n_meshes);
+    const bs_status setup = guarded([&] {
+        DeviceGuard g(ctx->device);
+        while ((int)ctx->workers.size() < n_workers) {
+            auto w = std::make_unique<bs_ctx>();
+            w->device = ctx->device;
+            BS_HIP(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+            ctx->workers.push_back(std::move(w));
+        }
+    });
+    if (setup != BS_OK) return setup;
+    // largest meshes first: the arenas reach their size early and the tail is short
+    std::vector<size_t> order(n_meshes);
+    for (size_t i = 0; i < n_meshes; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return n_tris[a] > n_tris[b]; });
+    std::atomic<size_t> next{0};
+    auto work = [&](int wi) {
+        bs_ctx* w = ctx->workers[wi].get();
+        w->paths = ctx->paths;
+        w->profiling = false;
+        for (size_t k; (k = next.fetch_add(1)) < n_meshes;) {
+            const size_t i = order[k];
+            st[i] = guarded([&] {
+                DeviceGuard g(w->device);
+                if (n_tris[i] == 0) throw Error(BS_ERR_EMPTY, "empty mesh: the reference returns None");
+                BS_REQUIRE(tri_xyz[i] != nullptr, BS_ERR_INVALID_ARG, "null mesh pointer");
+                const float* d = tri_xyz[i];
+                if (host_input) {
+                    float* buf = w->scratch.as<float>(S_INPUT, 9 * (uint64_t)n_tris[i]);
+                    BS_HIP(hipMemcpyAsync(buf, tri_xyz[i], sizeof(float) * 9 * n_tris[i], hipMemcpyHostToDevice,
+                                          w->stream));
+                    d = buf;
+                }
+                mesh_to_volume_device(w, d, n_tris[i], params[i], &out[i]);
+            });
+            if (st[i] != BS_OK) msg[i] = g_last_error;
+        }
+    };
+    std::vector<std::thread> threads;
+    for (int wi = 1; wi < n_workers; ++wi) threads.emplace_back(work, wi);
+    work(0);
+    for (auto& t : threads) t.join();
+    bs_status first = BS_OK;
+    for (size_t i = 0; i < n_meshes; ++i) {
+        if (status) status[i] = st[i];
+        if (st[i] != BS_OK && st[i] != BS_ERR_EMPTY && first == BS_OK) {
+            first = st[i];
+            g_last_error = "mesh " + std::to_string(i) + ": " + msg[i];
+        }
+    }
+    if (first == BS_OK) g_last_error.clear();
+    return first;
 }
 
 bs_status bs_volume_clone(const bs_volume* v, bs_volume** out) {

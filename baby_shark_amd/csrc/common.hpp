@@ -86,6 +86,7 @@ struct bs_ctx {
     bs_offset_stats last_offset = {};
     bs_dc_stats last_dc = {};
     bs_csg_stats last_csg = {};
+    std::vector<std::unique_ptr<bs_ctx>> workers;  // sub-contexts of batched calls (own stream, scratch)
     uint64_t* host_counters = nullptr;  // pinned, for small readbacks
 };
 

@@ -3,12 +3,33 @@
 #include "topology.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <thread>
 #include <numeric>
 
 namespace bs {
 
 namespace {
+
+// Runs f(i) for i in [0, n) on up to 16 host threads (serially when small).
+template <class F>
+void parallel_for(size_t n, F&& f) {
+    const size_t hw = std::max<size_t>(1, std::min<size_t>(16, std::thread::hardware_concurrency()));
+    const size_t T = std::min(hw, n / 8);
+    if (T <= 1) {
+        for (size_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+        for (size_t i; (i = next.fetch_add(1, std::memory_order_relaxed)) < n;) f(i);
+    };
+    std::vector<std::thread> pool;
+    for (size_t k = 1; k < T; ++k) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+}
 
 constexpr int32_t L1_SPAN = 4096;  // voxels covered by a Node1
 constexpr int32_t L2_SPAN = 128;   // voxels covered by a Node2
@@ -97,13 +118,18 @@ void fill_node(SlotNode<LOG2>& n, ChildFirst child_first, ChildLast child_last) 
 std::unique_ptr<Topology> Topology::from_leaves(const std::vector<std::array<int32_t, 3>>& origins,
                                                 const std::vector<TileRec>& tiles) {
     auto t = std::make_unique<Topology>();
+    Key last_key{INT32_MIN, 0, 0};
+    int32_t last_n1 = -1;
     auto node1_for = [&](int32_t x, int32_t y, int32_t z) -> Node1& {
         Key k{x & ~(L1_SPAN - 1), y & ~(L1_SPAN - 1), z & ~(L1_SPAN - 1)};
+        if (k == last_key) return t->n1[last_n1];
         auto it = t->root.find(k);
         if (it == t->root.end()) {
             t->n1.emplace_back(k[0], k[1], k[2]);
             it = t->root.emplace(k, (int32_t)t->n1.size() - 1).first;
         }
+        last_key = k;
+        last_n1 = it->second;
         return t->n1[it->second];
     };
     auto node2_for = [&](int32_t x, int32_t y, int32_t z) -> Node2& {
@@ -154,10 +180,11 @@ std::unique_ptr<Topology> Topology::from_leaves(const std::vector<std::array<int
 void Topology::flood_fill_upper() {
     auto leaf_first = [&](int32_t li) { return leaves[li].first_neg != 0; };
     auto leaf_last = [&](int32_t li) { return leaves[li].last_neg != 0; };
-    for (auto& n : n2) fill_node(n, leaf_first, leaf_last);
+    // nodes of one level are independent: each reads its children, writes itself
+    parallel_for(n2.size(), [&](size_t i) { fill_node(n2[i], leaf_first, leaf_last); });
     auto n2_first = [&](int32_t k) { return first_neg(n2[k], leaf_first); };
     auto n2_last = [&](int32_t k) { return last_neg(n2[k], leaf_first); };
-    for (auto& n : n1) fill_node(n, n2_first, n2_last);
+    parallel_for(n1.size(), [&](size_t i) { fill_node(n1[i], n2_first, n2_last); });
     // root_node/flood_fill.rs:14-41: negative tiles between two negative
     // children on one z line
     std::vector<Key> keys;
@@ -250,7 +277,7 @@ struct Csg {
                     continue;
                 }
                 if (s.kind[o] == SLOT_BRANCH && o_.kind[o] == SLOT_BRANCH)
-                    if (pair(s.child[o], o_.child[o])) make_inside(s, o);
+                    if (pair(s.child[o], o_.child[o], o)) make_inside(s, o);
             } else if (op == CSG_SUBTRACT) {
                 if (outside(s, o) || outside(o_, o)) continue;
                 if (inside(o_, o)) {
@@ -263,7 +290,7 @@ struct Csg {
                     else if (s.kind[o] == SLOT_TILE) s.tile[o] = -s.tile[o];
                     continue;
                 }
-                if (s.kind[o] == SLOT_BRANCH && o_.kind[o] == SLOT_BRANCH) pair(s.child[o], o_.child[o]);
+                if (s.kind[o] == SLOT_BRANCH && o_.kind[o] == SLOT_BRANCH) pair(s.child[o], o_.child[o], o);
             } else {
                 if (outside(s, o) || inside(o_, o)) continue;
                 if (outside(o_, o)) {
@@ -274,14 +301,14 @@ struct Csg {
                     take(s, o_, o);
                     continue;
                 }
-                if (s.kind[o] == SLOT_BRANCH && o_.kind[o] == SLOT_BRANCH) pair(s.child[o], o_.child[o]);
+                if (s.kind[o] == SLOT_BRANCH && o_.kind[o] == SLOT_BRANCH) pair(s.child[o], o_.child[o], o);
             }
         }
     }
 
     bool node2_pair(int32_t a, int32_t b) {
         slots(t.n2[a], t.n2[b],
-              [&](int32_t la, int32_t lb) {
+              [&](int32_t la, int32_t lb, int) {
                   leaf_pair(la, lb);
                   return t.leaves[la].empty != 0;
               },
@@ -289,9 +316,25 @@ struct Csg {
         return node2_empty(a);
     }
 
+    // The 16^3-node pairs of a 32^3 node touch disjoint slots and leaves, so
+    // they are collected during the slot pass and combined in parallel; a
+    // pair is the last step of its slot, so union's make_child_inside of an
+    // emptied pair is applied afterwards with the same effect.
     void node1_pair(int32_t a, int32_t b) {
-        slots(t.n1[a], t.n1[b], [&](int32_t ka, int32_t kb) { return node2_pair(ka, kb); },
+        std::vector<std::array<int32_t, 3>> pairs;  // (slot, node of self, node of other)
+        slots(t.n1[a], t.n1[b],
+              [&](int32_t ka, int32_t kb, int o) {
+                  pairs.push_back({o, ka, kb});
+                  return false;
+              },
               [&](int32_t k) { flip_node2(k); });
+        Node1& s = t.n1[a];
+        std::vector<uint8_t> empty(pairs.size(), 0);
+        parallel_for(pairs.size(), [&](size_t i) { empty[i] = node2_pair(pairs[i][1], pairs[i][2]); });
+        if (op == CSG_UNION) {
+            for (size_t i = 0; i < pairs.size(); ++i)
+                if (empty[i]) make_inside(s, pairs[i][0]);
+        }
     }
 };
 
@@ -454,17 +497,16 @@ std::unique_ptr<Topology> Topology::after_offset(const std::vector<uint32_t>& ke
     return t->compact(keep_leaf);
 }
 
-std::unique_ptr<Topology> Topology::compact(const std::vector<int32_t>& new_index) const {
+std::unique_ptr<Topology> Topology::compact(const std::vector<int32_t>& new_index) {
     auto t = std::make_unique<Topology>();
     int32_t max_leaf = -1;
     for (int32_t v : new_index) max_leaf = std::max(max_leaf, v);
     t->leaves.resize((size_t)(max_leaf + 1));
     for (const auto& kv : root) {
-        const Node1& a = n1[kv.second];
-        Node1 na = a;
+        Node1 na = std::move(n1[kv.second]);
         for (int o1 = 0; o1 < Node1::SIZE; ++o1) {
-            if (a.kind[o1] != SLOT_BRANCH) continue;
-            Node2 nb = n2[a.child[o1]];
+            if (na.kind[o1] != SLOT_BRANCH) continue;
+            Node2 nb = std::move(n2[na.child[o1]]);
             for (int o2 = 0; o2 < Node2::SIZE; ++o2) {
                 if (nb.kind[o2] != SLOT_BRANCH) continue;
                 const int32_t r = nb.child[o2];
@@ -483,6 +525,10 @@ std::unique_ptr<Topology> Topology::compact(const std::vector<int32_t>& new_inde
         t->n1.push_back(std::move(na));
         t->root[kv.first] = (int32_t)t->n1.size() - 1;
     }
+    root.clear();
+    n1.clear();
+    n2.clear();
+    leaves.clear();
     return t;
 }
 

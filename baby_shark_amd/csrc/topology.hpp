@@ -92,8 +92,9 @@ struct Topology {
                                            const std::vector<std::array<int32_t, 3>>& out_origins, float two_h) const;
 
     // The reachable part of the tree with leaf reference r renumbered to
-    // new_index[r] (a plain copy of the device leaf at that index).
-    std::unique_ptr<Topology> compact(const std::vector<int32_t>& new_index) const;
+    // new_index[r] (a plain copy of the device leaf at that index).  Moves the
+    // nodes out: this topology is empty afterwards.
+    std::unique_ptr<Topology> compact(const std::vector<int32_t>& new_index);
 };
 
 }  // namespace bs

@@ -371,11 +371,36 @@ void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op) {
     std::vector<int32_t> refs;
     std::vector<TileRec> tiles;
     ta.collect(refs, tiles);
-    // output leaves in lexicographic origin order, as every volume here keeps them
-    std::sort(refs.begin(), refs.end(), [&](int32_t p, int32_t q) {
-        const LeafRef &u = ta.leaves[p], &w = ta.leaves[q];
-        return std::tie(u.x, u.y, u.z) < std::tie(w.x, w.y, w.z);
-    });
+    // output leaves in lexicographic origin order, as every volume here keeps them:
+    // an LSD radix sort of (63-bit leaf key, ref) in four 16-bit digits
+    {
+        const size_t n = refs.size();
+        std::vector<uint64_t> key(n), key2(n);
+        std::vector<int32_t> tmp(n);
+        for (size_t i = 0; i < n; ++i) {
+            const LeafRef& u = ta.leaves[refs[i]];
+            key[i] = leaf_key(u.x >> 3, u.y >> 3, u.z >> 3);
+        }
+        std::vector<uint32_t> cnt((size_t)1 << 16);
+        for (int pass = 0; pass < 4; ++pass) {
+            const int sh = 16 * pass;
+            std::fill(cnt.begin(), cnt.end(), 0u);
+            for (size_t i = 0; i < n; ++i) ++cnt[(key[i] >> sh) & 0xFFFF];
+            uint32_t sum = 0;
+            for (auto& c : cnt) {
+                const uint32_t v = c;
+                c = sum;
+                sum += v;
+            }
+            for (size_t i = 0; i < n; ++i) {
+                const uint32_t d = cnt[(key[i] >> sh) & 0xFFFF]++;
+                key2[d] = key[i];
+                tmp[d] = refs[i];
+            }
+            key.swap(key2);
+            refs.swap(tmp);
+        }
+    }
     const uint64_t n_out = refs.size();
     std::vector<int32_t> new_index(ta.leaves.size(), -1);
     std::vector<int4> host(2 * n_out);

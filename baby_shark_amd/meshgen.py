@@ -162,3 +162,25 @@ def sphere_soup(level: int, radius: float = 1.0, center=(0.0, 0.0, 0.0)) -> np.n
 def signed_volume(soup: np.ndarray) -> float:
     s = soup.astype(np.float64)
     return float(np.einsum("ij,ij->i", s[:, 0], np.cross(s[:, 1], s[:, 2])).sum() / 6.0)
+
+
+def batch_meshes(count: int = 256, seed: int = 5, lo: float = 1e3, hi: float = 2e5, span_voxels: int = 256):
+    """The uneven batch: `count` closed meshes whose triangle counts are drawn
+    log-uniform in [lo, hi], alternating tori (2 * major * minor triangles,
+    major = 2 * minor, noise 0.01) and displaced icospheres (the level nearest
+    the drawn count, 3..6), each with the voxel size that makes its largest
+    bounding-box extent span `span_voxels` voxels.  Returns [(soup, h)]."""
+    u = uniform01(seed, 2 * count)
+    out = []
+    for i in range(count):
+        target = lo * (hi / lo) ** u[2 * i]
+        s = seed * 1000 + i
+        if i % 2 == 0:
+            minor = max(3, int(round((target / 4.0) ** 0.5)))
+            soup = torus(1.0, 0.35, 2 * minor, minor, 0.01, seed=s)
+        else:
+            level = int(min(6, max(3, round(np.log(target / 20.0) / np.log(4.0)))))
+            soup = displaced_icosphere(level, 1.0, 0.05, seed=s)
+        ext = float((soup.reshape(-1, 3).max(0) - soup.reshape(-1, 3).min(0)).max())
+        out.append((soup, ext / span_voxels))
+    return out

@@ -250,6 +250,45 @@ def volume_from_voxels(idx, values, voxel_size: float, ctx: Optional[Context] = 
     return Volume(h, ctx)
 
 
+def batch_mesh_to_volume(meshes, voxel_sizes, band: int, ctx: Optional[Context] = None, workers: int = 0):
+    """MeshToVolume::convert over many meshes in one library call.
+
+    `meshes`: CUDA tensors (device input) or arrays (host input), all of one
+    kind; `voxel_sizes`: one per mesh.  Returns a list of Volume or None
+    (empty mesh), in input order."""
+    ctx = ctx or default_context()
+    L = load()
+    n = len(meshes)
+    dev = n > 0 and hasattr(meshes[0], "is_cuda") and meshes[0].is_cuda
+    keep, ptrs, counts = [], (C.c_void_p * n)(), (C.c_size_t * n)()
+    params = (_lib.M2VParams * n)()
+    for i, m in enumerate(meshes):
+        if dev:
+            import torch
+            t = m.detach().reshape(-1, 9)
+            if t.dtype != torch.float32 or not t.is_contiguous():
+                t = t.to(torch.float32).contiguous()
+            keep.append(t)
+            ptrs[i] = t.data_ptr()
+            counts[i] = t.shape[0]
+        else:
+            a = np.ascontiguousarray(np.asarray(m, dtype=np.float32).reshape(-1, 9))
+            keep.append(a)
+            ptrs[i] = a.ctypes.data
+            counts[i] = a.shape[0]
+        L.bs_m2v_params_default(C.byref(params[i]))
+        params[i].voxel_size = float(voxel_sizes[i])
+        params[i].band = int(band)
+    if dev:
+        import torch
+        torch.cuda.synchronize()
+    out = (C.c_void_p * n)()
+    status = (C.c_int * n)()
+    check(L.bs_batch_mesh_to_volume(ctx.handle, n, ptrs, counts, params, 0 if dev else 1, int(workers), out,
+                                    status))
+    return [Volume(C.c_void_p(out[i]), ctx) if out[i] else None for i in range(n)]
+
+
 class DualContouringMesher:
     """meshing/dual_contouring.rs:12-83.  Default voxel size 1.0."""
 

@@ -126,6 +126,37 @@ def pipeline(ctx, level=7, h=0.001, band=3, reps=2):
     return res
 
 
+def batch(ctx, count=256, seed=5):
+    """The uneven batch: 256 closed meshes (tori and displaced icospheres)
+    with triangle counts log-uniform in [1e3, 2e5], each voxelized so its
+    bounding box spans 256 voxels, band 3, soups resident in HBM.  Timed as
+    one bs_batch_mesh_to_volume call over 4 streams, and as the same call on
+    1 stream (the per-mesh loop a caller would write)."""
+    import torch
+    import baby_shark_amd as bs
+    from baby_shark_amd import meshgen as mg
+    ms = mg.batch_meshes(count=count, seed=seed)
+    soups = [torch.from_numpy(s).cuda() for s, _ in ms]
+    hs = [h for _, h in ms]
+    torch.cuda.synchronize()
+    res = {"workload": f"{count} meshes (tori / displaced icospheres), {sum(s.shape[0] for s, _ in ms)} triangles "
+                       f"in all, {min(s.shape[0] for s, _ in ms)}-{max(s.shape[0] for s, _ in ms)} each, "
+                       "bounding box 256 voxels, band 3"}
+    for name, workers in (("streams_4", 4), ("streams_1", 1)):
+        best = None
+        for _ in range(2):
+            t0 = time.perf_counter()
+            vols = bs.batch_mesh_to_volume(soups, hs, 3, ctx=ctx, workers=workers)
+            dt = time.perf_counter() - t0
+            n_act = sum(v.info()["n_active"] for v in vols if v is not None)
+            for v in vols:
+                if v is not None:
+                    v.free()
+            best = dt if best is None else min(best, dt)
+        res[name] = {"ms": round(best * 1e3, 2), "active_voxels": n_act, "voxels_per_s": round(n_act / best)}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +168,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pipeline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU layout is a design "
@@ -218,6 +250,8 @@ def main():
     }
     if not args.no_pipeline:
         out["pipeline"] = pipeline(ctx)
+    if not args.no_batch:
+        out["batch"] = batch(ctx)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(soup, args.voxel, args.band)
     print(json.dumps(out), flush=True)

@@ -124,6 +124,20 @@ bs_status bs_mesh_to_volume(bs_ctx* ctx, const float* tri_xyz, size_t n, const b
 bs_status bs_mesh_to_volume_device(bs_ctx* ctx, const float* tri_xyz_dev, size_t n, const bs_m2v_params* p,
                                    bs_volume** out);
 
+/* Many MeshToVolume::convert calls in one: mesh i is tri_xyz[i] with n_tris[i]
+ * triangles (device pointers, or host pointers when host_input != 0) and
+ * params[i].  The meshes are spread over n_workers streams of this device
+ * (0 = 4), each with its own scratch arena and host thread, largest first, so
+ * small meshes' kernels overlap.  out[i] receives the volume or NULL;
+ * status[i] (nullable) the per-mesh status (BS_ERR_EMPTY where convert
+ * returns None).  Returns BS_OK when every mesh gave BS_OK or BS_ERR_EMPTY,
+ * else the first other status.  Results equal per-mesh bs_mesh_to_volume*
+ * calls bit for bit.  Config-4 batch of the workload definitions; the
+ * reference has no batched entry point (callers loop over convert). */
+bs_status bs_batch_mesh_to_volume(bs_ctx* ctx, size_t n_meshes, const float* const* tri_xyz, const size_t* n_tris,
+                                  const bs_m2v_params* params, int host_input, int n_workers, bs_volume** out,
+                                  bs_status* status);
+
 /* Volume::clone (volume/mod.rs:116-123) and drop. */
 bs_status bs_volume_clone(const bs_volume* v, bs_volume** out);
 void bs_volume_free(bs_volume* v);

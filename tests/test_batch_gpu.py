@@ -1,0 +1,52 @@
+"""The batched mesh-to-volume call equals per-mesh calls bit for bit."""
+import numpy as np
+import pytest
+
+import baby_shark_amd as bs
+from baby_shark_amd import _lib
+from baby_shark_amd import meshgen as mg
+
+from test_volume_ops_gpu import assert_same_volume
+
+pytestmark = pytest.mark.gpu
+
+
+def _meshes():
+    ms = mg.batch_meshes(count=10, seed=7, lo=500, hi=20000, span_voxels=96)
+    ms.insert(3, (np.zeros((0, 3, 3), np.float32), 0.05))   # an empty mesh: None, like convert
+    return ms
+
+
+@pytest.mark.parametrize("device_input", [True, False])
+def test_batch_matches_single_calls(gpu_ctx, device_input):
+    import torch
+    ms = _meshes()
+    soups = [torch.from_numpy(s).cuda() if device_input else s for s, _ in ms]
+    vols = bs.batch_mesh_to_volume(soups, [h for _, h in ms], 3, ctx=gpu_ctx, workers=4)
+    assert vols[3] is None
+    for (soup, h), v in zip(ms, vols):
+        single = bs.MeshToVolume(gpu_ctx).with_voxel_size(h).with_narrow_band_width(3).convert(soup)
+        if single is None:
+            assert v is None
+            continue
+        assert_same_volume(v.export(), single.export())
+
+
+def test_batch_reports_per_mesh_errors(gpu_ctx):
+    L = _lib.load()
+    import ctypes as C
+    soup = np.ascontiguousarray(mg.torus(1.0, 0.35, 16, 8, 0.0, seed=1).reshape(-1, 9))
+    ptrs = (C.c_void_p * 2)(soup.ctypes.data, soup.ctypes.data)
+    counts = (C.c_size_t * 2)(soup.shape[0], soup.shape[0])
+    params = (_lib.M2VParams * 2)()
+    for i in range(2):
+        L.bs_m2v_params_default(C.byref(params[i]))
+        params[i].voxel_size = 0.1
+        params[i].band = 1
+    params[1].voxel_size = -1.0          # invalid
+    out = (C.c_void_p * 2)()
+    status = (C.c_int * 2)()
+    rc = L.bs_batch_mesh_to_volume(gpu_ctx.handle, 2, ptrs, counts, params, 1, 2, out, status)
+    assert rc == _lib.BS_ERR_INVALID_ARG and status[0] == _lib.BS_OK and status[1] == _lib.BS_ERR_INVALID_ARG
+    assert out[0] and not out[1]
+    L.bs_volume_free(out[0])
