@@ -22,6 +22,8 @@ BS_ERR_INTERNAL = 5
 BS_ERR_NO_DEVICE = 6
 BS_PATH_FORCE_HASH = 1
 BS_PATH_FORCE_LDS_STACK = 2
+BS_PROFILE_EVENTS = 1
+BS_PROFILE_COUNTERS = 2
 
 STATUS_NAMES = {0: "BS_OK", 1: "BS_ERR_EMPTY", 2: "BS_ERR_INVALID_ARG", 3: "BS_ERR_HIP",
                 4: "BS_ERR_UNSUPPORTED_TILE", 5: "BS_ERR_INTERNAL", 6: "BS_ERR_NO_DEVICE"}

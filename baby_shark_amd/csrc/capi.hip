@@ -111,9 +111,11 @@ void bs_ctx_destroy(bs_ctx* ctx) {
 
 void* bs_ctx_stream(bs_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
-bs_status bs_ctx_set_profiling(bs_ctx* ctx, int enabled) {
+bs_status bs_ctx_set_profiling(bs_ctx* ctx, int enabled) {  // flags
     if (!ctx) return BS_ERR_INVALID_ARG;
-    ctx->profiling = enabled != 0;
+    if (enabled & ~(BS_PROFILE_EVENTS | BS_PROFILE_COUNTERS)) return BS_ERR_INVALID_ARG;
+    ctx->profiling = (enabled & BS_PROFILE_EVENTS) != 0;
+    ctx->counting = (enabled & BS_PROFILE_COUNTERS) != 0;
     return BS_OK;
 }
 
@@ -200,6 +202,7 @@ bs_status bs_batch_mesh_to_volume(bs_ctx* ctx, size_t n_meshes, const float* con
         bs_ctx* w = ctx->workers[wi].get();
         w->paths = ctx->paths;
         w->profiling = false;
+        w->counting = false;
         for (size_t k; (k = next.fetch_add(1)) < n_meshes;) {
             const size_t i = order[k];
             st[i] = guarded([&] {

@@ -79,7 +79,8 @@ struct Scratch {
 struct bs_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    bool profiling = false;
+    bool profiling = false;  // HIP events around the mesh-to-volume phases
+    bool counting = false;   // winding-traversal visit counters (a slower kernel variant)
     int paths = 0;  // BS_PATH_* flags
     bs::Scratch scratch;
     bs_m2v_stats last_m2v = {};

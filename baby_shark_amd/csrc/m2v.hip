@@ -808,13 +808,6 @@ __device__ inline double solid_angle_rel64(double ax, double ay, double az, doub
     return 2.0 * atan2(det, den);
 }
 
-// Kahan-compensated accumulation
-__device__ inline void kb_add(float& s, float& c, float x) {
-    const float y = x - c;
-    const float t = s + y;
-    c = (t - s) - y;
-    s = t;
-}
 
 // order-2 far-field term of a node at query offset r = c - p, with the node's
 // coefficients pre-scaled (acc_finalize): 17 VALU operations
@@ -886,8 +879,12 @@ __device__ inline int opaque_zero() {
     return z;
 }
 
+// Per-lane winding sum (plain f32: its rounding, ~1e-6 over a few hundred
+// terms, is far below the FP64 re-check band) and, when counting, per-wave
+// visit counters for the work accounting.
+template <bool COUNT>
 struct WnAcc {
-    float s = 0.0f, c = 0.0f;
+    float s = 0.0f;
     unsigned long long near_int = 0, far = 0, leaf = 0, tri = 0;
 };
 
@@ -896,22 +893,25 @@ struct WnAcc {
 // inverse_ballot): far lanes add the order-2 term, near lanes either sum the
 // exact solid angles of a leaf's triangles or ask for the node's children.
 // Returns the mask of lanes that need the children (0 for a leaf).
+template <bool COUNT>
 __device__ inline unsigned long long wn_visit(const NodeRec& r, unsigned long long m, float px, float py, float pz,
-                                              const float* __restrict__ btris, WnAcc& a) {
+                                              const float* __restrict__ btris, WnAcc<COUNT>& a) {
     const float rx = r.A.x - px, ry = r.A.y - py, rz = r.A.z - pz;
     const float r2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
     const unsigned long long far_all = __ballot(r2 > r.A.w);
     const unsigned long long farm = m & far_all;
     const unsigned long long nm = m & ~far_all;
-    a.far += __popcll(farm);
+    if (COUNT) a.far += __popcll(farm);
     if (farm) {
-        if (__builtin_amdgcn_inverse_ballot_w64(farm)) kb_add(a.s, a.c, dipole_term(r.B, r.C, r.D, rx, ry, rz, r2));
+        if (__builtin_amdgcn_inverse_ballot_w64(farm)) a.s += dipole_term(r.B, r.C, r.D, rx, ry, rz, r2);
     }
     if (nm == 0) return 0;
     if (r.L.y < 0) {  // leaf: triangles [L.x, L.x - L.y)
         const int cnt = -r.L.y;
-        a.leaf += __popcll(nm);
-        a.tri += (unsigned long long)__popcll(nm) * cnt;
+        if (COUNT) {
+            a.leaf += __popcll(nm);
+            a.tri += (unsigned long long)__popcll(nm) * cnt;
+        }
         if (__builtin_amdgcn_inverse_ballot_w64(nm)) {
             // all BVH_LEAF_SIZE triangles requested at once (btris is padded)
             const float4* tp = reinterpret_cast<const float4*>(btris) + 3 * (uint64_t)r.L.x + opaque_zero();
@@ -927,15 +927,15 @@ __device__ inline unsigned long long wn_visit(const NodeRec& r, unsigned long lo
                                                  tr[5] - pz, tr[6] - px, tr[7] - py, tr[8] - pz);
                 om += (k < cnt) ? sa : 0.0f;
             }
-            kb_add(a.s, a.c, om * INV_4PI);
+            a.s += om * INV_4PI;
         }
         return 0;
     }
-    a.near_int += __popcll(nm);
+    if (COUNT) a.near_int += __popcll(nm);
     return nm;
 }
 
-template <class Stack>
+template <class Stack, bool COUNT>
 __global__ __launch_bounds__(WN_BLOCK) void k_winding(const uint32_t* __restrict__ act, uint32_t n_act,
                                                       const int32_t* __restrict__ origins, float h,
                                                       const NodeRec* __restrict__ nodes, int n_nodes,
@@ -956,7 +956,7 @@ __global__ __launch_bounds__(WN_BLOCK) void k_winding(const uint32_t* __restrict
     const float px = (float)vx * h, py = (float)vy * h, pz = (float)vz * h;
     const unsigned long long init = __ballot(valid);
     if (init == 0) return;
-    WnAcc a;
+    WnAcc<COUNT> a;
     Stack stk;
     if constexpr (!std::is_same<Stack, LaneStack>::value) {
         __shared__ int4 lds_stack[WN_BLOCK / 64][LdsStack::DEPTH];
@@ -998,14 +998,14 @@ __global__ __launch_bounds__(WN_BLOCK) void k_winding(const uint32_t* __restrict
         if (n0) stk.push(sp++, r0.L.x, r0.L.y, n0, lane);
     }
     if (overflow && lane == 0) atomicOr(err, E_STACK);
-    if (lane == 0) {
+    if (COUNT && lane == 0) {
         atomicAdd(&counters[C_NEAR_INTERNAL], a.near_int);
         atomicAdd(&counters[C_FAR], a.far);
         atomicAdd(&counters[C_LEAF_VISITS], a.leaf);
         atomicAdd(&counters[C_LEAF_TRIS], a.tri);
     }
     if (!valid) return;
-    const float w = a.s - a.c;
+    const float w = a.s;
     float* vp = values + slot;
     const float d = *vp;
     *vp = (w < thr) ? copysignf(d, 1.0f) : copysignf(d, -1.0f);
@@ -1360,7 +1360,9 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     {
         // a tree of height H needs at most H stack entries
         use_lds = !(H + 2 < LaneStack::DEPTH) || (ctx->paths & BS_PATH_FORCE_LDS_STACK);
-        auto kern = use_lds ? k_winding<LdsStack> : k_winding<LaneStack>;
+        // the visit counters feed the work accounting of profiled runs only
+        auto kern = use_lds ? (ctx->counting ? k_winding<LdsStack, true> : k_winding<LdsStack, false>)
+                            : (ctx->counting ? k_winding<LaneStack, true> : k_winding<LaneStack, false>);
         kern<<<div_up(n_act, WN_BLOCK), WN_BLOCK, 0, st>>>(act, n_act, vol->origins, h, nodes, n_nodes, btris,
                                                            prm.sign_threshold, prm.fp64_band, vol->values, recheck,
                                                            counters, err);

@@ -40,8 +40,10 @@ class Context:
     def stream(self) -> int:
         return int(load().bs_ctx_stream(self._h) or 0)
 
-    def set_profiling(self, on: bool) -> None:
-        check(load().bs_ctx_set_profiling(self._h, 1 if on else 0))
+    def set_profiling(self, on: bool, counters: bool = True) -> None:
+        """Phase timing with HIP events; with `counters`, winding visit counts
+        too (a slower kernel variant: time with counters=False)."""
+        check(load().bs_ctx_set_profiling(self._h, (1 | (2 if counters else 0)) if on else 0))
 
     def set_paths(self, flags: int) -> None:
         """Force alternate code paths (BS_PATH_* flags of bs_voxel.h); 0 = by size."""

@@ -101,8 +101,13 @@ bs_status bs_ctx_create(int device, bs_ctx** out);
 void bs_ctx_destroy(bs_ctx* ctx);
 /* The context's hipStream_t (as void*), for callers that time or order work. */
 void* bs_ctx_stream(bs_ctx* ctx);
-/* Per-kernel event timing inside bs_mesh_to_volume* (adds syncs; off by default). */
-bs_status bs_ctx_set_profiling(bs_ctx* ctx, int enabled);
+/* Instrumentation of bs_mesh_to_volume*, off by default: BS_PROFILE_EVENTS
+ * times each phase with HIP events (adds syncs); BS_PROFILE_COUNTERS counts
+ * winding-number node visits (wn_* in the stats) with a slower kernel
+ * variant.  Results are identical either way. */
+#define BS_PROFILE_EVENTS 1
+#define BS_PROFILE_COUNTERS 2
+bs_status bs_ctx_set_profiling(bs_ctx* ctx, int flags);
 bs_status bs_ctx_last_m2v_stats(const bs_ctx* ctx, bs_m2v_stats* out);
 /* Alternate code paths, normally chosen by size (for tests and diagnosis):
  * BS_PATH_FORCE_HASH      leaf allocation through the GPU hash even when a dense

@@ -161,6 +161,18 @@ void or_winding_numbers(const float* tris, size_t n, const float* pts, size_t m,
     });
 }
 
+// Node visits of the reference tree's approximate() summed over m points:
+// counts = (internal nodes opened, far evaluations, leaves summed).
+void or_winding_visit_counts(const float* tris, size_t n, const float* pts, size_t m, float acc, uint64_t* counts) {
+    auto mesh = to_tris(tris, n);
+    WindingNumbers w = WindingNumbers::from_mesh(mesh);
+    counts[0] = counts[1] = counts[2] = 0;
+    for (size_t i = 0; i < m; ++i) {
+        if (w.nodes.empty()) break;
+        w.count_visits(w.nodes.size() - 1, {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]}, acc, counts);
+    }
+}
+
 float or_solid_angle(const float* tri, const float* q) {
     return solid_angle({{tri[0], tri[1], tri[2]}, {tri[3], tri[4], tri[5]}, {tri[6], tri[7], tri[8]}}, {q[0], q[1], q[2]});
 }

@@ -257,6 +257,22 @@ float WindingNumbers::fast_wn(size_t idx, const Vec3f& p, float acc) const {
     return l + r;
 }
 
+void WindingNumbers::count_visits(size_t idx, const Vec3f& p, float acc, uint64_t counts[3]) const {
+    const DipoleData& d = data[idx];
+    if (norm(sub(p, d.center)) > d.radius * acc) {
+        ++counts[1];
+        return;
+    }
+    const BvhNode& n = nodes[idx];
+    if (n.leaf) {
+        ++counts[2];
+        return;
+    }
+    ++counts[0];
+    count_visits(n.left, p, acc, counts);
+    count_visits(n.right, p, acc, counts);
+}
+
 float WindingNumbers::approximate(const Vec3f& p, float acc) const {
     if (nodes.empty()) return 0.0f;
     return fast_wn(nodes.size() - 1, p, acc);

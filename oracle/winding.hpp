@@ -76,6 +76,9 @@ public:
     std::vector<Box3f> object_boxes;
     std::vector<DipoleData> data;
 
+    // node visits of approximate(p): internal nodes opened, far evaluations, leaves summed
+    void count_visits(size_t node, const Vec3f& p, float acc, uint64_t counts[3]) const;
+
 private:
     float fast_wn(size_t node, const Vec3f& p, float acc) const;
     double fast_wn_f64(size_t node, const Vec3f& p, float acc) const;
