@@ -22,6 +22,29 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 vector, published
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per k_winding launch at the default workload, from the FETCH_SIZE
+# (x2.0 calibration) and WRITE_SIZE counters of `bench.py --traffic-probe`
+# under rocprofv3 --pmc
+WINDING_TRAFFIC_BYTES = 2.236e9
+
+
+def traffic_probe():
+    """For rocprofv3 --pmc runs: a 1 GiB fill and a 1 GiB sum (counter
+    calibration), then one mesh-to-volume step of the default workload."""
+    import torch
+    import baby_shark_amd as bs
+    from baby_shark_amd import meshgen as mg
+    n = 256 * 2 ** 20
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    total = float(src.sum())
+    torch.cuda.synchronize()
+    print("calibration: 1 GiB written, 1 GiB read", total, flush=True)
+    soup = torch.from_numpy(mg.displaced_icosphere(8, 1.0, 0.05, seed=2)).cuda()
+    torch.cuda.synchronize()
+    ctx = bs.default_context(0)
+    v = bs.MeshToVolume(ctx).with_voxel_size(0.002).with_narrow_band_width(3).convert(soup)
+    print("mesh_to_volume active voxels", ctx.last_stats()["n_active"], flush=True)
+    v.free()
 
 
 def distance_flop(stats):
@@ -169,7 +192,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pipeline", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true")
     args = ap.parse_args()
+    if args.traffic_probe:
+        traffic_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU layout is a design "
               "note, not a measured path.", file=sys.stderr)
@@ -228,7 +255,8 @@ def main():
     dom = max(kernels, key=lambda k: kernels[k]["ms"])
     roof = {"bound": "valu_fp32", "kernel": dom, "achieved": round(kernels[dom]["tflops"], 3),
             "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(kernels[dom]["tflops"] / PEAK_FP32_TFLOPS, 4),
-            "traffic": None}
+            "traffic": WINDING_TRAFFIC_BYTES if dom == "k_winding" else None,
+            "traffic_source": "profiles/m2v_icosphere_L8_traffic.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"}
 
     out = {
         "metric": "narrow-band SDF voxels/s at 1.3M tris (plain), err<=1e-4 vox",
