@@ -120,6 +120,7 @@ def load():
             "bs_volume_intersect": (C.c_int, [vp, vp, vp]),
             "bs_ctx_last_csg_stats": (C.c_int, [vp, C.POINTER(CsgStats)]),
             "bs_volume_flood_fill": (C.c_int, [vp, vp]),
+            "bs_volume_densify_tiles": (C.c_int, [vp, vp]),
             "bs_volume_offset": (C.c_int, [vp, vp, C.c_float]),
             "bs_dual_contour": (C.c_int, [vp, vp, C.c_float, C.POINTER(f32p), C.POINTER(C.c_size_t)]),
             "bs_dual_contour_device": (C.c_int, [vp, vp, C.c_float, C.POINTER(vp), C.POINTER(C.c_size_t)]),

@@ -18,6 +18,7 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
 void volume_sphere(bs_ctx* ctx, float radius, float cx, float cy, float cz, float h, bs_volume** out);
 void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op);
 void volume_flood_fill(bs_ctx* ctx, bs_volume* v);
+void volume_densify_tiles(bs_ctx* ctx, bs_volume* v);
 void volume_offset(bs_ctx* ctx, bs_volume* vol, float distance, bs_offset_stats* stats);
 void dual_contour_device(bs_ctx* ctx, bs_volume* vol, float voxel_size, float** out_dev, uint64_t* n_tris,
                          bs_dc_stats* stats);
@@ -464,6 +465,18 @@ void bs_free(void* p) { std::free(p); }
 
 void bs_device_free(void* p) {
     if (p) (void)hipFree(p);
+}
+
+bs_status bs_volume_densify_tiles(bs_ctx* ctx, bs_volume* v) {
+    if (!ctx || !v) {
+        g_last_error = "null argument";
+        return BS_ERR_INVALID_ARG;
+    }
+    return guarded([&] {
+        DeviceGuard g(ctx->device);
+        BS_REQUIRE(v->device == ctx->device, BS_ERR_INVALID_ARG, "volume lives on another device");
+        volume_densify_tiles(ctx, v);
+    });
 }
 
 bs_status bs_volume_signs_at(bs_ctx* ctx, bs_volume* v, const int32_t* idx, size_t n, uint8_t* out_negative) {

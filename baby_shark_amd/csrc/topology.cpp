@@ -497,6 +497,55 @@ std::unique_ptr<Topology> Topology::after_offset(const std::vector<uint32_t>& ke
     return t->compact(keep_leaf);
 }
 
+std::vector<TileRec> Topology::densify_tiles() {
+    std::vector<TileRec> made;
+    auto new_leaf = [&](int32_t x, int32_t y, int32_t z, float v) {
+        const int32_t idx = (int32_t)leaves.size();
+        LeafRef r;
+        r.x = x;
+        r.y = y;
+        r.z = z;
+        r.a = idx;
+        r.op = LEAF_COPY_A;
+        leaves.push_back(r);
+        made.push_back({x, y, z, 8, v});
+        return idx;
+    };
+    std::vector<int32_t> n1_ids;
+    for (const auto& kv : root) n1_ids.push_back(kv.second);
+    for (int32_t ai : n1_ids) {
+        for (int o1 = 0; o1 < Node1::SIZE; ++o1) {
+            const uint8_t k1 = n1[ai].kind[o1];
+            if (k1 == SLOT_TILE) {
+                const float v = n1[ai].tile[o1];
+                int32_t nx, ny, nz;
+                slot_origin(n1[ai], o1, L2_SPAN, nx, ny, nz);
+                Node2 nb(nx, ny, nz);
+                for (int o2 = 0; o2 < Node2::SIZE; ++o2) {
+                    int32_t lx, ly, lz;
+                    slot_origin(nb, o2, 8, lx, ly, lz);
+                    nb.kind[o2] = SLOT_BRANCH;
+                    nb.child[o2] = new_leaf(lx, ly, lz, v);
+                }
+                n2.push_back(std::move(nb));
+                n1[ai].kind[o1] = SLOT_BRANCH;
+                n1[ai].child[o1] = (int32_t)n2.size() - 1;
+            } else if (k1 == SLOT_BRANCH) {
+                const int32_t bi = n1[ai].child[o1];
+                for (int o2 = 0; o2 < Node2::SIZE; ++o2) {
+                    if (n2[bi].kind[o2] != SLOT_TILE) continue;
+                    int32_t lx, ly, lz;
+                    slot_origin(n2[bi], o2, 8, lx, ly, lz);
+                    const int32_t li = new_leaf(lx, ly, lz, n2[bi].tile[o2]);
+                    n2[bi].kind[o2] = SLOT_BRANCH;
+                    n2[bi].child[o2] = li;
+                }
+            }
+        }
+    }
+    return made;
+}
+
 std::unique_ptr<Topology> Topology::compact(const std::vector<int32_t>& new_index) {
     auto t = std::make_unique<Topology>();
     int32_t max_leaf = -1;

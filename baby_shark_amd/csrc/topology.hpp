@@ -91,6 +91,11 @@ struct Topology {
     std::unique_ptr<Topology> after_offset(const std::vector<uint32_t>& keep,
                                            const std::vector<std::array<int32_t, 3>>& out_origins, float two_h) const;
 
+    // Every active tile becomes the leaves it covers (a Node1-level tile gets a
+    // fresh 16^3 node of leaves): the new leaves are appended to `leaves` and
+    // returned as (origin, edge 8, tile value) in index order.
+    std::vector<TileRec> densify_tiles();
+
     // The reachable part of the tree with leaf reference r renumbered to
     // new_index[r] (a plain copy of the device leaf at that index).  Moves the
     // nodes out: this topology is empty afterwards.

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <chrono>
+#include <cstring>
 #include <numeric>
 #include <tuple>
 
@@ -506,6 +507,95 @@ void volume_from_voxels(bs_ctx* ctx, const int32_t* idx, const float* values, ui
         throw;
     }
     *out = vol;
+}
+
+// Output leaf i: rec[i] = (source leaf of the old arrays or -1, tile value
+// bits for a new leaf), org[i] = origin.  A new leaf is all active.
+__global__ __launch_bounds__(256) void k_densify_assemble(uint64_t n_out, const int2* __restrict__ rec,
+                                                          const int4* __restrict__ org, const uint64_t* __restrict__ om,
+                                                          const float* __restrict__ ov, int4* __restrict__ origins,
+                                                          uint64_t* __restrict__ masks, float* __restrict__ values) {
+    const uint64_t leaf = (uint64_t)blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x >> 6);
+    if (leaf >= n_out) return;
+    const int lane = threadIdx.x & 63;
+    const int2 r = rec[leaf];
+    float4* po = reinterpret_cast<float4*>(values + leaf * 512 + (uint64_t)lane * 8);
+    if (r.x >= 0) {
+        const float4* pi = reinterpret_cast<const float4*>(ov + (uint64_t)(uint32_t)r.x * 512 + lane * 8);
+        po[0] = pi[0];
+        po[1] = pi[1];
+        if (lane < 8) masks[leaf * 8 + lane] = om[(uint64_t)(uint32_t)r.x * 8 + lane];
+    } else {
+        const float v = __int_as_float(r.y);
+        po[0] = make_float4(v, v, v, v);
+        po[1] = make_float4(v, v, v, v);
+        if (lane < 8) masks[leaf * 8 + lane] = ~0ull;
+    }
+    if (lane == 0) origins[leaf] = org[leaf];
+}
+
+// Active tiles become full leaves of the tile value (an extension with no
+// reference counterpart: the reference's dual contouring stops at todo!() on
+// tiles, dual_contouring.rs:146).
+void volume_densify_tiles(bs_ctx* ctx, bs_volume* v) {
+    if (v->tiles.empty()) return;
+    Topology& t = ensure_topology(ctx, v);
+    const uint64_t n_old = v->n_leaves;
+    const std::vector<TileRec> made = t.densify_tiles();
+    const uint64_t n_out = n_old + made.size();
+    BS_REQUIRE(n_out < (1ull << 31), BS_ERR_INVALID_ARG, "densified volume exceeds 2^31 leaves");
+    std::vector<std::pair<uint64_t, int64_t>> order(n_out);  // (leaf key, source: old index or -1-k)
+    for (uint64_t i = 0; i < n_old; ++i) {
+        const LeafRef& r = t.leaves[i];
+        order[i] = {leaf_key(r.x >> 3, r.y >> 3, r.z >> 3), (int64_t)i};
+    }
+    for (size_t k = 0; k < made.size(); ++k)
+        order[n_old + k] = {leaf_key(made[k].x >> 3, made[k].y >> 3, made[k].z >> 3), -1 - (int64_t)k};
+    std::sort(order.begin(), order.end());
+    std::vector<int2> rec(n_out);
+    std::vector<int4> org(n_out);
+    std::vector<int32_t> new_index(n_out);
+    for (uint64_t i = 0; i < n_out; ++i) {
+        const int64_t src = order[i].second;
+        const uint64_t ref = src >= 0 ? (uint64_t)src : n_old + (uint64_t)(-1 - src);
+        new_index[ref] = (int32_t)i;
+        const LeafRef& r = t.leaves[ref];
+        org[i] = make_int4(r.x, r.y, r.z, 0);
+        if (src >= 0) {
+            rec[i] = make_int2((int)src, 0);
+        } else {
+            float fv = made[(size_t)(-1 - src)].value;
+            int bits;
+            std::memcpy(&bits, &fv, 4);
+            rec[i] = make_int2(-1, bits);
+        }
+    }
+    bs_volume out;
+    out.device = v->device;
+    try {
+        alloc_leaves(&out, n_out);
+        int2* d_rec = reinterpret_cast<int2*>(ctx->scratch.get(S_MISC0, sizeof(int2) * n_out));
+        int4* d_org = ctx->scratch.as<int4>(S_MISC1, n_out);
+        BS_HIP(hipMemcpyAsync(d_rec, rec.data(), sizeof(int2) * n_out, hipMemcpyHostToDevice, ctx->stream));
+        BS_HIP(hipMemcpyAsync(d_org, org.data(), sizeof(int4) * n_out, hipMemcpyHostToDevice, ctx->stream));
+        k_densify_assemble<<<div_up(n_out, WAVES_PER_BLOCK), 256, 0, ctx->stream>>>(
+            n_out, d_rec, d_org, v->masks, v->values, reinterpret_cast<int4*>(out.origins), out.masks, out.values);
+        BS_HIP(hipGetLastError());
+        BS_HIP(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+        free_leaves(&out);
+        throw;
+    }
+    free_leaves(v);
+    v->n_leaves = out.n_leaves;
+    v->origins = out.origins;
+    v->masks = out.masks;
+    v->values = out.values;
+    out.origins = nullptr;
+    out.masks = nullptr;
+    out.values = nullptr;
+    v->tiles.clear();
+    v->topo = t.compact(new_index);
 }
 
 // VolumeBuilder::sphere (builder.rs:21-30) through Volume::from_fn.

@@ -182,6 +182,12 @@ class Volume:
         check(load().bs_volume_flood_fill(self._ctx.handle, self._h))
         return self
 
+    def densify_tiles(self) -> "Volume":
+        """Active tiles become full leaves of the tile value (an extension:
+        lets dual contouring run after a union that left tiles)."""
+        check(load().bs_volume_densify_tiles(self._ctx.handle, self._h))
+        return self
+
     def signs_at(self, idx) -> np.ndarray:
         """FloodFill::sign_at for (n, 3) voxel indices: True where negative."""
         a = np.ascontiguousarray(np.asarray(idx, dtype=np.int32).reshape(-1, 3))

@@ -88,7 +88,7 @@ typedef struct {
     double ms_wall;           /* the whole call on the host clock             */
 } bs_csg_stats;
 
-/* Counters of the last bs_dual_contour* call. */
+/* Counters of the last bs_dual_contour* call (see below). */
 typedef struct {
     uint64_t n_leaves, n_tris, used_hash;
     double ms_total;  /* device time from the first kernel to the last (stream events) */
@@ -214,6 +214,14 @@ bs_status bs_dual_contour_device(bs_ctx* ctx, const bs_volume* v, float voxel_si
 bs_status bs_ctx_last_dc_stats(const bs_ctx* ctx, bs_dc_stats* out);
 void bs_free(void* p);
 void bs_device_free(void* p_dev);
+
+/* Extension with no reference counterpart: every active tile becomes the
+ * leaves it covers, all 512 entries active and equal to the tile value (a
+ * 128^3 tile gives 4096 leaves).  The reference's dual contouring stops at
+ * todo!() on tiles (dual_contouring.rs:146), and a union leaves active -MAX
+ * tiles (internal_node/csg.rs:42-48); densifying first lets bs_dual_contour
+ * run on such volumes. */
+bs_status bs_volume_densify_tiles(bs_ctx* ctx, bs_volume* v);
 
 /* FloodFill::sign_at (root_node/flood_fill.rs:56-63, internal_node/
  * flood_fill.rs:110-117, leaf_node/flood_fill.rs:72-75) for n voxel indices

@@ -443,6 +443,25 @@ void or_free(void* p) { std::free(p); }
 // fast_sweep.rs:509-537 (exposed for its closed-form checks)
 float or_fs_compute_distance(float a1, float a2, float a3, float h) { return fs_compute_distance(a1, a2, a3, h); }
 
+// Tile densification (an extension with no reference counterpart, for dual
+// contouring volumes that hold active tiles): every active tile becomes the
+// leaves it covers, all 512 entries active and equal to the tile value,
+// inserted with insert_leaf_at (internal_node/tree_node.rs:280-298).
+void or_volume_densify_tiles(or_volume* a) {
+    struct T { Vec3i o; int size; float v; };
+    std::vector<T> tiles;
+    a->grid->visit_leafs([&](Vec3i o, int s, float v) { tiles.push_back({o, s, v}); }, [&](const LeafNode&) {});
+    for (const T& t : tiles)
+        for (int64_t x = 0; x < t.size; x += 8)
+            for (int64_t y = 0; y < t.size; y += 8)
+                for (int64_t z = 0; z < t.size; z += 8) {
+                    auto leaf = std::make_unique<LeafNode>(Vec3i{t.o.x + x, t.o.y + y, t.o.z + z});
+                    leaf->fill(t.v);
+                    a->grid->insert_leaf_at(std::move(leaf));
+                }
+    a->index();
+}
+
 // Volume::offset (volume/mod.rs:95-108).  Returns 0, or 1 where the reference
 // panics (an active tile inside the band reaches the leaf queue).
 int or_volume_offset(or_volume* a, float distance) {

@@ -59,6 +59,7 @@ def lib():
         u8p = C.POINTER(C.c_uint8)
         L.or_volume_sphere.restype = vp
         L.or_volume_sphere.argtypes = [C.c_float] * 5
+        L.or_volume_densify_tiles.argtypes = [vp]
         L.or_dual_contour.restype = C.c_int
         L.or_dual_contour.argtypes = [vp, C.c_float, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_size_t)]
         L.or_free.argtypes = [vp]
@@ -135,6 +136,11 @@ class Volume:
     def offset(self, distance: float) -> "Volume":   # volume/mod.rs:95-108
         if lib().or_volume_offset(self.h, distance) != 0:
             raise RuntimeError("offset: the reference panics (tile origin in the leaf queue)")
+        return self
+
+    def densify_tiles(self) -> "Volume":
+        """Active tiles become full leaves of the tile value (an extension)."""
+        lib().or_volume_densify_tiles(self.h)
         return self
 
     def dual_contour(self, voxel_size: float) -> np.ndarray:

@@ -8,6 +8,7 @@ from baby_shark_amd import _lib
 from baby_shark_amd import meshgen as mg
 
 from test_offset_gpu import _import
+from test_volume_ops_gpu import assert_same_volume
 
 pytestmark = pytest.mark.gpu
 
@@ -75,3 +76,41 @@ def test_dc_error_cases(gpu_ctx, oracle_lib):
     # nothing to contour
     assert m.mesh(bs.volume_from_voxels([[0, 0, 0], [1, 0, 0]], [1.0, 1.0], 0.05, gpu_ctx)).shape == (0, 3, 3)
     assert m.mesh(B.sphere(-1.0, (0.0, 0.0, 0.0))).shape == (0, 3, 3)
+
+
+def test_densify_then_dc_matches_oracle(gpu_ctx, oracle_lib):
+    """Two nested spheres and the torus-and-sphere workload (torus 128x64,
+    h = 0.02, band 3, united with the sphere r = 0.5 at (1, 0, 0)): the union
+    leaves active 8^3 tiles; densified, both paths contour the same soup."""
+    h = 0.05
+    B = bs.VolumeBuilder(gpu_ctx, h)
+    g = B.sphere(0.3, (0.1, 0.0, 0.0)).union(B.sphere(1.0, (0.0, 0.0, 0.0)))
+    r = oracle_lib.sphere(0.3, (0.1, 0.0, 0.0), h).union(oracle_lib.sphere(1.0, (0.0, 0.0, 0.0), h))
+    assert g.export().tile_sizes.size > 0
+    g.densify_tiles()
+    r.densify_tiles()
+    assert_same_volume(g.export(), r.export())
+    _same(bs.DualContouringMesher(gpu_ctx).with_voxel_size(h).mesh(g), r.dual_contour(h))
+
+    soup = mg.torus(1.0, 0.35, 128, 64, 0.01, seed=1)
+    ov = oracle_lib.mesh_to_volume(soup, 0.02, 3)
+    g = _import(gpu_ctx, ov).union(bs.VolumeBuilder(gpu_ctx, 0.02).sphere(0.5, (1.0, 0.0, 0.0))).densify_tiles()
+    r = ov.union(oracle_lib.sphere(0.5, (1.0, 0.0, 0.0), 0.02)).densify_tiles()
+    assert_same_volume(g.export(), r.export())
+    got = bs.DualContouringMesher(gpu_ctx).with_voxel_size(0.02).mesh(g)
+    assert got.shape[0] > 100000
+    _same(got, r.dual_contour(0.02))
+
+
+def test_densify_node_level_tiles(gpu_ctx, oracle_lib):
+    """A 128^3 tile (the union of a small sphere with one of 225 voxels
+    radius) becomes 4096 leaves; a volume without tiles is left alone."""
+    h = 0.004
+    B = bs.VolumeBuilder(gpu_ctx, h)
+    g = B.sphere(0.2, (0.1, 0.0, 0.0)).union(B.sphere(0.9, (0.0, 0.0, 0.0)))
+    r = oracle_lib.sphere(0.2, (0.1, 0.0, 0.0), h).union(oracle_lib.sphere(0.9, (0.0, 0.0, 0.0), h))
+    assert set(g.export().tile_sizes.tolist()) == {128}
+    assert_same_volume(g.densify_tiles().export(), r.densify_tiles().export())
+    s = bs.VolumeBuilder(gpu_ctx, 0.05).sphere(0.4, (0.0, 0.0, 0.0))
+    before = s.export()
+    assert_same_volume(s.densify_tiles().export(), before)

@@ -76,3 +76,39 @@ def test_torus_and_sphere_union_has_tiles_so_dc_stops(oracle_lib):
     assert u.export().tile_values.size > 0
     with pytest.raises(RuntimeError, match="tile"):
         u.dual_contour(0.02)
+
+
+def test_densify_tiles_invariants(oracle_lib):
+    """Densifying replaces each active tile by the leaves it covers, all 512
+    entries active and equal to the tile value; the other leaves and the
+    signs everywhere are unchanged."""
+    for (r1, c1, r2, h) in [(0.3, (0.1, 0.0, 0.0), 1.0, H), (0.2, (0.1, 0.0, 0.0), 0.9, 0.004)]:
+        u = oracle_lib.sphere(r1, c1, h).union(oracle_lib.sphere(r2, (0.0, 0.0, 0.0), h))
+        before = u.export()
+        assert before.tile_sizes.size > 0
+        probe = np.random.default_rng(3).integers(-260, 260, size=(4000, 3)).astype(np.int32)
+        s0 = u.signs_at(probe)
+        after = u.clone().densify_tiles().export()
+        assert after.tile_sizes.size == 0
+        old = {tuple(o): i for i, o in enumerate(before.origins.tolist())}
+        covered = {}
+        for o, s, v in zip(before.tile_origins.tolist(), before.tile_sizes.tolist(), before.tile_values.tolist()):
+            for dx in range(0, s, 8):
+                for dy in range(0, s, 8):
+                    for dz in range(0, s, 8):
+                        covered[(o[0] + dx, o[1] + dy, o[2] + dz)] = v
+        assert set(old).isdisjoint(covered)
+        assert after.origins.shape[0] == len(old) + len(covered)
+        # sorted by origin, as every export
+        key = after.origins.astype(np.int64)
+        assert np.all(np.diff((key[:, 0] << 40) + (key[:, 1] << 20) + key[:, 2]) > 0)
+        for i, o in enumerate(after.origins.tolist()):
+            o = tuple(o)
+            if o in old:
+                j = old[o]
+                assert np.array_equal(after.masks[i], before.masks[j])
+                assert np.array_equal(after.values[i].view(np.uint32), before.values[j].view(np.uint32))
+            else:
+                assert np.all(after.masks[i] == np.uint64(2**64 - 1))
+                assert np.all(after.values[i] == np.float32(covered[o]))
+        assert np.array_equal(u.densify_tiles().signs_at(probe), s0)
