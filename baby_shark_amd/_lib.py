@@ -22,6 +22,8 @@ BS_ERR_INTERNAL = 5
 BS_ERR_NO_DEVICE = 6
 BS_PATH_FORCE_HASH = 1
 BS_PATH_FORCE_LDS_STACK = 2
+BS_SIGN_PER_VOXEL = 0
+BS_SIGN_SHARED = 1
 BS_PROFILE_EVENTS = 1
 BS_PROFILE_COUNTERS = 2
 
@@ -31,7 +33,7 @@ STATUS_NAMES = {0: "BS_OK", 1: "BS_ERR_EMPTY", 2: "BS_ERR_INVALID_ARG", 3: "BS_E
 
 class M2VParams(C.Structure):
     _fields_ = [("voxel_size", C.c_float), ("band", C.c_int64), ("wn_accuracy", C.c_float),
-                ("sign_threshold", C.c_float), ("fp64_band", C.c_float)]
+                ("sign_threshold", C.c_float), ("fp64_band", C.c_float), ("sign_mode", C.c_int32)]
 
 
 class VolumeInfo(C.Structure):
@@ -45,7 +47,8 @@ class M2VStats(C.Structure):
                                           "n_fp64_rechecks", "wn_leaf_visits", "used_hash", "used_lds_stack",
                                           "bvh_height")] + \
               [(k, C.c_double) for k in ("ms_total", "ms_subdivide", "ms_leaves", "ms_distance", "ms_bvh",
-                                         "ms_winding")]
+                                         "ms_winding")] + \
+              [("n_sign_evals", C.c_uint64), ("ms_sign_share", C.c_double)]
 
 
 class OffsetStats(C.Structure):

@@ -139,6 +139,7 @@ void bs_m2v_params_default(bs_m2v_params* p) {
     p->wn_accuracy = 2.0f;    // mesh_to_volume.rs:264
     p->sign_threshold = 0.2f; // mesh_to_volume.rs:266
     p->fp64_band = 0.1f;
+    p->sign_mode = BS_SIGN_PER_VOXEL;
 }
 
 bs_status bs_mesh_to_volume_device(bs_ctx* ctx, const float* tri_xyz_dev, size_t n, const bs_m2v_params* p,

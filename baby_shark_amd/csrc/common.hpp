@@ -44,7 +44,7 @@ enum ScratchSlot : int {
     S_KEYS, S_KEYS_SORTED, S_CAND_LEAF, S_CAND_CNT, S_CAND_OFF, S_CAND_LIST, S_CURSOR, S_ACT_CNT, S_ACT_OFF,
     S_ACT_LIST, S_COUNTERS, S_CUB, S_MORTON, S_MORTON_SORTED, S_IDX, S_IDX_SORTED, S_BVH_TRIS, S_BVH_NODES,
     S_BVH_LINKS, S_BVH_PARENT, S_BVH_FLAGS, S_BVH_INIT, S_BVH_BOX, S_RECHECK, S_ARGMIN, S_MISC0, S_MISC1,
-    S_MISC2, S_MISC3, S_INPUT, S_BVH_KIDS, S_FLAGS_A, S_FLAGS_B, S_CSG_PLAN, S_DC_MAP, S_DC_KEYS, S_DC_FAIL, S_DC_CELLS,
+    S_MISC2, S_MISC3, S_INPUT, S_BVH_KIDS, S_FLAGS_A, S_FLAGS_B, S_CSG_PLAN, S_DC_MAP, S_DC_KEYS, S_DC_FAIL, S_DC_CELLS, S_SIGN_ROOTS, S_REP_CNT, S_REP_OFF, S_REPS,
     S_DC_CELLMASK, S_DC_TRICNT, S_DC_TRIOFF, S_COUNT
 };
 

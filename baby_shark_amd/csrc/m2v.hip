@@ -1073,6 +1073,101 @@ __global__ void k_winding_fp64(const uint32_t* __restrict__ list, uint32_t n, co
 }
 
 // ---------------------------------------------------------------------------
+// Shared signs.  For face neighbours v, u of one leaf with |d_v| + |d_u| > thr
+// (thr bounds the sample spacing plus the distance and sub-triangle rounding),
+// no point of the surface lies on the segment between them: a crossing point s
+// would give |d_v| + |d_u| <= |v - s| + |s - u| = |v - u|.  A closed mesh's
+// winding number is constant off the surface, so such voxels share their sign.
+// Each leaf's voxels are grouped by a union-find over those links; the root of
+// a group is its smallest offset (a larger root is always hooked under a
+// smaller one), its winding number is evaluated and its sign is copied to the
+// group.  Groups never span leaves.
+
+__device__ inline int uf_find(int* par, int x) {
+    int p = __hip_atomic_load(&par[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    while (p != x) {
+        x = p;
+        p = __hip_atomic_load(&par[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    return x;
+}
+
+__device__ inline void uf_unite(int* par, int a, int b) {
+    // every failed exchange means another lane hooked b first: the loop ends
+    for (;;) {
+        a = uf_find(par, a);
+        b = uf_find(par, b);
+        if (a == b) return;
+        if (a > b) {
+            const int t = a;
+            a = b;
+            b = t;
+        }
+        if (atomicCAS(&par[b], b, a) == b) return;
+    }
+}
+
+// roots[slot] = offset of the group root, 0xFFFF for an inactive voxel;
+// rep_cnt[leaf] = number of groups.  Values are unsigned distances here.
+__global__ __launch_bounds__(512) void k_sign_groups(const uint64_t* __restrict__ masks,
+                                                     const float* __restrict__ values, float thr,
+                                                     uint16_t* __restrict__ roots, uint32_t* __restrict__ rep_cnt) {
+    __shared__ float d[LEAF_VOX];
+    __shared__ int par[LEAF_VOX];
+    __shared__ uint32_t wcnt[8];
+    const uint64_t leaf = blockIdx.x;
+    const int v = threadIdx.x;
+    const bool act = (masks[leaf * 8 + (v >> 6)] >> (v & 63)) & 1ull;
+    d[v] = act ? values[leaf * LEAF_VOX + v] : -1.0f;
+    par[v] = v;
+    __syncthreads();
+    if (act) {
+        const float dv = d[v];
+        // +x, +y, +z neighbours inside the leaf (offset = x<<6 | y<<3 | z)
+        if ((v >> 6) < 7 && d[v + 64] >= 0.0f && dv + d[v + 64] > thr) uf_unite(par, v, v + 64);
+        if (((v >> 3) & 7) < 7 && d[v + 8] >= 0.0f && dv + d[v + 8] > thr) uf_unite(par, v, v + 8);
+        if ((v & 7) < 7 && d[v + 1] >= 0.0f && dv + d[v + 1] > thr) uf_unite(par, v, v + 1);
+    }
+    __syncthreads();
+    const int r = act ? uf_find(par, v) : -1;
+    roots[leaf * LEAF_VOX + v] = act ? (uint16_t)r : (uint16_t)0xFFFF;
+    const unsigned long long b = __ballot(act && r == v);
+    if ((v & 63) == 0) wcnt[v >> 6] = (uint32_t)__popcll(b);
+    __syncthreads();
+    if (v == 0) {
+        uint32_t c = 0;
+        for (int k = 0; k < 8; ++k) c += wcnt[k];
+        rep_cnt[leaf] = c;
+    }
+}
+
+// the group roots of every leaf, slot = leaf * 512 + offset, in offset order
+__global__ __launch_bounds__(512) void k_sign_reps(const uint16_t* __restrict__ roots,
+                                                   const uint32_t* __restrict__ rep_off, uint32_t* __restrict__ reps) {
+    __shared__ uint32_t wcnt[8];
+    const uint64_t leaf = blockIdx.x;
+    const int v = threadIdx.x;
+    const int w = v >> 6, lane = v & 63;
+    const bool rep = roots[leaf * LEAF_VOX + v] == (uint16_t)v;
+    const unsigned long long b = __ballot(rep);
+    if (lane == 0) wcnt[w] = (uint32_t)__popcll(b);
+    __syncthreads();
+    uint32_t before = 0;
+    for (int k = 0; k < w; ++k) before += wcnt[k];
+    if (rep) reps[rep_off[leaf] + before + __popcll(b & ((1ull << lane) - 1))] = (uint32_t)leaf * 512u + (uint32_t)v;
+}
+
+// every non-root voxel takes the sign of its (already signed) root
+__global__ __launch_bounds__(512) void k_sign_broadcast(const uint16_t* __restrict__ roots, float* __restrict__ values) {
+    const uint64_t leaf = blockIdx.x;
+    const int v = threadIdx.x;
+    const uint16_t r = roots[leaf * LEAF_VOX + v];
+    if (r == (uint16_t)0xFFFF || r == (uint16_t)v) return;
+    float* vp = values + leaf * LEAF_VOX;
+    vp[v] = copysignf(vp[v], vp[r]);
+}
+
+// ---------------------------------------------------------------------------
 // host driver
 
 template <class T>
@@ -1156,6 +1251,8 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     *out = nullptr;
     BS_REQUIRE(prm.voxel_size > 0.0f && std::isfinite(prm.voxel_size), BS_ERR_INVALID_ARG, "voxel_size must be > 0");
     BS_REQUIRE(prm.band >= 0 && prm.band < (1 << 20), BS_ERR_INVALID_ARG, "band must be in [0, 2^20)");
+    BS_REQUIRE(prm.sign_mode == BS_SIGN_PER_VOXEL || prm.sign_mode == BS_SIGN_SHARED, BS_ERR_INVALID_ARG,
+               "sign_mode must be BS_SIGN_PER_VOXEL or BS_SIGN_SHARED");
     if (n == 0) throw Error(BS_ERR_EMPTY, "empty mesh: the reference returns None");
     BS_REQUIRE(n < (1ull << 30), BS_ERR_INVALID_ARG, "more than 2^30 triangles");
     hipStream_t st = ctx->stream;
@@ -1183,6 +1280,16 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     k_subdiv_count<<<div_up(n, TB), TB, 0, st>>>(d_tris, n, h, rows, subs, err);
     exclusive_scan(ctx, rows, row_off, n + 1);
     exclusive_scan(ctx, subs, sub_off, n + 1);
+    const bool shared_signs = prm.sign_mode == BS_SIGN_SHARED;
+    uint32_t max_rows = 1;  // largest subdivision count: bounds the sub-triangle rounding
+    if (shared_signs) {
+        uint32_t* d_max = reinterpret_cast<uint32_t*>(S.get(S_MISC2, 64));
+        size_t bytes = 0;
+        BS_HIP(hipcub::DeviceReduce::Max(nullptr, bytes, rows, d_max, (int)n, st));
+        void* tmp = S.get(S_CUB, bytes);
+        BS_HIP(hipcub::DeviceReduce::Max(tmp, bytes, rows, d_max, (int)n, st));
+        max_rows = std::max<uint32_t>(1u, read_scalar(ctx, d_max));
+    }
     uint64_t total_rows = read_scalar(ctx, row_off + n);
     uint64_t n_sub = read_scalar(ctx, sub_off + n);
     if (read_scalar(ctx, err) & E_TOO_MANY_SUBS)
@@ -1354,18 +1461,47 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     }
     tm.mark();
 
-    // ---- 6. winding numbers and signs
-    uint32_t* recheck = S.as<uint32_t>(S_RECHECK, n_act);
-    if (n_act > 0)
+    // ---- 6. sign groups (shared signs only): the winding number is evaluated
+    // at one voxel per group
+    const uint32_t* wn_list = act;
+    uint32_t n_wn = n_act;
+    uint16_t* roots = nullptr;
+    if (shared_signs && n_act > 0) {
+        double maxc = 0.0;
+        for (int k = 0; k < 3; ++k)
+            maxc = std::max({maxc, std::fabs((double)(long long)hc[C_BOX_LO + k]),
+                             std::fabs((double)(long long)hc[C_BOX_HI + k])});
+        maxc = (maxc + 1.0) * (double)h;
+        // sample spacing, sub-triangle vertex drift (two roundings per
+        // accumulation step) and distance rounding, each bounded in ulps of
+        // the largest coordinate
+        const double ulp = maxc * 0x1p-23;
+        const float thr = (float)((double)h * (1.0 + 0x1p-10) + (4.0 * max_rows + 200.0) * ulp);
+        roots = S.as<uint16_t>(S_SIGN_ROOTS, n_leaves * LEAF_VOX);
+        uint32_t* rep_cnt = S.as<uint32_t>(S_REP_CNT, n_leaves + 1);
+        uint32_t* rep_off = S.as<uint32_t>(S_REP_OFF, n_leaves + 1);
+        BS_HIP(hipMemsetAsync(rep_cnt + n_leaves, 0, sizeof(uint32_t), st));
+        k_sign_groups<<<(unsigned)n_leaves, 512, 0, st>>>(vol->masks, vol->values, thr, roots, rep_cnt);
+        exclusive_scan(ctx, rep_cnt, rep_off, n_leaves + 1);
+        n_wn = read_scalar(ctx, rep_off + n_leaves);
+        uint32_t* reps = S.as<uint32_t>(S_REPS, std::max<uint32_t>(n_wn, 1));
+        k_sign_reps<<<(unsigned)n_leaves, 512, 0, st>>>(roots, rep_off, reps);
+        wn_list = reps;
+    }
+    tm.mark();
+
+    // ---- 7. winding numbers and signs
+    uint32_t* recheck = S.as<uint32_t>(S_RECHECK, std::max<uint32_t>(n_wn, 1));
+    if (n_wn > 0)
     {
         // a tree of height H needs at most H stack entries
         use_lds = !(H + 2 < LaneStack::DEPTH) || (ctx->paths & BS_PATH_FORCE_LDS_STACK);
         // the visit counters feed the work accounting of profiled runs only
         auto kern = use_lds ? (ctx->counting ? k_winding<LdsStack, true> : k_winding<LdsStack, false>)
                             : (ctx->counting ? k_winding<LaneStack, true> : k_winding<LaneStack, false>);
-        kern<<<div_up(n_act, WN_BLOCK), WN_BLOCK, 0, st>>>(act, n_act, vol->origins, h, nodes, n_nodes, btris,
-                                                           prm.sign_threshold, prm.fp64_band, vol->values, recheck,
-                                                           counters, err);
+        kern<<<div_up(n_wn, WN_BLOCK), WN_BLOCK, 0, st>>>(wn_list, n_wn, vol->origins, h, nodes, n_nodes, btris,
+                                                          prm.sign_threshold, prm.fp64_band, vol->values, recheck,
+                                                          counters, err);
     }
     tm.mark();
     BS_HIP(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, st));
@@ -1374,6 +1510,8 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     if (n_re > 0)
         k_winding_fp64<<<div_up(n_re, 64), 64, 0, st>>>(recheck, n_re, vol->origins, h, nodes, n_nodes, btris,
                                                         prm.sign_threshold, vol->values, err);
+    if (roots) k_sign_broadcast<<<(unsigned)n_leaves, 512, 0, st>>>(roots, vol->values);
+    tm.mark();
     unsigned e = read_scalar(ctx, err);
     if (e & E_STACK) throw Error(BS_ERR_INTERNAL, "winding-number traversal stack overflow");
     if (e & E_TREE) throw Error(BS_ERR_INTERNAL, "BVH parent links are corrupt");
@@ -1401,7 +1539,9 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     stt.ms_leaves = tm.ms(1, 2);
     stt.ms_distance = tm.ms(2, 3);
     stt.ms_bvh = tm.ms(3, 4);
-    stt.ms_winding = tm.ms(4, 5);
+    stt.ms_winding = tm.ms(5, 6);
+    stt.n_sign_evals = n_wn;
+    stt.ms_sign_share = shared_signs ? tm.ms(4, 5) + tm.ms(6, 7) : 0.0;
     *out = guard.release();
 }
 

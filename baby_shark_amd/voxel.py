@@ -258,7 +258,8 @@ def volume_from_voxels(idx, values, voxel_size: float, ctx: Optional[Context] = 
     return Volume(h, ctx)
 
 
-def batch_mesh_to_volume(meshes, voxel_sizes, band: int, ctx: Optional[Context] = None, workers: int = 0):
+def batch_mesh_to_volume(meshes, voxel_sizes, band: int, ctx: Optional[Context] = None, workers: int = 0,
+                         shared_signs: bool = False):
     """MeshToVolume::convert over many meshes in one library call.
 
     `meshes`: CUDA tensors (device input) or arrays (host input), all of one
@@ -287,6 +288,7 @@ def batch_mesh_to_volume(meshes, voxel_sizes, band: int, ctx: Optional[Context] 
         L.bs_m2v_params_default(C.byref(params[i]))
         params[i].voxel_size = float(voxel_sizes[i])
         params[i].band = int(band)
+        params[i].sign_mode = _lib.BS_SIGN_SHARED if shared_signs else _lib.BS_SIGN_PER_VOXEL
     if dev:
         import torch
         torch.cuda.synchronize()
@@ -391,6 +393,13 @@ class MeshToVolume:
 
     def set_voxel_size(self, size: float) -> "MeshToVolume":
         self.params.voxel_size = float(size)
+        return self
+
+    def with_shared_signs(self, on: bool = True) -> "MeshToVolume":
+        """One winding-number evaluation per group of voxels that provably
+        share a sign (closed meshes only); off = the reference's per-voxel
+        signs."""
+        self.params.sign_mode = _lib.BS_SIGN_SHARED if on else _lib.BS_SIGN_PER_VOXEL
         return self
 
     def convert(self, mesh) -> Optional[Volume]:

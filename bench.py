@@ -149,6 +149,42 @@ def pipeline(ctx, level=7, h=0.001, band=3, reps=2):
     return res
 
 
+def shared_signs(ctx, dev, args, ms_plain):
+    """The same convert with shared signs (one winding-number evaluation per
+    group of voxels that no surface point separates): K timed steps after W
+    warm-ups, phase times from HIP events, and the whole volume compared bit
+    for bit with the per-voxel-sign volume."""
+    import numpy as np
+    import baby_shark_amd as bs
+    import torch
+    m2v = bs.MeshToVolume(ctx).with_voxel_size(args.voxel).with_narrow_band_width(args.band)
+    m2s = bs.MeshToVolume(ctx).with_voxel_size(args.voxel).with_narrow_band_width(args.band).with_shared_signs()
+    for _ in range(args.warmup):
+        m2s.convert(dev).free()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        m2s.convert(dev).free()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    ctx.set_profiling(True, counters=False)
+    v = m2s.convert(dev)
+    st = ctx.last_stats()
+    ctx.set_profiling(False)
+    a = v.export()
+    v.free()
+    p = m2v.convert(dev)
+    b = p.export()
+    p.free()
+    diff = int(np.count_nonzero(a.values.view(np.uint32) != b.values.view(np.uint32)))
+    sign_diff = int(np.count_nonzero(np.signbit(a.values[a.active()]) != np.signbit(b.values[b.active()])))
+    return {"ms_per_step": round(ms, 3), "voxels_per_s": st["n_active"] / (ms * 1e-3),
+            "speedup_vs_plain": round(ms_plain / ms, 3), "winding_evaluations": st["n_sign_evals"],
+            "evaluated_fraction": round(st["n_sign_evals"] / st["n_active"], 5),
+            "ms_groups": round(st["ms_sign_share"], 4), "ms_winding": round(st["ms_winding"], 4),
+            "values_differing_from_plain": diff, "sign_mismatches_vs_plain": sign_diff}
+
+
 def torus_and_sphere(ctx, reps=5):
     """The small end-to-end workload: torus (R 1, r 0.35, 128 x 64 segments)
     to a volume at h = 0.02, band 3, united with the sphere r = 0.5 at
@@ -317,6 +353,7 @@ def main():
         "phases_ms": {k: round(float(np.mean([p[k] for p in prof])), 4)
                       for k in ("ms_subdivide", "ms_leaves", "ms_distance", "ms_bvh", "ms_winding")},
     }
+    out["shared_signs"] = shared_signs(ctx, dev, args, ms_step)
     if not args.no_pipeline:
         out["pipeline"] = pipeline(ctx)
         out["torus_and_sphere"] = torus_and_sphere(ctx)

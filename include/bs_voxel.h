@@ -47,7 +47,17 @@ typedef struct {
     float wn_accuracy;    /* accuracy scale of approximate(); reference: 2.0      */
     float sign_threshold; /* wn < threshold => outside; reference: 0.2            */
     float fp64_band;      /* |wn - threshold| < fp64_band is re-evaluated in FP64 */
+    int32_t sign_mode;    /* BS_SIGN_PER_VOXEL (the reference) or BS_SIGN_SHARED  */
 } bs_m2v_params;
+
+/* Sign modes.  PER_VOXEL evaluates the winding number at every active voxel,
+ * as compute_sings does (mesh_to_volume.rs:198-215).  SHARED evaluates it once
+ * per group of a leaf's voxels joined by face links with |d1| + |d2| larger
+ * than the voxel size (plus a rounding margin): no surface point lies between
+ * two such voxels, so for a closed mesh (constant winding number off the
+ * surface) the group shares one sign.  Not for open meshes. */
+#define BS_SIGN_PER_VOXEL 0
+#define BS_SIGN_SHARED 1
 
 typedef struct {
     float voxel_size;
@@ -62,6 +72,8 @@ typedef struct {
     uint64_t n_bvh_nodes, wn_node_visits, wn_leaf_tris, wn_far_evals, n_fp64_rechecks;
     uint64_t wn_leaf_visits, used_hash, used_lds_stack, bvh_height;
     double ms_total, ms_subdivide, ms_leaves, ms_distance, ms_bvh, ms_winding;
+    uint64_t n_sign_evals; /* voxels whose winding number was evaluated          */
+    double ms_sign_share;  /* sign groups, representatives and sign copy (SHARED) */
 } bs_m2v_stats;
 
 /* Work counters of the last bs_volume_offset call. */
