@@ -20,6 +20,7 @@ BS_ERR_HIP = 3
 BS_ERR_UNSUPPORTED_TILE = 4
 BS_ERR_INTERNAL = 5
 BS_ERR_NO_DEVICE = 6
+BS_ERR_IO = 7
 BS_PATH_FORCE_HASH = 1
 BS_PATH_FORCE_LDS_STACK = 2
 BS_SIGN_PER_VOXEL = 0
@@ -28,7 +29,8 @@ BS_PROFILE_EVENTS = 1
 BS_PROFILE_COUNTERS = 2
 
 STATUS_NAMES = {0: "BS_OK", 1: "BS_ERR_EMPTY", 2: "BS_ERR_INVALID_ARG", 3: "BS_ERR_HIP",
-                4: "BS_ERR_UNSUPPORTED_TILE", 5: "BS_ERR_INTERNAL", 6: "BS_ERR_NO_DEVICE"}
+                4: "BS_ERR_UNSUPPORTED_TILE", 5: "BS_ERR_INTERNAL", 6: "BS_ERR_NO_DEVICE",
+                7: "BS_ERR_IO"}
 
 
 class M2VParams(C.Structure):
@@ -117,6 +119,8 @@ def load():
             "bs_volume_export": (C.c_int, [vp, i32p, u64p, f32p]),
             "bs_volume_export_tiles": (C.c_int, [vp, i32p, i32p, f32p]),
             "bs_volume_sphere": (C.c_int, [vp, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, pp]),
+            "bs_volume_cuboid": (C.c_int, [vp] + [C.c_float] * 7 + [pp]),
+            "bs_volume_iwp": (C.c_int, [vp] + [C.c_float] * 8 + [pp]),
             "bs_volume_from_voxels": (C.c_int, [vp, i32p, f32p, C.c_size_t, C.c_float, pp]),
             "bs_volume_union": (C.c_int, [vp, vp, vp]),
             "bs_volume_subtract": (C.c_int, [vp, vp, vp]),
@@ -129,6 +133,8 @@ def load():
             "bs_dual_contour_device": (C.c_int, [vp, vp, C.c_float, C.POINTER(vp), C.POINTER(C.c_size_t)]),
             "bs_ctx_last_dc_stats": (C.c_int, [vp, C.POINTER(DcStats)]),
             "bs_free": (None, [vp]),
+            "bs_stl_read": (C.c_int, [C.c_char_p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_size_t)]),
+            "bs_stl_write": (C.c_int, [C.c_char_p, vp, C.c_size_t]),
             "bs_device_free": (None, [vp]),
             "bs_ctx_last_offset_stats": (C.c_int, [vp, C.POINTER(OffsetStats)]),
             "bs_volume_signs_at": (C.c_int, [vp, vp, i32p, C.c_size_t, C.POINTER(C.c_uint8)]),

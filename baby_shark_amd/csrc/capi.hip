@@ -8,6 +8,7 @@
 #include <string>
 
 #include "common.hpp"
+#include "stl.hpp"
 
 namespace bs {
 
@@ -16,6 +17,8 @@ void set_last_error(const std::string& msg) { g_last_error = msg; }
 
 void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const bs_m2v_params& prm, bs_volume** out);
 void volume_sphere(bs_ctx* ctx, float radius, float cx, float cy, float cz, float h, bs_volume** out);
+void volume_cuboid(bs_ctx* ctx, const float mn[3], const float mx[3], float h, bs_volume** out);
+void volume_iwp(bs_ctx* ctx, const float mn[3], const float mx[3], float cell_size, float h, bs_volume** out);
 void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op);
 void volume_flood_fill(bs_ctx* ctx, bs_volume* v);
 void volume_densify_tiles(bs_ctx* ctx, bs_volume* v);
@@ -340,6 +343,34 @@ bs_status bs_volume_sphere(bs_ctx* ctx, float radius, float cx, float cy, float 
     });
 }
 
+bs_status bs_volume_cuboid(bs_ctx* ctx, float min_x, float min_y, float min_z, float max_x, float max_y,
+                           float max_z, float voxel_size, bs_volume** out) {
+    if (!ctx || !out) {
+        g_last_error = "null argument";
+        return BS_ERR_INVALID_ARG;
+    }
+    *out = nullptr;
+    return guarded([&] {
+        DeviceGuard g(ctx->device);
+        const float mn[3] = {min_x, min_y, min_z}, mx[3] = {max_x, max_y, max_z};
+        volume_cuboid(ctx, mn, mx, voxel_size, out);
+    });
+}
+
+bs_status bs_volume_iwp(bs_ctx* ctx, float min_x, float min_y, float min_z, float max_x, float max_y, float max_z,
+                        float cell_size, float voxel_size, bs_volume** out) {
+    if (!ctx || !out) {
+        g_last_error = "null argument";
+        return BS_ERR_INVALID_ARG;
+    }
+    *out = nullptr;
+    return guarded([&] {
+        DeviceGuard g(ctx->device);
+        const float mn[3] = {min_x, min_y, min_z}, mx[3] = {max_x, max_y, max_z};
+        volume_iwp(ctx, mn, mx, cell_size, voxel_size, out);
+    });
+}
+
 bs_status bs_volume_from_voxels(bs_ctx* ctx, const int32_t* idx, const float* values, size_t n, float voxel_size,
                                 bs_volume** out) {
     if (!ctx || !out || (n > 0 && (!idx || !values))) {
@@ -460,6 +491,37 @@ bs_status bs_ctx_last_dc_stats(const bs_ctx* ctx, bs_dc_stats* out) {
     if (!ctx || !out) return BS_ERR_INVALID_ARG;
     *out = ctx->last_dc;
     return BS_OK;
+}
+
+bs_status bs_stl_read(const char* path, float** out_xyz, size_t* n_tris) {
+    if (!path || !out_xyz || !n_tris) {
+        g_last_error = "null argument";
+        return BS_ERR_INVALID_ARG;
+    }
+    *out_xyz = nullptr;
+    *n_tris = 0;
+    return guarded([&] {
+        std::vector<float> xyz;
+        const std::string err = stl_read(path, xyz);
+        if (!err.empty()) throw Error(BS_ERR_IO, err);
+        if (xyz.empty()) return;
+        float* p = static_cast<float*>(std::malloc(sizeof(float) * xyz.size()));
+        if (!p) throw std::bad_alloc();
+        std::memcpy(p, xyz.data(), sizeof(float) * xyz.size());
+        *out_xyz = p;
+        *n_tris = xyz.size() / 9;
+    });
+}
+
+bs_status bs_stl_write(const char* path, const float* tri_xyz, size_t n_tris) {
+    if (!path || (n_tris > 0 && !tri_xyz)) {
+        g_last_error = "null argument";
+        return BS_ERR_INVALID_ARG;
+    }
+    return guarded([&] {
+        const std::string err = stl_write(path, tri_xyz, n_tris);
+        if (!err.empty()) throw Error(BS_ERR_IO, err);
+    });
 }
 
 void bs_free(void* p) { std::free(p); }

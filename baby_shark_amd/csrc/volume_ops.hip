@@ -173,81 +173,182 @@ __global__ __launch_bounds__(256) void k_csg_assemble(uint64_t n_out, const int4
     if (lane == 0) origins[leaf] = origins_in[leaf];
 }
 
-// ------------------------------------------------------------------ sphere
-// Volume::from_fn (volume/mod.rs:40-72) with f(p) = |p - o| - r, the function
-// of VolumeBuilder::sphere (builder.rs:21-30).  The reference visits every
-// index of the inclusive box [lo, hi]^3, evaluates f at p = idx * h and
-// inserts the value when |f| <= (1 + 1) * h; a leaf exists when it received a
-// value and its other entries keep the default 0.0.  Here one wavefront takes
-// one 8^3 cell of the box; cells whose distance bounds put every entry outside
-// the band (with a margin far above f32 rounding) are skipped without
-// evaluation.  Pass 1 flags cells that hold a value, pass 2 writes them at
-// their exclusive-scan position: cells are numbered in (x, y, z) order, so the
-// leaves come out in lexicographic origin order.
-struct SphereGrid {
-    float r, ox, oy, oz, h, band;
-    double margin;
+// ------------------------------------------------------------------ from_fn
+// Volume::from_fn (volume/mod.rs:40-72) for the functions of VolumeBuilder
+// (builder.rs:21-76).  The reference visits every index of the inclusive box
+// [lo, hi]^3, evaluates f at p = idx * h and inserts the value when
+// |f| <= (width + 1) * h; a leaf exists when it received a value and its other
+// entries keep the default 0.0.  Here one wavefront takes one 8^3 cell of the
+// box; a cell whose bound puts every entry outside the band (with a margin far
+// above f32 rounding) is skipped without evaluation.  Pass 1 flags cells that
+// hold a value, pass 2 writes them at their exclusive-scan position: cells are
+// numbered in (x, y, z) order, so the leaves come out in origin order.
+struct FnGrid {
+    float h, band;
     int32_t lo[3], hi[3];  // inclusive voxel index box
     int32_t c0[3];         // first cell (in units of 8 voxels)
     uint32_t nc[3];        // cells per axis
 };
 
-__device__ inline bool sphere_cell_culled(const SphereGrid& g, int32_t cx, int32_t cy, int32_t cz) {
-    const int32_t c[3] = {cx, cy, cz};
-    const double o[3] = {g.ox, g.oy, g.oz};
-    double dmin2 = 0.0, dmax2 = 0.0;
+// sample points of cell c inside the index box, per axis [p0, p1] (double)
+__device__ inline void cell_span(const FnGrid& g, const int32_t c[3], double p0[3], double p1[3]) {
     for (int k = 0; k < 3; ++k) {
-        const int32_t i0 = max(c[k] * 8, g.lo[k]), i1 = min(c[k] * 8 + 7, g.hi[k]);
-        const double p0 = (double)i0 * (double)g.h - o[k], p1 = (double)i1 * (double)g.h - o[k];
-        const double near = (p0 > 0.0) ? p0 : (p1 < 0.0 ? -p1 : 0.0);
-        const double far = fmax(fabs(p0), fabs(p1));
-        dmin2 += near * near;
-        dmax2 += far * far;
+        p0[k] = (double)max(c[k] * 8, g.lo[k]) * (double)g.h;
+        p1[k] = (double)min(c[k] * 8 + 7, g.hi[k]) * (double)g.h;
     }
-    const double lim = (double)g.band + g.margin;
-    return (sqrt(dmin2) - (double)g.r > lim) || ((double)g.r - sqrt(dmax2) > lim);
 }
 
-// f32 evaluation in the reference's operation order: p = idx as f32 * h,
-// d = p - o, |d| = sqrt((dx*dx + dy*dy) + dz*dz), f = |d| - r.
-__device__ inline float sphere_value(const SphereGrid& g, int32_t xi, int32_t yi, int32_t zi) {
-    const float dx = __fsub_rn(__fmul_rn((float)xi, g.h), g.ox);
-    const float dy = __fsub_rn(__fmul_rn((float)yi, g.h), g.oy);
-    const float dz = __fsub_rn(__fmul_rn((float)zi, g.h), g.oz);
-    const float n2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
-    return __fsub_rn(sqrt_rn(n2), g.r);
+// f32 box helpers in the reference's order: box3.rs:115-122, :96-112
+__device__ inline bool box_contains(const float mn[3], const float mx[3], float x, float y, float z) {
+    return x >= mn[0] && x <= mx[0] && y >= mn[1] && y <= mx[1] && z >= mn[2] && z <= mx[2];
+}
+__device__ inline float box_sq_dist(const float mn[3], const float mx[3], const float p[3]) {
+    float sq = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (p[i] < mn[i]) sq = __fadd_rn(sq, __fmul_rn(__fsub_rn(mn[i], p[i]), __fsub_rn(mn[i], p[i])));
+        if (p[i] > mx[i]) sq = __fadd_rn(sq, __fmul_rn(__fsub_rn(p[i], mx[i]), __fsub_rn(p[i], mx[i])));
+    }
+    return sq;
+}
+// exact signed distance to a box (double): 1-Lipschitz
+__device__ inline double box_sdf(const double mn[3], const double mx[3], const double p[3]) {
+    double out2 = 0.0, in = 1e300;
+    bool inside = true;
+    for (int k = 0; k < 3; ++k) {
+        const double d = p[k] < mn[k] ? mn[k] - p[k] : (p[k] > mx[k] ? p[k] - mx[k] : 0.0);
+        out2 += d * d;
+        inside &= d == 0.0;
+        in = fmin(in, fmin(p[k] - mn[k], mx[k] - p[k]));
+    }
+    return inside ? -in : sqrt(out2);
 }
 
-template <bool WRITE>
-__global__ __launch_bounds__(256) void k_sphere(SphereGrid g, uint64_t n_cells, uint32_t* __restrict__ cell_flag,
-                                                const uint32_t* __restrict__ cell_off, int4* __restrict__ origins,
-                                                uint64_t* __restrict__ masks, float* __restrict__ values) {
+// VolumeBuilder::sphere (builder.rs:21-30): f = |p - o| - r
+struct SphereFn {
+    float r, ox, oy, oz;
+    double margin;
+    __device__ bool culled(const FnGrid& g, const int32_t c[3]) const {
+        const double o[3] = {ox, oy, oz};
+        double p0[3], p1[3];
+        cell_span(g, c, p0, p1);
+        double dmin2 = 0.0, dmax2 = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            const double a = p0[k] - o[k], b = p1[k] - o[k];
+            const double near = (a > 0.0) ? a : (b < 0.0 ? -b : 0.0);
+            const double far = fmax(fabs(a), fabs(b));
+            dmin2 += near * near;
+            dmax2 += far * far;
+        }
+        const double lim = (double)g.band + margin;
+        return (sqrt(dmin2) - (double)r > lim) || ((double)r - sqrt(dmax2) > lim);
+    }
+    // f32 in the reference's order: d = p - o, |d| = sqrt((dx*dx + dy*dy) + dz*dz)
+    __device__ float value(const FnGrid& g, int32_t xi, int32_t yi, int32_t zi) const {
+        const float dx = __fsub_rn(__fmul_rn((float)xi, g.h), ox);
+        const float dy = __fsub_rn(__fmul_rn((float)yi, g.h), oy);
+        const float dz = __fsub_rn(__fmul_rn((float)zi, g.h), oz);
+        const float n2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+        return __fsub_rn(sqrt_rn(n2), r);
+    }
+};
+
+// VolumeBuilder::cuboid (builder.rs:32-51): inside, minus the f32::min chain
+// of the six face distances; outside, the distance to the box
+struct CuboidFn {
+    float mn[3], mx[3];
+    double margin;
+    __device__ bool culled(const FnGrid& g, const int32_t c[3]) const {
+        double p0[3], p1[3], ctr[3], bmn[3], bmx[3], r2 = 0.0;
+        cell_span(g, c, p0, p1);
+        for (int k = 0; k < 3; ++k) {
+            ctr[k] = 0.5 * (p0[k] + p1[k]);
+            r2 += 0.25 * (p1[k] - p0[k]) * (p1[k] - p0[k]);
+            bmn[k] = mn[k];
+            bmx[k] = mx[k];
+        }
+        return fabs(box_sdf(bmn, bmx, ctr)) - sqrt(r2) > (double)g.band + margin;
+    }
+    __device__ float value(const FnGrid& g, int32_t xi, int32_t yi, int32_t zi) const {
+        const float p[3] = {__fmul_rn((float)xi, g.h), __fmul_rn((float)yi, g.h), __fmul_rn((float)zi, g.h)};
+        if (box_contains(mn, mx, p[0], p[1], p[2])) {
+            float m = fminf(__fsub_rn(p[0], mn[0]), __fsub_rn(mx[0], p[0]));
+            m = fminf(m, __fsub_rn(p[1], mn[1]));
+            m = fminf(m, __fsub_rn(mx[1], p[1]));
+            m = fminf(m, __fsub_rn(p[2], mn[2]));
+            m = fminf(m, __fsub_rn(mx[2], p[2]));
+            return -m;
+        }
+        return sqrt_rn(box_sq_dist(mn, mx, p));
+    }
+};
+
+// VolumeBuilder::iwp (builder.rs:53-76): inside [mn, mx] the IWP surface
+// v = -(cx + cy + cz - 0.51 (cx cy + cy cz + cz cx) - 1) times the cell size,
+// c* = cos(p * (1 / cell)); outside it the distance to the box grown by one
+// voxel (zero in that one-voxel shell).  cos is correctly rounded (through
+// double); the reference's is the platform libm's cosf.
+struct IwpFn {
+    float mn[3], mx[3], imn[3], imx[3], inv, cell;
+    double margin;
+    __device__ bool culled(const FnGrid& g, const int32_t c[3]) const {
+        double p0[3], p1[3], ctr[3], r2 = 0.0;
+        cell_span(g, c, p0, p1);
+        for (int k = 0; k < 3; ++k) {
+            // only cells well inside the sampling box: the function jumps at its faces
+            if (p0[k] < (double)mn[k] + margin || p1[k] > (double)mx[k] - margin) return false;
+            ctr[k] = 0.5 * (p0[k] + p1[k]);
+            r2 += 0.25 * (p1[k] - p0[k]) * (p1[k] - p0[k]);
+        }
+        const double ic = 1.0 / (double)cell;
+        const double a = cos(ctr[0] * ic), b = cos(ctr[1] * ic), d = cos(ctr[2] * ic);
+        const double v = -(a + b + d - 0.51 * (a * b + b * d + d * a) - 1.0) * (double)cell;
+        // |grad v| <= sqrt(3) * (1 + 2 * 0.51) < 3.5
+        return fabs(v) - 3.5 * sqrt(r2) > (double)g.band + margin;
+    }
+    __device__ float value(const FnGrid& g, int32_t xi, int32_t yi, int32_t zi) const {
+        const float p[3] = {__fmul_rn((float)xi, g.h), __fmul_rn((float)yi, g.h), __fmul_rn((float)zi, g.h)};
+        if (!box_contains(mn, mx, p[0], p[1], p[2])) return sqrt_rn(box_sq_dist(imn, imx, p));
+        const float cx = (float)cos((double)__fmul_rn(p[0], inv));
+        const float cy = (float)cos((double)__fmul_rn(p[1], inv));
+        const float cz = (float)cos((double)__fmul_rn(p[2], inv));
+        const float s = __fadd_rn(__fadd_rn(cx, cy), cz);
+        const float q = __fadd_rn(__fadd_rn(__fmul_rn(cx, cy), __fmul_rn(cy, cz)), __fmul_rn(cz, cx));
+        const float v = -__fsub_rn(__fsub_rn(s, __fmul_rn(0.51f, q)), 1.0f);
+        return __fmul_rn(v, cell);
+    }
+};
+
+template <class F, bool WRITE>
+__global__ __launch_bounds__(256) void k_from_fn(FnGrid g, F f, uint64_t n_cells, uint32_t* __restrict__ cell_flag,
+                                                 const uint32_t* __restrict__ cell_off, int4* __restrict__ origins,
+                                                 uint64_t* __restrict__ masks, float* __restrict__ values) {
     const uint64_t cell = (uint64_t)blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x >> 6);
     if (cell >= n_cells) return;
     if (WRITE && !cell_flag[cell]) return;
     const uint32_t iz = (uint32_t)(cell % g.nc[2]);
     const uint32_t iy = (uint32_t)((cell / g.nc[2]) % g.nc[1]);
     const uint32_t ix = (uint32_t)(cell / ((uint64_t)g.nc[2] * g.nc[1]));
-    const int32_t cx = g.c0[0] + (int32_t)ix, cy = g.c0[1] + (int32_t)iy, cz = g.c0[2] + (int32_t)iz;
+    const int32_t c[3] = {g.c0[0] + (int32_t)ix, g.c0[1] + (int32_t)iy, g.c0[2] + (int32_t)iz};
     const int lane = threadIdx.x & 63;
-    if (!WRITE && sphere_cell_culled(g, cx, cy, cz)) {
+    if (!WRITE && f.culled(g, c)) {
         if (lane == 0) cell_flag[cell] = 0;
         return;
     }
     const int x = lane >> 3, y = lane & 7;
-    const int32_t xi = cx * 8 + x, yi = cy * 8 + y;
+    const int32_t xi = c[0] * 8 + x, yi = c[1] * 8 + y;
     const bool xy_in = xi >= g.lo[0] && xi <= g.hi[0] && yi >= g.lo[1] && yi <= g.hi[1];
     uint32_t row = 0;
     float v[8];
 #pragma unroll
     for (int z = 0; z < 8; ++z) {
-        const int32_t zi = cz * 8 + z;
+        const int32_t zi = c[2] * 8 + z;
         v[z] = 0.0f;
         if (xy_in && zi >= g.lo[2] && zi <= g.hi[2]) {
-            const float f = sphere_value(g, xi, yi, zi);
-            if (!(fabsf(f) > g.band)) {
+            const float fv = f.value(g, xi, yi, zi);
+            if (!(fabsf(fv) > g.band)) {
                 row |= 1u << z;
-                v[z] = f;
+                v[z] = fv;
             }
         }
     }
@@ -262,7 +363,7 @@ __global__ __launch_bounds__(256) void k_sphere(SphereGrid g, uint64_t n_cells, 
     po[1] = make_float4(v[4], v[5], v[6], v[7]);
     const uint64_t w = gather_word(row, y);
     if (y == 0) masks[out * 8 + x] = w;
-    if (lane == 0) origins[out] = make_int4(cx * 8, cy * 8, cz * 8, 0);
+    if (lane == 0) origins[out] = make_int4(c[0] * 8, c[1] * 8, c[2] * 8, 0);
 }
 
 // Sign bit of entry (leaf, offset) for each query q = leaf * 512 + offset.
@@ -598,56 +699,45 @@ void volume_densify_tiles(bs_ctx* ctx, bs_volume* v) {
     v->topo = t.compact(new_index);
 }
 
-// VolumeBuilder::sphere (builder.rs:21-30) through Volume::from_fn.
-void volume_sphere(bs_ctx* ctx, float radius, float cx, float cy, float cz, float h, bs_volume** out) {
-    BS_REQUIRE(std::isfinite(radius) && std::isfinite(cx) && std::isfinite(cy) && std::isfinite(cz),
-               BS_ERR_INVALID_ARG, "sphere radius and centre must be finite");
-    BS_REQUIRE(std::isfinite(h) && h > 0.0f, BS_ERR_INVALID_ARG, "voxel size must be finite and positive");
-    // builder.rs:22-25: offset = r + band_width * h with band_width 1; min/max = o -+ offset
-    const float offset = radius + 1.0f * h;
-    const float c[3] = {cx, cy, cz};
-    // volume/mod.rs:49-51: band (1 + 1) * h; index box floor(min / h) .. ceil(max / h)
-    SphereGrid g{};
-    g.r = radius;
-    g.ox = cx;
-    g.oy = cy;
-    g.oz = cz;
+// The index box of Volume::from_fn (volume/mod.rs:49-51): floor(min / h) ..=
+// ceil(max / h) per axis, in f32 as the reference divides; pmax is the largest
+// sample coordinate.  Returns false for an empty box.
+static bool fn_grid(FnGrid& g, const float mn[3], const float mx[3], float h, int width, double& pmax) {
     g.h = h;
-    g.band = 2.0f * h;
+    g.band = (float)(width + 1) * h;
     bool empty = false;
-    double pmax = 0.0;
+    pmax = 0.0;
     for (int k = 0; k < 3; ++k) {
-        const float mn = c[k] + (-offset), mx = c[k] + offset;
-        const double lo = std::floor((double)(mn / h)), hi = std::ceil((double)(mx / h));
+        const double lo = std::floor((double)(mn[k] / h)), hi = std::ceil((double)(mx[k] / h));
         BS_REQUIRE(std::fabs(lo) < (double)(1 << 22) && std::fabs(hi) < (double)(1 << 22), BS_ERR_INVALID_ARG,
-                   "sphere index box exceeds +-2^22 voxels");
+                   "from_fn index box exceeds +-2^22 voxels");
         g.lo[k] = (int32_t)lo;
         g.hi[k] = (int32_t)hi;
         empty |= g.lo[k] > g.hi[k];
         pmax = std::max(pmax, std::max(std::fabs(lo), std::fabs(hi)) * (double)h);
     }
-    // f32 rounding of idx*h, p - o, the norm and the subtraction stays below
-    // 8 ulps of the largest magnitude involved; the margin is far above it.
-    g.margin = 2.0 * (double)h + 1e-5 * (pmax + std::fabs((double)cx) + std::fabs((double)cy) +
-                                         std::fabs((double)cz) + std::fabs((double)radius));
-    uint64_t n_cells = 1;
     for (int k = 0; k < 3; ++k) {
         g.c0[k] = empty ? 0 : floor_div8(g.lo[k]);
         const int32_t c1 = empty ? -1 : floor_div8(g.hi[k]);
         g.nc[k] = (uint32_t)(c1 - g.c0[k] + 1);
-        n_cells *= g.nc[k];
     }
-    BS_REQUIRE(n_cells < (1ull << 31), BS_ERR_INVALID_ARG, "sphere spans more than 2^31 leaf cells");
+    return !empty;
+}
 
+// Both passes of k_from_fn over the cells of g.
+template <class F>
+static bs_volume* build_from_fn(bs_ctx* ctx, const FnGrid& g, const F& f) {
+    uint64_t n_cells = (uint64_t)g.nc[0] * g.nc[1] * g.nc[2];
+    BS_REQUIRE(n_cells < (1ull << 31), BS_ERR_INVALID_ARG, "from_fn box spans more than 2^31 leaf cells");
     auto* v = new bs_volume();
     v->device = ctx->device;
-    v->voxel_size = h;
+    v->voxel_size = g.h;
     try {
         if (n_cells) {
             uint32_t* flag = ctx->scratch.as<uint32_t>(S_MISC0, n_cells);
             uint32_t* off = ctx->scratch.as<uint32_t>(S_MISC1, n_cells);
             const unsigned grid = div_up(n_cells, WAVES_PER_BLOCK);
-            k_sphere<false><<<grid, 256, 0, ctx->stream>>>(g, n_cells, flag, nullptr, nullptr, nullptr, nullptr);
+            k_from_fn<F, false><<<grid, 256, 0, ctx->stream>>>(g, f, n_cells, flag, nullptr, nullptr, nullptr, nullptr);
             BS_HIP(hipGetLastError());
             size_t bytes = 0;
             BS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, flag, off, (int)n_cells, ctx->stream));
@@ -660,9 +750,9 @@ void volume_sphere(bs_ctx* ctx, float radius, float cx, float cy, float cz, floa
             const uint64_t n_leaves = (uint64_t)tail[0] + tail[1];
             alloc_leaves(v, n_leaves);
             if (n_leaves) {
-                k_sphere<true><<<grid, 256, 0, ctx->stream>>>(g, n_cells, flag, off,
-                                                               reinterpret_cast<int4*>(v->origins), v->masks,
-                                                               v->values);
+                k_from_fn<F, true><<<grid, 256, 0, ctx->stream>>>(g, f, n_cells, flag, off,
+                                                                  reinterpret_cast<int4*>(v->origins), v->masks,
+                                                                  v->values);
                 BS_HIP(hipGetLastError());
             }
             BS_HIP(hipStreamSynchronize(ctx->stream));
@@ -672,7 +762,85 @@ void volume_sphere(bs_ctx* ctx, float radius, float cx, float cy, float cz, floa
         delete v;
         throw;
     }
-    *out = v;
+    return v;
+}
+
+static bool all_finite(std::initializer_list<float> xs) {
+    for (float x : xs)
+        if (!std::isfinite(x)) return false;
+    return true;
+}
+
+// VolumeBuilder::sphere (builder.rs:21-30) through Volume::from_fn.
+void volume_sphere(bs_ctx* ctx, float radius, float cx, float cy, float cz, float h, bs_volume** out) {
+    BS_REQUIRE(all_finite({radius, cx, cy, cz}), BS_ERR_INVALID_ARG, "sphere radius and centre must be finite");
+    BS_REQUIRE(std::isfinite(h) && h > 0.0f, BS_ERR_INVALID_ARG, "voxel size must be finite and positive");
+    // builder.rs:22-25: offset = r + band_width * h with band_width 1; min/max = o -+ offset
+    const float offset = radius + 1.0f * h;
+    const float c[3] = {cx, cy, cz};
+    float mn[3], mx[3];
+    for (int k = 0; k < 3; ++k) {
+        mn[k] = c[k] + (-offset);
+        mx[k] = c[k] + offset;
+    }
+    FnGrid g{};
+    double pmax = 0.0;
+    fn_grid(g, mn, mx, h, 1, pmax);
+    // f32 rounding of idx*h, p - o, the norm and the subtraction stays below
+    // 8 ulps of the largest magnitude involved; the margin is far above it.
+    const double margin = 2.0 * (double)h + 1e-5 * (pmax + std::fabs((double)cx) + std::fabs((double)cy) +
+                                                    std::fabs((double)cz) + std::fabs((double)radius));
+    *out = build_from_fn(ctx, g, SphereFn{radius, cx, cy, cz, margin});
+}
+
+// VolumeBuilder::cuboid (builder.rs:32-51) through Volume::from_fn.
+void volume_cuboid(bs_ctx* ctx, const float mn[3], const float mx[3], float h, bs_volume** out) {
+    BS_REQUIRE(all_finite({mn[0], mn[1], mn[2], mx[0], mx[1], mx[2]}), BS_ERR_INVALID_ARG,
+               "cuboid corners must be finite");
+    BS_REQUIRE(std::isfinite(h) && h > 0.0f, BS_ERR_INVALID_ARG, "voxel size must be finite and positive");
+    const float offset = 1.0f * h;  // band_width 1
+    float gmn[3], gmx[3];
+    double bmax = 0.0;
+    CuboidFn f{};
+    for (int k = 0; k < 3; ++k) {
+        gmn[k] = mn[k] + (-offset);
+        gmx[k] = mx[k] + offset;
+        f.mn[k] = mn[k];
+        f.mx[k] = mx[k];
+        bmax = std::max({bmax, std::fabs((double)mn[k]), std::fabs((double)mx[k])});
+    }
+    FnGrid g{};
+    double pmax = 0.0;
+    fn_grid(g, gmn, gmx, h, 1, pmax);
+    f.margin = 2.0 * (double)h + 1e-5 * (pmax + bmax);
+    *out = build_from_fn(ctx, g, f);
+}
+
+// VolumeBuilder::iwp (builder.rs:53-76) through Volume::from_fn.
+void volume_iwp(bs_ctx* ctx, const float mn[3], const float mx[3], float cell_size, float h, bs_volume** out) {
+    BS_REQUIRE(all_finite({mn[0], mn[1], mn[2], mx[0], mx[1], mx[2]}), BS_ERR_INVALID_ARG,
+               "iwp corners must be finite");
+    BS_REQUIRE(std::isfinite(h) && h > 0.0f, BS_ERR_INVALID_ARG, "voxel size must be finite and positive");
+    BS_REQUIRE(std::isfinite(cell_size) && cell_size > 0.0f, BS_ERR_INVALID_ARG,
+               "iwp cell size must be finite and positive");
+    IwpFn f{};
+    f.inv = 1.0f / cell_size;
+    f.cell = cell_size;
+    double bmax = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        f.mn[k] = mn[k];
+        f.mx[k] = mx[k];
+        f.imn[k] = mn[k] + (-h);
+        f.imx[k] = mx[k] + h;
+        bmax = std::max({bmax, std::fabs((double)mn[k]), std::fabs((double)mx[k])});
+    }
+    FnGrid g{};
+    double pmax = 0.0;
+    fn_grid(g, f.imn, f.imx, h, 2, pmax);
+    // the cos arguments x = p / cell carry ~3 ulps of |x|; v sums six terms of them
+    const double xmax = (pmax + bmax) / (double)cell_size;
+    f.margin = 2.0 * (double)h + (double)cell_size * (1e-5 + 1e-6 * xmax) + 1e-5 * (pmax + bmax);
+    *out = build_from_fn(ctx, g, f);
 }
 
 }  // namespace bs

@@ -243,6 +243,22 @@ class VolumeBuilder:
                                       C.byref(h)))
         return Volume(h, self.ctx)
 
+    def cuboid(self, min_corner, max_corner) -> Volume:
+        """The box's signed distance, kept within 2 voxels (builder.rs:32-51)."""
+        a = [float(c) for c in min_corner]
+        b = [float(c) for c in max_corner]
+        h = C.c_void_p()
+        check(load().bs_volume_cuboid(self.ctx.handle, *a, *b, self.voxel_size, C.byref(h)))
+        return Volume(h, self.ctx)
+
+    def iwp(self, min_corner, max_corner, cell_size: float) -> Volume:
+        """The IWP triply periodic surface in a box, band width 2 (builder.rs:53-76)."""
+        a = [float(c) for c in min_corner]
+        b = [float(c) for c in max_corner]
+        h = C.c_void_p()
+        check(load().bs_volume_iwp(self.ctx.handle, *a, *b, float(cell_size), self.voxel_size, C.byref(h)))
+        return Volume(h, self.ctx)
+
 
 def volume_from_voxels(idx, values, voxel_size: float, ctx: Optional[Context] = None) -> Volume:
     """A grid built by inserting values[i] at voxel idx[i] in order."""
@@ -422,3 +438,23 @@ class MeshToVolume:
             return None
         check(st)
         return Volume(h, self.ctx)
+
+
+def read_stl(path) -> np.ndarray:
+    """Binary STL (StlReader, io/stl.rs:64-92) as an (n, 3, 3) float32 soup."""
+    L = load()
+    out = C.POINTER(C.c_float)()
+    n = C.c_size_t(0)
+    check(L.bs_stl_read(str(path).encode(), C.byref(out), C.byref(n)))
+    if n.value == 0:
+        return np.zeros((0, 3, 3), np.float32)
+    try:
+        return np.ctypeslib.as_array(out, shape=(n.value * 9,)).copy().reshape(-1, 3, 3)
+    finally:
+        L.bs_free(C.cast(out, C.c_void_p))
+
+
+def write_stl(path, soup) -> None:
+    """Binary STL (StlWriter, io/stl.rs:150-199) of an (n, 3, 3) soup."""
+    a = np.ascontiguousarray(np.asarray(soup, dtype=np.float32).reshape(-1, 9))
+    check(load().bs_stl_write(str(path).encode(), a.ctypes.data_as(C.c_void_p), a.shape[0]))

@@ -34,7 +34,8 @@ typedef enum {
     BS_ERR_HIP = 3,              /* HIP runtime error                        */
     BS_ERR_UNSUPPORTED_TILE = 4, /* reference panics with todo!() on a tile  */
     BS_ERR_INTERNAL = 5,         /* reference panics (unreachable!/unwrap)   */
-    BS_ERR_NO_DEVICE = 6         /* no HIP device / library built without GPU */
+    BS_ERR_NO_DEVICE = 6,        /* no HIP device / library built without GPU */
+    BS_ERR_IO = 7                /* a file could not be read or written       */
 } bs_status;
 
 typedef struct bs_ctx bs_ctx;
@@ -175,6 +176,22 @@ bs_status bs_volume_export_tiles(const bs_volume* v, int32_t* origins, int32_t* 
 bs_status bs_volume_sphere(bs_ctx* ctx, float radius, float cx, float cy, float cz, float voxel_size,
                            bs_volume** out);
 
+/* VolumeBuilder::cuboid (builder.rs:32-51) through Volume::from_fn: inside
+ * the box [min, max], minus the distance to the nearest face; outside, the
+ * distance to the box; sampled over floor((min - h)/h) .. ceil((max + h)/h),
+ * kept where |f| <= 2h. */
+bs_status bs_volume_cuboid(bs_ctx* ctx, float min_x, float min_y, float min_z, float max_x, float max_y,
+                           float max_z, float voxel_size, bs_volume** out);
+
+/* VolumeBuilder::iwp (builder.rs:53-76) through Volume::from_fn: inside
+ * [min, max], cell_size * -(cx + cy + cz - 0.51 (cx cy + cy cz + cz cx) - 1)
+ * with c* = cos(p* / cell_size); outside, the distance to the box grown by h;
+ * band width 2 (kept where |f| <= 3h).  cos is correctly rounded here; the
+ * reference uses the platform libm's cosf, so values may differ by an ulp.
+ * cell_size must be finite and > 0 (the reference fills NaN otherwise). */
+bs_status bs_volume_iwp(bs_ctx* ctx, float min_x, float min_y, float min_z, float max_x, float max_y, float max_z,
+                        float cell_size, float voxel_size, bs_volume** out);
+
 /* A grid built by inserting voxel values one by one (TreeNode::insert on an
  * empty VolumeGrid: root_node/tree_node.rs, internal_node/tree_node.rs:54-79,
  * leaf_node/tree_node.rs:46-50): idx n*3 int32 voxel indices,
@@ -226,6 +243,14 @@ bs_status bs_dual_contour_device(bs_ctx* ctx, const bs_volume* v, float voxel_si
 bs_status bs_ctx_last_dc_stats(const bs_ctx* ctx, bs_dc_stats* out);
 void bs_free(void* p);
 void bs_device_free(void* p_dev);
+
+/* Binary STL (io/stl.rs), host only, no device needed.  bs_stl_read is
+ * StlReader (:64-92): *out_xyz receives 9 floats per face (malloc'd, release
+ * with bs_free; NULL for zero faces); a missing or short file gives BS_ERR_IO.
+ * bs_stl_write is StlWriter (:150-199): unit face normals of (b-a) x (c-a),
+ * zeros for degenerate faces. */
+bs_status bs_stl_read(const char* path, float** out_xyz, size_t* n_tris);
+bs_status bs_stl_write(const char* path, const float* tri_xyz, size_t n_tris);
 
 /* Extension with no reference counterpart: every active tile becomes the
  * leaves it covers, all 512 entries active and equal to the tile value (a

@@ -377,15 +377,14 @@ void or_sample_unsigned(const float* tris, size_t n, float voxel_size, int64_t b
 }
 
 // ---------------------------------------------------------------- volume ops
-// Volume::from_fn (volume/mod.rs:40-72) for VolumeBuilder::sphere (builder.rs:21-30)
-or_volume* or_volume_sphere(float radius, float cx, float cy, float cz, float h) {
+extern "C++" {
+// Volume::from_fn (volume/mod.rs:40-72): every grid point of the index box
+// floor(min / h) ..= ceil(max / h), p = idx as f32 * h, kept when
+// |f(p)| <= (width + 1) as f32 * h.
+template <class F>
+static or_volume* from_fn(float h, const Vec3f& mn, const Vec3f& mx, int width, F f) {
     auto g = std::make_unique<VolumeGrid>();
-    const float band_offset = 1.0f * h;  // band_width = 1, `band_width as f32 * voxel_size`
-    const float offset = radius + band_offset;
-    const Vec3f o{cx, cy, cz};
-    const Vec3f mn{cx + (-offset), cy + (-offset), cz + (-offset)};
-    const Vec3f mx{cx + offset, cy + offset, cz + offset};
-    const float nbw = (float)(1 + 1) * h;
+    const float nbw = (float)(width + 1) * h;
     int64_t lo[3], hi[3];
     for (int k = 0; k < 3; ++k) {
         lo[k] = f2i(std::floor(mn[k] / h));
@@ -395,13 +394,73 @@ or_volume* or_volume_sphere(float radius, float cx, float cy, float cz, float h)
         for (int64_t y = lo[1]; y <= hi[1]; ++y)
             for (int64_t z = lo[2]; z <= hi[2]; ++z) {
                 Vec3f p{(float)x * h, (float)y * h, (float)z * h};
-                float v = norm(sub(p, o)) - radius;
+                float v = f(p);
                 if (std::fabs(v) > nbw) continue;
                 g->insert({x, y, z}, v);
             }
     auto* r = new or_volume{std::move(g), h, {}, {}};
     r->index();
     return r;
+}
+
+// box3.rs:115-122
+static bool box_contains(const Vec3f& mn, const Vec3f& mx, const Vec3f& p) {
+    return p.x >= mn.x && p.x <= mx.x && p.y >= mn.y && p.y <= mx.y && p.z >= mn.z && p.z <= mx.z;
+}
+
+// box3.rs:96-112
+static float box_squared_distance(const Vec3f& mn, const Vec3f& mx, const Vec3f& p) {
+    float sq = 0.0f;
+    for (int i = 0; i < 3; ++i) {
+        const float v = p[i];
+        if (v < mn[i]) sq += (mn[i] - v) * (mn[i] - v);
+        if (v > mx[i]) sq += (v - mx[i]) * (v - mx[i]);
+    }
+    return sq;
+}
+}  // extern "C++"
+
+// VolumeBuilder::sphere (builder.rs:21-30)
+or_volume* or_volume_sphere(float radius, float cx, float cy, float cz, float h) {
+    const float band_offset = 1.0f * h;  // band_width = 1, `band_width as f32 * voxel_size`
+    const float offset = radius + band_offset;
+    const Vec3f o{cx, cy, cz};
+    const Vec3f mn{cx + (-offset), cy + (-offset), cz + (-offset)};
+    const Vec3f mx{cx + offset, cy + offset, cz + offset};
+    return from_fn(h, mn, mx, 1, [&](const Vec3f& p) { return norm(sub(p, o)) - radius; });
+}
+
+// VolumeBuilder::cuboid (builder.rs:32-51): inside, minus the distance to the
+// nearest face (f32::min chain); outside, the distance to the box.
+or_volume* or_volume_cuboid(const float* mn_, const float* mx_, float h) {
+    const Vec3f mn{mn_[0], mn_[1], mn_[2]}, mx{mx_[0], mx_[1], mx_[2]};
+    const float offset = 1.0f * h;
+    const Vec3f gmn{mn.x + (-offset), mn.y + (-offset), mn.z + (-offset)};
+    const Vec3f gmx{mx.x + offset, mx.y + offset, mx.z + offset};
+    return from_fn(h, gmn, gmx, 1, [&](const Vec3f& p) {
+        if (box_contains(mn, mx, p))
+            return -std::fmin(std::fmin(std::fmin(std::fmin(std::fmin(p.x - mn.x, mx.x - p.x), p.y - mn.y), mx.y - p.y),
+                                        p.z - mn.z),
+                              mx.z - p.z);
+        return std::sqrt(box_squared_distance(mn, mx, p));
+    });
+}
+
+// VolumeBuilder::iwp (builder.rs:53-76): the IWP triply periodic surface of
+// period cell_size inside [min, max], the distance to the box grown by one
+// voxel outside it; band width 2.  cos is libm's cosf, as f32::cos on Linux.
+or_volume* or_volume_iwp(const float* mn_, const float* mx_, float cell_size, float h) {
+    const Vec3f mn{mn_[0], mn_[1], mn_[2]}, mx{mx_[0], mx_[1], mx_[2]};
+    const float inv = 1.0f / cell_size;
+    const Vec3f imn{mn.x + (-h), mn.y + (-h), mn.z + (-h)};
+    const Vec3f imx{mx.x + h, mx.y + h, mx.z + h};
+    return from_fn(h, imn, imx, 2, [&](const Vec3f& p) {
+        const float x = p.x * inv, y = p.y * inv, z = p.z * inv;
+        const float cx = std::cos(x), cy = std::cos(y), cz = std::cos(z);
+        const float v = -(cx + cy + cz - 0.51f * (cx * cy + cy * cz + cz * cx) - 1.0f);
+        if (!box_contains(mn, mx, p)) return std::sqrt(box_squared_distance(imn, imx, p));
+        return v * cell_size;
+    });
 }
 
 // An empty VolumeGrid receiving `insert(idx, value)` for each voxel in order

@@ -59,6 +59,10 @@ def lib():
         u8p = C.POINTER(C.c_uint8)
         L.or_volume_sphere.restype = vp
         L.or_volume_sphere.argtypes = [C.c_float] * 5
+        L.or_volume_cuboid.restype = vp
+        L.or_volume_cuboid.argtypes = [f32p, f32p, C.c_float]
+        L.or_volume_iwp.restype = vp
+        L.or_volume_iwp.argtypes = [f32p, f32p, C.c_float, C.c_float]
         L.or_volume_densify_tiles.argtypes = [vp]
         L.or_dual_contour.restype = C.c_int
         L.or_dual_contour.argtypes = [vp, C.c_float, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_size_t)]
@@ -281,6 +285,20 @@ def sample_unsigned(soup, voxel_size: float, band: int, voxels, threads: int = 0
 def sphere(radius: float, center, voxel_size: float) -> Volume:
     """VolumeBuilder::default().with_voxel_size(h).sphere(radius, center)."""
     return Volume(lib().or_volume_sphere(radius, center[0], center[1], center[2], voxel_size), voxel_size)
+
+
+def cuboid(mn, mx, voxel_size: float) -> Volume:
+    """VolumeBuilder::default().with_voxel_size(h).cuboid(min, max) (builder.rs:32-51)."""
+    a, pa = _f32(mn)
+    b, pb = _f32(mx)
+    return Volume(lib().or_volume_cuboid(pa, pb, voxel_size), voxel_size)
+
+
+def iwp(mn, mx, cell_size: float, voxel_size: float) -> Volume:
+    """VolumeBuilder::default().with_voxel_size(h).iwp(min, max, cell_size) (builder.rs:53-76)."""
+    a, pa = _f32(mn)
+    b, pb = _f32(mx)
+    return Volume(lib().or_volume_iwp(pa, pb, cell_size, voxel_size), voxel_size)
 
 
 def fs_compute_distance(a1, a2, a3, h) -> float:

@@ -1,0 +1,97 @@
+"""VolumeBuilder::cuboid / ::iwp on the GPU against the oracle, and the
+reference's lightweighting example (examples/lightweighting.rs: bunny at
+voxel 0.2, IWP lattice, offset -1, intersect / subtract / union / subtract
+with a clip box, dual contouring) run through the GPU path."""
+import os
+
+import numpy as np
+import pytest
+
+import baby_shark_amd as bs
+
+from test_offset_gpu import _import
+from test_volume_ops_gpu import assert_same_volume
+from test_dc_gpu import _same
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+BOXES = [
+    ((-0.33, -0.21, 0.07), (0.41, 0.18, 0.52), 0.05),
+    ((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), 0.01),      # 200 voxels wide: interior cells culled
+    ((40.0, -0.5, 40.9), (41.3, 0.25, 41.0), 0.01),    # straddles 4096-voxel root blocks, thin in z
+    ((0.3, 0.3, 0.3), (0.1, 0.5, 0.5), 0.05),          # min > max along x: no interior
+]
+
+
+@pytest.mark.parametrize("mn,mx,h", BOXES)
+def test_cuboid_matches_oracle(gpu_ctx, oracle_lib, mn, mx, h):
+    g = bs.VolumeBuilder(gpu_ctx, h).cuboid(mn, mx).export()
+    assert_same_volume(g, oracle_lib.cuboid(mn, mx, h).export())
+
+
+def _iwp_close(g, r, band, cell):
+    """Masks equal except at voxels whose reference value sits within a few
+    ulps of the band edge; values equal up to the libm cosf difference."""
+    assert np.array_equal(g.origins, r.origins)
+    ga, ra = g.active(), r.active()
+    diff = ga != ra
+    if diff.any():
+        assert np.all(np.abs(np.abs(np.where(ra, r.values, g.values)[diff]) - band) <= 4e-7 * band), "mask differs"
+    both = ga & ra
+    err = np.abs(g.values[both] - r.values[both])
+    exact = float(np.mean(err == 0))
+    print(f"iwp: {both.sum()} voxels, bit-exact {exact:.4f}, max |diff| {err.max():.3g}", flush=True)
+    assert err.max() <= 1e-6 * cell
+    return exact
+
+
+@pytest.mark.parametrize("mn,mx,cell,h", [((-0.6, -0.6, 0.0), (0.6, 0.6, 0.9), 0.5, 0.05),
+                                          ((-26.0, -26.0, 0.0), (26.0, 26.0, 50.0), 0.5, 0.2),
+                                          ((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), 0.3, 0.01)])
+def test_iwp_matches_oracle(gpu_ctx, oracle_lib, mn, mx, cell, h):
+    g = bs.VolumeBuilder(gpu_ctx, h).iwp(mn, mx, cell).export()
+    r = oracle_lib.iwp(mn, mx, cell, h).export()
+    assert _iwp_close(g, r, np.float32(3.0) * np.float32(h), cell) > 0.95
+
+
+def test_cuboid_sphere_csg_then_dc(gpu_ctx, oracle_lib):
+    h = 0.02
+    B = bs.VolumeBuilder(gpu_ctx, h)
+    g = B.cuboid((-0.4, -0.4, -0.4), (0.4, 0.4, 0.4)).subtract(B.sphere(0.5, (0.0, 0.0, 0.0)))
+    r = oracle_lib.cuboid((-0.4, -0.4, -0.4), (0.4, 0.4, 0.4), h).subtract(oracle_lib.sphere(0.5, (0, 0, 0), h))
+    assert_same_volume(g.export(), r.export())
+    _same(bs.DualContouringMesher(gpu_ctx).with_voxel_size(h).mesh(g), r.dual_contour(h))
+
+
+def _lightweight(m2v, iwp, cuboid):
+    """examples/lightweighting.rs:23-35 on a bunny volume."""
+    bunny = m2v
+    bunny_offset = bunny.clone().offset(-1.0)
+    bunny_iwp = bunny_offset.clone().intersect(iwp)
+    clip = cuboid((-30.0, -30.0, -2.0), (30.0, -5.0, 50.0))
+    return bunny.subtract(bunny_offset).union(bunny_iwp).subtract(clip)
+
+
+def test_lightweighting_example_matches_oracle(gpu_ctx, oracle_lib):
+    """Same inputs (the oracle's bunny volume and lattice imported), every
+    later step on the GPU: the final soup equals the oracle's bit for bit."""
+    h = 0.2
+    soup = bs.read_stl(os.path.join(GOLDEN, "bunny.stl"))
+    ob = oracle_lib.mesh_to_volume(soup, h, 0)
+    oi = oracle_lib.iwp((-26.0, -26.0, 0.0), (26.0, 26.0, 50.0), 0.5, h)
+    g = _lightweight(_import(gpu_ctx, ob), _import(gpu_ctx, oi), lambda a, b: bs.VolumeBuilder(gpu_ctx, h).cuboid(a, b))
+    r = _lightweight(ob.clone(), oi.clone(), lambda a, b: oracle_lib.cuboid(a, b, h))
+    assert_same_volume(g.export(), r.export())
+    got = bs.DualContouringMesher(gpu_ctx).with_voxel_size(h).mesh(g)
+    want = r.dual_contour(h)
+    assert want.shape[0] > 900000
+    _same(got, want)
+    # the whole example on the GPU (its own bunny volume and lattice): values
+    # differ from the oracle's by rounding only, so the soup matches closely
+    full = _lightweight(bs.MeshToVolume(gpu_ctx).with_voxel_size(h).convert(soup),
+                        bs.VolumeBuilder(gpu_ctx, h).iwp((-26.0, -26.0, 0.0), (26.0, 26.0, 50.0), 0.5),
+                        lambda a, b: bs.VolumeBuilder(gpu_ctx, h).cuboid(a, b))
+    tris = bs.DualContouringMesher(gpu_ctx).with_voxel_size(h).mesh(full)
+    print(f"lightweighting: {tris.shape[0]} triangles on the GPU path, {want.shape[0]} in the oracle", flush=True)
+    assert abs(tris.shape[0] - want.shape[0]) <= 1e-3 * want.shape[0]
