@@ -241,6 +241,19 @@ __global__ __launch_bounds__(64) void k_sweep_plane(SweepArgs a, const uint32_t*
     const int lane = threadIdx.x;
     const int lx = lane >> 3, ly = lane & 7;  // this lane's z-row of the leaf
     if (blockIdx.x == 0 && lane == 0) starts[plane + 1] = start;
+    // each lane's voxel on each of the 22 voxel planes (padded index, -1 for
+    // none), resolved once for all the leaves the block sweeps
+    __shared__ int16_t plane_vox[22][64];
+    for (int p = 0; p < 22; ++p) {
+        const int s0 = kVoxelPlanes.start[p], cnt = kVoxelPlanes.start[p + 1] - s0;
+        int16_t q = -1;
+        if (lane < cnt) {
+            const int e = kVoxelPlanes.entry[s0 + lane];
+            const int u = e >> 6, v = (e >> 3) & 7, w = e & 7;
+            q = (int16_t)pidx(a.sx > 0 ? u : 7 - u, a.sy > 0 ? v : 7 - v, a.sz > 0 ? w : 7 - w);
+        }
+        plane_vox[p][lane] = q;
+    }
     for (uint32_t item = blockIdx.x; item < n_old + n_new; item += gridDim.x) {
         const uint32_t slot = item < n_old ? order[b0 + item] : new_lo + (item - n_old);
         const int4 org = a.origins[slot];
@@ -275,20 +288,19 @@ __global__ __launch_bounds__(64) void k_sweep_plane(SweepArgs a, const uint32_t*
             uint8_t on = 0;
             float v = 0.0f;
             if (n >= 0) {
-                on = (a.masks[(uint64_t)n * 8 + (o >> 6)] >> (o & 63)) & 1ull;
-                if (on) v = a.values[(uint64_t)n * 512 + o];
+                // mask word and value in flight together
+                const uint64_t mw = a.masks[(uint64_t)n * 8 + (o >> 6)];
+                const float nv = a.values[(uint64_t)n * 512 + o];
+                on = (mw >> (o & 63)) & 1ull;
+                v = on ? nv : 0.0f;
             }
             L.st[po] = on;
             L.val[po] = v;
         }
         __syncthreads();
         for (int p = 0; p < 22; ++p) {
-            const int s0 = kVoxelPlanes.start[p], cnt = kVoxelPlanes.start[p + 1] - s0;
-            if (lane < cnt) {
-                const int e = kVoxelPlanes.entry[s0 + lane];
-                const int u = e >> 6, v = (e >> 3) & 7, w = e & 7;
-                sweep_voxel(L, a, pidx(a.sx > 0 ? u : 7 - u, a.sy > 0 ? v : 7 - v, a.sz > 0 ? w : 7 - w));
-            }
+            const int q = plane_vox[p][lane];
+            if (q >= 0) sweep_voxel(L, a, q);
             __syncthreads();
         }
         // write back
