@@ -306,6 +306,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pipeline", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-shared", action="store_true")
     ap.add_argument("--traffic-probe", action="store_true")
     args = ap.parse_args()
     if args.traffic_probe:
@@ -396,7 +397,8 @@ def main():
         "phases_ms": {k: round(float(np.mean([p[k] for p in prof])), 4)
                       for k in ("ms_subdivide", "ms_leaves", "ms_distance", "ms_bvh", "ms_winding")},
     }
-    out["shared_signs"] = shared_signs(ctx, dev, args, ms_step)
+    if not args.no_shared:
+        out["shared_signs"] = shared_signs(ctx, dev, args, ms_step)
     if not args.no_pipeline:
         out["pipeline"] = pipeline(ctx)
         out["torus_and_sphere"] = torus_and_sphere(ctx)
