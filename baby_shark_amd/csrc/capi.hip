@@ -95,15 +95,31 @@ bs_status bs_ctx_create(int device, bs_ctx** out) {
     });
 }
 
+// the second stream, its events and the pinned buffer of a context
+static void release_side(bs_ctx* c) {
+    if (c->stream2) {
+        (void)hipStreamSynchronize(c->stream2);
+        (void)hipStreamDestroy(c->stream2);
+        c->stream2 = nullptr;
+    }
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->host_pinned) (void)hipHostFree(c->host_pinned);
+    c->ev_fork = c->ev_join = nullptr;
+    c->host_pinned = nullptr;
+}
+
 void bs_ctx_destroy(bs_ctx* ctx) {
     if (!ctx) return;
     int prev = -1;
     (void)hipGetDevice(&prev);
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    release_side(ctx);
     ctx->scratch.release();
     for (auto& w : ctx->workers) {
         (void)hipStreamSynchronize(w->stream);
+        release_side(w.get());
         w->scratch.release();
         (void)hipStreamDestroy(w->stream);
     }

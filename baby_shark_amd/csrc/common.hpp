@@ -45,6 +45,7 @@ enum ScratchSlot : int {
     S_ACT_LIST, S_COUNTERS, S_CUB, S_MORTON, S_MORTON_SORTED, S_IDX, S_IDX_SORTED, S_BVH_TRIS, S_BVH_NODES,
     S_BVH_LINKS, S_BVH_PARENT, S_BVH_FLAGS, S_BVH_INIT, S_BVH_BOX, S_RECHECK, S_ARGMIN, S_MISC0, S_MISC1,
     S_MISC2, S_MISC3, S_INPUT, S_BVH_KIDS, S_FLAGS_A, S_FLAGS_B, S_CSG_PLAN, S_DC_MAP, S_DC_KEYS, S_DC_FAIL, S_DC_CELLS, S_SIGN_ROOTS, S_REP_CNT, S_REP_OFF, S_REPS,
+    S_BVH_HEIGHT, S_BVH_HEIGHT_S, S_BVH_INODE, S_BVH_INODE_S, S_BVH_MISC, S_BVH_STARTS, S_CUB_B,
     S_DC_CELLMASK, S_DC_TRICNT, S_DC_TRIOFF, S_COUNT
 };
 
@@ -88,7 +89,11 @@ struct bs_ctx {
     bs_dc_stats last_dc = {};
     bs_csg_stats last_csg = {};
     std::vector<std::unique_ptr<bs_ctx>> workers;  // sub-contexts of batched calls (own stream, scratch)
-    uint64_t* host_counters = nullptr;  // pinned, for small readbacks
+    // second stream for work that overlaps the main one (the BVH build), with
+    // its events and a pinned readback buffer; created on first use
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    uint32_t* host_pinned = nullptr;  // 512 words
 };
 
 struct bs_volume {

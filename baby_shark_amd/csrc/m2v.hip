@@ -1188,13 +1188,35 @@ void exclusive_scan(bs_ctx* ctx, const TIn* in, TOut* out, uint64_t n) {
 }
 
 template <class K, class V>
-void sort_pairs(bs_ctx* ctx, const K* k, K* ks, const V* v, V* vs, uint64_t n, int end_bit) {
+void sort_pairs_on(bs_ctx* ctx, hipStream_t stream, int cub_slot, const K* k, K* ks, const V* v, V* vs, uint64_t n,
+                   int end_bit) {
     BS_REQUIRE(n < (1ull << 31), BS_ERR_INVALID_ARG, "sort longer than 2^31 elements");
     size_t bytes = 0;
-    BS_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, ks, v, vs, (int)n, 0, end_bit, ctx->stream));
-    void* tmp = ctx->scratch.get(S_CUB, bytes);
-    BS_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, bytes, k, ks, v, vs, (int)n, 0, end_bit, ctx->stream));
+    BS_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, ks, v, vs, (int)n, 0, end_bit, stream));
+    void* tmp = ctx->scratch.get(cub_slot, bytes);
+    BS_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, bytes, k, ks, v, vs, (int)n, 0, end_bit, stream));
 }
+
+template <class K, class V>
+void sort_pairs(bs_ctx* ctx, const K* k, K* ks, const V* v, V* vs, uint64_t n, int end_bit) {
+    sort_pairs_on(ctx, ctx->stream, S_CUB, k, ks, v, vs, n, end_bit);
+}
+
+// The context's second stream, its fork/join events and pinned readback
+// buffer, created on first use.
+void ensure_side_stream(bs_ctx* ctx) {
+    if (!ctx->stream2) BS_HIP(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+    if (!ctx->ev_fork) BS_HIP(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+    if (!ctx->ev_join) BS_HIP(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+    if (!ctx->host_pinned) BS_HIP(hipHostMalloc((void**)&ctx->host_pinned, sizeof(uint32_t) * 512, hipHostMallocDefault));
+}
+
+// Waits for the second stream when a call leaves early, so that no kernel of
+// it still uses the scratch arena afterwards.
+struct SideJoin {
+    hipStream_t s;
+    ~SideJoin() { (void)hipStreamSynchronize(s); }
+};
 
 template <class It, class Out, class Pred>
 void select_if(bs_ctx* ctx, It in, Out out, uint32_t* d_count, uint64_t n, Pred pred) {
@@ -1269,6 +1291,38 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     BS_HIP(hipMemsetAsync(counters, 0, sizeof(unsigned long long) * C_COUNT, st));
     BS_HIP(hipMemsetAsync(err, 0, 64, st));
     const unsigned TB = 256;
+
+    // ---- 0. LBVH + dipoles over the original mesh, on the second stream: it
+    // needs only the triangles, so it is built beside the distance pass (its
+    // buffers are taken here); the winding pass waits for it
+    ensure_side_stream(ctx);
+    hipStream_t sb = ctx->stream2;
+    BS_HIP(hipEventRecord(ctx->ev_fork, st));
+    BS_HIP(hipStreamWaitEvent(sb, ctx->ev_fork, 0));
+    SideJoin side_join{sb};
+    const int nt = (int)n;
+    const int n_int = nt - 1;
+    int H = 0;  // tree height (root level)
+    bool use_lds = false;
+    const int n_nodes = 2 * nt - 1;
+    unsigned* bounds = reinterpret_cast<unsigned*>(S.get(S_BVH_MISC, 64));
+    const unsigned nb = div_up(n, 256);
+    uint4* bparts = S.as<uint4>(S_BVH_BOX, 2 * (uint64_t)nb);
+    uint64_t* codes = S.as<uint64_t>(S_MORTON, n);
+    uint64_t* codes_s = S.as<uint64_t>(S_MORTON_SORTED, n);
+    uint32_t* idx = S.as<uint32_t>(S_IDX, n);
+    uint32_t* idx_s = S.as<uint32_t>(S_IDX_SORTED, n);
+    int* parent = S.as<int>(S_BVH_PARENT, n_nodes);
+    unsigned* hword = S.as<unsigned>(S_BVH_FLAGS, std::max(n_int, 1));
+    uint32_t* height = S.as<uint32_t>(S_BVH_HEIGHT, std::max(n_int, 1));
+    uint32_t* height_s = S.as<uint32_t>(S_BVH_HEIGHT_S, std::max(n_int, 1));
+    uint32_t* inode = S.as<uint32_t>(S_BVH_INODE, std::max(n_int, 1));
+    uint32_t* inode_s = S.as<uint32_t>(S_BVH_INODE_S, std::max(n_int, 1));
+    NodeAcc* accs = S.as<NodeAcc>(S_BVH_INIT, n_nodes);
+    NodeRec* nodes = S.as<NodeRec>(S_BVH_NODES, (uint64_t)n_nodes);  // internal nodes, then leaves
+    float* btris = S.as<float>(S_BVH_TRIS, 12 * (n + BVH_LEAF_SIZE));  // padded: leaves read 4 triangles
+    int2* kids = S.as<int2>(S_BVH_KIDS, std::max(n_int, 1));  // binary children of internal nodes
+    uint32_t* starts = reinterpret_cast<uint32_t*>(S.get(S_BVH_STARTS, sizeof(uint32_t) * 258));
 
     // ---- 1. subdivision
     uint32_t* rows = S.as<uint32_t>(S_ROWS, n + 1);
@@ -1396,6 +1450,41 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
                                                           (uint32_t)n_cov, vol->values, vol->masks);
     tm.mark();
 
+    // ---- BVH on the second stream, beside the distance pass
+    k_centroid_bounds<<<nb, 256, 0, sb>>>(d_tris, n, bparts);
+    k_bounds_final<<<1, 1024, 0, sb>>>(bparts, nb, bounds);
+    k_morton<<<div_up(n, TB), TB, 0, sb>>>(d_tris, n, bounds, codes, idx);
+    sort_pairs_on(ctx, sb, S_CUB_B, codes, codes_s, idx, idx_s, n, 63);
+    BS_HIP(hipMemsetAsync(parent, 0xFF, sizeof(int) * n_nodes, sb));
+    BS_HIP(hipMemsetAsync(hword, 0, sizeof(unsigned) * std::max(n_int, 1), sb));
+    if (n_int > 0) k_lbvh_build<<<div_up(n_int, TB), TB, 0, sb>>>(codes_s, nt, nodes, kids, parent);
+    k_lbvh_leaves<<<div_up(n, TB), TB, 0, sb>>>(d_tris, nt, idx_s, prm.wn_accuracy, parent, hword, height, accs, nodes,
+                                                btris, err);
+    uint32_t* hs = ctx->host_pinned;  // level starts [0, 258), then the error word
+    if (n_int > 0) {
+        // internal nodes grouped by height; combined level by level further down
+        k_iota<<<div_up(n_int, TB), TB, 0, sb>>>(inode, (uint32_t)n_int);
+        sort_pairs_on(ctx, sb, S_CUB_B, height, height_s, inode, inode_s, (uint64_t)n_int, 8);
+        BS_HIP(hipMemsetAsync(starts, 0xFF, sizeof(uint32_t) * 258, sb));
+        k_level_starts<<<div_up(n_int, TB), TB, 0, sb>>>(height_s, (uint32_t)n_int, starts);
+        BS_HIP(hipMemcpyAsync(hs, starts, sizeof(uint32_t) * 258, hipMemcpyDeviceToHost, sb));
+        BS_HIP(hipMemcpyAsync(hs + 258, err, sizeof(uint32_t), hipMemcpyDeviceToHost, sb));
+    }
+    if (n_int > 0) {
+        BS_HIP(hipStreamSynchronize(sb));
+        if (hs[258] & E_TREE) throw Error(BS_ERR_INTERNAL, "BVH parent links are corrupt");
+        // heights run 1..H contiguously in the sorted order
+        H = 0;
+        for (int k = 1; k < 256; ++k)
+            if (hs[k] != 0xFFFFFFFFu) H = k;
+        for (int k = 1; k <= H; ++k) {
+            BS_REQUIRE(hs[k] != 0xFFFFFFFFu, BS_ERR_INTERNAL, "BVH height levels are not contiguous");
+            const uint32_t b = hs[k], e = (k == H) ? (uint32_t)n_int : hs[k + 1];
+            k_lbvh_level<<<div_up(e - b, TB), TB, 0, sb>>>(inode_s + b, e - b, prm.wn_accuracy, kids, accs, nodes);
+        }
+    }
+    BS_HIP(hipEventRecord(ctx->ev_join, sb));
+
     // ---- 4. active list
     uint32_t* act_cnt = S.as<uint32_t>(S_ACT_CNT, n_leaves + 1);
     uint32_t* act_off = S.as<uint32_t>(S_ACT_OFF, n_leaves + 1);
@@ -1406,59 +1495,8 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     uint32_t* act = S.as<uint32_t>(S_ACT_LIST, n_act);
     k_active_list<<<(unsigned)n_leaves, 512, 0, st>>>(vol->masks, act_off, act);
 
-    // ---- 5. LBVH + dipoles over the original mesh
-    const int nt = (int)n;
-    const int n_int = nt - 1;
-    int H = 0;  // tree height (root level)
-    bool use_lds = false;
-    const int n_nodes = 2 * nt - 1;
-    unsigned* bounds = reinterpret_cast<unsigned*>(S.get(S_MISC2, 64));
-    const unsigned nb = div_up(n, 256);
-    uint4* bparts = S.as<uint4>(S_BVH_BOX, 2 * (uint64_t)nb);
-    k_centroid_bounds<<<nb, 256, 0, st>>>(d_tris, n, bparts);
-    k_bounds_final<<<1, 1024, 0, st>>>(bparts, nb, bounds);
-    uint64_t* codes = S.as<uint64_t>(S_MORTON, n);
-    uint64_t* codes_s = S.as<uint64_t>(S_MORTON_SORTED, n);
-    uint32_t* idx = S.as<uint32_t>(S_IDX, n);
-    uint32_t* idx_s = S.as<uint32_t>(S_IDX_SORTED, n);
-    k_morton<<<div_up(n, TB), TB, 0, st>>>(d_tris, n, bounds, codes, idx);
-    sort_pairs(ctx, codes, codes_s, idx, idx_s, n, 63);
-    int* parent = S.as<int>(S_BVH_PARENT, n_nodes);
-    unsigned* hword = S.as<unsigned>(S_BVH_FLAGS, std::max(n_int, 1));
-    uint32_t* height = S.as<uint32_t>(S_ROWS, std::max(n_int, 1));       // subdivision scratch is free now
-    uint32_t* height_s = S.as<uint32_t>(S_ROW_OFF, std::max(n_int, 1));
-    uint32_t* inode = S.as<uint32_t>(S_SUBS, std::max(n_int, 1));
-    uint32_t* inode_s = S.as<uint32_t>(S_SUB_OFF, std::max(n_int, 1));
-    NodeAcc* accs = S.as<NodeAcc>(S_BVH_INIT, n_nodes);
-    NodeRec* nodes = S.as<NodeRec>(S_BVH_NODES, (uint64_t)n_nodes);  // internal nodes, then leaves
-    float* btris = S.as<float>(S_BVH_TRIS, 12 * (n + BVH_LEAF_SIZE));  // padded: leaves read 4 triangles
-    int2* kids = S.as<int2>(S_BVH_KIDS, std::max(n_int, 1));  // binary children of internal nodes
-    BS_HIP(hipMemsetAsync(parent, 0xFF, sizeof(int) * n_nodes, st));
-    BS_HIP(hipMemsetAsync(hword, 0, sizeof(unsigned) * std::max(n_int, 1), st));
-    if (n_int > 0) k_lbvh_build<<<div_up(n_int, TB), TB, 0, st>>>(codes_s, nt, nodes, kids, parent);
-    k_lbvh_leaves<<<div_up(n, TB), TB, 0, st>>>(d_tris, nt, idx_s, prm.wn_accuracy, parent, hword, height, accs, nodes,
-                                                btris, err);
-    if (n_int > 0) {
-        // internal nodes grouped by height, then combined level by level
-        k_iota<<<div_up(n_int, TB), TB, 0, st>>>(inode, (uint32_t)n_int);
-        sort_pairs(ctx, height, height_s, inode, inode_s, (uint64_t)n_int, 8);
-        uint32_t* starts = reinterpret_cast<uint32_t*>(S.get(S_MISC0, sizeof(uint32_t) * 258));
-        BS_HIP(hipMemsetAsync(starts, 0xFF, sizeof(uint32_t) * 258, st));
-        k_level_starts<<<div_up(n_int, TB), TB, 0, st>>>(height_s, (uint32_t)n_int, starts);
-        uint32_t hs[258];
-        BS_HIP(hipMemcpyAsync(hs, starts, sizeof(hs), hipMemcpyDeviceToHost, st));
-        BS_HIP(hipStreamSynchronize(st));
-        if (read_scalar(ctx, err) & E_TREE) throw Error(BS_ERR_INTERNAL, "BVH parent links are corrupt");
-        // heights run 1..H contiguously in the sorted order
-        H = 0;
-        for (int k = 1; k < 256; ++k)
-            if (hs[k] != 0xFFFFFFFFu) H = k;
-        for (int k = 1; k <= H; ++k) {
-            BS_REQUIRE(hs[k] != 0xFFFFFFFFu, BS_ERR_INTERNAL, "BVH height levels are not contiguous");
-            const uint32_t b = hs[k], e = (k == H) ? (uint32_t)n_int : hs[k + 1];
-            k_lbvh_level<<<div_up(e - b, TB), TB, 0, st>>>(inode_s + b, e - b, prm.wn_accuracy, kids, accs, nodes);
-        }
-    }
+    // ---- 5. the BVH (second stream) must be complete
+    BS_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
     tm.mark();
 
     // ---- 6. sign groups (shared signs only): the winding number is evaluated

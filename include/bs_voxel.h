@@ -67,7 +67,10 @@ typedef struct {
     uint64_t n_tiles;     /* active internal tiles                   */
 } bs_volume_info;
 
-/* Work counters and kernel timings of the last bs_mesh_to_volume* call. */
+/* Work counters and kernel timings of the last bs_mesh_to_volume* call.  The
+ * ms_* phases are HIP-event times on the context's stream (with profiling on):
+ * the BVH is built on a second stream beside the distance pass, so ms_bvh is
+ * the active-voxel list plus what remains of the BVH build after it. */
 typedef struct {
     uint64_t n_tris, n_sub, n_pairs, n_leaves, n_active, n_candidates;
     uint64_t n_bvh_nodes, wn_node_visits, wn_leaf_tris, wn_far_evals, n_fp64_rechecks;
