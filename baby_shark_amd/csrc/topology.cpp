@@ -5,30 +5,95 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
-#include <thread>
+#include <condition_variable>
+#include <deque>
+#include <functional>
+#include <mutex>
 #include <numeric>
+#include <thread>
 
 namespace bs {
 
 namespace {
 
-// Runs f(i) for i in [0, n) on up to 16 host threads (serially when small).
+// A persistent pool of host threads (up to 15 beside the caller), so that the
+// many short parallel loops of one operation do not each start threads.  Jobs
+// from concurrent callers queue; every caller works on its own job and returns
+// only when no helper is still inside it.
+struct PoolJob {
+    const std::function<void(size_t)>* f = nullptr;
+    size_t n = 0, helpers_max = 0;
+    std::atomic<size_t> next{0};
+    size_t helpers = 0, finished = 0;  // guarded by the pool's mutex
+};
+
+class WorkerPool {
+public:
+    // never destroyed: its threads sleep until the process ends
+    static WorkerPool& get() {
+        static WorkerPool* p = new WorkerPool();
+        return *p;
+    }
+    size_t threads() const { return workers_ + 1; }
+    void run(size_t n, size_t max_threads, const std::function<void(size_t)>& f) {
+        PoolJob job;
+        job.f = &f;
+        job.n = n;
+        job.helpers_max = max_threads - 1;
+        {
+            std::lock_guard<std::mutex> l(m_);
+            jobs_.push_back(&job);
+        }
+        cv_.notify_all();
+        work(job);
+        std::unique_lock<std::mutex> l(m_);
+        // every item is handed out: no further helper may take the job
+        auto it = std::find(jobs_.begin(), jobs_.end(), &job);
+        if (it != jobs_.end()) jobs_.erase(it);
+        done_.wait(l, [&] { return job.finished == job.helpers; });
+    }
+
+private:
+    WorkerPool() {
+        const size_t hw = std::max<size_t>(1, std::min<size_t>(16, std::thread::hardware_concurrency()));
+        workers_ = hw - 1;
+        for (size_t k = 0; k < workers_; ++k) std::thread([this] { loop(); }).detach();
+    }
+    static void work(PoolJob& j) {
+        for (size_t i; (i = j.next.fetch_add(1, std::memory_order_relaxed)) < j.n;) (*j.f)(i);
+    }
+    void loop() {
+        std::unique_lock<std::mutex> l(m_);
+        for (;;) {
+            cv_.wait(l, [&] { return !jobs_.empty(); });
+            PoolJob* j = jobs_.front();
+            ++j->helpers;
+            if (j->helpers >= j->helpers_max || j->next.load(std::memory_order_relaxed) >= j->n) jobs_.pop_front();
+            l.unlock();
+            work(*j);
+            l.lock();
+            ++j->finished;
+            done_.notify_all();
+        }
+    }
+    std::mutex m_;
+    std::condition_variable cv_, done_;
+    std::deque<PoolJob*> jobs_;
+    size_t workers_ = 0;
+};
+
+// Runs f(i) for i in [0, n) on up to 16 host threads, at least `grain` items
+// per thread (serially when that leaves one).
 template <class F>
-void parallel_for(size_t n, F&& f) {
-    const size_t hw = std::max<size_t>(1, std::min<size_t>(16, std::thread::hardware_concurrency()));
-    const size_t T = std::min(hw, n / 8);
+void parallel_for(size_t n, F&& f, size_t grain = 8) {
+    WorkerPool& pool = WorkerPool::get();
+    const size_t T = std::min(pool.threads(), n / std::max<size_t>(grain, 1));
     if (T <= 1) {
         for (size_t i = 0; i < n; ++i) f(i);
         return;
     }
-    std::atomic<size_t> next{0};
-    auto work = [&]() {
-        for (size_t i; (i = next.fetch_add(1, std::memory_order_relaxed)) < n;) f(i);
-    };
-    std::vector<std::thread> pool;
-    for (size_t k = 1; k < T; ++k) pool.emplace_back(work);
-    work();
-    for (auto& th : pool) th.join();
+    const std::function<void(size_t)> fn = [&](size_t i) { f(i); };
+    pool.run(n, T, fn);
 }
 
 constexpr int32_t L1_SPAN = 4096;  // voxels covered by a Node1
@@ -148,7 +213,63 @@ std::unique_ptr<Topology> Topology::from_leaves(const std::vector<std::array<int
         return t->n2[a.child[o]];
     };
     t->leaves.resize(origins.size());
-    for (size_t i = 0; i < origins.size(); ++i) {
+    const size_t n = origins.size();
+    bool sorted = n >= 65536;
+    for (size_t i = 1; sorted && i < n; ++i) sorted = origins[i - 1] < origins[i];
+    if (sorted) {
+        // Leaves sorted by origin: those of one 128-voxel x-slab are contiguous
+        // and own the 32^3-node slots of that slab, so slabs are built in
+        // parallel.  Nodes are numbered by first appearance, as serially.
+        for (size_t i = 0; i < n; ++i) node1_for(origins[i][0], origins[i][1], origins[i][2]);
+        std::vector<size_t> cut{0};
+        for (size_t i = 1; i < n; ++i)
+            if ((origins[i][0] >> 7) != (origins[i - 1][0] >> 7)) cut.push_back(i);
+        cut.push_back(n);
+        const size_t n_slab = cut.size() - 1;
+        std::vector<std::vector<Node2>> local(n_slab);
+        std::vector<std::vector<std::pair<int32_t, int>>> made(n_slab);  // (32^3 node, slot) per local node
+        parallel_for(n_slab, [&](size_t sl) {
+            Key lk{INT32_MIN, 0, 0};
+            int32_t ln1 = -1;
+            for (size_t i = cut[sl]; i < cut[sl + 1]; ++i) {
+                const auto& g = origins[i];
+                const Key k{g[0] & ~(L1_SPAN - 1), g[1] & ~(L1_SPAN - 1), g[2] & ~(L1_SPAN - 1)};
+                if (k != lk) {
+                    ln1 = t->root.find(k)->second;
+                    lk = k;
+                }
+                Node1& a = t->n1[ln1];
+                const int o1 = l1_slot(g[0], g[1], g[2]);
+                if (a.kind[o1] != SLOT_BRANCH) {
+                    int32_t nx, ny, nz;
+                    slot_origin(a, o1, L2_SPAN, nx, ny, nz);
+                    a.kind[o1] = SLOT_BRANCH;
+                    a.child[o1] = (int32_t)local[sl].size();
+                    local[sl].emplace_back(nx, ny, nz);
+                    made[sl].push_back({ln1, o1});
+                }
+                Node2& b = local[sl][a.child[o1]];
+                const int o = l2_slot(g[0], g[1], g[2]);
+                b.kind[o] = SLOT_BRANCH;
+                b.child[o] = (int32_t)i;
+                LeafRef& r = t->leaves[i];
+                r.x = g[0];
+                r.y = g[1];
+                r.z = g[2];
+                r.a = (int32_t)i;
+                r.op = LEAF_COPY_A;
+            }
+        }, 1);
+        size_t total = 0;
+        for (const auto& l : local) total += l.size();
+        t->n2.reserve(total);
+        for (size_t sl = 0; sl < n_slab; ++sl) {
+            const int32_t base = (int32_t)t->n2.size();
+            for (const auto& m : made[sl]) t->n1[m.first].child[m.second] += base;
+            for (auto& nd : local[sl]) t->n2.push_back(std::move(nd));
+        }
+    }
+    for (size_t i = 0; !sorted && i < n; ++i) {
         const auto& g = origins[i];
         Node2& b = node2_for(g[0], g[1], g[2]);
         const int o = l2_slot(g[0], g[1], g[2]);
@@ -181,10 +302,10 @@ void Topology::flood_fill_upper() {
     auto leaf_first = [&](int32_t li) { return leaves[li].first_neg != 0; };
     auto leaf_last = [&](int32_t li) { return leaves[li].last_neg != 0; };
     // nodes of one level are independent: each reads its children, writes itself
-    parallel_for(n2.size(), [&](size_t i) { fill_node(n2[i], leaf_first, leaf_last); });
+    parallel_for(n2.size(), [&](size_t i) { fill_node(n2[i], leaf_first, leaf_last); }, 1);
     auto n2_first = [&](int32_t k) { return first_neg(n2[k], leaf_first); };
     auto n2_last = [&](int32_t k) { return last_neg(n2[k], leaf_first); };
-    parallel_for(n1.size(), [&](size_t i) { fill_node(n1[i], n2_first, n2_last); });
+    parallel_for(n1.size(), [&](size_t i) { fill_node(n1[i], n2_first, n2_last); }, 1);
     // root_node/flood_fill.rs:14-41: negative tiles between two negative
     // children on one z line
     std::vector<Key> keys;
@@ -343,17 +464,25 @@ struct Csg {
 void Topology::csg(Topology&& other, CsgOp op) {
     // other's pools are appended; its leaves become B-side references
     const int32_t n1_base = (int32_t)n1.size(), n2_base = (int32_t)n2.size(), leaf_base = (int32_t)leaves.size();
-    for (auto r : other.leaves) {
-        r.b = r.a;
-        r.a = -1;
-        r.op = LEAF_COPY_B;
-        leaves.push_back(r);
+    for (auto& sg : other.leaves.seg) {
+        parallel_for((sg.size() + 4095) / 4096, [&](size_t chunk) {
+            const size_t e = std::min(sg.size(), (chunk + 1) * 4096);
+            for (size_t i = chunk * 4096; i < e; ++i) {
+                LeafRef& r = sg[i];
+                r.b = r.a;
+                r.a = -1;
+                r.op = LEAF_COPY_B;
+            }
+        });
+        leaves.seg.push_back(std::move(sg));
     }
-    for (auto& n : other.n2) {
+    parallel_for(other.n2.size(), [&](size_t i) {
+        Node2& n = other.n2[i];
         for (int o = 0; o < Node2::SIZE; ++o)
             if (n.kind[o] == SLOT_BRANCH) n.child[o] += leaf_base;
-        n2.push_back(std::move(n));
-    }
+    });
+    n2.reserve(n2.size() + other.n2.size());
+    for (auto& n : other.n2) n2.push_back(std::move(n));
     for (auto& n : other.n1) {
         for (int o = 0; o < Node1::SIZE; ++o)
             if (n.kind[o] == SLOT_BRANCH) n.child[o] += n2_base;
@@ -383,26 +512,50 @@ void Topology::csg(Topology&& other, CsgOp op) {
 }
 
 void Topology::collect(std::vector<int32_t>& leaf_refs, std::vector<TileRec>& tiles) const {
+    // the 32^3-level items in tree order (a tile, or a 16^3 node whose slots
+    // are gathered in parallel), then concatenated in that order
+    struct Item {
+        int32_t node;  // 16^3 node, or -1 for a tile
+        TileRec tile;
+    };
+    std::vector<Item> items;
     for (const auto& kv : root) {
         const Node1& a = n1[kv.second];
         for (int o1 = 0; o1 < Node1::SIZE; ++o1) {
             if (a.kind[o1] == SLOT_TILE) {
                 int32_t x, y, z;
                 slot_origin(a, o1, L2_SPAN, x, y, z);
-                tiles.push_back({x, y, z, L2_SPAN, a.tile[o1]});
+                items.push_back({-1, {x, y, z, L2_SPAN, a.tile[o1]}});
             } else if (a.kind[o1] == SLOT_BRANCH) {
-                const Node2& b = n2[a.child[o1]];
-                for (int o2 = 0; o2 < Node2::SIZE; ++o2) {
-                    if (b.kind[o2] == SLOT_BRANCH) {
-                        leaf_refs.push_back(b.child[o2]);
-                    } else if (b.kind[o2] == SLOT_TILE) {
-                        int32_t x, y, z;
-                        slot_origin(b, o2, 8, x, y, z);
-                        tiles.push_back({x, y, z, 8, b.tile[o2]});
-                    }
-                }
+                items.push_back({a.child[o1], {}});
             }
         }
+    }
+    std::vector<std::vector<int32_t>> refs(items.size());
+    std::vector<std::vector<TileRec>> tl(items.size());
+    parallel_for(items.size(), [&](size_t k) {
+        if (items[k].node < 0) return;
+        const Node2& b = n2[items[k].node];
+        for (int o2 = 0; o2 < Node2::SIZE; ++o2) {
+            if (b.kind[o2] == SLOT_BRANCH) {
+                refs[k].push_back(b.child[o2]);
+            } else if (b.kind[o2] == SLOT_TILE) {
+                int32_t x, y, z;
+                slot_origin(b, o2, 8, x, y, z);
+                tl[k].push_back({x, y, z, 8, b.tile[o2]});
+            }
+        }
+    });
+    size_t nr = 0;
+    for (const auto& r : refs) nr += r.size();
+    leaf_refs.reserve(leaf_refs.size() + nr);
+    for (size_t k = 0; k < items.size(); ++k) {
+        if (items[k].node < 0) {
+            tiles.push_back(items[k].tile);
+            continue;
+        }
+        leaf_refs.insert(leaf_refs.end(), refs[k].begin(), refs[k].end());
+        tiles.insert(tiles.end(), tl[k].begin(), tl[k].end());
     }
 }
 
@@ -462,7 +615,7 @@ std::unique_ptr<Topology> Topology::after_offset(const std::vector<uint32_t>& ke
         }
     };
     prune();
-    t->leaves.assign(out_origins.size(), LeafRef{});
+    t->leaves.resize(out_origins.size());
     for (size_t i = 0; i < out_origins.size(); ++i) {
         const int32_t x = out_origins[i][0], y = out_origins[i][1], z = out_origins[i][2];
         const Key k{x & ~(L1_SPAN - 1), y & ~(L1_SPAN - 1), z & ~(L1_SPAN - 1)};
@@ -551,33 +704,37 @@ std::unique_ptr<Topology> Topology::compact(const std::vector<int32_t>& new_inde
     int32_t max_leaf = -1;
     for (int32_t v : new_index) max_leaf = std::max(max_leaf, v);
     t->leaves.resize((size_t)(max_leaf + 1));
+    // nodes in tree order (moves only), then each 16^3 node's leaves in
+    // parallel: every leaf sits in exactly one slot
     for (const auto& kv : root) {
         Node1 na = std::move(n1[kv.second]);
         for (int o1 = 0; o1 < Node1::SIZE; ++o1) {
             if (na.kind[o1] != SLOT_BRANCH) continue;
-            Node2 nb = std::move(n2[na.child[o1]]);
-            for (int o2 = 0; o2 < Node2::SIZE; ++o2) {
-                if (nb.kind[o2] != SLOT_BRANCH) continue;
-                const int32_t r = nb.child[o2];
-                const int32_t ni = new_index[r];
-                LeafRef& lr = t->leaves[ni];
-                lr = leaves[r];
-                lr.a = ni;
-                lr.b = -1;
-                lr.op = LEAF_COPY_A;
-                lr.neg = 0;
-                nb.child[o2] = ni;
-            }
-            t->n2.push_back(std::move(nb));
+            t->n2.push_back(std::move(n2[na.child[o1]]));
             na.child[o1] = (int32_t)t->n2.size() - 1;
         }
         t->n1.push_back(std::move(na));
         t->root[kv.first] = (int32_t)t->n1.size() - 1;
     }
+    parallel_for(t->n2.size(), [&](size_t k) {
+        Node2& nb = t->n2[k];
+        for (int o2 = 0; o2 < Node2::SIZE; ++o2) {
+            if (nb.kind[o2] != SLOT_BRANCH) continue;
+            const int32_t r = nb.child[o2];
+            const int32_t ni = new_index[r];
+            LeafRef& lr = t->leaves[ni];
+            lr = leaves[r];
+            lr.a = ni;
+            lr.b = -1;
+            lr.op = LEAF_COPY_A;
+            lr.neg = 0;
+            nb.child[o2] = ni;
+        }
+    });
     root.clear();
     n1.clear();
     n2.clear();
-    leaves.clear();
+    leaves.seg.clear();
     return t;
 }
 

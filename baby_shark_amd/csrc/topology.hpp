@@ -36,6 +36,35 @@ struct LeafRef {
     uint8_t first_neg = 0, last_neg = 0, empty = 0;  // after the leaf flood fill
 };
 
+// Leaf references in segments, index i running over them in order: a CSG
+// operand's references join the result by moving its segments in, without
+// copying them.
+struct LeafStore {
+    std::vector<std::vector<LeafRef>> seg;
+    size_t size() const {
+        size_t n = 0;
+        for (const auto& s : seg) n += s.size();
+        return n;
+    }
+    LeafRef& operator[](size_t i) {
+        for (auto& s : seg) {
+            if (i < s.size()) return s[i];
+            i -= s.size();
+        }
+        return seg.back()[seg.back().size()];  // out of range: never reached by the callers
+    }
+    const LeafRef& operator[](size_t i) const { return const_cast<LeafStore&>(*this)[i]; }
+    // one segment of n default references (the store must be empty)
+    void resize(size_t n) {
+        seg.clear();
+        seg.emplace_back(n);
+    }
+    void push_back(const LeafRef& r) {
+        if (seg.empty()) seg.emplace_back();
+        seg.back().push_back(r);
+    }
+};
+
 template <int LOG2>
 struct SlotNode {
     static constexpr int DIM = 1 << LOG2;
@@ -54,7 +83,7 @@ struct Topology {
     std::map<Key, int32_t> root;         // -> Node1 index
     std::vector<Node1> n1;
     std::vector<Node2> n2;
-    std::vector<LeafRef> leaves;
+    LeafStore leaves;
 
     // Builds the tree of a leaf set sorted or not, plus active tiles
     // (origin, edge 8 or 128, value).

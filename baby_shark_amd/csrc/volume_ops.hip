@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <chrono>
+#include <future>
 #include <cstring>
 #include <numeric>
 #include <tuple>
@@ -54,8 +55,7 @@ void free_leaves(bs_volume* v) {
 
 // The host topology of a volume, built from its leaf origins and active tiles
 // the first time an operation needs it.
-Topology& ensure_topology(bs_ctx* ctx, bs_volume* v) {
-    if (v->topo) return *v->topo;
+static std::vector<std::array<int32_t, 3>> leaf_origins(bs_ctx* ctx, const bs_volume* v) {
     std::vector<int32_t> o4(4 * v->n_leaves);
     if (v->n_leaves) {
         BS_HIP(hipMemcpyAsync(o4.data(), v->origins, sizeof(int32_t) * o4.size(), hipMemcpyDeviceToHost,
@@ -64,9 +64,28 @@ Topology& ensure_topology(bs_ctx* ctx, bs_volume* v) {
     }
     std::vector<std::array<int32_t, 3>> origins(v->n_leaves);
     for (uint64_t i = 0; i < v->n_leaves; ++i) origins[i] = {o4[4 * i], o4[4 * i + 1], o4[4 * i + 2]};
-    v->topo = Topology::from_leaves(origins, v->tiles);
+    return origins;
+}
+
+Topology& ensure_topology(bs_ctx* ctx, bs_volume* v) {
+    if (v->topo) return *v->topo;
+    v->topo = Topology::from_leaves(leaf_origins(ctx, v), v->tiles);
     BS_REQUIRE(v->topo->leaves.size() == v->n_leaves, BS_ERR_INTERNAL, "volume has two leaves at one origin");
     return *v->topo;
+}
+
+// ensure_topology of two volumes, the second built on another host thread
+static void ensure_topologies(bs_ctx* ctx, bs_volume* a, bs_volume* b) {
+    std::vector<std::array<int32_t, 3>> oa, ob;
+    if (!a->topo) oa = leaf_origins(ctx, a);
+    if (!b->topo) ob = leaf_origins(ctx, b);
+    auto build_b = std::async(std::launch::async, [&] {
+        if (!b->topo) b->topo = Topology::from_leaves(ob, b->tiles);
+    });
+    if (!a->topo) a->topo = Topology::from_leaves(oa, a->tiles);
+    build_b.get();
+    BS_REQUIRE(a->topo->leaves.size() == a->n_leaves && b->topo->leaves.size() == b->n_leaves, BS_ERR_INTERNAL,
+               "volume has two leaves at one origin");
 }
 
 namespace {
@@ -451,8 +470,16 @@ void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op) {
     bs_csg_stats cs{};
     cs.n_a = a->n_leaves;
     cs.n_b = b->n_leaves;
-    Topology& ta = ensure_topology(ctx, a);
-    Topology& tb = ensure_topology(ctx, b);
+    auto lap = [last = std::chrono::steady_clock::now()]() mutable {
+        const auto now = std::chrono::steady_clock::now();
+        const double ms = std::chrono::duration<double, std::milli>(now - last).count();
+        last = now;
+        return ms;
+    };
+    ensure_topologies(ctx, a, b);
+    Topology& ta = *a->topo;
+    Topology& tb = *b->topo;
+    cs.ms_topology = lap();
     std::vector<uint8_t> fa, fb;
     EventPair ev_flood, ev_asm;
     ev_flood.start(ctx->stream);
@@ -461,10 +488,16 @@ void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op) {
     ev_flood.stop(ctx->stream);
     BS_HIP(hipStreamSynchronize(ctx->stream));
     cs.ms_flood = ev_flood.ms();
-    apply_flags(ta, fa);
-    apply_flags(tb, fb);
-    ta.flood_fill_upper();
-    tb.flood_fill_upper();
+    lap();
+    {
+        auto fill_b = std::async(std::launch::async, [&] {
+            apply_flags(tb, fb);
+            tb.flood_fill_upper();
+        });
+        apply_flags(ta, fa);
+        ta.flood_fill_upper();
+        fill_b.get();
+    }
 
     std::unique_ptr<Topology> tb_owned = std::move(b->topo);
     ta.csg(std::move(*tb_owned), op);
@@ -474,17 +507,37 @@ void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op) {
     std::vector<TileRec> tiles;
     ta.collect(refs, tiles);
     // output leaves in lexicographic origin order, as every volume here keeps them:
-    // an LSD radix sort of (63-bit leaf key, ref) in four 16-bit digits
+    // an LSD radix sort of (key, ref) in 16-bit digits, the key being the
+    // leaf's rank in the (x, y, z) box of the output (or the 63-bit leaf key
+    // when that box has 2^48 cells or more)
     {
         const size_t n = refs.size();
         std::vector<uint64_t> key(n), key2(n);
         std::vector<int32_t> tmp(n);
+        int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
         for (size_t i = 0; i < n; ++i) {
             const LeafRef& u = ta.leaves[refs[i]];
-            key[i] = leaf_key(u.x >> 3, u.y >> 3, u.z >> 3);
+            const int32_t c[3] = {u.x >> 3, u.y >> 3, u.z >> 3};
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::min(lo[k], c[k]);
+                hi[k] = std::max(hi[k], c[k]);
+            }
+        }
+        uint64_t ext[3] = {1, 1, 1};
+        for (int k = 0; k < 3 && n; ++k) ext[k] = (uint64_t)((int64_t)hi[k] - lo[k] + 1);
+        const bool dense = n && ext[0] < (1ull << 21) && ext[1] < (1ull << 21) && ext[2] < (1ull << 21) &&
+                           ext[0] * ext[1] * ext[2] < (1ull << 48);
+        const uint64_t cells = dense ? ext[0] * ext[1] * ext[2] : 0;
+        int passes = 4;
+        if (dense) passes = cells <= (1ull << 16) ? 1 : (cells <= (1ull << 32) ? 2 : 3);
+        for (size_t i = 0; i < n; ++i) {
+            const LeafRef& u = ta.leaves[refs[i]];
+            key[i] = dense ? (((uint64_t)((u.x >> 3) - lo[0]) * ext[1] + (uint64_t)((u.y >> 3) - lo[1])) * ext[2] +
+                              (uint64_t)((u.z >> 3) - lo[2]))
+                           : leaf_key(u.x >> 3, u.y >> 3, u.z >> 3);
         }
         std::vector<uint32_t> cnt((size_t)1 << 16);
-        for (int pass = 0; pass < 4; ++pass) {
+        for (int pass = 0; pass < passes; ++pass) {
             const int sh = 16 * pass;
             std::fill(cnt.begin(), cnt.end(), 0u);
             for (size_t i = 0; i < n; ++i) ++cnt[(key[i] >> sh) & 0xFFFF];
@@ -518,10 +571,12 @@ void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op) {
         host[n_out + i] = make_int4(r.x, r.y, r.z, 0);
     }
 
+    cs.ms_plan = lap();
     bs_volume out;
     out.device = a->device;
     try {
         alloc_leaves(&out, n_out);
+        cs.ms_memory = lap();
         ev_asm.start(ctx->stream);
         if (n_out) {
             int4* d_host = ctx->scratch.as<int4>(S_CSG_PLAN, 2 * n_out);
@@ -538,8 +593,10 @@ void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op) {
         free_leaves(&out);
         throw;
     }
+    lap();
     free_leaves(a);
     free_leaves(b);
+    cs.ms_memory += lap();
     a->n_leaves = out.n_leaves;
     a->origins = out.origins;
     a->masks = out.masks;
@@ -549,6 +606,7 @@ void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op) {
     out.values = nullptr;
     a->tiles = std::move(tiles);
     a->topo = ta.compact(new_index);
+    cs.ms_compact = lap();
     cs.n_out = n_out;
     cs.ms_wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wall).count();
     ctx->last_csg = cs;

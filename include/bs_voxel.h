@@ -102,6 +102,11 @@ typedef struct {
     double ms_flood;          /* leaf flood fill of both operands (events)    */
     double ms_assemble;       /* result assembly kernel (events)              */
     double ms_wall;           /* the whole call on the host clock             */
+    /* host-clock parts of ms_wall: building the operands' upper levels from
+     * their leaves (when they carry none), the upper-level flood fill, CSG
+     * and output plan, device allocation and release, and the result's
+     * upper levels */
+    double ms_topology, ms_plan, ms_memory, ms_compact;
 } bs_csg_stats;
 
 /* Counters of the last bs_dual_contour* call (see below). */
