@@ -21,6 +21,9 @@ BS_ERR_UNSUPPORTED_TILE = 4
 BS_ERR_INTERNAL = 5
 BS_ERR_NO_DEVICE = 6
 BS_ERR_IO = 7
+BS_ERR_UNSUPPORTED = 8
+BS_MESHING_FEATURE_PRESERVING = 0
+BS_MESHING_MANIFOLD = 1
 BS_PATH_FORCE_HASH = 1
 BS_PATH_FORCE_LDS_STACK = 2
 BS_SIGN_PER_VOXEL = 0
@@ -30,7 +33,7 @@ BS_PROFILE_COUNTERS = 2
 
 STATUS_NAMES = {0: "BS_OK", 1: "BS_ERR_EMPTY", 2: "BS_ERR_INVALID_ARG", 3: "BS_ERR_HIP",
                 4: "BS_ERR_UNSUPPORTED_TILE", 5: "BS_ERR_INTERNAL", 6: "BS_ERR_NO_DEVICE",
-                7: "BS_ERR_IO"}
+                7: "BS_ERR_IO", 8: "BS_ERR_UNSUPPORTED"}
 
 
 class M2VParams(C.Structure):
@@ -134,6 +137,8 @@ def load():
             "bs_dual_contour_device": (C.c_int, [vp, vp, C.c_float, C.POINTER(vp), C.POINTER(C.c_size_t)]),
             "bs_ctx_last_dc_stats": (C.c_int, [vp, C.POINTER(DcStats)]),
             "bs_free": (None, [vp]),
+            "bs_voxel_remesh": (C.c_int, [vp, vp, C.c_size_t, C.c_int, C.c_float, C.c_int,
+                                          C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_size_t)]),
             "bs_stl_read": (C.c_int, [C.c_char_p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_size_t)]),
             "bs_stl_write": (C.c_int, [C.c_char_p, vp, C.c_size_t]),
             "bs_device_free": (None, [vp]),

@@ -503,6 +503,58 @@ bs_status bs_dual_contour(bs_ctx* ctx, const bs_volume* v, float voxel_size, flo
     });
 }
 
+bs_status bs_voxel_remesh(bs_ctx* ctx, const float* tri_xyz, size_t n, int device_input, float voxel_size,
+                          int method, float** out_xyz, size_t* n_tris) {
+    if (!ctx || !out_xyz || !n_tris || (n > 0 && !tri_xyz)) {
+        g_last_error = "null argument";
+        return BS_ERR_INVALID_ARG;
+    }
+    *out_xyz = nullptr;
+    *n_tris = 0;
+    return guarded([&] {
+        DeviceGuard g(ctx->device);
+        BS_REQUIRE(method == BS_MESHING_FEATURE_PRESERVING || method == BS_MESHING_MANIFOLD, BS_ERR_INVALID_ARG,
+                   "unknown meshing method");
+        if (method == BS_MESHING_MANIFOLD)
+            throw Error(BS_ERR_UNSUPPORTED, "marching cubes (MeshingMethod::Manifold) is not part of this library");
+        if (n == 0) throw Error(BS_ERR_EMPTY, "empty mesh: the reference returns None");
+        // VoxelRemesher::default(): MeshToVolume with band 0 (remeshing/voxel.rs:83-90)
+        bs_m2v_params p;
+        bs_m2v_params_default(&p);
+        p.voxel_size = voxel_size;
+        p.band = 0;
+        const float* d = tri_xyz;
+        if (!device_input) {
+            float* dd = ctx->scratch.as<float>(S_INPUT, 9 * (uint64_t)n);
+            BS_HIP(hipMemcpyAsync(dd, tri_xyz, sizeof(float) * 9 * n, hipMemcpyHostToDevice, ctx->stream));
+            d = dd;
+        }
+        bs_volume* vol = nullptr;
+        mesh_to_volume_device(ctx, d, n, p, &vol);
+        std::unique_ptr<bs_volume, void (*)(bs_volume*)> hold(vol, [](bs_volume* v) {
+            free_volume_arrays(v);
+            delete v;
+        });
+        float* dt = nullptr;
+        uint64_t nt = 0;
+        dual_contour_device(ctx, vol, voxel_size, &dt, &nt, &ctx->last_dc);
+        if (!nt) return;
+        float* h = static_cast<float*>(std::malloc(sizeof(float) * 9 * nt));
+        if (!h) {
+            (void)hipFree(dt);
+            throw Error(BS_ERR_INTERNAL, "host out of memory");
+        }
+        hipError_t e = hipMemcpy(h, dt, sizeof(float) * 9 * nt, hipMemcpyDeviceToHost);
+        (void)hipFree(dt);
+        if (e != hipSuccess) {
+            std::free(h);
+            BS_HIP(e);
+        }
+        *out_xyz = h;
+        *n_tris = (size_t)nt;
+    });
+}
+
 bs_status bs_ctx_last_dc_stats(const bs_ctx* ctx, bs_dc_stats* out) {
     if (!ctx || !out) return BS_ERR_INVALID_ARG;
     *out = ctx->last_dc;

@@ -458,3 +458,61 @@ def write_stl(path, soup) -> None:
     """Binary STL (StlWriter, io/stl.rs:150-199) of an (n, 3, 3) soup."""
     a = np.ascontiguousarray(np.asarray(soup, dtype=np.float32).reshape(-1, 9))
     check(load().bs_stl_write(str(path).encode(), a.ctypes.data_as(C.c_void_p), a.shape[0]))
+
+
+class MeshingMethod:
+    """remeshing/voxel.rs:10-15."""
+    FEATURE_PRESERVING = _lib.BS_MESHING_FEATURE_PRESERVING  # dual contouring
+    MANIFOLD = _lib.BS_MESHING_MANIFOLD                      # marching cubes: not provided here
+
+
+class VoxelRemesher:
+    """remeshing/voxel.rs:44-95: mesh -> band-0 SDF -> mesh, in one library
+    call.  The reference's default method is MANIFOLD (marching cubes), which
+    this library does not provide; choose FEATURE_PRESERVING."""
+
+    def __init__(self, ctx: Optional[Context] = None):
+        self._ctx = ctx
+        self.voxel_size = 1.0
+        self.method = MeshingMethod.MANIFOLD
+
+    @property
+    def ctx(self) -> Context:
+        if self._ctx is None:
+            self._ctx = default_context()
+        return self._ctx
+
+    def with_voxel_size(self, size: float) -> "VoxelRemesher":
+        self.voxel_size = float(size)
+        return self
+
+    def with_meshing_method(self, method: int) -> "VoxelRemesher":
+        self.method = int(method)
+        return self
+
+    def remesh(self, mesh) -> Optional[np.ndarray]:
+        """(m, 3, 3) float32 soup; None for an empty mesh."""
+        L = load()
+        out = C.POINTER(C.c_float)()
+        n = C.c_size_t(0)
+        if hasattr(mesh, "is_cuda") and mesh.is_cuda:
+            import torch
+            t = mesh.detach().reshape(-1, 9)
+            if t.dtype != torch.float32 or not t.is_contiguous():
+                t = t.to(torch.float32).contiguous()
+            torch.cuda.current_stream(t.device).synchronize()
+            st = L.bs_voxel_remesh(self.ctx.handle, C.c_void_p(t.data_ptr()), t.shape[0], 1, self.voxel_size,
+                                   self.method, C.byref(out), C.byref(n))
+        else:
+            a = np.ascontiguousarray(np.asarray(mesh, dtype=np.float32).reshape(-1, 9))
+            st = L.bs_voxel_remesh(self.ctx.handle, a.ctypes.data_as(C.c_void_p), a.shape[0], 0, self.voxel_size,
+                                   self.method, C.byref(out), C.byref(n))
+        if st == _lib.BS_ERR_EMPTY:
+            return None
+        check(st)
+        if n.value == 0:
+            return np.zeros((0, 3, 3), np.float32)
+        try:
+            return np.ctypeslib.as_array(out, shape=(n.value * 9,)).copy().reshape(-1, 3, 3)
+        finally:
+            L.bs_free(C.cast(out, C.c_void_p))

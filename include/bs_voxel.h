@@ -35,7 +35,8 @@ typedef enum {
     BS_ERR_UNSUPPORTED_TILE = 4, /* reference panics with todo!() on a tile  */
     BS_ERR_INTERNAL = 5,         /* reference panics (unreachable!/unwrap)   */
     BS_ERR_NO_DEVICE = 6,        /* no HIP device / library built without GPU */
-    BS_ERR_IO = 7                /* a file could not be read or written       */
+    BS_ERR_IO = 7,               /* a file could not be read or written       */
+    BS_ERR_UNSUPPORTED = 8       /* a method this library does not provide    */
 } bs_status;
 
 typedef struct bs_ctx bs_ctx;
@@ -251,6 +252,18 @@ bs_status bs_dual_contour_device(bs_ctx* ctx, const bs_volume* v, float voxel_si
 bs_status bs_ctx_last_dc_stats(const bs_ctx* ctx, bs_dc_stats* out);
 void bs_free(void* p);
 void bs_device_free(void* p_dev);
+
+/* VoxelRemesher::remesh (remeshing/voxel.rs:65-81): MeshToVolume with band 0
+ * (VoxelRemesher::default, :83-90) and voxel size h, then the mesher of
+ * `method`: BS_MESHING_FEATURE_PRESERVING is DualContouringMesher (the soup
+ * comes back as bs_dual_contour's, malloc'd, release with bs_free);
+ * BS_MESHING_MANIFOLD is MarchingCubesMesher, which this library does not
+ * provide (BS_ERR_UNSUPPORTED).  tri_xyz is device memory when device_input
+ * is non-zero.  An empty mesh gives BS_ERR_EMPTY (the reference's None). */
+#define BS_MESHING_FEATURE_PRESERVING 0
+#define BS_MESHING_MANIFOLD 1
+bs_status bs_voxel_remesh(bs_ctx* ctx, const float* tri_xyz, size_t n, int device_input, float voxel_size,
+                          int method, float** out_xyz, size_t* n_tris);
 
 /* Binary STL (io/stl.rs), host only, no device needed.  bs_stl_read is
  * StlReader (:64-92): *out_xyz receives 9 floats per face (malloc'd, release
