@@ -135,6 +135,7 @@ __constant__ int8_t kCell[3][4][3] = {{{0, 0, 0}, {0, 0, -1}, {0, -1, -1}, {0, -
 
 __global__ __launch_bounds__(64) void k_dc_cells(uint64_t n_leaves, DcArgs a) {
     __shared__ Block B;
+    __shared__ uint16_t cell_list[512];  // offset x*64 + y*8 + z of each listed cell
     const int lane = threadIdx.x;
     const int x = lane >> 3, y = lane & 7;
     for (uint64_t leaf = blockIdx.x; leaf < n_leaves; leaf += gridDim.x) {
@@ -158,8 +159,11 @@ __global__ __launch_bounds__(64) void k_dc_cells(uint64_t n_leaves, DcArgs a) {
             }
         }
         if (bad) atomicOr(a.fail, 1u);
-        // cells (dual_contouring.rs:176-247)
+        // cells (dual_contouring.rs:176-247): the leaf's cells with all eight
+        // corners active and mixed signs are listed first, then each lane
+        // takes one listed cell, so the feature-point work runs on full waves
         uint32_t present = 0;
+        int n_cells = 0;
         for (int z = 0; z < 8; ++z) {
             bool all = true;
             uint32_t neg = 0;
@@ -169,7 +173,18 @@ __global__ __launch_bounds__(64) void k_dc_cells(uint64_t n_leaves, DcArgs a) {
                 all &= B.act[c] != 0;
                 neg |= (__float_as_uint(B.val[c]) >> 31) << i;
             }
-            if (!all || neg == 0 || neg == 0xFFu) continue;
+            const bool surface = all && neg != 0 && neg != 0xFFu;
+            const unsigned long long b = __ballot(surface);
+            if (surface) {
+                present |= 1u << z;
+                cell_list[n_cells + __popcll(b & ((1ull << lane) - 1))] = (uint16_t)((lane << 3) | z);
+            }
+            n_cells += __popcll(b);
+        }
+        __syncthreads();
+        for (int k = lane; k < n_cells; k += 64) {
+            const int cell = cell_list[k];
+            const int x = cell >> 6, y = (cell >> 3) & 7, z = cell & 7;
             // crossings in EDGE_OFFSETS order; all indexing static so they stay in registers
             float P[12][3], N[12][3];
             uint32_t valid = 0;
@@ -207,11 +222,10 @@ __global__ __launch_bounds__(64) void k_dc_cells(uint64_t n_leaves, DcArgs a) {
                 cz = cz + (fz * damping) / fc;
                 if ((fx * fx + fy * fy) + fz * fz < 1e-6f) break;
             }
-            float* out = a.cell_pts + (leaf * 512 + (uint64_t)(lane * 8 + z)) * 3;
+            float* out = a.cell_pts + (leaf * 512 + (uint64_t)cell) * 3;
             out[0] = cx;
             out[1] = cy;
             out[2] = cz;
-            present |= 1u << z;
         }
         // cell presence mask: word x, bits y*8+z
         const uint64_t w = gather_word(present, y);
