@@ -256,10 +256,37 @@ __device__ inline bool degenerate(const float a[3], const float b[3], const floa
     return (cx * cx + cy * cy) + cz * cz == 0.0f;
 }
 
-// Triangles of the z-row owned by one lane (dual_contouring.rs:93-174,
-// :62-79): written at dst when dst is not null; returns how many.
-__device__ inline uint32_t row_triangles(const Block& B, const DcArgs& a, int x, int y, float* dst) {
-    uint32_t cnt = 0;
+// The quad of crossing edge d from voxel (x, y, z) with sign bit s1
+// (dual_contouring.rs:93-174, :62-79): its two triangles, scaled, in q, and
+// whether each survives the degenerate-triangle filter; false when one of the
+// four cells around the edge is missing.
+__device__ inline bool edge_quad(const Block& B, const DcArgs& a, int x, int y, int z, int d, uint32_t s1,
+                                 float q[6][3], bool keep[2]) {
+    float p[4][3];
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        ok = ok && cell_at(B, a, x + kCell[d][k][0], y + kCell[d][k][1], z + kCell[d][k][2], p[k]);
+    if (!ok) return false;
+    // [p0, p1, p2, p2, p3, p0], with 1<->2 and 4<->5 swapped when v is negative
+    const int f1 = s1 ? 2 : 1, f2 = s1 ? 1 : 2, f4 = s1 ? 0 : 3, f5 = s1 ? 3 : 0;
+    const int f[6] = {0, f1, f2, 2, f4, f5};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int src = f[k];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            q[k][j] = (src == 0 ? p[0][j] : src == 1 ? p[1][j] : src == 2 ? p[2][j] : p[3][j]) * a.voxel_size;
+    }
+    keep[0] = !degenerate(q[0], q[1], q[2]);
+    keep[1] = !degenerate(q[3], q[4], q[5]);
+    return true;
+}
+
+// Crossing edges from the voxels of this lane's z-row, in the sequential
+// order (z, then axis); emit(z, d) is called for each.
+template <class F>
+__device__ inline void row_edges(const Block& B, int x, int y, F&& emit) {
     for (int z = 0; z < 8; ++z) {
         const int c = bidx(x, y, z);
         if (!B.act[c]) continue;
@@ -267,63 +294,67 @@ __device__ inline uint32_t row_triangles(const Block& B, const DcArgs& a, int x,
         for (int d = 0; d < 3; ++d) {
             const int c2 = c + (d == 0 ? HB * HB : (d == 1 ? HB : 1));
             if (!B.act[c2] || (__float_as_uint(B.val[c2]) >> 31) == s1) continue;
-            float p[4][3];
-            bool ok = true;
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-                ok = ok && cell_at(B, a, x + kCell[d][k][0], y + kCell[d][k][1], z + kCell[d][k][2], p[k]);
-            if (!ok) continue;
-            // [p0, p1, p2, p2, p3, p0], with 1<->2 and 4<->5 swapped when v is negative
-            const int f1 = s1 ? 2 : 1, f2 = s1 ? 1 : 2, f4 = s1 ? 0 : 3, f5 = s1 ? 3 : 0;
-            const int f[6] = {0, f1, f2, 2, f4, f5};
-            float q[6][3];
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                const int src = f[k];
-#pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    q[k][j] = (src == 0 ? p[0][j] : src == 1 ? p[1][j] : src == 2 ? p[2][j] : p[3][j]) * a.voxel_size;
-            }
-#pragma unroll
-            for (int tr = 0; tr < 2; ++tr) {
-                if (degenerate(q[3 * tr], q[3 * tr + 1], q[3 * tr + 2])) continue;
-                if (dst) {
-                    float* o = dst + (uint64_t)cnt * 9;
-#pragma unroll
-                    for (int k = 0; k < 3; ++k)
-#pragma unroll
-                        for (int j = 0; j < 3; ++j) o[k * 3 + j] = q[3 * tr + k][j];
-                }
-                ++cnt;
-            }
+            emit(z, d);
         }
     }
-    return cnt;
 }
 
 // WRITE = false: triangles per leaf into tri_count.  WRITE = true: the
-// triangles at tri_off[leaf], lanes (z-rows) in order.
+// triangles at tri_off[leaf].  A leaf's crossing edges are listed in the
+// sequential order (z-rows = lanes, then z, then axis), then taken 64 at a
+// time, one per lane; each lane's triangles go after those of the edges
+// before it (two ballots give the prefix), so the soup keeps that order.
 template <bool WRITE>
 __global__ __launch_bounds__(64) void k_dc_triangles(uint64_t n_leaves, DcArgs a, uint32_t* __restrict__ tri_count,
                                                      const uint64_t* __restrict__ tri_off, float* __restrict__ out) {
     __shared__ Block B;
     __shared__ uint32_t lane_cnt[64];
+    __shared__ uint16_t edges[512 * 3];  // lane << 5 | z << 2 | axis
     const int lane = threadIdx.x;
     const int x = lane >> 3, y = lane & 7;
+    const unsigned long long lt = (1ull << lane) - 1;
     for (uint64_t leaf = blockIdx.x; leaf < n_leaves; leaf += gridDim.x) {
         stage_block(B, a, a.origins[leaf], lane);
-        lane_cnt[lane] = row_triangles(B, a, x, y, nullptr);
+        uint32_t cnt = 0;
+        row_edges(B, x, y, [&](int, int) { ++cnt; });
+        lane_cnt[lane] = cnt;
         __syncthreads();
-        uint32_t before = 0, total = 0;
+        uint32_t before = 0, n_edges = 0;
         for (int l = 0; l < 64; ++l) {
             before += l < lane ? lane_cnt[l] : 0;
-            total += lane_cnt[l];
+            n_edges += lane_cnt[l];
         }
-        if (!WRITE) {
-            if (lane == 0) tri_count[leaf] = total;
-        } else {
-            row_triangles(B, a, x, y, out + (tri_off[leaf] + before) * 9);
+        row_edges(B, x, y, [&](int z, int d) { edges[before++] = (uint16_t)((lane << 5) | (z << 2) | d); });
+        __syncthreads();
+        const uint64_t base = WRITE ? tri_off[leaf] : 0;
+        uint32_t done = 0;
+        for (uint32_t k0 = 0; k0 < n_edges; k0 += 64) {
+            const uint32_t k = k0 + (uint32_t)lane;
+            float q[6][3];
+            bool keep[2] = {false, false};
+            if (k < n_edges) {
+                const int e = edges[k];
+                const int el = e >> 5, ex = el >> 3, ey = el & 7, ez = (e >> 2) & 7, d = e & 3;
+                const uint32_t s1 = __float_as_uint(B.val[bidx(ex, ey, ez)]) >> 31;
+                if (!edge_quad(B, a, ex, ey, ez, d, s1, q, keep)) keep[0] = keep[1] = false;
+            }
+            const unsigned long long b0 = __ballot(keep[0]), b1 = __ballot(keep[1]);
+            if (WRITE) {
+                uint64_t at = base + done + (uint64_t)(__popcll(b0 & lt) + __popcll(b1 & lt));
+#pragma unroll
+                for (int tr = 0; tr < 2; ++tr) {
+                    if (!keep[tr]) continue;
+                    float* o = out + at * 9;
+#pragma unroll
+                    for (int kk = 0; kk < 3; ++kk)
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) o[kk * 3 + j] = q[3 * tr + kk][j];
+                    ++at;
+                }
+            }
+            done += (uint32_t)(__popcll(b0) + __popcll(b1));
         }
+        if (!WRITE && lane == 0) tri_count[leaf] = done;
         __syncthreads();
     }
 }
