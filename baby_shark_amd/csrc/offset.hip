@@ -315,7 +315,12 @@ __global__ __launch_bounds__(64) void k_sweep_plane(SweepArgs a, const uint32_t*
             const uint64_t w = gather_word(row, ly);
             if (ly == 0) a.masks[(uint64_t)slot * 8 + lx] = w;
         }
-        // fast_sweep.rs:186-273: the +direction face of each axis
+        // fast_sweep.rs:186-273: the +direction face of each axis; lanes 0..2
+        // create the pushed neighbours of the three axes at the same time.
+        // The slot is published without a fence before it: in this launch a
+        // lookup of the new leaf reads only its masks and values, which hold
+        // the empty leaf already, and the origin is read from the next launch.
+        bool push[3];
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
             const int dir = ax == 0 ? a.sx : (ax == 1 ? a.sy : a.sz);
@@ -324,20 +329,22 @@ __global__ __launch_bounds__(64) void k_sweep_plane(SweepArgs a, const uint32_t*
             const int o = ax == 0 ? pidx(f, i, j) : (ax == 1 ? pidx(i, f, j) : pidx(i, j, f));
             const float v = L.val[o];
             const bool q = (L.st[o] & 1) && (int)(__float_as_uint(v) >> 31) == a.neg && fabsf(v) < a.limit_abs;
-            if (__ballot(q) != 0 && lane == 0) {
-                if (plane + 1 >= n_planes) atomicOr(&st->fail, 1u | 4u);
-                const int32_t nx = (org.x >> 3) + (ax == 0 ? dir : 0), ny = (org.y >> 3) + (ax == 1 ? dir : 0),
-                              nz = (org.z >> 3) + (ax == 2 ? dir : 0);
-                int32_t* cell = a.map.claim(nx, ny, nz, &st->fail);
-                if (cell) {
-                    const uint32_t s = start + atomicAdd(&created[plane + 1], 1u);
-                    if (s < a.capacity) {
-                        a.origins[s] = make_int4(nx * 8, ny * 8, nz * 8, 0);
-                        __threadfence();
-                        atomicExch(cell, (int32_t)s);
-                    } else {
-                        atomicOr(&st->fail, 2u);  // pool overflow: retried with a larger pool
-                    }
+            push[ax] = __ballot(q) != 0;
+        }
+        if (lane < 3 && (lane == 0 ? push[0] : (lane == 1 ? push[1] : push[2]))) {
+            const int ax = lane;
+            const int dir = ax == 0 ? a.sx : (ax == 1 ? a.sy : a.sz);
+            if (plane + 1 >= n_planes) atomicOr(&st->fail, 1u | 4u);
+            const int32_t nx = (org.x >> 3) + (ax == 0 ? dir : 0), ny = (org.y >> 3) + (ax == 1 ? dir : 0),
+                          nz = (org.z >> 3) + (ax == 2 ? dir : 0);
+            int32_t* cell = a.map.claim(nx, ny, nz, &st->fail);
+            if (cell) {
+                const uint32_t s = start + atomicAdd(&created[plane + 1], 1u);
+                if (s < a.capacity) {
+                    a.origins[s] = make_int4(nx * 8, ny * 8, nz * 8, 0);
+                    atomicExch(cell, (int32_t)s);
+                } else {
+                    atomicOr(&st->fail, 2u);  // pool overflow: retried with a larger pool
                 }
             }
         }
