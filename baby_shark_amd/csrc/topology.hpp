@@ -8,6 +8,7 @@
 // and root_node/{flood_fill,csg}.rs; each is cited where implemented.
 #pragma once
 #include <array>
+#include <cstdlib>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -51,7 +52,7 @@ struct LeafStore {
             if (i < s.size()) return s[i];
             i -= s.size();
         }
-        return seg.back()[seg.back().size()];  // out of range: never reached by the callers
+        std::abort();  // an index past the end is a bug of the caller
     }
     const LeafRef& operator[](size_t i) const { return const_cast<LeafStore&>(*this)[i]; }
     // one segment of n default references (the store must be empty)
