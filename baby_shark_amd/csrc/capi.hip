@@ -200,7 +200,7 @@ bs_status bs_batch_mesh_to_volume(bs_ctx* ctx, size_t n_meshes, const float* con
         out[i] = nullptr;
         if (status) status[i] = BS_OK;
     }
-    if (n_workers <= 0) n_workers = 8;
+    if (n_workers <= 0) n_workers = 4;
     n_workers = (int)std::min<size_t>((size_t)std::min(n_workers, 16), std::max<size_t>(n_meshes, 1));
     std::vector<bs_status> st(n_meshes, BS_OK);
     std::vector<std::string> msg(n_meshes);

@@ -23,7 +23,7 @@ namespace bs {
 namespace {
 
 constexpr float INV_4PI = 0.0795774715459476678844f;
-constexpr int BVH_LEAF_SIZE = 4;     // a node covering <= 4 triangles is evaluated as a leaf
+constexpr int BVH_LEAF_SIZE = 2;     // a node covering <= 2 triangles is evaluated as a leaf
 constexpr int WN_STACK = 128;        // per-wave traversal stack depth
 constexpr int WN_BLOCK = 256;
 
@@ -612,7 +612,7 @@ __device__ inline int lbvh_delta(const uint64_t* __restrict__ k, int n, int i, i
 // B = (awn.xyz, trace O2), C = (S00, S11, S22, 2 S01), D = (2 S02, 2 S12) with
 // S the symmetric part of order2 = sum a c n^T - c_d awn^T (only S enters
 // r^T O2 r), and links L = (left, right) for an internal node or
-// (first triangle, -count) for a node evaluated as a leaf (count <= 4).
+// (first triangle, -count) for a node evaluated as a leaf (count <= BVH_LEAF_SIZE).
 struct alignas(16) NodeRec {
     float4 A, B, C;
     float2 D;
@@ -1320,7 +1320,7 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     uint32_t* inode_s = S.as<uint32_t>(S_BVH_INODE_S, std::max(n_int, 1));
     NodeAcc* accs = S.as<NodeAcc>(S_BVH_INIT, n_nodes);
     NodeRec* nodes = S.as<NodeRec>(S_BVH_NODES, (uint64_t)n_nodes);  // internal nodes, then leaves
-    float* btris = S.as<float>(S_BVH_TRIS, 12 * (n + BVH_LEAF_SIZE));  // padded: leaves read 4 triangles
+    float* btris = S.as<float>(S_BVH_TRIS, 12 * (n + BVH_LEAF_SIZE));  // padded: leaves read BVH_LEAF_SIZE triangles
     int2* kids = S.as<int2>(S_BVH_KIDS, std::max(n_int, 1));  // binary children of internal nodes
     uint32_t* starts = reinterpret_cast<uint32_t*>(S.get(S_BVH_STARTS, sizeof(uint32_t) * 258));
 

@@ -25,7 +25,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per k_winding launch at the default workload, from the FETCH_SIZE
 # (x2.0 calibration) and WRITE_SIZE counters of `bench.py --traffic-probe`
 # under rocprofv3 --pmc
-WINDING_TRAFFIC_BYTES = 2.236e9
+WINDING_TRAFFIC_BYTES = 2.889e9
 
 
 def traffic_probe():
@@ -292,7 +292,7 @@ def batch(ctx, count=256, seed=5):
     """The uneven batch: 256 closed meshes (tori and displaced icospheres)
     with triangle counts log-uniform in [1e3, 2e5], each voxelized so its
     bounding box spans 256 voxels, band 3, soups resident in HBM.  Timed as
-    one bs_batch_mesh_to_volume call over 8 streams, and as the same call on
+    one bs_batch_mesh_to_volume call over 4 streams, and as the same call on
     1 stream (the per-mesh loop a caller would write)."""
     import torch
     import baby_shark_amd as bs
@@ -304,7 +304,7 @@ def batch(ctx, count=256, seed=5):
     res = {"workload": f"{count} meshes (tori / displaced icospheres), {sum(s.shape[0] for s, _ in ms)} triangles "
                        f"in all, {min(s.shape[0] for s, _ in ms)}-{max(s.shape[0] for s, _ in ms)} each, "
                        "bounding box 256 voxels, band 3"}
-    for name, workers in (("streams_8", 8), ("streams_1", 1)):
+    for name, workers in (("streams_4", 4), ("streams_1", 1)):
         best = None
         for _ in range(2):
             t0 = time.perf_counter()

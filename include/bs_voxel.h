@@ -154,7 +154,7 @@ bs_status bs_mesh_to_volume_device(bs_ctx* ctx, const float* tri_xyz_dev, size_t
 /* Many MeshToVolume::convert calls in one: mesh i is tri_xyz[i] with n_tris[i]
  * triangles (device pointers, or host pointers when host_input != 0) and
  * params[i].  The meshes are spread over n_workers streams of this device
- * (0 = 8), each with its own scratch arena and host thread, largest first, so
+ * (0 = 4), each with its own scratch arena and host thread, largest first, so
  * small meshes' kernels overlap.  out[i] receives the volume or NULL;
  * status[i] (nullable) the per-mesh status (BS_ERR_EMPTY where convert
  * returns None).  Returns BS_OK when every mesh gave BS_OK or BS_ERR_EMPTY,
