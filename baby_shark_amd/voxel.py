@@ -119,6 +119,11 @@ class Volume:
         self._h = handle
         self._ctx = ctx
 
+    @staticmethod
+    def with_voxel_size(voxel_size: float, ctx: Optional[Context] = None) -> "Volume":
+        """An empty volume (Volume::with_voxel_size, volume/mod.rs:16-24)."""
+        return volume_from_voxels(np.zeros((0, 3), np.int32), np.zeros(0, np.float32), voxel_size, ctx)
+
     # ---- ownership ------------------------------------------------------
     def _take(self) -> C.c_void_p:
         if not self._h:
