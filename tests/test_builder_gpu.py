@@ -95,3 +95,23 @@ def test_lightweighting_example_matches_oracle(gpu_ctx, oracle_lib):
     tris = bs.DualContouringMesher(gpu_ctx).with_voxel_size(h).mesh(full)
     print(f"lightweighting: {tris.shape[0]} triangles on the GPU path, {want.shape[0]} in the oracle", flush=True)
     assert abs(tris.shape[0] - want.shape[0]) <= 1e-3 * want.shape[0]
+
+
+def test_builder_argument_errors(gpu_ctx):
+    B = bs.VolumeBuilder(gpu_ctx, 0.05)
+    for call in (lambda: B.cuboid((0, 0, float("nan")), (1, 1, 1)),
+                 lambda: B.iwp((0, 0, 0), (1, 1, 1), 0.0),
+                 lambda: B.iwp((0, 0, 0), (1, 1, 1), float("inf")),
+                 lambda: bs.VolumeBuilder(gpu_ctx, -0.1).cuboid((0, 0, 0), (1, 1, 1)),
+                 lambda: bs.VolumeBuilder(gpu_ctx, 0.05).sphere(float("inf"), (0, 0, 0))):
+        with pytest.raises(bs.BSError) as e:
+            call()
+        assert e.value.status == bs._lib.BS_ERR_INVALID_ARG
+    # an empty volume (Volume::with_voxel_size) unites like any other
+    v = bs.Volume.with_voxel_size(0.05, gpu_ctx)
+    assert v.info()["n_leaves"] == 0
+    s = B.sphere(0.3, (0, 0, 0))
+    before = s.export()
+    from test_volume_ops_gpu import assert_same_volume
+    got = v.union(s).export()
+    assert got.origins.shape == before.origins.shape

@@ -38,3 +38,27 @@ def test_ctx_without_device_reports_no_device():
     h = C.c_void_p()
     assert L.bs_ctx_create(0, C.byref(h)) == _lib.BS_ERR_NO_DEVICE
     assert b"device" in L.bs_last_error()
+
+
+def test_new_entry_points_reject_null_arguments():
+    L = _lib.load()
+    out = C.c_void_p()
+    n = C.c_size_t(0)
+    fp = C.POINTER(C.c_float)()
+    assert L.bs_volume_cuboid(None, 0, 0, 0, 1, 1, 1, 0.1, C.byref(out)) == _lib.BS_ERR_INVALID_ARG
+    assert L.bs_volume_iwp(None, 0, 0, 0, 1, 1, 1, 0.5, 0.1, C.byref(out)) == _lib.BS_ERR_INVALID_ARG
+    assert L.bs_volume_densify_tiles(None, None) == _lib.BS_ERR_INVALID_ARG
+    assert L.bs_voxel_remesh(None, None, 0, 0, 0.1, 0, C.byref(fp), C.byref(n)) == _lib.BS_ERR_INVALID_ARG
+    assert L.bs_stl_read(None, C.byref(fp), C.byref(n)) == _lib.BS_ERR_INVALID_ARG
+    assert L.bs_stl_write(b"/tmp/x.stl", None, 3) == _lib.BS_ERR_INVALID_ARG
+    assert L.bs_ctx_set_paths(None, 0) == _lib.BS_ERR_INVALID_ARG
+    assert L.bs_ctx_set_profiling(None, 1) == _lib.BS_ERR_INVALID_ARG
+
+
+def test_m2v_params_struct_matches_header():
+    """The ctypes mirror has the header's fields in order (sign_mode last)."""
+    names = [f for f, _ in _lib.M2VParams._fields_]
+    assert names == ["voxel_size", "band", "wn_accuracy", "sign_threshold", "fp64_band", "sign_mode"]
+    p = _lib.M2VParams()
+    _lib.load().bs_m2v_params_default(C.byref(p))
+    assert p.sign_mode == _lib.BS_SIGN_PER_VOXEL and abs(p.fp64_band - 0.1) < 1e-7

@@ -88,3 +88,18 @@ def test_shared_signs_batch_and_params(gpu_ctx):
     with pytest.raises(bs.BSError) as e:
         m.convert(soups[0])
     assert e.value.status == bs._lib.BS_ERR_INVALID_ARG
+
+
+def test_shared_signs_on_alternate_paths(gpu_ctx):
+    """Shared signs through the GPU leaf hash and the LDS traversal stack
+    equal the per-voxel signs on the default paths."""
+    soup = mg.displaced_icosphere(4, 0.8, 0.05, seed=6)
+    p, _ = _convert(gpu_ctx, soup, 0.01, 3, False)
+    pe = p.export()
+    gpu_ctx.set_paths(bs._lib.BS_PATH_FORCE_HASH | bs._lib.BS_PATH_FORCE_LDS_STACK)
+    try:
+        s, st = _convert(gpu_ctx, soup, 0.01, 3, True)
+        assert st["used_hash"] == 1 and st["used_lds_stack"] == 1
+    finally:
+        gpu_ctx.set_paths(0)
+    assert np.array_equal(pe.values.view(np.uint32), s.export().values.view(np.uint32))
