@@ -398,14 +398,48 @@ __constant__ uint32_t kMagic[65] = {
     1873, 1821, 1772, 1725, 1681, 1639, 1599, 1561, 1525, 1490, 1457, 1425, 1395, 1366, 1338, 1311, 1286, 1261,
     1237, 1214, 1192, 1171, 1150, 1130, 1111, 1093, 1075, 1058, 1041, 1025};
 
+// Union-find over a leaf's voxels in LDS, for the shared-sign groups (see the
+// shared-sign kernels further down for the argument).
+__device__ inline int uf_find(int* par, int x) {
+    int p = __hip_atomic_load(&par[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    while (p != x) {
+        x = p;
+        p = __hip_atomic_load(&par[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    return x;
+}
+
+__device__ inline void uf_unite(int* par, int a, int b) {
+    // every failed exchange means another lane hooked b first: the loop ends
+    for (;;) {
+        a = uf_find(par, a);
+        b = uf_find(par, b);
+        if (a == b) return;
+        if (a > b) {
+            const int t = a;
+            a = b;
+            b = t;
+        }
+        if (atomicCAS(&par[b], b, a) == b) return;
+    }
+}
+
 constexpr int DIST_BLOCK = 256;
 
+// GROUPS: the leaf's shared-sign groups are formed here too, from the
+// distances still in the workgroup: face-adjacent active voxels whose
+// distances sum above group_thr are united (LDS union-find, root = smallest
+// offset), roots[leaf * 512 + v] = root offset (0xFFFF inactive), rep_cnt[leaf]
+// = number of groups.
+template <bool GROUPS>
 __global__ __launch_bounds__(DIST_BLOCK) void k_distance(const int32_t* __restrict__ origins,
                                                          const uint32_t* __restrict__ cand_off,
                                                          const uint32_t* __restrict__ cand_list,
                                                          const DistRec* __restrict__ recs,
                                                          const float* __restrict__ sub, float h, uint32_t n_cand,
-                                                         float* __restrict__ values, uint64_t* __restrict__ masks) {
+                                                         float* __restrict__ values, uint64_t* __restrict__ masks,
+                                                         float group_thr, uint16_t* __restrict__ roots,
+                                                         uint32_t* __restrict__ rep_cnt) {
     __shared__ unsigned long long best[LEAF_VOX];
     const int leaf = blockIdx.x;
     const int tid = threadIdx.x;
@@ -458,8 +492,11 @@ __global__ __launch_bounds__(DIST_BLOCK) void k_distance(const int32_t* __restri
         }
     }
     __syncthreads();
+    constexpr int PER = LEAF_VOX / DIST_BLOCK;
+    float vk[PER];
+    bool ak[PER];
 #pragma unroll
-    for (int k = 0; k < LEAF_VOX / DIST_BLOCK; ++k) {
+    for (int k = 0; k < PER; ++k) {
         const int off = tid + DIST_BLOCK * k;
         const unsigned long long key = best[off];
         const bool active = key != ~0ull;
@@ -473,6 +510,45 @@ __global__ __launch_bounds__(DIST_BLOCK) void k_distance(const int32_t* __restri
         values[(uint64_t)leaf * LEAF_VOX + off] = v;
         const unsigned long long m = __ballot(active);
         if (lane == 0) masks[(uint64_t)leaf * 8 + (off >> 6)] = m;
+        vk[k] = v;
+        ak[k] = active;
+    }
+    if constexpr (GROUPS) {
+        // best[] is free once every key has been read: 512 distances (-1 for an
+        // inactive voxel) and 512 union-find parents take its place
+        __shared__ uint32_t n_roots;
+        float* d = reinterpret_cast<float*>(best);
+        int* par = reinterpret_cast<int*>(best) + LEAF_VOX;
+        __syncthreads();
+        if (tid == 0) n_roots = 0;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int off = tid + DIST_BLOCK * k;
+            d[off] = ak[k] ? vk[k] : -1.0f;
+            par[off] = off;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int v = tid + DIST_BLOCK * k;
+            if (!ak[k]) continue;
+            const float dv = vk[k];
+            if ((v >> 6) < 7 && d[v + 64] >= 0.0f && dv + d[v + 64] > group_thr) uf_unite(par, v, v + 64);
+            if (((v >> 3) & 7) < 7 && d[v + 8] >= 0.0f && dv + d[v + 8] > group_thr) uf_unite(par, v, v + 8);
+            if ((v & 7) < 7 && d[v + 1] >= 0.0f && dv + d[v + 1] > group_thr) uf_unite(par, v, v + 1);
+        }
+        __syncthreads();
+        uint32_t mine = 0;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int v = tid + DIST_BLOCK * k;
+            const int r = ak[k] ? uf_find(par, v) : -1;
+            roots[(uint64_t)leaf * LEAF_VOX + v] = ak[k] ? (uint16_t)r : (uint16_t)0xFFFF;
+            mine += (uint32_t)__popcll(__ballot(ak[k] && r == v));
+        }
+        if (lane == 0) atomicAdd(&n_roots, mine);
+        __syncthreads();
+        if (tid == 0) rep_cnt[leaf] = n_roots;
     }
 }
 
@@ -1083,64 +1159,6 @@ __global__ void k_winding_fp64(const uint32_t* __restrict__ list, uint32_t n, co
 // smaller one), its winding number is evaluated and its sign is copied to the
 // group.  Groups never span leaves.
 
-__device__ inline int uf_find(int* par, int x) {
-    int p = __hip_atomic_load(&par[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    while (p != x) {
-        x = p;
-        p = __hip_atomic_load(&par[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
-    return x;
-}
-
-__device__ inline void uf_unite(int* par, int a, int b) {
-    // every failed exchange means another lane hooked b first: the loop ends
-    for (;;) {
-        a = uf_find(par, a);
-        b = uf_find(par, b);
-        if (a == b) return;
-        if (a > b) {
-            const int t = a;
-            a = b;
-            b = t;
-        }
-        if (atomicCAS(&par[b], b, a) == b) return;
-    }
-}
-
-// roots[slot] = offset of the group root, 0xFFFF for an inactive voxel;
-// rep_cnt[leaf] = number of groups.  Values are unsigned distances here.
-__global__ __launch_bounds__(512) void k_sign_groups(const uint64_t* __restrict__ masks,
-                                                     const float* __restrict__ values, float thr,
-                                                     uint16_t* __restrict__ roots, uint32_t* __restrict__ rep_cnt) {
-    __shared__ float d[LEAF_VOX];
-    __shared__ int par[LEAF_VOX];
-    __shared__ uint32_t wcnt[8];
-    const uint64_t leaf = blockIdx.x;
-    const int v = threadIdx.x;
-    const bool act = (masks[leaf * 8 + (v >> 6)] >> (v & 63)) & 1ull;
-    d[v] = act ? values[leaf * LEAF_VOX + v] : -1.0f;
-    par[v] = v;
-    __syncthreads();
-    if (act) {
-        const float dv = d[v];
-        // +x, +y, +z neighbours inside the leaf (offset = x<<6 | y<<3 | z)
-        if ((v >> 6) < 7 && d[v + 64] >= 0.0f && dv + d[v + 64] > thr) uf_unite(par, v, v + 64);
-        if (((v >> 3) & 7) < 7 && d[v + 8] >= 0.0f && dv + d[v + 8] > thr) uf_unite(par, v, v + 8);
-        if ((v & 7) < 7 && d[v + 1] >= 0.0f && dv + d[v + 1] > thr) uf_unite(par, v, v + 1);
-    }
-    __syncthreads();
-    const int r = act ? uf_find(par, v) : -1;
-    roots[leaf * LEAF_VOX + v] = act ? (uint16_t)r : (uint16_t)0xFFFF;
-    const unsigned long long b = __ballot(act && r == v);
-    if ((v & 63) == 0) wcnt[v >> 6] = (uint32_t)__popcll(b);
-    __syncthreads();
-    if (v == 0) {
-        uint32_t c = 0;
-        for (int k = 0; k < 8; ++k) c += wcnt[k];
-        rep_cnt[leaf] = c;
-    }
-}
-
 // the group roots of every leaf, slot = leaf * 512 + offset, in offset order
 __global__ __launch_bounds__(512) void k_sign_reps(const uint16_t* __restrict__ roots,
                                                    const uint32_t* __restrict__ rep_off, uint32_t* __restrict__ reps) {
@@ -1446,8 +1464,32 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     tm.mark();
 
     // ---- 3. distance
-    k_distance<<<(unsigned)n_leaves, DIST_BLOCK, 0, st>>>(vol->origins, cand_off, cand_list, recs, sub, h,
-                                                          (uint32_t)n_cov, vol->values, vol->masks);
+    // shared signs: the link threshold and the group buffers, filled by the distance pass
+    uint16_t* roots = nullptr;
+    uint32_t* rep_cnt = nullptr;
+    float group_thr = 0.0f;
+    if (shared_signs) {
+        double maxc = 0.0;
+        for (int k = 0; k < 3; ++k)
+            maxc = std::max({maxc, std::fabs((double)(long long)hc[C_BOX_LO + k]),
+                             std::fabs((double)(long long)hc[C_BOX_HI + k])});
+        maxc = (maxc + 1.0) * (double)h;
+        // sample spacing, sub-triangle vertex drift (two roundings per
+        // accumulation step) and distance rounding, each bounded in ulps of
+        // the largest coordinate
+        const double ulp = maxc * 0x1p-23;
+        group_thr = (float)((double)h * (1.0 + 0x1p-10) + (4.0 * max_rows + 200.0) * ulp);
+        roots = S.as<uint16_t>(S_SIGN_ROOTS, n_leaves * LEAF_VOX);
+        rep_cnt = S.as<uint32_t>(S_REP_CNT, n_leaves + 1);
+        BS_HIP(hipMemsetAsync(rep_cnt + n_leaves, 0, sizeof(uint32_t), st));
+        k_distance<true><<<(unsigned)n_leaves, DIST_BLOCK, 0, st>>>(vol->origins, cand_off, cand_list, recs, sub, h,
+                                                                    (uint32_t)n_cov, vol->values, vol->masks,
+                                                                    group_thr, roots, rep_cnt);
+    } else {
+        k_distance<false><<<(unsigned)n_leaves, DIST_BLOCK, 0, st>>>(vol->origins, cand_off, cand_list, recs, sub, h,
+                                                                     (uint32_t)n_cov, vol->values, vol->masks,
+                                                                     0.0f, nullptr, nullptr);
+    }
     tm.mark();
 
     // ---- BVH on the second stream, beside the distance pass
@@ -1499,27 +1541,12 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     BS_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
     tm.mark();
 
-    // ---- 6. sign groups (shared signs only): the winding number is evaluated
+    // ---- 6. group roots (shared signs only): the winding number is evaluated
     // at one voxel per group
     const uint32_t* wn_list = act;
     uint32_t n_wn = n_act;
-    uint16_t* roots = nullptr;
     if (shared_signs && n_act > 0) {
-        double maxc = 0.0;
-        for (int k = 0; k < 3; ++k)
-            maxc = std::max({maxc, std::fabs((double)(long long)hc[C_BOX_LO + k]),
-                             std::fabs((double)(long long)hc[C_BOX_HI + k])});
-        maxc = (maxc + 1.0) * (double)h;
-        // sample spacing, sub-triangle vertex drift (two roundings per
-        // accumulation step) and distance rounding, each bounded in ulps of
-        // the largest coordinate
-        const double ulp = maxc * 0x1p-23;
-        const float thr = (float)((double)h * (1.0 + 0x1p-10) + (4.0 * max_rows + 200.0) * ulp);
-        roots = S.as<uint16_t>(S_SIGN_ROOTS, n_leaves * LEAF_VOX);
-        uint32_t* rep_cnt = S.as<uint32_t>(S_REP_CNT, n_leaves + 1);
         uint32_t* rep_off = S.as<uint32_t>(S_REP_OFF, n_leaves + 1);
-        BS_HIP(hipMemsetAsync(rep_cnt + n_leaves, 0, sizeof(uint32_t), st));
-        k_sign_groups<<<(unsigned)n_leaves, 512, 0, st>>>(vol->masks, vol->values, thr, roots, rep_cnt);
         exclusive_scan(ctx, rep_cnt, rep_off, n_leaves + 1);
         n_wn = read_scalar(ctx, rep_off + n_leaves);
         uint32_t* reps = S.as<uint32_t>(S_REPS, std::max<uint32_t>(n_wn, 1));
