@@ -136,6 +136,9 @@ def load():
             "bs_dual_contour": (C.c_int, [vp, vp, C.c_float, C.POINTER(f32p), C.POINTER(C.c_size_t)]),
             "bs_dual_contour_device": (C.c_int, [vp, vp, C.c_float, C.POINTER(vp), C.POINTER(C.c_size_t)]),
             "bs_ctx_last_dc_stats": (C.c_int, [vp, C.POINTER(DcStats)]),
+            "bs_marching_cubes": (C.c_int, [vp, vp, C.c_float, C.POINTER(f32p), C.POINTER(C.c_size_t)]),
+            "bs_marching_cubes_device": (C.c_int, [vp, vp, C.c_float, C.POINTER(vp), C.POINTER(C.c_size_t)]),
+            "bs_ctx_last_mc_stats": (C.c_int, [vp, C.POINTER(DcStats)]),
             "bs_free": (None, [vp]),
             "bs_voxel_remesh": (C.c_int, [vp, vp, C.c_size_t, C.c_int, C.c_float, C.c_int,
                                           C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_size_t)]),

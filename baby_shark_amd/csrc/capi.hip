@@ -25,6 +25,8 @@ void volume_densify_tiles(bs_ctx* ctx, bs_volume* v);
 void volume_offset(bs_ctx* ctx, bs_volume* vol, float distance, bs_offset_stats* stats);
 void dual_contour_device(bs_ctx* ctx, bs_volume* vol, float voxel_size, float** out_dev, uint64_t* n_tris,
                          bs_dc_stats* stats);
+void marching_cubes_device(bs_ctx* ctx, bs_volume* vol, float voxel_size, float** out_dev, uint64_t* n_tris,
+                           bs_dc_stats* stats);
 void volume_from_voxels(bs_ctx* ctx, const int32_t* idx, const float* values, uint64_t n, float h, bs_volume** out);
 void volume_signs_at(bs_ctx* ctx, bs_volume* v, const int32_t* idx, uint64_t n, uint8_t* out_neg);
 
@@ -503,6 +505,62 @@ bs_status bs_dual_contour(bs_ctx* ctx, const bs_volume* v, float voxel_size, flo
     });
 }
 
+// a device soup of n triangles to malloc'd host memory; the device copy is freed
+static float* soup_to_host(float* d, uint64_t n) {
+    float* h = static_cast<float*>(std::malloc(sizeof(float) * 9 * n));
+    if (!h) {
+        (void)hipFree(d);
+        throw Error(BS_ERR_INTERNAL, "host out of memory");
+    }
+    hipError_t e = hipMemcpy(h, d, sizeof(float) * 9 * n, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) {
+        std::free(h);
+        BS_HIP(e);
+    }
+    return h;
+}
+
+bs_status bs_marching_cubes_device(bs_ctx* ctx, const bs_volume* v, float voxel_size, float** out_xyz_dev,
+                                   size_t* n_tris) {
+    if (!ctx || !v || !out_xyz_dev || !n_tris) {
+        g_last_error = "null argument";
+        return BS_ERR_INVALID_ARG;
+    }
+    *out_xyz_dev = nullptr;
+    *n_tris = 0;
+    return guarded([&] {
+        DeviceGuard g(ctx->device);
+        uint64_t n = 0;
+        marching_cubes_device(ctx, const_cast<bs_volume*>(v), voxel_size, out_xyz_dev, &n, &ctx->last_mc);
+        *n_tris = (size_t)n;
+    });
+}
+
+bs_status bs_marching_cubes(bs_ctx* ctx, const bs_volume* v, float voxel_size, float** out_xyz, size_t* n_tris) {
+    if (!ctx || !v || !out_xyz || !n_tris) {
+        g_last_error = "null argument";
+        return BS_ERR_INVALID_ARG;
+    }
+    *out_xyz = nullptr;
+    *n_tris = 0;
+    return guarded([&] {
+        DeviceGuard g(ctx->device);
+        float* d = nullptr;
+        uint64_t n = 0;
+        marching_cubes_device(ctx, const_cast<bs_volume*>(v), voxel_size, &d, &n, &ctx->last_mc);
+        if (!n) return;
+        *out_xyz = soup_to_host(d, n);
+        *n_tris = (size_t)n;
+    });
+}
+
+bs_status bs_ctx_last_mc_stats(const bs_ctx* ctx, bs_dc_stats* out) {
+    if (!ctx || !out) return BS_ERR_INVALID_ARG;
+    *out = ctx->last_mc;
+    return BS_OK;
+}
+
 bs_status bs_voxel_remesh(bs_ctx* ctx, const float* tri_xyz, size_t n, int device_input, float voxel_size,
                           int method, float** out_xyz, size_t* n_tris) {
     if (!ctx || !out_xyz || !n_tris || (n > 0 && !tri_xyz)) {
@@ -515,8 +573,6 @@ bs_status bs_voxel_remesh(bs_ctx* ctx, const float* tri_xyz, size_t n, int devic
         DeviceGuard g(ctx->device);
         BS_REQUIRE(method == BS_MESHING_FEATURE_PRESERVING || method == BS_MESHING_MANIFOLD, BS_ERR_INVALID_ARG,
                    "unknown meshing method");
-        if (method == BS_MESHING_MANIFOLD)
-            throw Error(BS_ERR_UNSUPPORTED, "marching cubes (MeshingMethod::Manifold) is not part of this library");
         if (n == 0) throw Error(BS_ERR_EMPTY, "empty mesh: the reference returns None");
         // VoxelRemesher::default(): MeshToVolume with band 0 (remeshing/voxel.rs:83-90)
         bs_m2v_params p;
@@ -537,20 +593,12 @@ bs_status bs_voxel_remesh(bs_ctx* ctx, const float* tri_xyz, size_t n, int devic
         });
         float* dt = nullptr;
         uint64_t nt = 0;
-        dual_contour_device(ctx, vol, voxel_size, &dt, &nt, &ctx->last_dc);
+        if (method == BS_MESHING_MANIFOLD)
+            marching_cubes_device(ctx, vol, voxel_size, &dt, &nt, &ctx->last_mc);
+        else
+            dual_contour_device(ctx, vol, voxel_size, &dt, &nt, &ctx->last_dc);
         if (!nt) return;
-        float* h = static_cast<float*>(std::malloc(sizeof(float) * 9 * nt));
-        if (!h) {
-            (void)hipFree(dt);
-            throw Error(BS_ERR_INTERNAL, "host out of memory");
-        }
-        hipError_t e = hipMemcpy(h, dt, sizeof(float) * 9 * nt, hipMemcpyDeviceToHost);
-        (void)hipFree(dt);
-        if (e != hipSuccess) {
-            std::free(h);
-            BS_HIP(e);
-        }
-        *out_xyz = h;
+        *out_xyz = soup_to_host(dt, nt);
         *n_tris = (size_t)nt;
     });
 }

@@ -46,7 +46,8 @@ enum ScratchSlot : int {
     S_BVH_LINKS, S_BVH_PARENT, S_BVH_FLAGS, S_BVH_INIT, S_BVH_BOX, S_RECHECK, S_ARGMIN, S_MISC0, S_MISC1,
     S_MISC2, S_MISC3, S_INPUT, S_BVH_KIDS, S_FLAGS_A, S_FLAGS_B, S_CSG_PLAN, S_DC_MAP, S_DC_KEYS, S_DC_FAIL, S_DC_CELLS, S_SIGN_ROOTS, S_REP_CNT, S_REP_OFF, S_REPS,
     S_BVH_HEIGHT, S_BVH_HEIGHT_S, S_BVH_INODE, S_BVH_INODE_S, S_BVH_MISC, S_BVH_STARTS, S_CUB_B,
-    S_DC_CELLMASK, S_DC_TRICNT, S_DC_TRIOFF, S_COUNT
+    S_DC_CELLMASK, S_DC_TRICNT, S_DC_TRIOFF, S_MC_TAB, S_MC_UNITS, S_MC_TILES, S_MC_T8, S_MC_T8K, S_MC_T128,
+    S_MC_T128K, S_MC_TORG, S_COUNT
 };
 
 struct Scratch {
@@ -87,6 +88,7 @@ struct bs_ctx {
     bs_m2v_stats last_m2v = {};
     bs_offset_stats last_offset = {};
     bs_dc_stats last_dc = {};
+    bs_dc_stats last_mc = {};
     bs_csg_stats last_csg = {};
     std::vector<std::unique_ptr<bs_ctx>> workers;  // sub-contexts of batched calls (own stream, scratch)
     // second stream for work that overlaps the main one (the BVH build), with

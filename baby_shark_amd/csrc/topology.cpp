@@ -559,6 +559,33 @@ void Topology::collect(std::vector<int32_t>& leaf_refs, std::vector<TileRec>& ti
     }
 }
 
+// internal_node/tree_node.rs:228-239, root_node/tree_node.rs:109-113
+void Topology::visit_units(std::vector<int64_t>& units, std::vector<TileRec>& tiles) const {
+    for (const auto& kv : root) {
+        const Node1& a = n1[kv.second];
+        for (int o1 = 0; o1 < Node1::SIZE; ++o1) {
+            if (a.kind[o1] == SLOT_TILE) {
+                int32_t x, y, z;
+                slot_origin(a, o1, L2_SPAN, x, y, z);
+                units.push_back(-1 - (int64_t)tiles.size());
+                tiles.push_back({x, y, z, L2_SPAN, a.tile[o1]});
+            } else if (a.kind[o1] == SLOT_BRANCH) {
+                const Node2& b = n2[a.child[o1]];
+                for (int o2 = 0; o2 < Node2::SIZE; ++o2) {
+                    if (b.kind[o2] == SLOT_BRANCH) {
+                        units.push_back(b.child[o2]);
+                    } else if (b.kind[o2] == SLOT_TILE) {
+                        int32_t x, y, z;
+                        slot_origin(b, o2, 8, x, y, z);
+                        units.push_back(-1 - (int64_t)tiles.size());
+                        tiles.push_back({x, y, z, 8, b.tile[o2]});
+                    }
+                }
+            }
+        }
+    }
+}
+
 int32_t Topology::locate(int32_t x, int32_t y, int32_t z, bool* neg_out) const {
     auto it = root.find(Key{x & ~(L1_SPAN - 1), y & ~(L1_SPAN - 1), z & ~(L1_SPAN - 1)});
     if (it == root.end()) {

@@ -105,6 +105,10 @@ struct Topology {
     // Leaves (in tree order) and active tiles of the result.
     void collect(std::vector<int32_t>& leaf_refs, std::vector<TileRec>& tiles) const;
 
+    // The serial visit order of visit_leafs (root in key order, slots in offset
+    // order): units[i] >= 0 is a leaf index, -1 - k is active tile tiles[k].
+    void visit_units(std::vector<int64_t>& units, std::vector<TileRec>& tiles) const;
+
     // sign_at down to the leaf level (root_node/flood_fill.rs:56-63,
     // internal_node/flood_fill.rs:110-117): the index of the leaf holding
     // voxel (x, y, z), or -1 with *neg set from the slot's tile value (active

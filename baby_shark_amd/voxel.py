@@ -64,6 +64,11 @@ class Context:
         check(load().bs_ctx_last_dc_stats(self.handle, C.byref(s)))
         return {k: getattr(s, k) for k, _ in s._fields_}
 
+    def last_mc_stats(self) -> dict:
+        s = _lib.DcStats()
+        check(load().bs_ctx_last_mc_stats(self.handle, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
     def last_stats(self) -> dict:
         s = _lib.M2VStats()
         check(load().bs_ctx_last_m2v_stats(self._h, C.byref(s)))
@@ -358,6 +363,51 @@ class DualContouringMesher:
         return DeviceTriangles(ptr.value or 0, n.value)
 
 
+class MarchingCubesMesher:
+    """meshing/marching_cubes.rs:17-63 (Marching Cubes 33).  Default voxel
+    size 1.0.  Cube by cube the reference's crossings, topology and
+    orientation; the diagonal that splits a polygon and the order inside a
+    cube are this library's own."""
+
+    def __init__(self, ctx: Optional[Context] = None):
+        self._ctx = ctx
+        self.voxel_size = 1.0
+
+    @property
+    def ctx(self) -> Context:
+        if self._ctx is None:
+            self._ctx = default_context()
+        return self._ctx
+
+    def with_voxel_size(self, size: float) -> "MarchingCubesMesher":
+        self.voxel_size = float(size)
+        return self
+
+    def set_voxel_size(self, size: float) -> "MarchingCubesMesher":
+        return self.with_voxel_size(size)
+
+    def mesh(self, volume: "Volume") -> np.ndarray:
+        """Triangle soup (n, 3, 3) float32 on the host."""
+        L = load()
+        out = C.POINTER(C.c_float)()
+        n = C.c_size_t(0)
+        check(L.bs_marching_cubes(self.ctx.handle, volume._h, self.voxel_size, C.byref(out), C.byref(n)))
+        if n.value == 0:
+            return np.zeros((0, 3, 3), np.float32)
+        try:
+            return np.ctypeslib.as_array(out, shape=(n.value * 9,)).copy().reshape(-1, 3, 3)
+        finally:
+            L.bs_free(C.cast(out, C.c_void_p))
+
+    def mesh_device(self, volume: "Volume") -> "DeviceTriangles":
+        """Triangle soup left in device memory (see DeviceTriangles)."""
+        ptr = C.c_void_p()
+        n = C.c_size_t(0)
+        check(load().bs_marching_cubes_device(self.ctx.handle, volume._h, self.voxel_size, C.byref(ptr),
+                                              C.byref(n)))
+        return DeviceTriangles(ptr.value or 0, n.value)
+
+
 class DeviceTriangles:
     """A device-resident (n, 3, 3) float32 soup owned by the library's
     allocator; exposes __cuda_array_interface__ so torch.as_tensor can view it
@@ -468,13 +518,13 @@ def write_stl(path, soup) -> None:
 class MeshingMethod:
     """remeshing/voxel.rs:10-15."""
     FEATURE_PRESERVING = _lib.BS_MESHING_FEATURE_PRESERVING  # dual contouring
-    MANIFOLD = _lib.BS_MESHING_MANIFOLD                      # marching cubes: not provided here
+    MANIFOLD = _lib.BS_MESHING_MANIFOLD                      # marching cubes (the default)
 
 
 class VoxelRemesher:
     """remeshing/voxel.rs:44-95: mesh -> band-0 SDF -> mesh, in one library
-    call.  The reference's default method is MANIFOLD (marching cubes), which
-    this library does not provide; choose FEATURE_PRESERVING."""
+    call.  The default method is MANIFOLD (marching cubes), as in the
+    reference (:87-95)."""
 
     def __init__(self, ctx: Optional[Context] = None):
         self._ctx = ctx

@@ -253,13 +253,27 @@ bs_status bs_ctx_last_dc_stats(const bs_ctx* ctx, bs_dc_stats* out);
 void bs_free(void* p);
 void bs_device_free(void* p_dev);
 
+/* MarchingCubesMesher::default().with_voxel_size(voxel_size).mesh(volume)
+ * (src/voxel/meshing/marching_cubes.rs:32-63): Marching Cubes 33 over every
+ * cube whose eight corners have values, leaves and active tiles in the tree's
+ * visit order (a tile meets the cubes of its six faces, :970-994).  *out_xyz
+ * receives 9 floats per triangle (malloc'd, release with bs_free).  Each cube
+ * emits the reference's crossings, topology and orientation; the diagonal that
+ * splits a polygon and the order inside a cube are this library's own. */
+bs_status bs_marching_cubes(bs_ctx* ctx, const bs_volume* v, float voxel_size, float** out_xyz, size_t* n_tris);
+/* Same, with the soup left in device memory (release with bs_device_free). */
+bs_status bs_marching_cubes_device(bs_ctx* ctx, const bs_volume* v, float voxel_size, float** out_xyz_dev,
+                                   size_t* n_tris);
+/* Counters of the last bs_marching_cubes* call (n_leaves, n_tris, used_hash, ms_total). */
+bs_status bs_ctx_last_mc_stats(const bs_ctx* ctx, bs_dc_stats* out);
+
 /* VoxelRemesher::remesh (remeshing/voxel.rs:65-81): MeshToVolume with band 0
  * (VoxelRemesher::default, :83-90) and voxel size h, then the mesher of
  * `method`: BS_MESHING_FEATURE_PRESERVING is DualContouringMesher (the soup
  * comes back as bs_dual_contour's, malloc'd, release with bs_free);
- * BS_MESHING_MANIFOLD is MarchingCubesMesher, which this library does not
- * provide (BS_ERR_UNSUPPORTED).  tri_xyz is device memory when device_input
- * is non-zero.  An empty mesh gives BS_ERR_EMPTY (the reference's None). */
+ * BS_MESHING_MANIFOLD (the reference's default, :92) is MarchingCubesMesher,
+ * as bs_marching_cubes.  tri_xyz is device memory when device_input is
+ * non-zero.  An empty mesh gives BS_ERR_EMPTY (the reference's None). */
 #define BS_MESHING_FEATURE_PRESERVING 0
 #define BS_MESHING_MANIFOLD 1
 bs_status bs_voxel_remesh(bs_ctx* ctx, const float* tri_xyz, size_t n, int device_input, float voxel_size,

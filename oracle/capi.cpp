@@ -16,6 +16,7 @@
 #include <unordered_map>
 
 #include "dual_contouring.hpp"
+#include "marching_cubes.hpp"
 #include "fast_sweep.hpp"
 #include "mesh_to_volume.hpp"
 #include "parallel.hpp"
@@ -495,6 +496,30 @@ int or_dual_contour(const or_volume* a, float h, float** out, size_t* n_tris) {
         return e.code;
     }
     return 0;
+}
+
+// MarchingCubesMesher::default().with_voxel_size(h).mesh(volume)
+// (marching_cubes.rs:43-63).  *out receives n_tris*9 floats (free with or_free).
+int or_marching_cubes(const or_volume* a, float h, float** out, size_t* n_tris) {
+    std::vector<Vec3f> v = mc::mesh(*a->grid, h);
+    *n_tris = v.size() / 3;
+    *out = (float*)std::malloc(sizeof(float) * 3 * (v.size() ? v.size() : 1));
+    for (size_t i = 0; i < v.size(); ++i) {
+        (*out)[3 * i] = v[i].x;
+        (*out)[3 * i + 1] = v[i].y;
+        (*out)[3 * i + 2] = v[i].z;
+    }
+    return 0;
+}
+
+// The triangles of one cube as crossing-edge triples (12 = centre), for the
+// per-cube checks: c = 8 clamped corner values; returns the triangle count.
+int or_mc_cube_tiling(const float* c, int* out_edges) {
+    std::vector<mc::Tri> t;
+    mc::cube_tiling(c, t);
+    for (size_t i = 0; i < t.size(); ++i)
+        for (int k = 0; k < 3; ++k) out_edges[3 * i + k] = t[i][k];
+    return (int)t.size();
 }
 
 void or_free(void* p) { std::free(p); }

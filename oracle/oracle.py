@@ -66,6 +66,10 @@ def lib():
         L.or_volume_densify_tiles.argtypes = [vp]
         L.or_dual_contour.restype = C.c_int
         L.or_dual_contour.argtypes = [vp, C.c_float, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_size_t)]
+        L.or_marching_cubes.restype = C.c_int
+        L.or_marching_cubes.argtypes = [vp, C.c_float, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_size_t)]
+        L.or_mc_cube_tiling.restype = C.c_int
+        L.or_mc_cube_tiling.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_int)]
         L.or_free.argtypes = [vp]
         L.or_fs_compute_distance.restype = C.c_float
         L.or_fs_compute_distance.argtypes = [C.c_float] * 4
@@ -147,6 +151,17 @@ class Volume:
         lib().or_volume_densify_tiles(self.h)
         return self
 
+    def marching_cubes(self, voxel_size: float) -> np.ndarray:
+        """MarchingCubesMesher (marching_cubes.rs:43-63): (n_tris, 3, 3) float32."""
+        out = C.POINTER(C.c_float)()
+        n = C.c_size_t(0)
+        lib().or_marching_cubes(self.h, voxel_size, C.byref(out), C.byref(n))
+        try:
+            return np.ctypeslib.as_array(out, shape=(n.value * 9,)).copy().reshape(-1, 3, 3) if n.value else \
+                np.zeros((0, 3, 3), np.float32)
+        finally:
+            lib().or_free(C.cast(out, C.c_void_p))
+
     def dual_contour(self, voxel_size: float) -> np.ndarray:
         """DualContouringMesher (dual_contouring.rs:23-83): (n_tris, 3, 3) float32."""
         out = C.POINTER(C.c_float)()
@@ -193,6 +208,14 @@ class Volume:
             L.or_volume_export_tiles(self.h, to.ctypes.data_as(C.POINTER(C.c_int32)),
                                      ts.ctypes.data_as(C.POINTER(C.c_int32)), tv.ctypes.data_as(C.POINTER(C.c_float)))
         return VolumeExport(self.voxel_size, o, m, v, to, ts, tv)
+
+
+def mc_cube_tiling(corner_values) -> list:
+    """One cube's triangles as crossing-edge triples (12 = centre vertex)."""
+    c = np.ascontiguousarray(corner_values, np.float32)
+    out = (C.c_int * 48)()
+    n = lib().or_mc_cube_tiling(c.ctypes.data_as(C.POINTER(C.c_float)), out)
+    return [tuple(out[3 * i:3 * i + 3]) for i in range(n)]
 
 
 def mesh_to_volume(soup, voxel_size: float, band: int, threads: int = 0, timing: bool = False):

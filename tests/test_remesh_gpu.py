@@ -53,9 +53,13 @@ def test_feature_preserving_remesh_matches_oracle(gpu_ctx, oracle_lib, name, h):
 
 def test_remesh_methods_and_empty(gpu_ctx):
     r = bs.VoxelRemesher(gpu_ctx).with_voxel_size(0.1)
-    with pytest.raises(bs.BSError) as e:   # the default, marching cubes, is not provided
-        r.remesh(unit_cube())
-    assert e.value.status == _lib.BS_ERR_UNSUPPORTED
+    default = r.remesh(unit_cube())   # the reference's own test: marching cubes, faces > 0
+    assert default.shape[0] > 0
+    r.with_meshing_method(bs.MeshingMethod.MANIFOLD)
+    assert np.array_equal(r.remesh(unit_cube()).view(np.uint32), default.view(np.uint32))
+    with pytest.raises(bs.BSError) as e:
+        r.with_meshing_method(7).remesh(unit_cube())
+    assert e.value.status == _lib.BS_ERR_INVALID_ARG
     r.with_meshing_method(bs.MeshingMethod.FEATURE_PRESERVING)
     assert r.remesh(np.zeros((0, 3, 3), np.float32)) is None
     import torch
