@@ -413,12 +413,28 @@ __device__ inline bool mc_interior13(const float* c) {
     return true;
 }
 
-// One cube (min corner g, clamped corner values c): the number of kept
-// triangles; with `out`, they are written there, 9 floats each.
-__device__ uint32_t mc_cube(const McArgs& a, const float* c, int gx, int gy, int gz, float* out) {
-    int id = 0;
-    for (int i = 0; i < 8; ++i) id |= (c[i] < 0.0f) << i;
-    if (id == 0 || id == 255) return 0;
+// Position of the crossing on cube edge e (ComputeEdgeIntersections,
+// :1070-1098: t = |v1| / (|v1| + |v2|) from the lower corner).
+__device__ inline void mc_vertex(const float* c, int e, int gx, int gy, int gz, float& x, float& y, float& z) {
+    const int c0 = c_edge[e][0], c1 = c_edge[e][1];
+    const float va = fabsf(c[c0]), vb = fabsf(c[c1]);
+    const float t = va / (va + vb);
+    x = (float)(gx + c_corner[c0][0]);
+    y = (float)(gy + c_corner[c0][1]);
+    z = (float)(gz + c_corner[c0][2]);
+    const int ax = c_axis[e];
+    if (ax == 0)
+        x = x + t;
+    else if (ax == 1)
+        y = y + t;
+    else
+        z = z + t;
+}
+
+// One cube with a surface (min corner g; c: its eight clamped corner values,
+// in LDS, configuration id): the number of kept triangles; with `out`, they
+// are written there, 9 floats each.
+__device__ uint32_t mc_cube(const McArgs& a, const float* c, int id, int gx, int gy, int gz, float* out) {
     const int pc = __popc(id);
     const uint8_t am = a.amb_mask[id];
     int p = 0, j = 0;
@@ -442,44 +458,39 @@ __device__ uint32_t mc_cube(const McArgs& a, const float* c, int gx, int gy, int
     }
     const uint32_t e = a.entry[slot * 2 + tube];
     const uint32_t off = e & 0xFFFFFFu, cnt = (e >> 24) & 15u;
-    if (!cnt) return 0;
-    // ComputeEdgeIntersections (:1070-1098): t = |v1| / (|v1| + |v2|) from the lower corner
-    float px[13], py[13], pz[13];
-    for (int k = 0; k < 12; ++k) {
-        const int c0 = c_edge[k][0], c1 = c_edge[k][1];
-        if ((c[c0] < 0.0f) == (c[c1] < 0.0f)) continue;
-        const float va = fabsf(c[c0]), vb = fabsf(c[c1]);
-        const float t = va / (va + vb);
-        float q[3] = {(float)(gx + c_corner[c0][0]), (float)(gy + c_corner[c0][1]), (float)(gz + c_corner[c0][2])};
-        q[c_axis[k]] = q[c_axis[k]] + t;
-        px[k] = q[0], py[k] = q[1], pz[k] = q[2];
-    }
-    if ((e >> 28) & 1u) {  // compute_c_vertex (:919-940)
-        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    float mx = 0.0f, my = 0.0f, mz = 0.0f;
+    if ((e >> 28) & 1u) {  // compute_c_vertex (:919-940): the mean of the crossings, summed in edge order
         int n = 0;
         for (int k = 0; k < 12; ++k) {
-            const int c0 = c_edge[k][0], c1 = c_edge[k][1];
-            if ((c[c0] < 0.0f) == (c[c1] < 0.0f)) continue;
-            sx = sx + px[k], sy = sy + py[k], sz = sz + pz[k];
+            if ((c[c_edge[k][0]] < 0.0f) == (c[c_edge[k][1]] < 0.0f)) continue;
+            float x, y, z;
+            mc_vertex(c, k, gx, gy, gz, x, y, z);
+            mx = mx + x, my = my + y, mz = mz + z;
             ++n;
         }
-        px[12] = sx / (float)n, py[12] = sy / (float)n, pz[12] = sz / (float)n;
+        mx = mx / (float)n, my = my / (float)n, mz = mz / (float)n;
     }
     const float h = a.voxel_size;
     uint32_t kept = 0;
     for (uint32_t t = 0; t < cnt; ++t) {
         const uint16_t tr = a.tris[off + t];
-        const int e1 = tr & 15, e2 = (tr >> 4) & 15, e3 = (tr >> 8) & 15;
-        const float ax = px[e1] * h, ay = py[e1] * h, az = pz[e1] * h;
-        const float bx = px[e2] * h, by = py[e2] * h, bz = pz[e2] * h;
-        const float cx = px[e3] * h, cy = py[e3] * h, cz = pz[e3] * h;
+        float v[9];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int ed = (tr >> (4 * q)) & 15;
+            float x = mx, y = my, z = mz;
+            if (ed != 12) mc_vertex(c, ed, gx, gy, gz, x, y, z);
+            v[3 * q] = x * h, v[3 * q + 1] = y * h, v[3 * q + 2] = z * h;
+        }
         // Triangle3::is_degenerate (triangle3.rs:272-275)
-        const float ux = bx - ax, uy = by - ay, uz = bz - az, vx = cx - ax, vy = cy - ay, vz = cz - az;
-        const float nx = uy * vz - uz * vy, ny = uz * vx - ux * vz, nz = ux * vy - uy * vx;
+        const float ux = v[3] - v[0], uy = v[4] - v[1], uz = v[5] - v[2];
+        const float wx = v[6] - v[0], wy = v[7] - v[1], wz = v[8] - v[2];
+        const float nx = uy * wz - uz * wy, ny = uz * wx - ux * wz, nz = ux * wy - uy * wx;
         if (nx * nx + ny * ny + nz * nz == 0.0f) continue;
         if (out) {
             float* o = out + 9 * (uint64_t)kept;
-            o[0] = ax, o[1] = ay, o[2] = az, o[3] = bx, o[4] = by, o[5] = bz, o[6] = cx, o[7] = cy, o[8] = cz;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) o[q] = v[q];
         }
         ++kept;
     }
@@ -497,62 +508,98 @@ __device__ inline uint32_t wave_exclusive(uint32_t v, int lane, uint32_t& total)
 }
 
 constexpr int MB = 9;  // staged block edge: leaf-local 0 .. 8
+constexpr int CS = 9;  // per-lane corner slot stride in LDS
 
-// WRITE = false: unit_cnt[u] = kept triangles of unit u.  WRITE = true: the
-// triangles at out + 9 * unit_off[u].
+// offset of cube corner k in the staged 9^3 block
+__device__ inline int mc_corner_off(int k) {
+    return (((k + 1) >> 1) & 1) * (MB * MB) + ((k >> 1) & 1) * MB + ((k >> 2) & 1);
+}
+
+// WRITE = false: unit_cnt[u] = kept triangles of unit u, lane_cnt[64 u + l] =
+// those of lane l of a leaf unit.  WRITE = true: the triangles at
+// out + 9 * unit_off[u].
 template <bool WRITE>
 __global__ __launch_bounds__(64) void k_mc_units(McArgs a, uint32_t* __restrict__ unit_cnt,
-                                                 const uint64_t* __restrict__ unit_off, float* __restrict__ out) {
+                                                 uint16_t* __restrict__ lane_cnt, const uint64_t* __restrict__ unit_off,
+                                                 float* __restrict__ out) {
     __shared__ float val[MB * MB * MB];
     __shared__ uint8_t act[MB * MB * MB];
+    __shared__ float cv[64 * CS];
+    __shared__ int32_t nb_leaf[8];
+    __shared__ float nb_val[8];
+    __shared__ uint8_t nb_kind[8];  // 0 none, 1 leaf, 2 tile
     const int lane = threadIdx.x;
+    float* mine_c = cv + lane * CS;
     for (uint64_t u = blockIdx.x; u < a.n_units; u += gridDim.x) {
         const int64_t unit = a.units[u];
         uint64_t running = WRITE ? unit_off[u] : 0;
         if (unit >= 0) {
             const int4 org = a.origins[unit];
+            if (lane < 8) {  // the leaf and its +x/+y/+z neighbour blocks
+                const int dx = lane >> 2, dy = (lane >> 1) & 1, dz = lane & 1;
+                const int bx = (org.x >> 3) + dx, by = (org.y >> 3) + dy, bz = (org.z >> 3) + dz;
+                int32_t s = lane == 0 ? (int32_t)unit : a.leaves.find(bx, by, bz);
+                uint8_t kind = s >= 0 ? 1 : 0;
+                float tv = 0.0f;
+                if (s < 0 && a.n8) {
+                    const int32_t t = a.t8.find(bx, by, bz);
+                    if (t >= 0) kind = 2, tv = a.t8_val[t];
+                }
+                if (s < 0 && kind == 0 && a.n128) {
+                    const int32_t t = a.t128.find(bx >> 4, by >> 4, bz >> 4);
+                    if (t >= 0) kind = 2, tv = a.t128_val[t];
+                }
+                nb_leaf[lane] = s;
+                nb_kind[lane] = kind;
+                nb_val[lane] = tv;
+            }
+            __syncthreads();
             for (int i = lane; i < MB * MB * MB; i += 64) {
                 const int x = i / (MB * MB), y = (i / MB) % MB, z = i % MB;
+                const int nb = ((x >> 3) << 2) | ((y >> 3) << 1) | (z >> 3);
                 float v = 0.0f;
-                bool on;
-                if (x < 8 && y < 8 && z < 8) {
-                    const int o = (x << 6) | (y << 3) | z;
-                    on = (a.masks[(uint64_t)unit * 8 + (o >> 6)] >> (o & 63)) & 1ull;
-                    if (on) v = a.values[(uint64_t)unit * LEAF_VOX + o];
-                } else {
-                    on = mc_at(a, org.x + x, org.y + y, org.z + z, v);
+                bool on = false;
+                if (nb_kind[nb] == 1) {
+                    const uint64_t s = (uint64_t)nb_leaf[nb];
+                    const int o = ((x & 7) << 6) | ((y & 7) << 3) | (z & 7);
+                    on = (a.masks[s * 8 + (o >> 6)] >> (o & 63)) & 1ull;
+                    if (on) v = a.values[s * LEAF_VOX + o];
+                } else if (nb_kind[nb] == 2) {
+                    on = true;
+                    v = nb_val[nb];
                 }
                 val[i] = mc_clamp(v);
                 act[i] = on;
             }
             __syncthreads();
             const int x = lane >> 3, y = lane & 7;
+            uint32_t base = 0;
+            if (WRITE) {
+                uint32_t total;
+                base = wave_exclusive(lane_cnt[u * 64 + lane], lane, total);
+            }
             uint32_t mine = 0;
-            float c[8];
-            for (int pass = 0; pass < (WRITE ? 2 : 1); ++pass) {
-                uint32_t base = 0;
-                if (pass == 1) {
-                    uint32_t total;
-                    base = wave_exclusive(mine, lane, total);
+            for (int z = 0; z < 8; ++z) {
+                const int b0 = (x * MB + y) * MB + z;
+                bool full = true;
+                int id = 0;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int i = b0 + mc_corner_off(k);
+                    full = full && act[i];
+                    id |= (val[i] < 0.0f) << k;
                 }
-                for (int z = 0; z < 8; ++z) {
-                    bool full = true;
-                    for (int k = 0; k < 8; ++k) {
-                        const int i = ((x + c_corner[k][0]) * MB + (y + c_corner[k][1])) * MB + (z + c_corner[k][2]);
-                        full = full && act[i];
-                        c[k] = val[i];
-                    }
-                    if (!full) continue;
-                    if (pass == 0) {
-                        mine += mc_cube(a, c, org.x + x, org.y + y, org.z + z, nullptr);
-                    } else {
-                        base += mc_cube(a, c, org.x + x, org.y + y, org.z + z, out + 9 * (running + base));
-                    }
-                }
+                if (!full || id == 0 || id == 255) continue;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) mine_c[k] = val[b0 + mc_corner_off(k)];
+                const uint32_t kept = mc_cube(a, mine_c, id, org.x + x, org.y + y, org.z + z,
+                                              WRITE ? out + 9 * (running + base + mine) : nullptr);
+                mine += kept;
             }
             if (!WRITE) {
                 uint32_t total;
                 wave_exclusive(mine, lane, total);
+                lane_cnt[u * 64 + lane] = (uint16_t)mine;
                 if (lane == 0) unit_cnt[u] = total;
             }
             __syncthreads();
@@ -565,12 +612,12 @@ __global__ __launch_bounds__(64) void k_mc_units(McArgs a, uint32_t* __restrict_
             for (uint32_t q0 = 0; q0 < pairs; q0 += 64) {
                 const uint32_t q = q0 + lane;
                 const int i = (int)(q / (uint32_t)s), j = (int)(q % (uint32_t)s);
-                uint32_t mine = 0;
-                uint32_t base = 0;
+                uint32_t mine = 0, base = 0;
                 for (int pass = 0; pass < (WRITE ? 2 : 1); ++pass) {
                     if (pass == 1) {
                         uint32_t total;
                         base = wave_exclusive(mine, lane, total);
+                        mine = 0;
                     }
                     for (int f = 0; f < 6 && q < pairs; ++f) {
                         int lx, ly, lz;
@@ -583,23 +630,21 @@ __global__ __launch_bounds__(64) void k_mc_units(McArgs a, uint32_t* __restrict_
                             default: lx = i, ly = 0, lz = j; break;
                         }
                         if (lx < s - 1 && ly < s - 1 && lz < s - 1) continue;  // one value throughout
-                        float c[8];
                         bool full = true;
+                        int id = 0;
                         for (int k = 0; k < 8 && full; ++k) {
                             const int cx = lx + c_corner[k][0], cy = ly + c_corner[k][1], cz = lz + c_corner[k][2];
-                            if (cx < s && cy < s && cz < s) {
-                                c[k] = tv;
-                            } else {
-                                float v;
+                            float v = tv;
+                            if (!(cx < s && cy < s && cz < s)) {
                                 full = mc_at(a, to.x + cx, to.y + cy, to.z + cz, v);
-                                c[k] = mc_clamp(v);
+                                v = mc_clamp(v);
                             }
+                            mine_c[k] = v;
+                            id |= (v < 0.0f) << k;
                         }
-                        if (!full) continue;
-                        if (pass == 0)
-                            mine += mc_cube(a, c, to.x + lx, to.y + ly, to.z + lz, nullptr);
-                        else
-                            base += mc_cube(a, c, to.x + lx, to.y + ly, to.z + lz, out + 9 * (running + base));
+                        if (!full || id == 0 || id == 255) continue;
+                        mine += mc_cube(a, mine_c, id, to.x + lx, to.y + ly, to.z + lz,
+                                        pass == 1 ? out + 9 * (running + base + mine) : nullptr);
                     }
                 }
                 uint32_t total;
@@ -763,7 +808,8 @@ void marching_cubes_device(bs_ctx* ctx, bs_volume* vol, float voxel_size, float*
     const unsigned grid = (unsigned)std::min<uint64_t>(n_units, 16384);
     uint32_t* cnt = ctx->scratch.as<uint32_t>(S_DC_TRICNT, n_units);
     uint64_t* off = ctx->scratch.as<uint64_t>(S_DC_TRIOFF, n_units + 1);
-    k_mc_units<false><<<grid, 64, 0, st>>>(a, cnt, nullptr, nullptr);
+    uint16_t* lane_cnt = ctx->scratch.as<uint16_t>(S_MC_LANES, n_units * 64);
+    k_mc_units<false><<<grid, 64, 0, st>>>(a, cnt, lane_cnt, nullptr, nullptr);
     BS_HIP(hipGetLastError());
     mc_scan(ctx, cnt, off, n_units);
     uint32_t h_fail = 0, last_cnt = 0;
@@ -777,7 +823,7 @@ void marching_cubes_device(bs_ctx* ctx, bs_volume* vol, float voxel_size, float*
     if (total) {
         float* out = nullptr;
         BS_HIP(hipMalloc(&out, sizeof(float) * 9 * total));
-        k_mc_units<true><<<grid, 64, 0, st>>>(a, nullptr, off, out);
+        k_mc_units<true><<<grid, 64, 0, st>>>(a, nullptr, lane_cnt, off, out);
         hipError_t le = hipGetLastError();
         if (le == hipSuccess) le = hipStreamSynchronize(st);
         if (le != hipSuccess) {
