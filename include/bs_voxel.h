@@ -36,7 +36,7 @@ typedef enum {
     BS_ERR_INTERNAL = 5,         /* reference panics (unreachable!/unwrap)   */
     BS_ERR_NO_DEVICE = 6,        /* no HIP device / library built without GPU */
     BS_ERR_IO = 7,               /* a file could not be read or written       */
-    BS_ERR_UNSUPPORTED = 8       /* a method this library does not provide    */
+    BS_ERR_UNSUPPORTED = 8       /* reserved: a method the library lacks      */
 } bs_status;
 
 typedef struct bs_ctx bs_ctx;

@@ -1,10 +1,9 @@
 """The reference's examples (examples/{dual_contouring,boolean,offset}.rs)
-on the GPU path against the oracle.  dual_contouring.rs runs whole and its
-soup is compared bit for bit; boolean.rs and offset.rs mesh with marching
-cubes, which this library does not provide, so their volumes are compared
-bit for bit instead.  Mesh conversions start from the oracle's bunny volume
-(the GPU's agrees to rounding, checked elsewhere) so that every later step
-is compared exactly."""
+on the GPU path against the oracle, each run whole: volumes and soups are
+compared bit for bit (dual contouring for dual_contouring.rs, marching cubes
+for boolean.rs and offset.rs).  Mesh conversions start from the oracle's
+bunny volume (the GPU's agrees to rounding, checked elsewhere) so that every
+later step is compared exactly."""
 import os
 
 import numpy as np
@@ -38,8 +37,8 @@ def test_dual_contouring_example(gpu_ctx, oracle_lib):
 
 
 def test_boolean_example_volumes(gpu_ctx, oracle_lib, bunny):
-    """examples/boolean.rs up to its marching-cubes meshing: an empty volume
-    united with 17 boxes, then intersect / union / subtract with the bunny."""
+    """examples/boolean.rs: an empty volume united with 17 boxes, then
+    intersect / union / subtract with the bunny, each meshed by marching cubes."""
     B = bs.VolumeBuilder(gpu_ctx, H)
     boxes = bs.Volume.with_voxel_size(H, gpu_ctx)
     rboxes = oracle_lib.from_voxels(np.zeros((0, 3), np.int32), np.zeros(0, np.float32), H)
@@ -53,11 +52,18 @@ def test_boolean_example_volumes(gpu_ctx, oracle_lib, bunny):
         g = getattr(gb.clone(), op)(boxes.clone())
         r = getattr(bunny.clone(), op)(rboxes.clone())
         assert_same_volume(g.export(), r.export())
+        got = bs.MarchingCubesMesher(gpu_ctx).with_voxel_size(H).mesh(g)
+        assert got.shape[0] > 1000
+        _same(got, r.marching_cubes(H))
 
 
 def test_offset_example_volumes(gpu_ctx, oracle_lib, bunny):
-    """examples/offset.rs up to its marching-cubes meshing: the bunny offset
-    by -2.5, -1.5, 1.5 and 3.0."""
+    """examples/offset.rs: the bunny offset by -2.5, -1.5, 1.5 and 3.0, each
+    meshed by marching cubes."""
     gb = _import(gpu_ctx, bunny)
     for d in (-2.5, -1.5, 1.5, 3.0):
-        assert_same_volume(gb.clone().offset(d).export(), bunny.clone().offset(d).export())
+        g, r = gb.clone().offset(d), bunny.clone().offset(d)
+        assert_same_volume(g.export(), r.export())
+        got = bs.MarchingCubesMesher(gpu_ctx).with_voxel_size(H).mesh(g)
+        assert got.shape[0] > 1000
+        _same(got, r.marching_cubes(H))
