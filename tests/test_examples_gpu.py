@@ -1,4 +1,4 @@
-"""The reference's examples (examples/{dual_contouring,boolean,offset}.rs)
+"""The reference's examples (examples/{dual_contouring,boolean,offset,voxel_remeshing}.rs)
 on the GPU path against the oracle, each run whole: volumes and soups are
 compared bit for bit (dual contouring for dual_contouring.rs, marching cubes
 for boolean.rs and offset.rs).  Mesh conversions start from the oracle's
@@ -67,3 +67,18 @@ def test_offset_example_volumes(gpu_ctx, oracle_lib, bunny):
         got = bs.MarchingCubesMesher(gpu_ctx).with_voxel_size(H).mesh(g)
         assert got.shape[0] > 1000
         _same(got, r.marching_cubes(H))
+
+
+def test_voxel_remeshing_example(gpu_ctx, oracle_lib, tmp_path):
+    """examples/voxel_remeshing.rs: box.stl remeshed at voxel 0.01 with the
+    default method (marching cubes), written as STL and read back."""
+    soup = bs.read_stl(os.path.join(GOLDEN, "box.stl"))
+    got = bs.VoxelRemesher(gpu_ctx).with_voxel_size(0.01).remesh(soup)
+    e = bs.MeshToVolume(gpu_ctx).with_voxel_size(0.01).with_narrow_band_width(0).convert(soup).export()
+    li, off = np.nonzero(e.active())
+    idx = e.origins[li] + np.stack([off >> 6, (off >> 3) & 7, off & 7], 1)
+    _same(got, oracle_lib.from_voxels(idx, e.values[li, off], 0.01).marching_cubes(0.01))
+    path = str(tmp_path / "remeshed.stl")
+    bs.write_stl(path, got)
+    back = bs.read_stl(path)
+    assert np.array_equal(back.reshape(-1).view(np.uint32), got.reshape(-1).view(np.uint32))
