@@ -96,6 +96,13 @@ void parallel_for(size_t n, F&& f, size_t grain = 8) {
     pool.run(n, T, fn);
 }
 
+}  // namespace
+
+void host_parallel_for(size_t n, const std::function<void(size_t)>& f) { parallel_for(n, f, 1); }
+size_t host_threads() { return WorkerPool::get().threads(); }
+
+namespace {
+
 constexpr int32_t L1_SPAN = 4096;  // voxels covered by a Node1
 constexpr int32_t L2_SPAN = 128;   // voxels covered by a Node2
 
@@ -536,15 +543,21 @@ void Topology::collect(std::vector<int32_t>& leaf_refs, std::vector<TileRec>& ti
     parallel_for(items.size(), [&](size_t k) {
         if (items[k].node < 0) return;
         const Node2& b = n2[items[k].node];
+        // filled in locals and moved once: pushing into refs[k] directly makes
+        // the threads write the neighbouring vectors' headers, one cache line
+        std::vector<int32_t> r;
+        std::vector<TileRec> t;
         for (int o2 = 0; o2 < Node2::SIZE; ++o2) {
             if (b.kind[o2] == SLOT_BRANCH) {
-                refs[k].push_back(b.child[o2]);
+                r.push_back(b.child[o2]);
             } else if (b.kind[o2] == SLOT_TILE) {
                 int32_t x, y, z;
                 slot_origin(b, o2, 8, x, y, z);
-                tl[k].push_back({x, y, z, 8, b.tile[o2]});
+                t.push_back({x, y, z, 8, b.tile[o2]});
             }
         }
+        refs[k] = std::move(r);
+        tl[k] = std::move(t);
     });
     size_t nr = 0;
     for (const auto& r : refs) nr += r.size();

@@ -10,6 +10,7 @@
 #include <array>
 #include <cstdlib>
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <memory>
 #include <vector>
@@ -65,6 +66,12 @@ struct LeafStore {
         seg.back().push_back(r);
     }
 };
+
+// Runs f(0) .. f(n-1) on the library's persistent host thread pool (the
+// caller works too); returns when every call has finished.  f must not throw.
+void host_parallel_for(size_t n, const std::function<void(size_t)>& f);
+// The number of threads host_parallel_for may use (caller included).
+size_t host_threads();
 
 template <int LOG2>
 struct SlotNode {

@@ -10,6 +10,8 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <array>
+#include <atomic>
 #include <cmath>
 #include <chrono>
 #include <future>
@@ -509,20 +511,36 @@ void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op) {
     // output leaves in lexicographic origin order, as every volume here keeps them:
     // an LSD radix sort of (key, ref) in 16-bit digits, the key being the
     // leaf's rank in the (x, y, z) box of the output (or the 63-bit leaf key
-    // when that box has 2^48 cells or more)
+    // when that box has 2^48 cells or more).  Every per-leaf loop runs in T
+    // contiguous chunks on the host pool; the sort keeps per-chunk digit
+    // counts, so each pass stays stable.
+    const size_t n_refs = refs.size();
+    const size_t T = std::max<size_t>(1, std::min(host_threads(), (n_refs + 16383) / 16384));
+    const size_t chunk = (n_refs + T - 1) / T;
+    auto chunk_end = [&](size_t c) { return std::min(n_refs, (c + 1) * chunk); };
     {
-        const size_t n = refs.size();
+        const size_t n = n_refs;
         std::vector<uint64_t> key(n), key2(n);
         std::vector<int32_t> tmp(n);
-        int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
-        for (size_t i = 0; i < n; ++i) {
-            const LeafRef& u = ta.leaves[refs[i]];
-            const int32_t c[3] = {u.x >> 3, u.y >> 3, u.z >> 3};
-            for (int k = 0; k < 3; ++k) {
-                lo[k] = std::min(lo[k], c[k]);
-                hi[k] = std::max(hi[k], c[k]);
+        std::vector<std::array<int32_t, 3>> lc(n);
+        std::vector<std::array<int32_t, 6>> box(T, {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN, INT32_MIN});
+        host_parallel_for(T, [&](size_t c) {
+            auto& b = box[c];
+            for (size_t i = c * chunk; i < chunk_end(c); ++i) {
+                const LeafRef& u = ta.leaves[refs[i]];
+                lc[i] = {u.x >> 3, u.y >> 3, u.z >> 3};
+                for (int k = 0; k < 3; ++k) {
+                    b[k] = std::min(b[k], lc[i][k]);
+                    b[3 + k] = std::max(b[3 + k], lc[i][k]);
+                }
             }
-        }
+        });
+        int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+        for (const auto& b : box)
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::min(lo[k], b[k]);
+                hi[k] = std::max(hi[k], b[3 + k]);
+            }
         uint64_t ext[3] = {1, 1, 1};
         for (int k = 0; k < 3 && n; ++k) ext[k] = (uint64_t)((int64_t)hi[k] - lo[k] + 1);
         const bool dense = n && ext[0] < (1ull << 21) && ext[1] < (1ull << 21) && ext[2] < (1ull << 21) &&
@@ -530,28 +548,38 @@ void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op) {
         const uint64_t cells = dense ? ext[0] * ext[1] * ext[2] : 0;
         int passes = 4;
         if (dense) passes = cells <= (1ull << 16) ? 1 : (cells <= (1ull << 32) ? 2 : 3);
-        for (size_t i = 0; i < n; ++i) {
-            const LeafRef& u = ta.leaves[refs[i]];
-            key[i] = dense ? (((uint64_t)((u.x >> 3) - lo[0]) * ext[1] + (uint64_t)((u.y >> 3) - lo[1])) * ext[2] +
-                              (uint64_t)((u.z >> 3) - lo[2]))
-                           : leaf_key(u.x >> 3, u.y >> 3, u.z >> 3);
-        }
-        std::vector<uint32_t> cnt((size_t)1 << 16);
-        for (int pass = 0; pass < passes; ++pass) {
+        host_parallel_for(T, [&](size_t c) {
+            for (size_t i = c * chunk; i < chunk_end(c); ++i) {
+                const auto& q = lc[i];
+                key[i] = dense ? (((uint64_t)(q[0] - lo[0]) * ext[1] + (uint64_t)(q[1] - lo[1])) * ext[2] +
+                                  (uint64_t)(q[2] - lo[2]))
+                               : leaf_key(q[0], q[1], q[2]);
+            }
+        });
+        constexpr size_t DIG = (size_t)1 << 16;
+        std::vector<uint32_t> cnt(T * DIG);
+        for (int pass = 0; pass < passes && n; ++pass) {
             const int sh = 16 * pass;
-            std::fill(cnt.begin(), cnt.end(), 0u);
-            for (size_t i = 0; i < n; ++i) ++cnt[(key[i] >> sh) & 0xFFFF];
-            uint32_t sum = 0;
-            for (auto& c : cnt) {
-                const uint32_t v = c;
-                c = sum;
-                sum += v;
-            }
-            for (size_t i = 0; i < n; ++i) {
-                const uint32_t d = cnt[(key[i] >> sh) & 0xFFFF]++;
-                key2[d] = key[i];
-                tmp[d] = refs[i];
-            }
+            host_parallel_for(T, [&](size_t c) {
+                uint32_t* h = cnt.data() + c * DIG;
+                std::fill(h, h + DIG, 0u);
+                for (size_t i = c * chunk; i < chunk_end(c); ++i) ++h[(key[i] >> sh) & 0xFFFF];
+            });
+            uint32_t sum = 0;  // digit-major, chunk-minor: stable across chunks
+            for (size_t d = 0; d < DIG; ++d)
+                for (size_t c = 0; c < T; ++c) {
+                    const uint32_t v = cnt[c * DIG + d];
+                    cnt[c * DIG + d] = sum;
+                    sum += v;
+                }
+            host_parallel_for(T, [&](size_t c) {
+                uint32_t* h = cnt.data() + c * DIG;
+                for (size_t i = c * chunk; i < chunk_end(c); ++i) {
+                    const uint32_t d = h[(key[i] >> sh) & 0xFFFF]++;
+                    key2[d] = key[i];
+                    tmp[d] = refs[i];
+                }
+            });
             key.swap(key2);
             refs.swap(tmp);
         }
@@ -559,17 +587,22 @@ void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op) {
     const uint64_t n_out = refs.size();
     std::vector<int32_t> new_index(ta.leaves.size(), -1);
     std::vector<int4> host(2 * n_out);
-    for (uint64_t i = 0; i < n_out; ++i) {
-        const LeafRef& r = ta.leaves[refs[i]];
-        new_index[refs[i]] = (int32_t)i;
-        BS_REQUIRE(r.op == LEAF_COPY_B || (r.a >= 0 && (uint64_t)r.a < a->n_leaves), BS_ERR_INTERNAL,
-                   "CSG plan refers to a missing leaf of the first operand");
-        BS_REQUIRE(r.op == LEAF_COPY_A || (r.b >= 0 && (uint64_t)r.b < b->n_leaves), BS_ERR_INTERNAL,
-                   "CSG plan refers to a missing leaf of the second operand");
-        host[i] = make_int4(std::max(r.a, 0), std::max(r.b, 0), (int)r.op | ((int)r.neg << 8), 0);
-        cs.n_paired += r.op == LEAF_UNION || r.op == LEAF_SUBTRACT || r.op == LEAF_INTERSECT;
-        host[n_out + i] = make_int4(r.x, r.y, r.z, 0);
-    }
+    std::atomic<int> bad{0};
+    std::vector<uint64_t> paired(T, 0);
+    host_parallel_for(T, [&](size_t c) {
+        for (size_t i = c * chunk; i < chunk_end(c); ++i) {
+            const LeafRef& r = ta.leaves[refs[i]];
+            new_index[refs[i]] = (int32_t)i;
+            if (!(r.op == LEAF_COPY_B || (r.a >= 0 && (uint64_t)r.a < a->n_leaves))) bad.fetch_or(1);
+            if (!(r.op == LEAF_COPY_A || (r.b >= 0 && (uint64_t)r.b < b->n_leaves))) bad.fetch_or(2);
+            host[i] = make_int4(std::max(r.a, 0), std::max(r.b, 0), (int)r.op | ((int)r.neg << 8), 0);
+            paired[c] += r.op == LEAF_UNION || r.op == LEAF_SUBTRACT || r.op == LEAF_INTERSECT;
+            host[n_out + i] = make_int4(r.x, r.y, r.z, 0);
+        }
+    });
+    BS_REQUIRE(!(bad.load() & 1), BS_ERR_INTERNAL, "CSG plan refers to a missing leaf of the first operand");
+    BS_REQUIRE(!(bad.load() & 2), BS_ERR_INTERNAL, "CSG plan refers to a missing leaf of the second operand");
+    for (uint64_t p : paired) cs.n_paired += p;
 
     cs.ms_plan = lap();
     bs_volume out;
