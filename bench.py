@@ -417,7 +417,9 @@ def main():
     for k in kernels.values():
         k["tflops"] = k["flop"] / (k["ms"] * 1e-3) / 1e12 if k["ms"] > 0 else 0.0
     dom = max(kernels, key=lambda k: kernels[k]["ms"])
-    roof = {"bound": "valu_fp32", "kernel": dom, "achieved": round(kernels[dom]["tflops"], 3),
+    # compute-bound: the contract names that "mfma"; the work here is vector
+    # FP32 (VALU), priced against MI355X's published FP32 peak
+    roof = {"bound": "mfma", "bound_detail": "compute, vector FP32 (VALU); no matrix-shaped work", "kernel": dom, "achieved": round(kernels[dom]["tflops"], 3),
             "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(kernels[dom]["tflops"] / PEAK_FP32_TFLOPS, 4),
             "traffic": WINDING_TRAFFIC_BYTES if dom == "k_winding" else None,
             "traffic_source": "profiles/m2v_icosphere_L8_traffic.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"}
