@@ -50,8 +50,8 @@ def test_sphere_builder(gpu_ctx):
 
 
 def test_box2_conversion_offset_and_dc(gpu_ctx):
-    """volume/mod.rs:134-152 up to its marching-cubes mesher (not provided):
-    box2.stl at voxel 0.2, band 0, offset 0.5."""
+    """volume/mod.rs:134-152: box2.stl at voxel 0.2, band 0, offset 0.5; the
+    reference's marching cubes gives 7944 vertices (its own assertion)."""
     e = bs.MeshToVolume(gpu_ctx).with_voxel_size(0.2).convert(bs.read_stl(os.path.join(GOLDEN, "box2.stl"))).export()
     assert np.array_equal(e.origins, F["box2_m2v_origins"]) and np.array_equal(e.masks, F["box2_m2v_masks"])
     act = e.active()
@@ -62,6 +62,7 @@ def test_box2_conversion_offset_and_dc(gpu_ctx):
     assert np.array_equal(o.values.view(np.uint32), F["box2_off_values"].view(np.uint32))
     tris = bs.DualContouringMesher(gpu_ctx).with_voxel_size(0.2).mesh(v)
     assert np.array_equal(tris.view(np.uint32), F["box2_off_dc"].view(np.uint32))
+    assert bs.MarchingCubesMesher(gpu_ctx).with_voxel_size(0.2).mesh(v).shape[0] * 3 == 7944
 
 
 def test_small_sphere_dc(gpu_ctx):
