@@ -1040,38 +1040,50 @@ __global__ __launch_bounds__(WN_BLOCK) void k_winding(const uint32_t* __restrict
     }
     int sp = 0;
     bool overflow = false;
+    // the pair to visit next (wave-uniform); the first child pair of a visit
+    // is taken here directly instead of through the stack, which only holds
+    // the second ones: the visiting order is the same depth-first order
+    int c0 = 0, c1 = 0;
+    unsigned long long m = 0;
+    bool have = false;
     {
         const NodeRec root = nodes[0];  // internal node 0, or the only leaf when n == 1
         pin_record(root);
         const unsigned long long nm = wn_visit(root, init, px, py, pz, btris, a);
         if (nm) {
-            stk.push(0, root.L.x, root.L.y, nm, lane);
-            sp = 1;
+            c0 = root.L.x, c1 = root.L.y, m = nm;
+            have = true;
         }
     }
-    // every sibling pair is pushed at most once per wave in a tree: more pops
-    // than nodes means corrupt links, and the walk stops
-    for (int pops = 0; sp > 0; ++pops) {
+    // every sibling pair is visited at most once per wave in a tree: more
+    // visits than nodes means corrupt links, and the walk stops
+    for (int pops = 0;; ++pops) {
+        if (!have) {
+            if (sp == 0) break;
+            --sp;
+            stk.pop(sp, c0, c1, m);
+        }
         if (pops > n_nodes) {
             overflow = true;
             break;
         }
-        --sp;
-        int c0, c1;
-        unsigned long long m;
-        stk.pop(sp, c0, c1, m);
         const NodeRec r0 = nodes[c0];
         const NodeRec r1 = nodes[c1];
         pin_record(r0);
         pin_record(r1);
         const unsigned long long n0 = wn_visit(r0, m, px, py, pz, btris, a);
         const unsigned long long n1 = wn_visit(r1, m, px, py, pz, btris, a);
-        if (sp + 2 > Stack::DEPTH) {
+        if (sp + 1 > Stack::DEPTH) {
             overflow = true;
             break;
         }
-        if (n1) stk.push(sp++, r1.L.x, r1.L.y, n1, lane);
-        if (n0) stk.push(sp++, r0.L.x, r0.L.y, n0, lane);
+        have = (n0 | n1) != 0;
+        if (n0) {
+            if (n1) stk.push(sp++, r1.L.x, r1.L.y, n1, lane);
+            c0 = r0.L.x, c1 = r0.L.y, m = n0;
+        } else if (n1) {
+            c0 = r1.L.x, c1 = r1.L.y, m = n1;
+        }
     }
     if (overflow && lane == 0) atomicOr(err, E_STACK);
     if (COUNT && lane == 0) {
