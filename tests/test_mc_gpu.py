@@ -139,3 +139,32 @@ def test_mc_empty_and_errors(gpu_ctx):
     assert m.mesh(bs.VolumeBuilder(gpu_ctx, 0.05).sphere(-1.0, (0.0, 0.0, 0.0))).shape == (0, 3, 3)
     # a lone pair of voxels: no cube has all eight corners
     assert m.mesh(bs.volume_from_voxels([[0, 0, 0], [1, 0, 0]], [-1.0, 1.0], 0.05, gpu_ctx)).shape == (0, 3, 3)
+
+
+def test_hash_maps_tiles_far_clusters_and_empty(gpu_ctx, oracle_lib):
+    """The leaf map as a hash (forced, with the tile maps beside it, and
+    chosen by itself for two clusters 2^21 voxels apart), and an empty
+    volume."""
+    h = 0.05
+    B = bs.VolumeBuilder(gpu_ctx, h)
+    g = B.sphere(0.3, (0.1, 0.0, 0.0)).union(B.sphere(1.0, (0.0, 0.0, 0.0)))
+    r = oracle_lib.sphere(0.3, (0.1, 0.0, 0.0), h).union(oracle_lib.sphere(1.0, (0.0, 0.0, 0.0), h))
+    ref = r.marching_cubes(h)
+    try:
+        gpu_ctx.set_paths(_lib.BS_PATH_FORCE_HASH)
+        got = _mc(gpu_ctx, g, h)
+        assert gpu_ctx.last_mc_stats()["used_hash"] == 1
+    finally:
+        gpu_ctx.set_paths(0)
+    _same(got, ref)
+    # two clusters far apart: the dense box would exceed 2^28 cells
+    n = 12
+    grid = np.stack(np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"), -1).reshape(-1, 3)
+    d = np.sqrt(((grid - 5.5) ** 2).sum(1)) - 4.0
+    idx = np.concatenate([grid - 1000000, grid + 1000000]).astype(np.int32)
+    vals = np.concatenate([d, d]).astype(np.float32)
+    got = _mc(gpu_ctx, bs.volume_from_voxels(idx, vals, h, gpu_ctx), h)
+    assert gpu_ctx.last_mc_stats()["used_hash"] == 1
+    assert got.shape[0] > 0
+    _same(got, oracle_lib.from_voxels(idx, vals, h).marching_cubes(h))
+    assert _mc(gpu_ctx, bs.Volume.with_voxel_size(h, gpu_ctx), h).shape == (0, 3, 3)
