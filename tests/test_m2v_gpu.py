@@ -60,7 +60,7 @@ def check_case(ctx, oracle_lib, soup, h, band, fp64=False, closed=True):
     return res
 
 
-def test_cfg1_torus_parity(gpu_ctx, oracle_lib):
+def test_torus_128x64_parity(gpu_ctx, oracle_lib):
     """Torus R=1, r=0.35, 128x64 segments, radial noise +-0.01 (seed 1), h=0.02, band 3."""
     soup = mg.torus(1.0, 0.35, 128, 64, 0.01, seed=1)
     res = check_case(gpu_ctx, oracle_lib, soup, 0.02, 3, fp64=True)
@@ -115,7 +115,7 @@ def test_repeatable(gpu_ctx):
     assert np.array_equal(a.values, b.values) and np.array_equal(a.masks, b.masks)
 
 
-def test_cfg2_full_size(gpu_ctx, oracle_lib):
+def test_icosphere_L8_full_size(gpu_ctx, oracle_lib):
     """Displaced icosphere L8 (1,310,720 tris), h=0.002, band 3: active set and
     leaf keys bit-exact against the union of dilated boxes; values at 20000
     sampled voxels against the reference arithmetic and FP64; signs at 4000
@@ -127,7 +127,7 @@ def test_cfg2_full_size(gpu_ctx, oracle_lib):
     g = run_gpu(ctx, soup, h, band)
     stats = ctx.last_stats()
     ctx.set_profiling(False)
-    print("cfg2 stats:", stats, flush=True)
+    print("icosphere L8 stats:", stats, flush=True)
     ge = g.export()
     o, m = oracle_lib.active_set(soup, h, band)
     assert np.array_equal(ge.origins, o), "leaf keys differ"
@@ -140,7 +140,7 @@ def test_cfg2_full_size(gpu_ctx, oracle_lib):
     r32, r64 = oracle_lib.sample_unsigned(soup, h, band, vox)
     err = np.abs(np.abs(v).astype(np.float64) - r32) / h
     err64 = np.abs(np.abs(v).astype(np.float64) - r64) / h
-    print(f"cfg2 sampled: max err {err.max():.3g} vox, rms {np.sqrt(np.mean(err**2)):.3g}, "
+    print(f"icosphere L8 sampled: max err {err.max():.3g} vox, rms {np.sqrt(np.mean(err**2)):.3g}, "
           f"bit-exact {np.mean(np.abs(v) == r32):.4f}; vs FP64 max {err64.max():.3g} rms "
           f"{np.sqrt(np.mean(err64**2)):.3g}", flush=True)
     assert err.max() <= TOL_VOX
@@ -151,7 +151,7 @@ def test_cfg2_full_size(gpu_ctx, oracle_lib):
     big = ~on_surface(ge.values[li[sp], off[sp]], pts, h)
     ref_neg = ~(w_ref < 0.2)
     mism = (ge.values[li[sp], off[sp]] < 0) != ref_neg
-    print(f"cfg2 sign mismatches vs reference-tree winding number: {int(mism[big].sum())} / {int(big.sum())}",
+    print(f"icosphere L8 sign mismatches vs reference-tree winding number: {int(mism[big].sum())} / {int(big.sum())}",
           flush=True)
     assert int(mism[big].sum()) == 0
 
