@@ -1,6 +1,9 @@
 // Device lookup of leaves by leaf coordinate (origin >> 3), shared by the
 // kernels that walk across leaf borders.
 #pragma once
+#include <algorithm>
+#include <climits>
+
 #include "common.hpp"
 
 namespace bs {
@@ -81,6 +84,52 @@ static __global__ void k_leafmap_insert(uint64_t n, const int4* __restrict__ ori
     if (i >= n) return;
     const int4 o = origins[i];
     if (!map.put(o.x >> 3, o.y >> 3, o.z >> 3, (int32_t)i)) atomicOr(fail, 1u);
+}
+
+// Leaf-coordinate box (origin >> 3) of leaves [0, n): box[0..2] the minimum,
+// box[3..5] the maximum; box must hold k_leaf_box_init's start values.
+static __global__ void k_leaf_box_init(int32_t* box) {
+    if (threadIdx.x < 6) box[threadIdx.x] = threadIdx.x < 3 ? INT32_MAX : INT32_MIN;
+}
+static __global__ __launch_bounds__(256) void k_leaf_box(uint64_t n, const int4* __restrict__ origins, int32_t* box) {
+    int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const int4 o = origins[i];
+        const int32_t c[3] = {o.x >> 3, o.y >> 3, o.z >> 3};
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = min(lo[k], c[k]);
+            hi[k] = max(hi[k], c[k]);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = min(lo[k], __shfl_xor(lo[k], off, 64));
+            hi[k] = max(hi[k], __shfl_xor(hi[k], off, 64));
+        }
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 3; ++k) {
+            atomicMin(&box[k], lo[k]);
+            atomicMax(&box[3 + k], hi[k]);
+        }
+}
+
+// The leaf-coordinate box of a volume's n > 0 leaves, computed on the device
+// (six integers come back instead of every origin).
+inline void leaf_box(bs_ctx* ctx, const int32_t* origins, uint64_t n, int32_t lo[3], int32_t hi[3]) {
+    hipStream_t st = ctx->stream;
+    int32_t* d = ctx->scratch.as<int32_t>(S_LEAF_BOX, 6);
+    k_leaf_box_init<<<1, 64, 0, st>>>(d);
+    BS_HIP(hipGetLastError());
+    k_leaf_box<<<(unsigned)std::min<uint64_t>(div_up(n, 256), 1024), 256, 0, st>>>(
+        n, reinterpret_cast<const int4*>(origins), d);
+    BS_HIP(hipGetLastError());
+    int32_t h[6];
+    BS_HIP(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, st));
+    BS_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = h[k];
+        hi[k] = h[3 + k];
+    }
 }
 
 }  // namespace bs

@@ -725,15 +725,8 @@ void marching_cubes_device(bs_ctx* ctx, bs_volume* vol, float voxel_size, float*
     a.leaves.dense = 1;
     a.leaves.nx = a.leaves.ny = a.leaves.nz = 0;
     if (n) {
-        std::vector<int32_t> o4(4 * n);
-        BS_HIP(hipMemcpyAsync(o4.data(), vol->origins, sizeof(int32_t) * o4.size(), hipMemcpyDeviceToHost, st));
-        BS_HIP(hipStreamSynchronize(st));
-        int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
-        for (uint64_t i = 0; i < n; ++i)
-            for (int k = 0; k < 3; ++k) {
-                lo[k] = std::min(lo[k], o4[4 * i + k] >> 3);
-                hi[k] = std::max(hi[k], o4[4 * i + k] >> 3);
-            }
+        int32_t lo[3], hi[3];
+        leaf_box(ctx, vol->origins, n, lo, hi);
         uint64_t cells = 1;
         for (int k = 0; k < 3; ++k) cells *= (uint64_t)(hi[k] - lo[k] + 1);
         LeafMap& m = a.leaves;
