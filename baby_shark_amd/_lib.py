@@ -65,7 +65,8 @@ class OffsetStats(C.Structure):
 class CsgStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("n_a", "n_b", "n_out", "n_paired")] + \
               [(k, C.c_double) for k in ("ms_flood", "ms_assemble", "ms_wall", "ms_topology", "ms_plan",
-                                         "ms_memory", "ms_compact")]
+                                         "ms_memory", "ms_compact", "ms_plan_fill", "ms_plan_tree",
+                                         "ms_plan_order")]
 
 
 class DcStats(C.Structure):

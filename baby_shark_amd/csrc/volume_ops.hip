@@ -409,12 +409,23 @@ void launch_leaf_flood_fill(bs_ctx* ctx, bs_volume* v, int slot, std::vector<uin
     BS_HIP(hipMemcpyAsync(flags_host.data(), d_flags, v->n_leaves, hipMemcpyDeviceToHost, ctx->stream));
 }
 
+// flags[i] of leaf i (the store's segments in order), in parallel chunks
 void apply_flags(Topology& t, const std::vector<uint8_t>& flags) {
-    for (size_t i = 0; i < flags.size(); ++i) {
-        LeafRef& r = t.leaves[i];
-        r.first_neg = flags[i] & 1;
-        r.last_neg = (flags[i] >> 1) & 1;
-        r.empty = (flags[i] >> 2) & 1;
+    size_t base = 0;
+    for (auto& sg : t.leaves.seg) {
+        const size_t n = std::min(sg.size(), flags.size() - std::min(base, flags.size()));
+        constexpr size_t CH = 16384;
+        host_parallel_for((n + CH - 1) / CH, [&, base](size_t c) {
+            const size_t e = std::min(n, (c + 1) * CH);
+            for (size_t i = c * CH; i < e; ++i) {
+                LeafRef& r = sg[i];
+                const uint8_t f = flags[base + i];
+                r.first_neg = f & 1;
+                r.last_neg = (f >> 1) & 1;
+                r.empty = (f >> 2) & 1;
+            }
+        });
+        base += sg.size();
     }
 }
 
@@ -491,6 +502,13 @@ void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op) {
     BS_HIP(hipStreamSynchronize(ctx->stream));
     cs.ms_flood = ev_flood.ms();
     lap();
+    auto t_plan = std::chrono::steady_clock::now();
+    auto split = [&t_plan]() {
+        const auto now = std::chrono::steady_clock::now();
+        const double ms = std::chrono::duration<double, std::milli>(now - t_plan).count();
+        t_plan = now;
+        return ms;
+    };
     {
         auto fill_b = std::async(std::launch::async, [&] {
             apply_flags(tb, fb);
@@ -500,6 +518,7 @@ void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op) {
         ta.flood_fill_upper();
         fill_b.get();
     }
+    cs.ms_plan_fill = split();
 
     std::unique_ptr<Topology> tb_owned = std::move(b->topo);
     ta.csg(std::move(*tb_owned), op);
@@ -508,81 +527,61 @@ void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op) {
     std::vector<int32_t> refs;
     std::vector<TileRec> tiles;
     ta.collect(refs, tiles);
-    // output leaves in lexicographic origin order, as every volume here keeps them:
-    // an LSD radix sort of (key, ref) in 16-bit digits, the key being the
-    // leaf's rank in the (x, y, z) box of the output (or the 63-bit leaf key
-    // when that box has 2^48 cells or more).  Every per-leaf loop runs in T
-    // contiguous chunks on the host pool; the sort keeps per-chunk digit
-    // counts, so each pass stays stable.
+    cs.ms_plan_tree = split();
+    // output leaves in lexicographic origin order, as every volume here keeps
+    // them.  Every result leaf is a leaf of an operand (copied, negated or
+    // combined with the other operand's leaf at the same origin), and each
+    // operand keeps its leaves in that order: the kept leaves of `a` (paired
+    // ones included) and the copied leaves of `b` are two sorted lists, and
+    // one merge orders the result.
     const size_t n_refs = refs.size();
     const size_t T = std::max<size_t>(1, std::min(host_threads(), (n_refs + 16383) / 16384));
     const size_t chunk = (n_refs + T - 1) / T;
     auto chunk_end = [&](size_t c) { return std::min(n_refs, (c + 1) * chunk); };
     {
-        const size_t n = n_refs;
-        std::vector<uint64_t> key(n), key2(n);
-        std::vector<int32_t> tmp(n);
-        std::vector<std::array<int32_t, 3>> lc(n);
-        std::vector<std::array<int32_t, 6>> box(T, {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN, INT32_MIN});
+        std::vector<int32_t> ra(a->n_leaves, -1), rb(b->n_leaves, -1);
+        std::atomic<int> bad_ref{0};
         host_parallel_for(T, [&](size_t c) {
-            auto& b = box[c];
             for (size_t i = c * chunk; i < chunk_end(c); ++i) {
                 const LeafRef& u = ta.leaves[refs[i]];
-                lc[i] = {u.x >> 3, u.y >> 3, u.z >> 3};
-                for (int k = 0; k < 3; ++k) {
-                    b[k] = std::min(b[k], lc[i][k]);
-                    b[3 + k] = std::max(b[3 + k], lc[i][k]);
+                if (u.op == LEAF_COPY_B) {
+                    if (u.b < 0 || (uint64_t)u.b >= b->n_leaves) bad_ref.fetch_or(2);
+                    else rb[u.b] = refs[i];
+                } else {
+                    if (u.a < 0 || (uint64_t)u.a >= a->n_leaves) bad_ref.fetch_or(1);
+                    else ra[u.a] = refs[i];
                 }
             }
         });
-        int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
-        for (const auto& b : box)
-            for (int k = 0; k < 3; ++k) {
-                lo[k] = std::min(lo[k], b[k]);
-                hi[k] = std::max(hi[k], b[3 + k]);
-            }
-        uint64_t ext[3] = {1, 1, 1};
-        for (int k = 0; k < 3 && n; ++k) ext[k] = (uint64_t)((int64_t)hi[k] - lo[k] + 1);
-        const bool dense = n && ext[0] < (1ull << 21) && ext[1] < (1ull << 21) && ext[2] < (1ull << 21) &&
-                           ext[0] * ext[1] * ext[2] < (1ull << 48);
-        const uint64_t cells = dense ? ext[0] * ext[1] * ext[2] : 0;
-        int passes = 4;
-        if (dense) passes = cells <= (1ull << 16) ? 1 : (cells <= (1ull << 32) ? 2 : 3);
-        host_parallel_for(T, [&](size_t c) {
-            for (size_t i = c * chunk; i < chunk_end(c); ++i) {
-                const auto& q = lc[i];
-                key[i] = dense ? (((uint64_t)(q[0] - lo[0]) * ext[1] + (uint64_t)(q[1] - lo[1])) * ext[2] +
-                                  (uint64_t)(q[2] - lo[2]))
-                               : leaf_key(q[0], q[1], q[2]);
-            }
-        });
-        constexpr size_t DIG = (size_t)1 << 16;
-        std::vector<uint32_t> cnt(T * DIG);
-        for (int pass = 0; pass < passes && n; ++pass) {
-            const int sh = 16 * pass;
-            host_parallel_for(T, [&](size_t c) {
-                uint32_t* h = cnt.data() + c * DIG;
-                std::fill(h, h + DIG, 0u);
-                for (size_t i = c * chunk; i < chunk_end(c); ++i) ++h[(key[i] >> sh) & 0xFFFF];
-            });
-            uint32_t sum = 0;  // digit-major, chunk-minor: stable across chunks
-            for (size_t d = 0; d < DIG; ++d)
-                for (size_t c = 0; c < T; ++c) {
-                    const uint32_t v = cnt[c * DIG + d];
-                    cnt[c * DIG + d] = sum;
-                    sum += v;
+        BS_REQUIRE(!(bad_ref.load() & 1), BS_ERR_INTERNAL, "CSG plan refers to a missing leaf of the first operand");
+        BS_REQUIRE(!(bad_ref.load() & 2), BS_ERR_INTERNAL, "CSG plan refers to a missing leaf of the second operand");
+        struct Item {
+            int32_t x, y, z, ref;
+        };
+        auto gather = [&](const std::vector<int32_t>& r) {
+            std::vector<Item> out;
+            out.reserve(r.size());
+            for (int32_t k : r)
+                if (k >= 0) {
+                    const LeafRef& u = ta.leaves[k];
+                    out.push_back({u.x, u.y, u.z, k});
                 }
-            host_parallel_for(T, [&](size_t c) {
-                uint32_t* h = cnt.data() + c * DIG;
-                for (size_t i = c * chunk; i < chunk_end(c); ++i) {
-                    const uint32_t d = h[(key[i] >> sh) & 0xFFFF]++;
-                    key2[d] = key[i];
-                    tmp[d] = refs[i];
-                }
-            });
-            key.swap(key2);
-            refs.swap(tmp);
+            return out;
+        };
+        std::vector<Item> la, lb;
+        {
+            auto fb = std::async(std::launch::async, [&] { lb = gather(rb); });
+            la = gather(ra);
+            fb.get();
         }
+        BS_REQUIRE(la.size() + lb.size() == n_refs, BS_ERR_INTERNAL, "CSG plan lists an operand leaf twice");
+        auto less = [](const Item& p, const Item& q) {
+            return p.x != q.x ? p.x < q.x : (p.y != q.y ? p.y < q.y : p.z < q.z);
+        };
+        size_t ia = 0, ib = 0, o = 0;
+        while (ia < la.size() && ib < lb.size()) refs[o++] = less(lb[ib], la[ia]) ? lb[ib++].ref : la[ia++].ref;
+        while (ia < la.size()) refs[o++] = la[ia++].ref;
+        while (ib < lb.size()) refs[o++] = lb[ib++].ref;
     }
     const uint64_t n_out = refs.size();
     std::vector<int32_t> new_index(ta.leaves.size(), -1);
@@ -604,6 +603,7 @@ void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op) {
     BS_REQUIRE(!(bad.load() & 2), BS_ERR_INTERNAL, "CSG plan refers to a missing leaf of the second operand");
     for (uint64_t p : paired) cs.n_paired += p;
 
+    cs.ms_plan_order = split();
     cs.ms_plan = lap();
     bs_volume out;
     out.device = a->device;

@@ -116,7 +116,8 @@ def pipeline(ctx, level=7, h=0.001, band=3, reps=2):
                        "assemble_gbs": round(asm_b / (st["ms_assemble"] * 1e6), 1) if st["ms_assemble"] > 0 else None,
                        "kernels_gbs": round((flood_b + asm_b) / (dev_ms * 1e6), 1) if dev_ms > 0 else None,
                        "wall_gbs": round((flood_b + asm_b) / (wall * 1e6), 1),
-                       "host_ms": {k: round(st[k], 2) for k in ("ms_topology", "ms_plan", "ms_memory", "ms_compact")}}
+                       "host_ms": {k: round(st[k], 2) for k in ("ms_topology", "ms_plan", "ms_plan_fill", "ms_plan_tree",
+                                                                 "ms_plan_order", "ms_memory", "ms_compact")}}
             r["_" + name] = out
         for name, src, dist in (("offset_S_+0.02", "_subtract", 0.02), ("offset_I_-0.01", "_intersect", -0.01)):
             x = r[src].clone()

@@ -108,6 +108,9 @@ typedef struct {
      * and output plan, device allocation and release, and the result's
      * upper levels */
     double ms_topology, ms_plan, ms_memory, ms_compact;
+    /* ms_plan split: flags and upper-level flood fill, the tree CSG with the
+     * collection of the result's leaves, the output order and plan */
+    double ms_plan_fill, ms_plan_tree, ms_plan_order;
 } bs_csg_stats;
 
 /* Counters of the last bs_dual_contour* call (see below). */
