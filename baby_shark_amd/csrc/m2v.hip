@@ -1323,8 +1323,8 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     const unsigned TB = 256;
 
     // ---- 0. LBVH + dipoles over the original mesh, on the second stream: it
-    // needs only the triangles, so it is built beside the distance pass (its
-    // buffers are taken here); the winding pass waits for it
+    // needs only the triangles, so it is built beside the subdivision, leaf
+    // allocation and distance passes; the winding pass waits for it
     ensure_side_stream(ctx);
     hipStream_t sb = ctx->stream2;
     BS_HIP(hipEventRecord(ctx->ev_fork, st));
@@ -1353,6 +1353,30 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     float* btris = S.as<float>(S_BVH_TRIS, 12 * (n + BVH_LEAF_SIZE));  // padded: leaves read BVH_LEAF_SIZE triangles
     int2* kids = S.as<int2>(S_BVH_KIDS, std::max(n_int, 1));  // binary children of internal nodes
     uint32_t* starts = reinterpret_cast<uint32_t*>(S.get(S_BVH_STARTS, sizeof(uint32_t) * 258));
+    // Morton order, binary tree, leaves and the height grouping run while the
+    // host drives the subdivision and leaf allocation below (mostly scalar
+    // readbacks, so the device is largely idle there); only the per-level
+    // combination, which needs the level starts on the host, waits for the
+    // distance launch
+    k_centroid_bounds<<<nb, 256, 0, sb>>>(d_tris, n, bparts);
+    k_bounds_final<<<1, 1024, 0, sb>>>(bparts, nb, bounds);
+    k_morton<<<div_up(n, TB), TB, 0, sb>>>(d_tris, n, bounds, codes, idx);
+    sort_pairs_on(ctx, sb, S_CUB_B, codes, codes_s, idx, idx_s, n, 63);
+    BS_HIP(hipMemsetAsync(parent, 0xFF, sizeof(int) * n_nodes, sb));
+    BS_HIP(hipMemsetAsync(hword, 0, sizeof(unsigned) * std::max(n_int, 1), sb));
+    if (n_int > 0) k_lbvh_build<<<div_up(n_int, TB), TB, 0, sb>>>(codes_s, nt, nodes, kids, parent);
+    k_lbvh_leaves<<<div_up(n, TB), TB, 0, sb>>>(d_tris, nt, idx_s, prm.wn_accuracy, parent, hword, height, accs, nodes,
+                                                btris, err);
+    uint32_t* hs = ctx->host_pinned;  // level starts [0, 258), then the error word
+    if (n_int > 0) {
+        // internal nodes grouped by height; combined level by level further down
+        k_iota<<<div_up(n_int, TB), TB, 0, sb>>>(inode, (uint32_t)n_int);
+        sort_pairs_on(ctx, sb, S_CUB_B, height, height_s, inode, inode_s, (uint64_t)n_int, 8);
+        BS_HIP(hipMemsetAsync(starts, 0xFF, sizeof(uint32_t) * 258, sb));
+        k_level_starts<<<div_up(n_int, TB), TB, 0, sb>>>(height_s, (uint32_t)n_int, starts);
+        BS_HIP(hipMemcpyAsync(hs, starts, sizeof(uint32_t) * 258, hipMemcpyDeviceToHost, sb));
+        BS_HIP(hipMemcpyAsync(hs + 258, err, sizeof(uint32_t), hipMemcpyDeviceToHost, sb));
+    }
 
     // ---- 1. subdivision
     uint32_t* rows = S.as<uint32_t>(S_ROWS, n + 1);
@@ -1504,26 +1528,7 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     }
     tm.mark();
 
-    // ---- BVH on the second stream, beside the distance pass
-    k_centroid_bounds<<<nb, 256, 0, sb>>>(d_tris, n, bparts);
-    k_bounds_final<<<1, 1024, 0, sb>>>(bparts, nb, bounds);
-    k_morton<<<div_up(n, TB), TB, 0, sb>>>(d_tris, n, bounds, codes, idx);
-    sort_pairs_on(ctx, sb, S_CUB_B, codes, codes_s, idx, idx_s, n, 63);
-    BS_HIP(hipMemsetAsync(parent, 0xFF, sizeof(int) * n_nodes, sb));
-    BS_HIP(hipMemsetAsync(hword, 0, sizeof(unsigned) * std::max(n_int, 1), sb));
-    if (n_int > 0) k_lbvh_build<<<div_up(n_int, TB), TB, 0, sb>>>(codes_s, nt, nodes, kids, parent);
-    k_lbvh_leaves<<<div_up(n, TB), TB, 0, sb>>>(d_tris, nt, idx_s, prm.wn_accuracy, parent, hword, height, accs, nodes,
-                                                btris, err);
-    uint32_t* hs = ctx->host_pinned;  // level starts [0, 258), then the error word
-    if (n_int > 0) {
-        // internal nodes grouped by height; combined level by level further down
-        k_iota<<<div_up(n_int, TB), TB, 0, sb>>>(inode, (uint32_t)n_int);
-        sort_pairs_on(ctx, sb, S_CUB_B, height, height_s, inode, inode_s, (uint64_t)n_int, 8);
-        BS_HIP(hipMemsetAsync(starts, 0xFF, sizeof(uint32_t) * 258, sb));
-        k_level_starts<<<div_up(n_int, TB), TB, 0, sb>>>(height_s, (uint32_t)n_int, starts);
-        BS_HIP(hipMemcpyAsync(hs, starts, sizeof(uint32_t) * 258, hipMemcpyDeviceToHost, sb));
-        BS_HIP(hipMemcpyAsync(hs + 258, err, sizeof(uint32_t), hipMemcpyDeviceToHost, sb));
-    }
+    // ---- BVH levels on the second stream, beside the distance pass
     if (n_int > 0) {
         BS_HIP(hipStreamSynchronize(sb));
         if (hs[258] & E_TREE) throw Error(BS_ERR_INTERNAL, "BVH parent links are corrupt");
