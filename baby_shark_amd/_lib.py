@@ -120,6 +120,7 @@ def load():
                                                   C.POINTER(C.c_int)]),
             "bs_volume_clone": (C.c_int, [vp, pp]),
             "bs_volume_free": (None, [vp]),
+            "bs_release_cached_memory": (None, []),
             "bs_volume_info_get": (C.c_int, [vp, C.POINTER(VolumeInfo)]),
             "bs_volume_export": (C.c_int, [vp, i32p, u64p, f32p]),
             "bs_volume_export_tiles": (C.c_int, [vp, i32p, i32p, f32p]),

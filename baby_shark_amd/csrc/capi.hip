@@ -60,14 +60,7 @@ struct DeviceGuard {
     }
 };
 
-static void free_volume_arrays(bs_volume* v) {
-    if (v->origins) (void)hipFree(v->origins);
-    if (v->masks) (void)hipFree(v->masks);
-    if (v->values) (void)hipFree(v->values);
-    v->origins = nullptr;
-    v->masks = nullptr;
-    v->values = nullptr;
-}
+static void free_volume_arrays(bs_volume* v) { free_leaves(v); }
 
 }  // namespace bs
 
@@ -273,9 +266,7 @@ bs_status bs_volume_clone(const bs_volume* v, bs_volume** out) {
         if (v->topo) c->topo = v->topo->clone();
         try {
             if (v->n_leaves) {
-                BS_HIP(hipMalloc(&c->origins, sizeof(int32_t) * 4 * v->n_leaves));
-                BS_HIP(hipMalloc(&c->masks, sizeof(uint64_t) * 8 * v->n_leaves));
-                BS_HIP(hipMalloc(&c->values, sizeof(float) * 512 * v->n_leaves));
+                alloc_leaves(c, v->n_leaves);
                 BS_HIP(hipMemcpy(c->origins, v->origins, sizeof(int32_t) * 4 * v->n_leaves, hipMemcpyDeviceToDevice));
                 BS_HIP(hipMemcpy(c->masks, v->masks, sizeof(uint64_t) * 8 * v->n_leaves, hipMemcpyDeviceToDevice));
                 BS_HIP(hipMemcpy(c->values, v->values, sizeof(float) * 512 * v->n_leaves, hipMemcpyDeviceToDevice));
@@ -288,6 +279,8 @@ bs_status bs_volume_clone(const bs_volume* v, bs_volume** out) {
         *out = c;
     });
 }
+
+void bs_release_cached_memory(void) { release_cached_leaves(); }
 
 void bs_volume_free(bs_volume* v) {
     if (!v) return;

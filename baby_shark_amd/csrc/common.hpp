@@ -111,6 +111,12 @@ struct bs_volume {
 
 namespace bs {
 
+// Leaf arrays of a volume (origins, masks, values) for n leaves on the current
+// device, drawn from and returned to a per-device pool of released sets.
+void alloc_leaves(bs_volume* v, uint64_t n);
+void free_leaves(bs_volume* v);
+void release_cached_leaves();
+
 // ---------------------------------------------------------------- device helpers
 constexpr int LEAF_DIM = 8;
 constexpr int LEAF_VOX = 512;

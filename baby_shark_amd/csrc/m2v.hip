@@ -1475,14 +1475,10 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     vol->voxel_size = h;
     vol->n_leaves = n_leaves;
     std::unique_ptr<bs_volume, void (*)(bs_volume*)> guard(vol, [](bs_volume* v) {
-        (void)hipFree(v->origins);
-        (void)hipFree(v->masks);
-        (void)hipFree(v->values);
+        free_leaves(v);
         delete v;
     });
-    BS_HIP(hipMalloc(&vol->origins, sizeof(int32_t) * 4 * n_leaves));
-    BS_HIP(hipMalloc(&vol->masks, sizeof(uint64_t) * 8 * n_leaves));
-    BS_HIP(hipMalloc(&vol->values, sizeof(float) * 512 * n_leaves));
+    alloc_leaves(vol, n_leaves);
     if (ix.dense)
         k_leaf_assign_dense<<<div_up(n_leaves, TB), TB, 0, st>>>(leaf_idx, n_leaves, ix, vol->origins);
     else

@@ -15,6 +15,9 @@
 #include <cmath>
 #include <chrono>
 #include <future>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
 #include <cstring>
 #include <numeric>
 #include <tuple>
@@ -37,22 +40,160 @@ __device__ inline int last_at_or_below(uint64_t m, int top) {
 
 }  // namespace
 
+// ---------------------------------------------------------------- leaf arrays
+// A volume's leaf arrays (origins, masks, values: 2128 B per leaf) are taken
+// from a small per-device pool of released sets.  A mesh-to-volume call that
+// is repeated on meshes of similar size then reuses the arrays of the volume
+// freed before it instead of paying hipFree (which unmaps and synchronises)
+// and hipMalloc for a few hundred MB each time.  Release synchronises the
+// device first, as hipFree does, so no kernel still reads a set that is handed
+// out again.  The pool keeps at most POOL_SETS sets and POOL_BYTES bytes per
+// device; bs_release_cached_memory empties it.
+namespace {
+
+constexpr uint64_t LEAF_BYTES = sizeof(int32_t) * 4 + sizeof(uint64_t) * 8 + sizeof(float) * 512;
+constexpr int POOL_SETS = 4;
+constexpr uint64_t POOL_BYTES = 8ull << 30;
+constexpr int POOL_DEVICES = 64;
+
+struct LeafSet {
+    int32_t* origins;
+    uint64_t* masks;
+    float* values;
+    uint64_t cap;  // leaves
+};
+
+struct LeafPool {
+    std::mutex mu;
+    std::unordered_map<const float*, uint64_t> live_cap;  // values pointer -> capacity, sets in use
+    std::vector<LeafSet> idle[POOL_DEVICES];              // released sets, oldest first
+    uint64_t idle_bytes[POOL_DEVICES] = {};
+};
+
+LeafPool& leaf_pool() {
+    static LeafPool* p = new LeafPool();  // never destroyed: device memory may outlive static teardown
+    return *p;
+}
+
+void free_set(const LeafSet& s) {
+    (void)hipFree(s.origins);
+    (void)hipFree(s.masks);
+    (void)hipFree(s.values);
+}
+
+}  // namespace
+
 void alloc_leaves(bs_volume* v, uint64_t n) {
     v->n_leaves = n;
+    v->origins = nullptr;
+    v->masks = nullptr;
+    v->values = nullptr;
     if (!n) return;
-    BS_HIP(hipMalloc(&v->origins, sizeof(int32_t) * 4 * n));
-    BS_HIP(hipMalloc(&v->masks, sizeof(uint64_t) * 8 * n));
-    BS_HIP(hipMalloc(&v->values, sizeof(float) * 512 * n));
+    LeafPool& P = leaf_pool();
+    const int dev = v->device;
+    if (dev >= 0 && dev < POOL_DEVICES) {
+        std::lock_guard<std::mutex> lk(P.mu);
+        auto& idle = P.idle[dev];
+        int best = -1;
+        for (int i = 0; i < (int)idle.size(); ++i)  // smallest set that holds n without wasting more than n
+            if (idle[i].cap >= n && idle[i].cap <= 2 * n && (best < 0 || idle[i].cap < idle[best].cap)) best = i;
+        if (best >= 0) {
+            const LeafSet s = idle[best];
+            idle.erase(idle.begin() + best);
+            P.idle_bytes[dev] -= s.cap * LEAF_BYTES;
+            P.live_cap[s.values] = s.cap;
+            v->origins = s.origins;
+            v->masks = s.masks;
+            v->values = s.values;
+            return;
+        }
+    }
+    auto attempt = [&]() -> hipError_t {
+        hipError_t e = hipMalloc(&v->origins, sizeof(int32_t) * 4 * n);
+        if (e == hipSuccess) e = hipMalloc(&v->masks, sizeof(uint64_t) * 8 * n);
+        if (e == hipSuccess) e = hipMalloc(&v->values, sizeof(float) * 512 * n);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipFree(v->origins);
+            (void)hipFree(v->masks);
+            (void)hipFree(v->values);
+            v->origins = nullptr;
+            v->masks = nullptr;
+            v->values = nullptr;
+        }
+        return e;
+    };
+    hipError_t e = attempt();
+    if (e == hipErrorOutOfMemory) {  // give the pooled sets back and try once more
+        release_cached_leaves();
+        e = attempt();
+    }
+    BS_HIP(e);
+    std::lock_guard<std::mutex> lk(P.mu);
+    P.live_cap[v->values] = n;
 }
 
 void free_leaves(bs_volume* v) {
-    if (v->origins) BS_HIP(hipFree(v->origins));
-    if (v->masks) BS_HIP(hipFree(v->masks));
-    if (v->values) BS_HIP(hipFree(v->values));
+    if (v->values || v->origins || v->masks) {
+        LeafPool& P = leaf_pool();
+        const int dev = v->device;
+        bool pooled = false;
+        std::vector<LeafSet> evict;
+        int cur = -1;
+        if (dev >= 0 && dev < POOL_DEVICES && v->origins && v->masks && v->values && hipGetDevice(&cur) == hipSuccess &&
+            cur == dev && hipDeviceSynchronize() == hipSuccess) {
+            std::lock_guard<std::mutex> lk(P.mu);
+            auto it = P.live_cap.find(v->values);
+            if (it != P.live_cap.end()) {
+                const LeafSet s{v->origins, v->masks, v->values, it->second};
+                P.live_cap.erase(it);
+                if (s.cap * LEAF_BYTES <= POOL_BYTES) {
+                    auto& idle = P.idle[dev];
+                    idle.push_back(s);
+                    P.idle_bytes[dev] += s.cap * LEAF_BYTES;
+                    while ((int)idle.size() > POOL_SETS || P.idle_bytes[dev] > POOL_BYTES) {
+                        evict.push_back(idle.front());
+                        P.idle_bytes[dev] -= idle.front().cap * LEAF_BYTES;
+                        idle.erase(idle.begin());
+                    }
+                    pooled = true;
+                }
+            }
+        }
+        if (!pooled) {
+            {
+                std::lock_guard<std::mutex> lk(P.mu);
+                if (v->values) P.live_cap.erase(v->values);
+            }
+            (void)hipFree(v->origins);
+            (void)hipFree(v->masks);
+            (void)hipFree(v->values);
+        }
+        for (const LeafSet& s : evict) free_set(s);
+    }
     v->origins = nullptr;
     v->masks = nullptr;
     v->values = nullptr;
     v->n_leaves = 0;
+}
+
+void release_cached_leaves() {
+    LeafPool& P = leaf_pool();
+    std::vector<LeafSet> all[POOL_DEVICES];
+    {
+        std::lock_guard<std::mutex> lk(P.mu);
+        for (int d = 0; d < POOL_DEVICES; ++d) {
+            all[d].swap(P.idle[d]);
+            P.idle_bytes[d] = 0;
+        }
+    }
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    for (int d = 0; d < POOL_DEVICES; ++d) {
+        if (all[d].empty() || hipSetDevice(d) != hipSuccess) continue;
+        for (const LeafSet& s : all[d]) free_set(s);
+    }
+    if (prev >= 0) (void)hipSetDevice(prev);
 }
 
 // The host topology of a volume, built from its leaf origins and active tiles

@@ -270,6 +270,13 @@ class VolumeBuilder:
         return Volume(h, self.ctx)
 
 
+def release_cached_memory() -> None:
+    """Frees the leaf arrays that dropped volumes left in the library's
+    per-device pool (at most 4 sets and 8 GiB per device), for callers that
+    need that device memory for other work."""
+    load().bs_release_cached_memory()
+
+
 def volume_from_voxels(idx, values, voxel_size: float, ctx: Optional[Context] = None) -> Volume:
     """A grid built by inserting values[i] at voxel idx[i] in order."""
     ctx = ctx or default_context()

@@ -162,15 +162,23 @@ bs_status bs_mesh_to_volume_device(bs_ctx* ctx, const float* tri_xyz_dev, size_t
  * status[i] (nullable) the per-mesh status (BS_ERR_EMPTY where convert
  * returns None).  Returns BS_OK when every mesh gave BS_OK or BS_ERR_EMPTY,
  * else the first other status.  Results equal per-mesh bs_mesh_to_volume*
- * calls bit for bit.  Config-4 batch of the workload definitions; the
- * reference has no batched entry point (callers loop over convert). */
+ * calls bit for bit.  The reference has no batched entry point (callers
+ * loop over convert). */
 bs_status bs_batch_mesh_to_volume(bs_ctx* ctx, size_t n_meshes, const float* const* tri_xyz, const size_t* n_tris,
                                   const bs_m2v_params* params, int host_input, int n_workers, bs_volume** out,
                                   bs_status* status);
 
-/* Volume::clone (volume/mod.rs:116-123) and drop. */
+/* Volume::clone (volume/mod.rs:116-123) and drop.  A dropped volume's leaf
+ * arrays go to a per-device pool (at most 4 sets and 8 GiB per device) that
+ * later volumes of similar size draw from; bs_volume_free synchronises the
+ * volume's device first, as hipFree does. */
 bs_status bs_volume_clone(const bs_volume* v, bs_volume** out);
 void bs_volume_free(bs_volume* v);
+
+/* Frees every pooled leaf array set on every device.  No counterpart in the
+ * reference (Rust drop returns memory at once): for callers that need the
+ * device memory back for other work. */
+void bs_release_cached_memory(void);
 
 /* Volume::voxel_size and sizes. */
 bs_status bs_volume_info_get(const bs_volume* v, bs_volume_info* out);
