@@ -574,29 +574,72 @@ void Topology::collect(std::vector<int32_t>& leaf_refs, std::vector<TileRec>& ti
 
 // internal_node/tree_node.rs:228-239, root_node/tree_node.rs:109-113
 void Topology::visit_units(std::vector<int64_t>& units, std::vector<TileRec>& tiles) const {
+    // The order: roots in key order, Node1 slots, then Node2 slots.  The
+    // Node1 slots are listed serially (a Node2 index, or a 128^3 tile as
+    // -1 - slot); the Node2 scans, the bulk of the work, count and then fill
+    // their share of the output in parallel.
+    struct Seg {
+        int32_t n1i, o1, n2i;  // n2i < 0: a 128^3 tile at (n1i, o1)
+        uint64_t u0, t0;       // first unit and first tile of this segment
+    };
+    std::vector<Seg> segs;
     for (const auto& kv : root) {
         const Node1& a = n1[kv.second];
         for (int o1 = 0; o1 < Node1::SIZE; ++o1) {
-            if (a.kind[o1] == SLOT_TILE) {
-                int32_t x, y, z;
-                slot_origin(a, o1, L2_SPAN, x, y, z);
-                units.push_back(-1 - (int64_t)tiles.size());
-                tiles.push_back({x, y, z, L2_SPAN, a.tile[o1]});
-            } else if (a.kind[o1] == SLOT_BRANCH) {
-                const Node2& b = n2[a.child[o1]];
-                for (int o2 = 0; o2 < Node2::SIZE; ++o2) {
-                    if (b.kind[o2] == SLOT_BRANCH) {
-                        units.push_back(b.child[o2]);
-                    } else if (b.kind[o2] == SLOT_TILE) {
-                        int32_t x, y, z;
-                        slot_origin(b, o2, 8, x, y, z);
-                        units.push_back(-1 - (int64_t)tiles.size());
-                        tiles.push_back({x, y, z, 8, b.tile[o2]});
-                    }
-                }
-            }
+            if (a.kind[o1] == SLOT_TILE)
+                segs.push_back({kv.second, o1, -1, 0, 0});
+            else if (a.kind[o1] == SLOT_BRANCH)
+                segs.push_back({kv.second, o1, a.child[o1], 0, 0});
         }
     }
+    std::vector<uint32_t> n_units(segs.size()), n_tiles(segs.size());
+    host_parallel_for(segs.size(), [&](size_t i) {
+        if (segs[i].n2i < 0) {
+            n_units[i] = n_tiles[i] = 1;
+            return;
+        }
+        const Node2& b = n2[segs[i].n2i];
+        uint32_t nu = 0, nt = 0;
+        for (int o2 = 0; o2 < Node2::SIZE; ++o2) {
+            nu += b.kind[o2] != SLOT_EMPTY;
+            nt += b.kind[o2] == SLOT_TILE;
+        }
+        n_units[i] = nu;
+        n_tiles[i] = nt;
+    });
+    const uint64_t u_base = units.size(), t_base = tiles.size();
+    uint64_t nu = 0, nt = 0;
+    for (size_t i = 0; i < segs.size(); ++i) {
+        segs[i].u0 = nu;
+        segs[i].t0 = nt;
+        nu += n_units[i];
+        nt += n_tiles[i];
+    }
+    units.resize(u_base + nu);
+    tiles.resize(t_base + nt);
+    host_parallel_for(segs.size(), [&](size_t i) {
+        const Seg& g = segs[i];
+        uint64_t u = u_base + g.u0, t = t_base + g.t0;
+        const Node1& a = n1[g.n1i];
+        if (g.n2i < 0) {
+            int32_t x, y, z;
+            slot_origin(a, g.o1, L2_SPAN, x, y, z);
+            units[u] = -1 - (int64_t)t;
+            tiles[t] = {x, y, z, L2_SPAN, a.tile[g.o1]};
+            return;
+        }
+        const Node2& b = n2[g.n2i];
+        for (int o2 = 0; o2 < Node2::SIZE; ++o2) {
+            if (b.kind[o2] == SLOT_BRANCH) {
+                units[u++] = b.child[o2];
+            } else if (b.kind[o2] == SLOT_TILE) {
+                int32_t x, y, z;
+                slot_origin(b, o2, 8, x, y, z);
+                units[u++] = -1 - (int64_t)t;
+                tiles[t++] = {x, y, z, 8, b.tile[o2]};
+            }
+        }
+    });
 }
 
 int32_t Topology::locate(int32_t x, int32_t y, int32_t z, bool* neg_out) const {
