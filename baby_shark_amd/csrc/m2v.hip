@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <memory>
 #include <type_traits>
 
@@ -1208,6 +1209,39 @@ T read_scalar(bs_ctx* ctx, const T* dptr) {
     return v;
 }
 
+// Several device scalars back in one round trip: asynchronous copies into the
+// context's pinned buffer (words [320, 512); the side stream's BVH readback
+// uses [0, 259)), one synchronisation of the main stream, then a host copy.
+class Readback {
+    bs_ctx* ctx_;
+    uint8_t* base_;
+    struct Item {
+        size_t off, size;
+        void* dst;
+    };
+    Item items_[8];
+    int n_ = 0;
+    size_t used_ = 0;
+
+public:
+    static constexpr size_t BYTES = 4 * (512 - 320);
+    explicit Readback(bs_ctx* c) : ctx_(c), base_(reinterpret_cast<uint8_t*>(c->host_pinned + 320)) {}
+    template <class T>
+    void add(const T* dptr, T* host, size_t count = 1) {
+        const size_t off = (used_ + 7) & ~size_t(7), bytes = sizeof(T) * count;
+        if (n_ == 8 || off + bytes > BYTES) throw Error(BS_ERR_INTERNAL, "readback batch too large");
+        BS_HIP(hipMemcpyAsync(base_ + off, dptr, bytes, hipMemcpyDeviceToHost, ctx_->stream));
+        items_[n_++] = {off, bytes, host};
+        used_ = off + bytes;
+    }
+    void wait() {
+        BS_HIP(hipStreamSynchronize(ctx_->stream));
+        for (int i = 0; i < n_; ++i) std::memcpy(items_[i].dst, base_ + items_[i].off, items_[i].size);
+        n_ = 0;
+        used_ = 0;
+    }
+};
+
 template <class TIn, class TOut>
 void exclusive_scan(bs_ctx* ctx, const TIn* in, TOut* out, uint64_t n) {
     BS_REQUIRE(n < (1ull << 31), BS_ERR_INVALID_ARG, "scan longer than 2^31 elements");
@@ -1390,17 +1424,25 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     exclusive_scan(ctx, subs, sub_off, n + 1);
     const bool shared_signs = prm.sign_mode == BS_SIGN_SHARED;
     uint32_t max_rows = 1;  // largest subdivision count: bounds the sub-triangle rounding
+    uint32_t* d_max = nullptr;
     if (shared_signs) {
-        uint32_t* d_max = reinterpret_cast<uint32_t*>(S.get(S_MISC2, 64));
+        d_max = reinterpret_cast<uint32_t*>(S.get(S_MISC2, 64));
         size_t bytes = 0;
         BS_HIP(hipcub::DeviceReduce::Max(nullptr, bytes, rows, d_max, (int)n, st));
         void* tmp = S.get(S_CUB, bytes);
         BS_HIP(hipcub::DeviceReduce::Max(tmp, bytes, rows, d_max, (int)n, st));
-        max_rows = std::max<uint32_t>(1u, read_scalar(ctx, d_max));
     }
-    uint64_t total_rows = read_scalar(ctx, row_off + n);
-    uint64_t n_sub = read_scalar(ctx, sub_off + n);
-    if (read_scalar(ctx, err) & E_TOO_MANY_SUBS)
+    Readback rb(ctx);
+    uint32_t h_max_rows = 1, h_rows = 0, h_err = 0;
+    uint64_t n_sub = 0;
+    if (d_max) rb.add(d_max, &h_max_rows);
+    rb.add(row_off + n, &h_rows);
+    rb.add(sub_off + n, &n_sub);
+    rb.add(err, &h_err);
+    rb.wait();
+    max_rows = std::max<uint32_t>(1u, h_max_rows);
+    const uint64_t total_rows = h_rows;
+    if (h_err & E_TOO_MANY_SUBS)
         throw Error(BS_ERR_INVALID_ARG, "a triangle is subdivided into more than 2^32 sub-triangles");
     if (n_sub == 0) throw Error(BS_ERR_EMPTY, "no sub-triangles: the reference returns None");
     BS_REQUIRE(n_sub < (1ull << 31) - 1, BS_ERR_INVALID_ARG, "more than 2^31 sub-triangles");
@@ -1419,9 +1461,13 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     k_reduce_partials<<<1, 1024, 0, st>>>(partials, setup_blocks, counters);
     exclusive_scan(ctx, cov, cov_off, n_sub + 1);
     unsigned long long hc[C_COUNT];
-    BS_HIP(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, st));
-    const uint64_t n_cov = read_scalar(ctx, cov_off + n_sub);
-    if (read_scalar(ctx, err) & E_RANGE)
+    uint32_t h_cov = 0;
+    rb.add(counters, hc, C_COUNT);
+    rb.add(cov_off + n_sub, &h_cov);
+    rb.add(err, &h_err);
+    rb.wait();
+    const uint64_t n_cov = h_cov;
+    if (h_err & E_RANGE)
         throw Error(BS_ERR_INVALID_ARG, "mesh extent exceeds 8e6 voxels from the origin");
     BS_REQUIRE(n_cov < (1ull << 31), BS_ERR_INVALID_ARG, "more than 2^31 box/leaf overlaps");
     const uint64_t n_pairs = hc[C_PAIRS];
@@ -1467,8 +1513,12 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
         void* tmp = S.get(S_CUB, bytes);
         BS_HIP(hipcub::DeviceRadixSort::SortKeys(tmp, bytes, keys, hkeys_sorted, (int)nl, 0, 63, st));
     }
-    const uint64_t n_leaves = read_scalar(ctx, d_count);
-    if (read_scalar(ctx, err) & E_HASH) throw Error(BS_ERR_INTERNAL, "leaf hash table overflow");
+    uint32_t h_leaves = 0;
+    rb.add(d_count, &h_leaves);
+    rb.add(err, &h_err);
+    rb.wait();
+    const uint64_t n_leaves = h_leaves;
+    if (h_err & E_HASH) throw Error(BS_ERR_INTERNAL, "leaf hash table overflow");
     // output volume
     auto* vol = new bs_volume();
     vol->device = ctx->device;
@@ -1492,7 +1542,8 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     k_cand_pairs<<<div_up(n_sub, TB), TB, 0, st>>>(recs, n_sub, cov_off, ix, (uint32_t)n_leaves, ckeys, cvals, err);
     sort_pairs(ctx, ckeys, ckeys_s, cvals, cand_list, n_cov, bits_for(n_leaves));
     k_cand_offsets<<<div_up(n_cov, TB), TB, 0, st>>>(ckeys_s, (uint32_t)n_cov, (uint32_t)n_leaves, cand_off);
-    if (read_scalar(ctx, err) & E_HASH) throw Error(BS_ERR_INTERNAL, "leaf lookup failed");
+    // a failed lookup (E_HASH) is reported with the other error bits at the
+    // end: its pairs sort past every list, so the passes below never read them
     tm.mark();
 
     // ---- 3. distance
@@ -1582,15 +1633,18 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
                                                           counters, err);
     }
     tm.mark();
-    BS_HIP(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, st));
-    BS_HIP(hipStreamSynchronize(st));
+    rb.add(counters, hc, C_COUNT);
+    rb.add(err, &h_err);
+    rb.wait();
+    if (h_err & E_HASH) throw Error(BS_ERR_INTERNAL, "leaf lookup failed");
     const uint32_t n_re = (uint32_t)hc[C_RECHECK];
     if (n_re > 0)
         k_winding_fp64<<<div_up(n_re, 64), 64, 0, st>>>(recheck, n_re, vol->origins, h, nodes, n_nodes, btris,
                                                         prm.sign_threshold, vol->values, err);
     if (roots) k_sign_broadcast<<<(unsigned)n_leaves, 512, 0, st>>>(roots, vol->values);
     tm.mark();
-    unsigned e = read_scalar(ctx, err);
+    // the error word again only when kernels ran after the last readback
+    const unsigned e = (n_re > 0 || roots) ? read_scalar(ctx, err) : h_err;
     if (e & E_STACK) throw Error(BS_ERR_INTERNAL, "winding-number traversal stack overflow");
     if (e & E_TREE) throw Error(BS_ERR_INTERNAL, "BVH parent links are corrupt");
     auto wall1 = std::chrono::steady_clock::now();
