@@ -666,7 +666,11 @@ int32_t Topology::locate(int32_t x, int32_t y, int32_t z, bool* neg_out) const {
 std::unique_ptr<Topology> Topology::after_offset(const std::vector<uint32_t>& keep,
                                                 const std::vector<std::array<int32_t, 3>>& out_origins,
                                                 float two_h) const {
-    auto t = std::make_unique<Topology>(*this);
+    // the nodes only: the leaf records are replaced below
+    auto t = std::make_unique<Topology>();
+    t->root = root;
+    t->n1 = n1;
+    t->n2 = n2;
     // remove_if: tiles beyond the band go inactive, emptied leaves go
     for (auto& n : t->n1)
         for (int o = 0; o < Node1::SIZE; ++o)
@@ -698,8 +702,28 @@ std::unique_ptr<Topology> Topology::after_offset(const std::vector<uint32_t>& ke
         }
     };
     prune();
-    t->leaves.resize(out_origins.size());
-    for (size_t i = 0; i < out_origins.size(); ++i) {
+    // the new leaves: nodes first, created serially in leaf order for the
+    // leaves whose 128^3 slot is no branch yet (the numbering of a one-by-one
+    // insertion), then every leaf's 8^3 slot and record in parallel (each leaf
+    // owns one slot)
+    const size_t n_out = out_origins.size();
+    t->leaves.resize(n_out);
+    constexpr size_t CHUNK = 8192;
+    const size_t n_chunks = (n_out + CHUNK - 1) / CHUNK;
+    auto find_n1 = [&](const Topology& tt, int32_t x, int32_t y, int32_t z) -> int32_t {
+        auto it = tt.root.find(Key{x & ~(L1_SPAN - 1), y & ~(L1_SPAN - 1), z & ~(L1_SPAN - 1)});
+        return it == tt.root.end() ? -1 : it->second;
+    };
+    std::vector<uint8_t> need(n_out);
+    host_parallel_for(n_chunks, [&](size_t c) {
+        for (size_t i = c * CHUNK, e = std::min(n_out, i + CHUNK); i < e; ++i) {
+            const int32_t x = out_origins[i][0], y = out_origins[i][1], z = out_origins[i][2];
+            const int32_t ai = find_n1(*t, x, y, z);
+            need[i] = ai < 0 || t->n1[ai].kind[l1_slot(x, y, z)] != SLOT_BRANCH;
+        }
+    });
+    for (size_t i = 0; i < n_out; ++i) {
+        if (!need[i]) continue;
         const int32_t x = out_origins[i][0], y = out_origins[i][1], z = out_origins[i][2];
         const Key k{x & ~(L1_SPAN - 1), y & ~(L1_SPAN - 1), z & ~(L1_SPAN - 1)};
         auto it = t->root.find(k);
@@ -716,16 +740,23 @@ std::unique_ptr<Topology> Topology::after_offset(const std::vector<uint32_t>& ke
             t->n1[ai].kind[o1] = SLOT_BRANCH;
             t->n1[ai].child[o1] = (int32_t)t->n2.size() - 1;
         }
-        Node2& b = t->n2[t->n1[ai].child[o1]];
-        const int o2 = l2_slot(x, y, z);
-        b.kind[o2] = SLOT_BRANCH;
-        b.child[o2] = (int32_t)i;
-        LeafRef& r = t->leaves[i];
-        r.x = x;
-        r.y = y;
-        r.z = z;
-        r.a = (int32_t)i;
     }
+    std::vector<LeafRef>& recs = t->leaves.seg[0];
+    host_parallel_for(n_chunks, [&](size_t c) {
+        for (size_t i = c * CHUNK, e = std::min(n_out, i + CHUNK); i < e; ++i) {
+            const int32_t x = out_origins[i][0], y = out_origins[i][1], z = out_origins[i][2];
+            const Node1& a = t->n1[find_n1(*t, x, y, z)];
+            Node2& b = t->n2[a.child[l1_slot(x, y, z)]];
+            const int o2 = l2_slot(x, y, z);
+            b.kind[o2] = SLOT_BRANCH;
+            b.child[o2] = (int32_t)i;
+            LeafRef& r = recs[i];
+            r.x = x;
+            r.y = y;
+            r.z = z;
+            r.a = (int32_t)i;
+        }
+    });
     prune();
     // drop the node records no slot refers to any more
     std::vector<int32_t> keep_leaf(out_origins.size());
