@@ -174,6 +174,52 @@ void or_winding_visit_counts(const float* tris, size_t n, const float* pts, size
     }
 }
 
+// The reference-tree BVH of a soup (AABBTree::from_mesh + min 3 objects per
+// leaf + top_down::<Area>, aabb_tree.rs:636-644), nodes in push order (root
+// last).  Returns the node count; when the arrays are non-null (sized by a
+// first call with NULLs) fills leaf flags, left/right (object range of a leaf,
+// child nodes of a branch), node boxes (6 floats), the reordered objects
+// (9 floats each) and the per-node dipole centre and radius.
+size_t or_bvh_export(const float* tris, size_t n, uint8_t* leaf, uint64_t* left, uint64_t* right, float* bbox,
+                     float* objects, float* centre, float* radius) {
+    auto mesh = to_tris(tris, n);
+    WindingNumbers w = WindingNumbers::from_mesh(mesh);
+    if (!leaf) return w.nodes.size();
+    for (size_t i = 0; i < w.nodes.size(); ++i) {
+        const BvhNode& nd = w.nodes[i];
+        leaf[i] = nd.leaf ? 1 : 0;
+        left[i] = nd.left;
+        right[i] = nd.right;
+        const float b[6] = {nd.bbox.min.x, nd.bbox.min.y, nd.bbox.min.z, nd.bbox.max.x, nd.bbox.max.y, nd.bbox.max.z};
+        std::copy(b, b + 6, bbox + 6 * i);
+        if (centre) {
+            centre[3 * i] = w.data[i].center.x;
+            centre[3 * i + 1] = w.data[i].center.y;
+            centre[3 * i + 2] = w.data[i].center.z;
+        }
+        if (radius) radius[i] = w.data[i].radius;
+    }
+    for (size_t t = 0; t < w.objects.size(); ++t) {
+        const Tri& o = w.objects[t];
+        const float v[9] = {o.a.x, o.a.y, o.a.z, o.b.x, o.b.y, o.b.z, o.c.x, o.c.y, o.c.z};
+        std::copy(v, v + 9, objects + 9 * t);
+    }
+    return w.nodes.size();
+}
+
+// find_feature_point (dual_contouring.rs:429-458) of n intersection points
+// (points n*3, unit normals n*3, voxel units).
+void or_dc_feature_point(const float* points, const float* normals, size_t n, float* out) {
+    std::vector<IntPoint> pts(n);
+    for (size_t i = 0; i < n; ++i)
+        pts[i] = {{points[3 * i], points[3 * i + 1], points[3 * i + 2]},
+                  {normals[3 * i], normals[3 * i + 1], normals[3 * i + 2]}};
+    const Vec3f c = DualContouring::feature_point(pts);
+    out[0] = c.x;
+    out[1] = c.y;
+    out[2] = c.z;
+}
+
 float or_solid_angle(const float* tri, const float* q) {
     return solid_angle({{tri[0], tri[1], tri[2]}, {tri[3], tri[4], tri[5]}, {tri[6], tri[7], tri[8]}}, {q[0], q[1], q[2]});
 }

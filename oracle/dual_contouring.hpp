@@ -117,6 +117,7 @@ private:
                 }
     }
 
+public:
     // dual_contouring.rs:433-458
     static Vec3f feature_point(const std::vector<IntPoint>& pts) {
         const float cnt = (float)pts.size();
@@ -138,6 +139,7 @@ private:
         return c;
     }
 
+private:
     // dual_contouring.rs:176-247
     void cells(const LeafNode& l) {
         std::vector<IntPoint> pts;

@@ -48,6 +48,9 @@ def lib():
         L.or_pair_count.argtypes = [f32p, C.c_size_t, C.c_float, C.c_int64]
         L.or_winding_numbers.argtypes = [f32p, C.c_size_t, f32p, C.c_size_t, C.c_float, f32p, f64p, f32p, C.c_int]
         L.or_solid_angle.restype = C.c_float
+        L.or_bvh_export.restype = C.c_size_t
+        L.or_bvh_export.argtypes = [f32p, C.c_size_t, C.POINTER(C.c_uint8), u64p, u64p, f32p, f32p, f32p, f32p]
+        L.or_dc_feature_point.argtypes = [f32p, f32p, C.c_size_t, f32p]
         L.or_solid_angle.argtypes = [f32p, f32p]
         L.or_closest_point.argtypes = [f32p, f32p, f32p]
         L.or_unsigned_distance_f64.argtypes = [vp, f32p, C.c_size_t, C.c_int64, f64p, C.c_int]
@@ -270,6 +273,47 @@ def closest_point(tri, q) -> np.ndarray:
     qq, qp = _f32(q)
     out = np.zeros(3, np.float32)
     lib().or_closest_point(tp, qp, out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
+
+
+@dataclass
+class Bvh:
+    """The reference-tree BVH (aabb_tree.rs:636-644): nodes in push order, root last."""
+    leaf: np.ndarray     # (N,) bool
+    left: np.ndarray     # (N,) object start (leaf) or left child (branch)
+    right: np.ndarray    # (N,) object end (leaf) or right child (branch)
+    bbox: np.ndarray     # (N,6) min xyz, max xyz
+    objects: np.ndarray  # (T,3,3) triangles in tree order
+    centre: np.ndarray   # (N,3) dipole centre
+    radius: np.ndarray   # (N,) dipole radius
+
+
+def bvh_export(soup) -> Bvh:
+    a, p = _f32(soup)
+    n = a.size // 9
+    L = lib()
+    nn = L.or_bvh_export(p, n, None, None, None, None, None, None, None)
+    leaf = np.zeros(nn, np.uint8)
+    left = np.zeros(nn, np.uint64)
+    right = np.zeros(nn, np.uint64)
+    bbox = np.zeros((nn, 6), np.float32)
+    objs = np.zeros((n, 3, 3), np.float32)
+    cen = np.zeros((nn, 3), np.float32)
+    rad = np.zeros(nn, np.float32)
+    fp = lambda x: x.ctypes.data_as(C.POINTER(C.c_float))  # noqa: E731
+    up = lambda x: x.ctypes.data_as(C.POINTER(C.c_uint64))  # noqa: E731
+    L.or_bvh_export(p, n, leaf.ctypes.data_as(C.POINTER(C.c_uint8)), up(left), up(right), fp(bbox), fp(objs),
+                    fp(cen), fp(rad))
+    return Bvh(leaf.astype(bool), left.astype(np.int64), right.astype(np.int64), bbox, objs, cen, rad)
+
+
+def dc_feature_point(points, normals) -> np.ndarray:
+    """find_feature_point (dual_contouring.rs:429-458) in voxel units."""
+    pa, pp = _f32(points)
+    na, npp = _f32(normals)
+    assert pa.shape == na.shape and pa.size % 3 == 0
+    out = np.zeros(3, np.float32)
+    lib().or_dc_feature_point(pp, npp, pa.size // 3, out.ctypes.data_as(C.POINTER(C.c_float)))
     return out
 
 

@@ -11,6 +11,7 @@ crossing edges it uses, its Euler characteristic, whether it holds the
 centre vertex and its triangle count, plus the directed boundary edges of
 the whole list (its orientation)."""
 import os
+import ast
 import re
 import sys
 
@@ -36,7 +37,7 @@ def load_tables():
     out = {}
     for m in re.finditer(r"pub const (\w+)\s*:\s*[^=]+=\s*(\[.*?\]);", s, flags=re.S):
         if "convert" not in m.group(2):
-            out[m.group(1)] = eval(m.group(2), {})
+            out[m.group(1)] = ast.literal_eval(m.group(2))
     return out
 
 

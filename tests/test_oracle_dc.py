@@ -112,3 +112,91 @@ def test_densify_tiles_invariants(oracle_lib):
                 assert np.all(after.masks[i] == np.uint64(2**64 - 1))
                 assert np.all(after.values[i] == np.float32(covered[o]))
         assert np.array_equal(u.densify_tiles().signs_at(probe), s0)
+
+
+# ---- find_feature_point (dual_contouring.rs:429-458) -------------------------
+
+F32 = np.float32
+
+
+def _feature_point_f32(points, normals):
+    """The reference's 30 lines in numpy float32, one rounding per operation:
+    c = mean; 50 steps of F = sum(-n (n . (c - p))), c += F (1 - i/50) / k,
+    stopping after the update once |F|^2 < 1e-6."""
+    P, N = points.astype(F32), normals.astype(F32)
+    k = F32(len(P))
+    c = np.zeros(3, F32)
+    for p in P:
+        c = (c + p).astype(F32)
+    c = (c / k).astype(F32)
+    for i in range(50):
+        f = np.zeros(3, F32)
+        for p, n in zip(P, N):
+            r = (c - p).astype(F32)
+            d = F32(F32(F32(n[0] * r[0]) + F32(n[1] * r[1])) + F32(n[2] * r[2]))
+            f = (f + ((n * F32(-1.0)).astype(F32) * d).astype(F32)).astype(F32)
+        damping = F32(F32(1.0) - F32(F32(i) / F32(50.0)))
+        c = (c + ((f * damping).astype(F32) / k).astype(F32)).astype(F32)
+        if F32(F32(F32(f[0] * f[0]) + F32(f[1] * f[1])) + F32(f[2] * f[2])) < F32(1e-6):
+            break
+    return c
+
+
+def _plane_points(rng, axis, offset, count):
+    p = rng.uniform(0.0, 1.0, (count, 3))
+    p[:, axis] = offset
+    n = np.zeros((count, 3))
+    n[:, axis] = 1.0
+    return p.astype(F32), n.astype(F32)
+
+
+def test_feature_point_converges_to_a_corner(oracle_lib):
+    # points on the planes x = 0.3, y = 0.4, z = 0.6 with their axis normals:
+    # the particle settles on the corner (0.3, 0.4, 0.6), far from the mean
+    rng = np.random.default_rng(3)
+    for counts in [(1, 1, 1), (2, 3, 4), (4, 4, 4)]:
+        parts = [_plane_points(rng, a, o, c) for a, o, c in zip(range(3), (0.3, 0.4, 0.6), counts)]
+        P = np.concatenate([p for p, _ in parts])
+        N = np.concatenate([n for _, n in parts])
+        c = oracle_lib.dc_feature_point(P, N)
+        corner = np.array([0.3, 0.4, 0.6])
+        assert np.linalg.norm(c - corner) < 2e-3, (counts, c)
+        assert np.linalg.norm(P.mean(axis=0) - corner) > 0.1
+
+
+def test_feature_point_moves_only_across_an_edge(oracle_lib):
+    # two planes x = 0.7 and y = 0.2: the particle reaches the edge line and its
+    # z stays the mean's (no force along the edge)
+    rng = np.random.default_rng(4)
+    px, nx = _plane_points(rng, 0, 0.7, 3)
+    py, ny = _plane_points(rng, 1, 0.2, 3)
+    P, N = np.concatenate([px, py]), np.concatenate([nx, ny])
+    c = oracle_lib.dc_feature_point(P, N)
+    assert abs(c[0] - 0.7) < 2e-3 and abs(c[1] - 0.2) < 2e-3
+    assert c[2] == _feature_point_f32(P, N)[2]
+    assert abs(c[2] - P[:, 2].mean()) < 1e-6
+    assert np.hypot(*(P.mean(axis=0)[:2] - [0.7, 0.2])) > 0.1
+
+
+def test_feature_point_on_one_plane_is_the_mean(oracle_lib):
+    # every point on z = 0.5 with normal +z: the force is zero at the mean and
+    # the loop stops after its first step
+    rng = np.random.default_rng(5)
+    P, N = _plane_points(rng, 2, 0.5, 5)
+    c = oracle_lib.dc_feature_point(P, N)
+    m = np.zeros(3, F32)
+    for p in P:
+        m = (m + p).astype(F32)
+    assert np.array_equal(c, (m / F32(5)).astype(F32))
+
+
+def test_feature_point_bits_match_a_float32_restatement(oracle_lib):
+    # random sets (1..12 points, random unit normals): some stop early, most run
+    # all fifty damped steps; the oracle must agree bit for bit
+    rng = np.random.default_rng(6)
+    for _ in range(300):
+        n = int(rng.integers(1, 13))
+        P = rng.uniform(0.0, 1.0, (n, 3)).astype(F32)
+        N = rng.normal(size=(n, 3))
+        N = (N / np.linalg.norm(N, axis=1, keepdims=True)).astype(F32)
+        assert np.array_equal(oracle_lib.dc_feature_point(P, N), _feature_point_f32(P, N)), (P, N)
