@@ -16,6 +16,7 @@ static thread_local std::string g_last_error;
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
 void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const bs_m2v_params& prm, bs_volume** out);
+void init_context_streams(bs_ctx* ctx);
 void volume_sphere(bs_ctx* ctx, float radius, float cx, float cy, float cz, float h, bs_volume** out);
 void volume_cuboid(bs_ctx* ctx, const float mn[3], const float mx[3], float h, bs_volume** out);
 void volume_iwp(bs_ctx* ctx, const float mn[3], const float mx[3], float cell_size, float h, bs_volume** out);
@@ -205,7 +206,11 @@ bs_status bs_batch_mesh_to_volume(bs_ctx* ctx, size_t n_meshes, const float* con
             auto w = std::make_unique<bs_ctx>();
             w->device = ctx->device;
             BS_HIP(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+            bs_ctx* wp = w.get();
             ctx->workers.push_back(std::move(w));
+            // every stream, event and pinned buffer a worker uses exists before
+            // the worker threads start: none is created while others launch
+            init_context_streams(wp);
         }
     });
     if (setup != BS_OK) return setup;

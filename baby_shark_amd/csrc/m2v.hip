@@ -1679,4 +1679,6 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     *out = guard.release();
 }
 
+void init_context_streams(bs_ctx* ctx) { ensure_side_stream(ctx); }
+
 }  // namespace bs
