@@ -112,9 +112,11 @@ struct bs_volume {
 namespace bs {
 
 // Leaf arrays of a volume (origins, masks, values) for n leaves on the current
-// device, drawn from and returned to a per-device pool of released sets.
+// device, drawn from and returned to a per-device pool of released sets.  A
+// release waits for `pending` (a stream that may still run kernels on the
+// arrays, after a call failed part-way) and for nothing else.
 void alloc_leaves(bs_volume* v, uint64_t n);
-void free_leaves(bs_volume* v);
+void free_leaves(bs_volume* v, hipStream_t pending = nullptr);
 void release_cached_leaves();
 
 // ---------------------------------------------------------------- device helpers

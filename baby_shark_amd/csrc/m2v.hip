@@ -1525,10 +1525,16 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     vol->device = ctx->device;
     vol->voxel_size = h;
     vol->n_leaves = n_leaves;
-    std::unique_ptr<bs_volume, void (*)(bs_volume*)> guard(vol, [](bs_volume* v) {
-        free_leaves(v);
-        delete v;
-    });
+    // on failure both streams may still run kernels on the volume's arrays
+    struct Release {
+        hipStream_t a, b;
+        void operator()(bs_volume* v) const {
+            if (b) (void)hipStreamSynchronize(b);
+            free_leaves(v, a);
+            delete v;
+        }
+    };
+    std::unique_ptr<bs_volume, Release> guard(vol, Release{st, ctx->stream2});
     alloc_leaves(vol, n_leaves);
     if (ix.dense)
         k_leaf_assign_dense<<<div_up(n_leaves, TB), TB, 0, st>>>(leaf_idx, n_leaves, ix, vol->origins);

@@ -437,8 +437,6 @@ struct OffsetBuffers {
 
 }  // namespace
 
-void alloc_leaves(bs_volume* v, uint64_t n);
-void free_leaves(bs_volume* v);
 Topology& ensure_topology(bs_ctx* ctx, bs_volume* v);
 
 namespace {
@@ -696,7 +694,7 @@ void volume_offset(bs_ctx* ctx, bs_volume* vol, float distance, bs_offset_stats*
         try {
             r = offset_attempt(ctx, vol, distance, capacity, (ctx->paths & BS_PATH_FORCE_HASH) != 0, out, stats);
         } catch (...) {
-            free_leaves(&out);
+            free_leaves(&out, ctx->stream);
             throw;
         }
         if (!r.retry) break;

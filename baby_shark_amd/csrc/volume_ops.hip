@@ -133,15 +133,19 @@ void alloc_leaves(bs_volume* v, uint64_t n) {
     P.live_cap[v->values] = n;
 }
 
-void free_leaves(bs_volume* v) {
+void free_leaves(bs_volume* v, hipStream_t pending) {
     if (v->values || v->origins || v->masks) {
         LeafPool& P = leaf_pool();
         const int dev = v->device;
         bool pooled = false;
         std::vector<LeafSet> evict;
         int cur = -1;
-        if (dev >= 0 && dev < POOL_DEVICES && v->origins && v->masks && v->values && hipGetDevice(&cur) == hipSuccess &&
-            cur == dev && hipDeviceSynchronize() == hipSuccess) {
+        // Every library call returns with its stream's work done, so only a
+        // call that fails part-way (`pending`: its stream) can leave kernels
+        // that still touch the arrays; that stream alone is waited for.
+        const bool idle = !pending || hipStreamSynchronize(pending) == hipSuccess;
+        if (idle && dev >= 0 && dev < POOL_DEVICES && v->origins && v->masks && v->values &&
+            hipGetDevice(&cur) == hipSuccess && cur == dev) {
             std::lock_guard<std::mutex> lk(P.mu);
             auto it = P.live_cap.find(v->values);
             if (it != P.live_cap.end()) {
@@ -764,7 +768,7 @@ void volume_csg(bs_ctx* ctx, bs_volume* a, bs_volume* b, CsgOp op) {
         BS_HIP(hipStreamSynchronize(ctx->stream));
         cs.ms_assemble = ev_asm.ms();
     } catch (...) {
-        free_leaves(&out);
+        free_leaves(&out, ctx->stream);
         throw;
     }
     lap();
@@ -835,7 +839,7 @@ void volume_from_voxels(bs_ctx* ctx, const int32_t* idx, const float* values, ui
         }
         BS_HIP(hipStreamSynchronize(ctx->stream));
     } catch (...) {
-        free_leaves(vol);
+        free_leaves(vol, ctx->stream);
         delete vol;
         throw;
     }
@@ -916,7 +920,7 @@ void volume_densify_tiles(bs_ctx* ctx, bs_volume* v) {
         BS_HIP(hipGetLastError());
         BS_HIP(hipStreamSynchronize(ctx->stream));
     } catch (...) {
-        free_leaves(&out);
+        free_leaves(&out, ctx->stream);
         throw;
     }
     free_leaves(v);
@@ -990,7 +994,7 @@ static bs_volume* build_from_fn(bs_ctx* ctx, const FnGrid& g, const F& f) {
             BS_HIP(hipStreamSynchronize(ctx->stream));
         }
     } catch (...) {
-        free_leaves(v);
+        free_leaves(v, ctx->stream);
         delete v;
         throw;
     }

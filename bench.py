@@ -9,7 +9,8 @@ device-resident volume; value = active voxels / second.
 Prints one JSON line.  `roofline` is for the dominant kernel (FP32 VALU work
 from the kernels' own counters / its HIP-event time on the library stream);
 `cpu_baseline` times the CPU oracle (the reference algorithm restated in C++,
-same parallel decomposition) on a bounded sample of the same mesh.
+same parallel decomposition) on the whole mesh, and `error` checks the GPU's
+volume against that oracle volume (and against FP64 distances on a sample).
 """
 import argparse
 import json
@@ -60,22 +61,64 @@ def winding_flop(stats):
         77.0 * stats["wn_leaf_tris"]
 
 
-def cpu_baseline(soup, h, band, max_threads=16):
-    """The oracle on a polar cap of the same mesh (triangles with centroid z > 0.4,
-    about 30% of the surface: 10-30 s of CPU work)."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_and_error(soup, h, band, gpu_export, max_threads=16, n_sample=20000):
+    """The CPU oracle (the reference's algorithm and parallel decomposition,
+    restated in C++) converts the whole mesh on the host cores -- the CPU
+    baseline -- and its volume is the checker of the GPU's: leaf keys and
+    masks, every value (max and RMS error in voxels), every sign off the
+    surface; distances at a fixed sample of voxels against FP64."""
     import numpy as np
     from oracle import oracle as O
     if not os.path.exists(O.LIB_PATH):
         O.build()
-    cent = soup.mean(axis=1)
-    cap = np.ascontiguousarray(soup[cent[:, 2] > 0.4])
-    threads = max(1, min(max_threads, os.cpu_count() or 1))
-    vol, sec = O.mesh_to_volume(cap, h, band, threads=threads, timing=True)
-    n_act = vol.export().n_active
-    return {"value": n_act / sec, "unit": "voxels/s", "cores": threads, "kind": "port",
-            "sample": f"polar cap z>0.4 of the same mesh: {cap.shape[0]} triangles, {n_act} active voxels, "
-                      f"{sec:.2f} s",
-            "seconds": round(sec, 3)}
+    nproc = os.cpu_count() or 1
+    threads = max(1, min(max_threads, nproc))
+    vol, sec = O.mesh_to_volume(soup, h, band, threads=threads, timing=True)
+    ref = vol.export()
+    vol = None
+    n_act = ref.n_active
+    base = {"value": n_act / sec, "unit": "voxels/s", "cores": threads, "kind": "port",
+            "sample": f"the whole mesh: {soup.shape[0]} triangles, {n_act} active voxels, {sec:.2f} s, one run",
+            "seconds": round(sec, 3), "nproc": nproc, "cpu_model": _cpu_model()}
+    g = gpu_export
+    err = {"checked_against": "oracle whole volume (reference algorithm, f32); FP64 distances on a fixed sample"}
+    same_keys = g.origins.shape == ref.origins.shape and np.array_equal(g.origins, ref.origins)
+    same_masks = same_keys and np.array_equal(g.masks, ref.masks)
+    err["keys_equal"], err["masks_equal"] = bool(same_keys), bool(same_masks)
+    if same_masks:
+        act = ref.active()
+        gv = g.values[act].astype(np.float64)
+        rv = ref.values[act].astype(np.float64)
+        e = np.abs(gv - rv) / h
+        err["max_vox"] = float(e.max())
+        err["rms_vox"] = float(np.sqrt(np.mean(e ** 2)))
+        err["bit_exact_frac"] = float(np.mean(g.values[act] == ref.values[act]))
+        li, off = np.nonzero(act)
+        vox = ref.origins[li] + np.stack([off >> 6, (off >> 3) & 7, off & 7], 1)
+        pts = vox.astype(np.float32) * np.float32(h)
+        ulp = np.spacing(np.max(np.abs(pts), axis=1).astype(np.float32)).astype(np.float64)
+        off_surface = np.abs(rv) > np.maximum(1e-6 * h, 16.0 * ulp)
+        err["sign_mismatches"] = int(np.sum(np.signbit(gv[off_surface]) != np.signbit(rv[off_surface])))
+        err["signs_checked"] = int(off_surface.sum())
+        rng = np.random.default_rng(12345)
+        pick = rng.choice(li.size, min(n_sample, li.size), replace=False)
+        _, r64 = O.sample_unsigned(soup, h, band, vox[pick])
+        e64 = np.abs(np.abs(gv[pick]) - r64) / h
+        err["max_vs_fp64_vox"] = float(e64.max())
+        err["rms_vs_fp64_vox"] = float(np.sqrt(np.mean(e64 ** 2)))
+        err["fp64_sample"] = int(pick.size)
+    return base, err
 
 
 def pipeline(ctx, level=7, h=0.001, band=3, reps=2):
@@ -317,8 +360,8 @@ def batch(ctx, count=256, seed=5):
     """The uneven batch: 256 closed meshes (tori and displaced icospheres)
     with triangle counts log-uniform in [1e3, 2e5], each voxelized so its
     bounding box spans 256 voxels, band 3, soups resident in HBM.  Timed as
-    one bs_batch_mesh_to_volume call over 4 streams, and as the same call on
-    1 stream (the per-mesh loop a caller would write)."""
+    one bs_batch_mesh_to_volume call over 8 and 4 streams, and as the same
+    call on 1 stream (the per-mesh loop a caller would write)."""
     import torch
     import baby_shark_amd as bs
     from baby_shark_amd import meshgen as mg
@@ -329,7 +372,7 @@ def batch(ctx, count=256, seed=5):
     res = {"workload": f"{count} meshes (tori / displaced icospheres), {sum(s.shape[0] for s, _ in ms)} triangles "
                        f"in all, {min(s.shape[0] for s, _ in ms)}-{max(s.shape[0] for s, _ in ms)} each, "
                        "bounding box 256 voxels, band 3"}
-    for name, workers in (("streams_4", 4), ("streams_1", 1)):
+    for name, workers in (("streams_8", 8), ("streams_4", 4), ("streams_1", 1)):
         best = None
         for _ in range(2):
             t0 = time.perf_counter()
@@ -459,7 +502,10 @@ def main():
     if not args.no_batch:
         out["batch"] = batch(ctx)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(soup, args.voxel, args.band)
+        v = m2v.convert(dev)
+        ge = v.export()
+        v.free()
+        out["cpu_baseline"], out["error"] = cpu_baseline_and_error(soup, args.voxel, args.band, ge)
     print(json.dumps(out), flush=True)
 
 

@@ -521,6 +521,27 @@ or_volume* or_volume_from_voxels(const int32_t* idx, const float* values, size_t
     return r;
 }
 
+// A grid holding n given leaves as they are: origin (x, y, z per leaf), the
+// 512-bit active mask (bit i % 64 of word i / 64 is voxel offset i) and all
+// 512 values, inactive ones included; the leaves are placed with
+// insert_leaf_at (internal_node/tree_node.rs:280-298).  Lets the tests run the
+// oracle's CSG and offset on operands another implementation produced.
+or_volume* or_volume_from_leaves(const int32_t* origins, const uint64_t* masks, const float* values, size_t n,
+                                 float h) {
+    auto g = std::make_unique<VolumeGrid>();
+    for (size_t l = 0; l < n; ++l) {
+        auto leaf = std::make_unique<LeafNode>(Vec3i{origins[3 * l], origins[3 * l + 1], origins[3 * l + 2]});
+        for (int o = 0; o < LeafNode::SIZE; ++o) {
+            leaf->values[o] = values[l * LeafNode::SIZE + o];
+            if ((masks[l * 8 + (o >> 6)] >> (o & 63)) & 1ull) leaf->value_mask.on(o);
+        }
+        g->insert_leaf_at(std::move(leaf));
+    }
+    auto* r = new or_volume{std::move(g), h, {}, {}};
+    r->index();
+    return r;
+}
+
 // DualContouringMesher::default().with_voxel_size(h).mesh(volume)
 // (dual_contouring.rs:23-83).  *out receives n_tris*9 floats (free with
 // or_free).  Returns 0, 1 where the reference reaches todo!() (a tile) or 2

@@ -80,6 +80,9 @@ def lib():
         L.or_volume_offset.argtypes = [vp, C.c_float]
         L.or_volume_from_voxels.restype = vp
         L.or_volume_from_voxels.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_float), C.c_size_t, C.c_float]
+        L.or_volume_from_leaves.restype = vp
+        L.or_volume_from_leaves.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_uint64), C.POINTER(C.c_float),
+                                            C.c_size_t, C.c_float]
         L.or_volume_csg.argtypes = [vp, vp, C.c_int]
         L.or_volume_flood_fill.argtypes = [vp]
         L.or_volume_clone.restype = vp
@@ -379,6 +382,19 @@ def from_voxels(idx, values, voxel_size: float) -> Volume:
     assert a.shape[0] == v.shape[0]
     return Volume(lib().or_volume_from_voxels(a.ctypes.data_as(C.POINTER(C.c_int32)),
                                               v.ctypes.data_as(C.POINTER(C.c_float)), a.shape[0], voxel_size),
+                  voxel_size)
+
+
+def from_leaves(exp, voxel_size: float) -> Volume:
+    """A grid holding the leaves of a canonical export as they are (masks and
+    all 512 values per leaf); the export's tiles are not carried."""
+    o = np.ascontiguousarray(exp.origins, np.int32)
+    m = np.ascontiguousarray(exp.masks, np.uint64)
+    v = np.ascontiguousarray(exp.values, np.float32)
+    assert o.shape[0] == m.shape[0] == v.shape[0] and exp.tile_origins.shape[0] == 0
+    return Volume(lib().or_volume_from_leaves(o.ctypes.data_as(C.POINTER(C.c_int32)),
+                                              m.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                              v.ctypes.data_as(C.POINTER(C.c_float)), o.shape[0], voxel_size),
                   voxel_size)
 
 
