@@ -22,6 +22,7 @@ BS_ERR_INTERNAL = 5
 BS_ERR_NO_DEVICE = 6
 BS_ERR_IO = 7
 BS_ERR_UNSUPPORTED = 8
+BS_ERR_CAPACITY = 9
 BS_MESHING_FEATURE_PRESERVING = 0
 BS_MESHING_MANIFOLD = 1
 BS_PATH_FORCE_HASH = 1
@@ -33,7 +34,7 @@ BS_PROFILE_COUNTERS = 2
 
 STATUS_NAMES = {0: "BS_OK", 1: "BS_ERR_EMPTY", 2: "BS_ERR_INVALID_ARG", 3: "BS_ERR_HIP",
                 4: "BS_ERR_UNSUPPORTED_TILE", 5: "BS_ERR_INTERNAL", 6: "BS_ERR_NO_DEVICE",
-                7: "BS_ERR_IO", 8: "BS_ERR_UNSUPPORTED"}
+                7: "BS_ERR_IO", 8: "BS_ERR_UNSUPPORTED", 9: "BS_ERR_CAPACITY"}
 
 
 class M2VParams(C.Structure):
@@ -137,6 +138,7 @@ def load():
             "bs_volume_offset": (C.c_int, [vp, vp, C.c_float]),
             "bs_dual_contour": (C.c_int, [vp, vp, C.c_float, C.POINTER(f32p), C.POINTER(C.c_size_t)]),
             "bs_dual_contour_device": (C.c_int, [vp, vp, C.c_float, C.POINTER(vp), C.POINTER(C.c_size_t)]),
+            "bs_dual_contour_into": (C.c_int, [vp, vp, C.c_float, vp, C.c_size_t, C.POINTER(C.c_size_t)]),
             "bs_ctx_last_dc_stats": (C.c_int, [vp, C.POINTER(DcStats)]),
             "bs_marching_cubes": (C.c_int, [vp, vp, C.c_float, C.POINTER(f32p), C.POINTER(C.c_size_t)]),
             "bs_marching_cubes_device": (C.c_int, [vp, vp, C.c_float, C.POINTER(vp), C.POINTER(C.c_size_t)]),

@@ -26,6 +26,9 @@ void volume_densify_tiles(bs_ctx* ctx, bs_volume* v);
 void volume_offset(bs_ctx* ctx, bs_volume* vol, float distance, bs_offset_stats* stats);
 void dual_contour_device(bs_ctx* ctx, bs_volume* vol, float voxel_size, float** out_dev, uint64_t* n_tris,
                          bs_dc_stats* stats);
+void dual_contour_into(bs_ctx* ctx, const bs_volume* vol, float voxel_size, float* out_dev, uint64_t capacity_tris,
+                       uint64_t* n_tris);
+void dc_noalloc_init(int device);
 void marching_cubes_device(bs_ctx* ctx, bs_volume* vol, float voxel_size, float** out_dev, uint64_t* n_tris,
                            bs_dc_stats* stats);
 void volume_from_voxels(bs_ctx* ctx, const int32_t* idx, const float* values, uint64_t n, float h, bs_volume** out);
@@ -80,6 +83,7 @@ bs_status bs_ctx_create(int device, bs_ctx** out) {
         if (e != hipSuccess || count == 0) throw Error(BS_ERR_NO_DEVICE, "no HIP device available");
         BS_REQUIRE(device >= 0 && device < count, BS_ERR_INVALID_ARG, "device index out of range");
         DeviceGuard g(device);
+        dc_noalloc_init(device);  // this device's address of bs_dual_contour_into's workspace
         auto* c = new bs_ctx();
         c->device = device;
         hipError_t se = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -470,6 +474,26 @@ bs_status bs_dual_contour_device(bs_ctx* ctx, const bs_volume* v, float voxel_si
         DeviceGuard g(ctx->device);
         uint64_t n = 0;
         dual_contour_device(ctx, const_cast<bs_volume*>(v), voxel_size, out_xyz_dev, &n, &ctx->last_dc);
+        *n_tris = (size_t)n;
+    });
+}
+
+bs_status bs_dual_contour_into(bs_ctx* ctx, const bs_volume* v, float voxel_size, float* out_xyz_dev,
+                               size_t capacity_tris, size_t* n_tris) {
+    if (!ctx || !v || !n_tris) {
+        g_last_error = "null argument";
+        return BS_ERR_INVALID_ARG;
+    }
+    *n_tris = 0;
+    return guarded([&] {
+        DeviceGuard g(ctx->device);
+        uint64_t n = 0;
+        try {
+            dual_contour_into(ctx, v, voxel_size, out_xyz_dev, capacity_tris, &n);
+        } catch (...) {
+            *n_tris = (size_t)n;
+            throw;
+        }
         *n_tris = (size_t)n;
     });
 }

@@ -20,6 +20,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 
 #include "common.hpp"
 #include "leafmap.hpp"
@@ -359,14 +361,144 @@ __global__ __launch_bounds__(64) void k_dc_triangles(uint64_t n_leaves, DcArgs a
     }
 }
 
-void exclusive_scan_u64(bs_ctx* ctx, const uint32_t* in, uint64_t* out, uint64_t n) {
-    size_t bytes = 0;
-    BS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, ctx->stream));
-    void* tmp = ctx->scratch.get(S_CUB, bytes);
-    BS_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, (int)n, ctx->stream));
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
 }
 
+// Fills n 32-bit words with a kernel (not hipMemsetAsync, so that the call
+// stays valid inside a caller's graph capture).
+void fill_u32(hipStream_t st, void* p, uint64_t n, uint32_t v) {
+    k_fill_u32<<<(unsigned)std::min<uint64_t>(div_up(n, 256), 2048), 256, 0, st>>>(static_cast<uint32_t*>(p), n, v);
+    BS_HIP(hipGetLastError());
+}
+
+// Device buffers of one dual-contouring call.  The usual entry point takes
+// them from the context's scratch arena; bs_dual_contour_into from the
+// per-device static workspace below.
+struct DcWork {
+    LeafMap map;
+    uint32_t* fail;
+    float* cell_pts;
+    uint64_t* cell_mask;
+    uint32_t* tri_count;
+    uint64_t* tri_off;
+    void* scan_tmp;
+    size_t scan_bytes;
+};
+
+uint64_t hash_capacity(uint64_t n) {
+    uint64_t cap = 1024;
+    while (cap < 2 * n) cap <<= 1;
+    return cap;
+}
+
+// Everything up to the triangle count: leaf map, cell points, per-leaf
+// triangle counts and their scan.  Returns the soup's triangle count.
+uint64_t dc_count(hipStream_t st, const bs_volume* vol, float voxel_size, DcWork& w, DcArgs& a) {
+    const uint64_t n = vol->n_leaves;
+    if (w.map.dense) {
+        fill_u32(st, w.map.cells, (uint64_t)w.map.nx * w.map.ny * w.map.nz, 0xFFFFFFFFu);
+    } else {
+        const uint64_t cap = (uint64_t)w.map.cap_mask + 1;
+        fill_u32(st, w.map.keys, 2 * cap, 0xFFFFFFFFu);
+        fill_u32(st, w.map.cells, cap, 0xFFFFFFFFu);
+    }
+    fill_u32(st, w.fail, 1, 0);
+    k_leafmap_insert<<<div_up(n, 256), 256, 0, st>>>(n, reinterpret_cast<const int4*>(vol->origins), w.map, w.fail);
+    BS_HIP(hipGetLastError());
+
+    a = DcArgs{};
+    a.map = w.map;
+    a.origins = reinterpret_cast<const int4*>(vol->origins);
+    a.masks = vol->masks;
+    a.values = vol->values;
+    a.cell_pts = w.cell_pts;
+    a.cell_mask = w.cell_mask;
+    a.voxel_size = voxel_size;
+    a.fail = w.fail;
+    const unsigned grid = (unsigned)std::min<uint64_t>(n, 8192);
+    k_dc_cells<<<grid, 64, 0, st>>>(n, a);
+    BS_HIP(hipGetLastError());
+    k_dc_triangles<false><<<grid, 64, 0, st>>>(n, a, w.tri_count, nullptr, nullptr);
+    BS_HIP(hipGetLastError());
+    size_t bytes = w.scan_bytes;
+    BS_HIP(hipcub::DeviceScan::ExclusiveSum(w.scan_tmp, bytes, w.tri_count, w.tri_off, (int)n, st));
+    uint32_t h_fail = 0, last_cnt = 0;
+    uint64_t last_off = 0;
+    BS_HIP(hipMemcpyAsync(&h_fail, w.fail, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    BS_HIP(hipMemcpyAsync(&last_cnt, w.tri_count + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    BS_HIP(hipMemcpyAsync(&last_off, w.tri_off + n - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    BS_HIP(hipStreamSynchronize(st));
+    BS_REQUIRE(!(h_fail & 1u), BS_ERR_INTERNAL,
+               "dual contouring: a crossing voxel has no neighbour along an axis; the reference reaches "
+               "unreachable!() (dual_contouring.rs:347)");
+    return last_off + last_cnt;
+}
+
+void dc_write(hipStream_t st, uint64_t n, const DcArgs& a, const DcWork& w, float* out) {
+    const unsigned grid = (unsigned)std::min<uint64_t>(n, 8192);
+    k_dc_triangles<true><<<grid, 64, 0, st>>>(n, a, nullptr, w.tri_off, out);
+    BS_HIP(hipGetLastError());
+}
+
+void check_meshable(bs_ctx* ctx, const bs_volume* vol) {
+    BS_REQUIRE(vol->device == ctx->device, BS_ERR_INVALID_ARG, "volume lives on another device");
+    // visit_leafs_par meets every active tile: todo!() (dual_contouring.rs:146,181,354)
+    BS_REQUIRE(vol->tiles.empty(), BS_ERR_UNSUPPORTED_TILE,
+               "dual contouring of a volume with active tiles: the reference stops at todo!()");
+}
+
+// ---------------------------------------------------------------- workspace of bs_dual_contour_into
+// bs_dual_contour_into allocates nothing: its buffers live in this array,
+// which each device holds from the library's load until the process ends.
+// Sized for DC_NOALLOC_MAX_LEAVES leaves with the hash leaf map.
+constexpr uint64_t DC_NOALLOC_MAX_LEAVES = BS_DC_INTO_MAX_LEAVES;
+constexpr uint64_t DC_NOALLOC_HASH_CAP = 2 * DC_NOALLOC_MAX_LEAVES;  // power of two >= 1024
+constexpr uint64_t DC_NOALLOC_SCAN_BYTES = 1ull << 20;
+constexpr uint64_t al256(uint64_t b) { return (b + 255) & ~255ull; }
+constexpr uint64_t NA_KEYS = 0;
+constexpr uint64_t NA_CELLS = NA_KEYS + al256(8 * DC_NOALLOC_HASH_CAP);
+constexpr uint64_t NA_FAIL = NA_CELLS + al256(4 * DC_NOALLOC_HASH_CAP);
+constexpr uint64_t NA_PTS = NA_FAIL + 256;
+constexpr uint64_t NA_CMASK = NA_PTS + al256(4ull * 512 * 3 * DC_NOALLOC_MAX_LEAVES);
+constexpr uint64_t NA_TCNT = NA_CMASK + al256(64ull * DC_NOALLOC_MAX_LEAVES);
+constexpr uint64_t NA_TOFF = NA_TCNT + al256(4ull * DC_NOALLOC_MAX_LEAVES);
+constexpr uint64_t NA_SCAN = NA_TOFF + al256(8ull * (DC_NOALLOC_MAX_LEAVES + 1));
+constexpr uint64_t NA_BYTES = NA_SCAN + DC_NOALLOC_SCAN_BYTES;
+static_assert(DC_NOALLOC_HASH_CAP >= 1024 && (DC_NOALLOC_HASH_CAP & (DC_NOALLOC_HASH_CAP - 1)) == 0, "hash capacity");
+
 }  // namespace
+
+// The array itself.  In host code this name is a stand-in, not a device
+// address: host code uses the address hipGetSymbolAddress returned for the
+// device (dc_noalloc_ptr), never the name.
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+namespace {
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex g_na_mutex;
+unsigned char* g_na_ptr[NA_MAX_DEVICES] = {};
+}  // namespace
+
+// Looks up this device's copy of the workspace once, with `device` current.
+// Called by bs_ctx_create, so the entry point itself needs no set-up call.
+void dc_noalloc_init(int device) {
+    BS_REQUIRE(device >= 0 && device < NA_MAX_DEVICES, BS_ERR_INVALID_ARG, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_na_mutex);
+    if (g_na_ptr[device]) return;
+    void* p = nullptr;
+    BS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    BS_REQUIRE(p != nullptr, BS_ERR_HIP, "hipGetSymbolAddress returned no address for the DC workspace");
+    g_na_ptr[device] = static_cast<unsigned char*>(p);
+}
+
+static unsigned char* dc_noalloc_ptr(int device) {
+    std::lock_guard<std::mutex> lock(g_na_mutex);
+    BS_REQUIRE(device >= 0 && device < NA_MAX_DEVICES && g_na_ptr[device], BS_ERR_INTERNAL,
+               "the DC workspace of this device was not set up (bs_ctx_create does it)");
+    return g_na_ptr[device];
+}
 
 // DualContouringMesher::default().with_voxel_size(h).mesh(volume): the
 // triangle soup (n_tris * 9 floats) in device memory owned by the caller
@@ -376,10 +508,7 @@ void dual_contour_device(bs_ctx* ctx, bs_volume* vol, float voxel_size, float** 
     hipStream_t st = ctx->stream;
     *out_dev = nullptr;
     *n_tris = 0;
-    BS_REQUIRE(vol->device == ctx->device, BS_ERR_INVALID_ARG, "volume lives on another device");
-    // visit_leafs_par meets every active tile: todo!() (dual_contouring.rs:146,181,354)
-    BS_REQUIRE(vol->tiles.empty(), BS_ERR_UNSUPPORTED_TILE,
-               "dual contouring of a volume with active tiles: the reference stops at todo!()");
+    check_meshable(ctx, vol);
     const uint64_t n = vol->n_leaves;
     if (stats) *stats = bs_dc_stats{};
     if (!n) return;
@@ -393,62 +522,41 @@ void dual_contour_device(bs_ctx* ctx, bs_volume* vol, float voxel_size, float** 
     // leaf map over the leaves' box (one leaf of margin for the halo lookups)
     int32_t lo[3], hi[3];
     leaf_box(ctx, vol->origins, n, lo, hi);
-    LeafMap map{};
+    DcWork w{};
     uint64_t cells = 1;
     for (int k = 0; k < 3; ++k) cells *= (uint64_t)(hi[k] - lo[k] + 1);
-    map.dense = !(ctx->paths & BS_PATH_FORCE_HASH) && cells <= (1ull << 28);
-    if (map.dense) {
-        map.x0 = lo[0], map.y0 = lo[1], map.z0 = lo[2];
-        map.nx = hi[0] - lo[0] + 1, map.ny = hi[1] - lo[1] + 1, map.nz = hi[2] - lo[2] + 1;
-        map.cells = ctx->scratch.as<int32_t>(S_DC_MAP, cells);
-        BS_HIP(hipMemsetAsync(map.cells, 0xFF, sizeof(int32_t) * cells, st));
+    w.map.dense = !(ctx->paths & BS_PATH_FORCE_HASH) && cells <= (1ull << 28);
+    if (w.map.dense) {
+        w.map.x0 = lo[0], w.map.y0 = lo[1], w.map.z0 = lo[2];
+        w.map.nx = hi[0] - lo[0] + 1, w.map.ny = hi[1] - lo[1] + 1, w.map.nz = hi[2] - lo[2] + 1;
+        w.map.cells = ctx->scratch.as<int32_t>(S_DC_MAP, cells);
     } else {
-        uint64_t cap = 1024;
-        while (cap < 2 * n) cap <<= 1;
-        map.cap_mask = (uint32_t)(cap - 1);
-        map.keys = ctx->scratch.as<uint64_t>(S_DC_KEYS, cap);
-        map.cells = ctx->scratch.as<int32_t>(S_DC_MAP, cap);
-        BS_HIP(hipMemsetAsync(map.keys, 0xFF, sizeof(uint64_t) * cap, st));
-        BS_HIP(hipMemsetAsync(map.cells, 0xFF, sizeof(int32_t) * cap, st));
+        const uint64_t cap = hash_capacity(n);
+        w.map.cap_mask = (uint32_t)(cap - 1);
+        w.map.keys = ctx->scratch.as<uint64_t>(S_DC_KEYS, cap);
+        w.map.cells = ctx->scratch.as<int32_t>(S_DC_MAP, cap);
     }
-    uint32_t* fail = ctx->scratch.as<uint32_t>(S_DC_FAIL, 1);
-    BS_HIP(hipMemsetAsync(fail, 0, sizeof(uint32_t), st));
-    k_leafmap_insert<<<div_up(n, 256), 256, 0, st>>>(n, reinterpret_cast<const int4*>(vol->origins), map, fail);
-    BS_HIP(hipGetLastError());
+    w.fail = ctx->scratch.as<uint32_t>(S_DC_FAIL, 1);
+    w.cell_pts = ctx->scratch.as<float>(S_DC_CELLS, n * 512 * 3);
+    w.cell_mask = ctx->scratch.as<uint64_t>(S_DC_CELLMASK, n * 8);
+    w.tri_count = ctx->scratch.as<uint32_t>(S_DC_TRICNT, n);
+    w.tri_off = ctx->scratch.as<uint64_t>(S_DC_TRIOFF, n + 1);
+    BS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, w.scan_bytes, w.tri_count, w.tri_off, (int)n, st));
+    w.scan_tmp = ctx->scratch.get(S_CUB, w.scan_bytes);
 
     DcArgs a{};
-    a.map = map;
-    a.origins = reinterpret_cast<const int4*>(vol->origins);
-    a.masks = vol->masks;
-    a.values = vol->values;
-    a.cell_pts = ctx->scratch.as<float>(S_DC_CELLS, n * 512 * 3);
-    a.cell_mask = ctx->scratch.as<uint64_t>(S_DC_CELLMASK, n * 8);
-    a.voxel_size = voxel_size;
-    a.fail = fail;
-    const unsigned grid = (unsigned)std::min<uint64_t>(n, 8192);
-    k_dc_cells<<<grid, 64, 0, st>>>(n, a);
-    BS_HIP(hipGetLastError());
-    uint32_t* tri_count = ctx->scratch.as<uint32_t>(S_DC_TRICNT, n);
-    uint64_t* tri_off = ctx->scratch.as<uint64_t>(S_DC_TRIOFF, n + 1);
-    k_dc_triangles<false><<<grid, 64, 0, st>>>(n, a, tri_count, nullptr, nullptr);
-    BS_HIP(hipGetLastError());
-    exclusive_scan_u64(ctx, tri_count, tri_off, n);
-    uint32_t h_fail = 0, last_cnt = 0;
-    uint64_t last_off = 0;
-    BS_HIP(hipMemcpyAsync(&h_fail, fail, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    BS_HIP(hipMemcpyAsync(&last_cnt, tri_count + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    BS_HIP(hipMemcpyAsync(&last_off, tri_off + n - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-    BS_HIP(hipStreamSynchronize(st));
-    BS_REQUIRE(!(h_fail & 1u), BS_ERR_INTERNAL,
-               "dual contouring: a crossing voxel has no neighbour along an axis; the reference reaches "
-               "unreachable!() (dual_contouring.rs:347)");
-    const uint64_t total = last_off + last_cnt;
+    const uint64_t total = dc_count(st, vol, voxel_size, w, a);
     if (total) {
         float* out = nullptr;
         BS_HIP(hipMalloc(&out, sizeof(float) * 9 * total));
-        k_dc_triangles<true><<<grid, 64, 0, st>>>(n, a, nullptr, tri_off, out);
-        hipError_t le = hipGetLastError();
-        if (le == hipSuccess) le = hipStreamSynchronize(st);
+        hipError_t le = hipSuccess;
+        try {
+            dc_write(st, n, a, w, out);
+        } catch (...) {
+            (void)hipFree(out);
+            throw;
+        }
+        le = hipStreamSynchronize(st);
         if (le != hipSuccess) {
             (void)hipFree(out);
             BS_HIP(le);
@@ -464,9 +572,56 @@ void dual_contour_device(bs_ctx* ctx, bs_volume* vol, float voxel_size, float** 
         stats->ms_total = ms;
         stats->n_leaves = n;
         stats->n_tris = total;
-        stats->used_hash = !map.dense;
+        stats->used_hash = !w.map.dense;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+    }
+}
+
+// bs_dual_contour_into: the same soup written to the caller's device buffer
+// of capacity_tris triangles, with every intermediate buffer in this
+// device's static workspace and no device or host allocation.  *n_tris is
+// the soup's triangle count; a soup larger than the capacity writes nothing
+// and reports BS_ERR_CAPACITY.  Calls that overlap on one device race on the
+// shared workspace.
+void dual_contour_into(bs_ctx* ctx, const bs_volume* vol, float voxel_size, float* out_dev, uint64_t capacity_tris,
+                       uint64_t* n_tris) {
+    hipStream_t st = ctx->stream;
+    *n_tris = 0;
+    check_meshable(ctx, vol);
+    const uint64_t n = vol->n_leaves;
+    BS_REQUIRE(n <= DC_NOALLOC_MAX_LEAVES, BS_ERR_CAPACITY,
+               "bs_dual_contour_into: the volume has more leaves than the workspace holds (" +
+                   std::to_string(n) + " > " + std::to_string(DC_NOALLOC_MAX_LEAVES) + ")");
+    if (!n) return;
+    unsigned char* base = dc_noalloc_ptr(ctx->device);
+    DcWork w{};
+    const uint64_t cap = hash_capacity(n);
+    w.map.dense = 0;
+    w.map.cap_mask = (uint32_t)(cap - 1);
+    w.map.keys = reinterpret_cast<uint64_t*>(base + NA_KEYS);
+    w.map.cells = reinterpret_cast<int32_t*>(base + NA_CELLS);
+    w.fail = reinterpret_cast<uint32_t*>(base + NA_FAIL);
+    w.cell_pts = reinterpret_cast<float*>(base + NA_PTS);
+    w.cell_mask = reinterpret_cast<uint64_t*>(base + NA_CMASK);
+    w.tri_count = reinterpret_cast<uint32_t*>(base + NA_TCNT);
+    w.tri_off = reinterpret_cast<uint64_t*>(base + NA_TOFF);
+    w.scan_tmp = base + NA_SCAN;
+    size_t need = 0;
+    BS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, w.tri_count, w.tri_off, (int)n, st));
+    BS_REQUIRE(need <= DC_NOALLOC_SCAN_BYTES, BS_ERR_INTERNAL, "scan workspace too small");
+    w.scan_bytes = need;
+
+    DcArgs a{};
+    const uint64_t total = dc_count(st, vol, voxel_size, w, a);
+    *n_tris = total;
+    BS_REQUIRE(total <= capacity_tris, BS_ERR_CAPACITY,
+               "bs_dual_contour_into: the soup has " + std::to_string(total) +
+                   " triangles, more than the output capacity of " + std::to_string(capacity_tris));
+    if (total) {
+        BS_REQUIRE(out_dev != nullptr, BS_ERR_INVALID_ARG, "null output buffer");
+        dc_write(st, n, a, w, out_dev);
+        BS_HIP(hipStreamSynchronize(st));
     }
 }
 

@@ -36,7 +36,8 @@ typedef enum {
     BS_ERR_INTERNAL = 5,         /* reference panics (unreachable!/unwrap)   */
     BS_ERR_NO_DEVICE = 6,        /* no HIP device / library built without GPU */
     BS_ERR_IO = 7,               /* a file could not be read or written       */
-    BS_ERR_UNSUPPORTED = 8       /* reserved: a method the library lacks      */
+    BS_ERR_UNSUPPORTED = 8,      /* reserved: a method the library lacks      */
+    BS_ERR_CAPACITY = 9          /* input or output beyond a caller-sized or fixed buffer */
 } bs_status;
 
 typedef struct bs_ctx bs_ctx;
@@ -260,6 +261,21 @@ bs_status bs_dual_contour(bs_ctx* ctx, const bs_volume* v, float voxel_size, flo
 /* Same, with the soup left in device memory (release with bs_device_free). */
 bs_status bs_dual_contour_device(bs_ctx* ctx, const bs_volume* v, float voxel_size, float** out_xyz_dev,
                                  size_t* n_tris);
+/* No counterpart in the reference: a variant of bs_dual_contour_device for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It writes the same soup into the
+ * caller's device buffer out_xyz_dev of capacity_tris triangles (9 floats
+ * each) and allocates no device or host memory: every intermediate buffer
+ * lives in a workspace each device holds from the library's load until the
+ * process ends (about 50 MB), set up by bs_ctx_create.  Limits: at most
+ * BS_DC_INTO_MAX_LEAVES leaves (else BS_ERR_CAPACITY, nothing written).
+ * *n_tris receives the soup's triangle count; when it exceeds capacity_tris
+ * nothing is written and the call returns BS_ERR_CAPACITY, so the caller can
+ * size the buffer and call again.  All calls on one device share the
+ * workspace: calls that overlap on one device race. */
+#define BS_DC_INTO_MAX_LEAVES 8192
+bs_status bs_dual_contour_into(bs_ctx* ctx, const bs_volume* v, float voxel_size, float* out_xyz_dev,
+                               size_t capacity_tris, size_t* n_tris);
 bs_status bs_ctx_last_dc_stats(const bs_ctx* ctx, bs_dc_stats* out);
 void bs_free(void* p);
 void bs_device_free(void* p_dev);
