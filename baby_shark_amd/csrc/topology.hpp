@@ -128,7 +128,8 @@ struct Topology {
     // 312-337, root_node/tree_node.rs:150-158), then the result's leaves
     // (origins in output order; leaf r is device leaf r) are inserted, creating
     // fresh nodes where needed, and empty nodes are pruned again
-    // (remove_empty_branches, fast_sweep.rs:61-64).
+    // (remove_empty_branches, fast_sweep.rs:61-64).  out_origins must hold
+    // every kept leaf (a sweep never empties a leaf that has an active voxel).
     std::unique_ptr<Topology> after_offset(const std::vector<uint32_t>& keep,
                                            const std::vector<std::array<int32_t, 3>>& out_origins, float two_h) const;
 

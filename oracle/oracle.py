@@ -15,7 +15,9 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+# BS_ORACLE_LIB names another build of the same library (the sanitizer build
+# of `make -C oracle asan`, for tests/test_sanitizers.py)
+LIB_PATH = os.environ.get("BS_ORACLE_LIB") or os.path.join(_HERE, "_build", "liboracle.so")
 _lib = None
 
 
