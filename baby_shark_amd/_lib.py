@@ -29,6 +29,8 @@ BS_PATH_FORCE_HASH = 1
 BS_PATH_FORCE_LDS_STACK = 2
 BS_SIGN_PER_VOXEL = 0
 BS_SIGN_SHARED = 1
+BS_SIGN_SURROGATE = 2
+BS_MLP_PARAMS = 8641
 BS_PROFILE_EVENTS = 1
 BS_PROFILE_COUNTERS = 2
 
@@ -54,7 +56,13 @@ class M2VStats(C.Structure):
                                           "bvh_height")] + \
               [(k, C.c_double) for k in ("ms_total", "ms_subdivide", "ms_leaves", "ms_distance", "ms_bvh",
                                          "ms_winding")] + \
-              [("n_sign_evals", C.c_uint64), ("ms_sign_share", C.c_double)]
+              [("n_sign_evals", C.c_uint64), ("ms_sign_share", C.c_double)] + \
+              [(k, C.c_uint64) for k in ("n_sg_tiles", "n_sg_culled", "n_sg_misclassified", "n_sg_fallback")]
+
+
+class MlpTrainParams(C.Structure):
+    _fields_ = [("epochs", C.c_uint32), ("batch", C.c_uint32), ("lr", C.c_float), ("lr_final", C.c_float),
+                ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint64)]
 
 
 class OffsetStats(C.Structure):
@@ -153,6 +161,16 @@ def load():
             "bs_volume_signs_at": (C.c_int, [vp, vp, i32p, C.c_size_t, C.POINTER(C.c_uint8)]),
             "bs_volume_device_leaves": (C.c_int, [vp, C.POINTER(i32p), C.POINTER(u64p), C.POINTER(f32p),
                                                   C.POINTER(C.c_uint64)]),
+            "bs_mlp_create": (C.c_int, [vp, C.c_uint64, pp]),
+            "bs_mlp_destroy": (None, [vp]),
+            "bs_mlp_set_params": (C.c_int, [vp, vp, f32p]),
+            "bs_mlp_get_params": (C.c_int, [vp, vp, f32p]),
+            "bs_mlp_set_input_map": (C.c_int, [vp, C.c_float, C.c_float, C.c_float, C.c_float]),
+            "bs_mlp_forward_device": (C.c_int, [vp, vp, vp, C.c_size_t, vp]),
+            "bs_mlp_gradient_device": (C.c_int, [vp, vp, vp, vp, C.c_size_t, f32p, C.POINTER(C.c_double)]),
+            "bs_mlp_train_device": (C.c_int, [vp, vp, vp, vp, C.c_size_t, C.POINTER(MlpTrainParams), f32p]),
+            "bs_volume_tile_samples": (C.c_int, [vp, vp, vp, vp, C.c_size_t, C.POINTER(C.c_size_t)]),
+            "bs_ctx_set_surrogate": (C.c_int, [vp, vp, C.c_float]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)

@@ -47,7 +47,7 @@ enum ScratchSlot : int {
     S_MISC2, S_MISC3, S_INPUT, S_BVH_KIDS, S_FLAGS_A, S_FLAGS_B, S_CSG_PLAN, S_DC_MAP, S_DC_KEYS, S_DC_FAIL, S_DC_CELLS, S_SIGN_ROOTS, S_REP_CNT, S_REP_OFF, S_REPS,
     S_BVH_HEIGHT, S_BVH_HEIGHT_S, S_BVH_INODE, S_BVH_INODE_S, S_BVH_MISC, S_BVH_STARTS, S_CUB_B,
     S_DC_CELLMASK, S_DC_TRICNT, S_DC_TRIOFF, S_MC_TAB, S_MC_UNITS, S_MC_TILES, S_MC_T8, S_MC_T8K, S_MC_T128,
-    S_MC_T128K, S_MC_TORG, S_MC_LANES, S_LEAF_BOX, S_COUNT
+    S_MC_T128K, S_MC_TORG, S_MC_LANES, S_LEAF_BOX, S_SG_PRED, S_MLP_PART, S_MLP_GRAD, S_MLP_PERM, S_MC_FLAGS, S_MC_CV, S_MC_V12, S_COUNT
 };
 
 struct Scratch {
@@ -96,6 +96,10 @@ struct bs_ctx {
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     uint32_t* host_pinned = nullptr;  // 512 words
+    // tile classifier of BS_SIGN_SURROGATE (not owned), and its cull threshold
+    // in voxels; without a network the gate uses each tile's true smallest |d|
+    const struct bs_mlp* surrogate = nullptr;
+    float surrogate_threshold = 0.0f;
 };
 
 struct bs_volume {

@@ -17,6 +17,7 @@
 #include <type_traits>
 
 #include "common.hpp"
+#include "surrogate.hpp"
 #include "m2v_kernels.hpp"
 
 namespace bs {
@@ -30,7 +31,7 @@ constexpr int WN_BLOCK = 256;
 
 enum Counter : int {
     C_ERR = 0, C_LEAVES, C_RECHECK, C_NEAR_INTERNAL, C_FAR, C_LEAF_VISITS, C_LEAF_TRIS, C_BOX_LO, C_BOX_HI = C_BOX_LO + 3,
-    C_PAIRS = C_BOX_HI + 3, C_COUNT
+    C_PAIRS = C_BOX_HI + 3, C_SG_TILES, C_SG_CULLED, C_SG_MISS, C_SG_FALLBACK, C_COUNT
 };
 enum ErrBits : unsigned { E_RANGE = 1, E_TOO_MANY_SUBS = 2, E_STACK = 4, E_HASH = 8, E_TREE = 16 };
 
@@ -1338,8 +1339,9 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     *out = nullptr;
     BS_REQUIRE(prm.voxel_size > 0.0f && std::isfinite(prm.voxel_size), BS_ERR_INVALID_ARG, "voxel_size must be > 0");
     BS_REQUIRE(prm.band >= 0 && prm.band < (1 << 20), BS_ERR_INVALID_ARG, "band must be in [0, 2^20)");
-    BS_REQUIRE(prm.sign_mode == BS_SIGN_PER_VOXEL || prm.sign_mode == BS_SIGN_SHARED, BS_ERR_INVALID_ARG,
-               "sign_mode must be BS_SIGN_PER_VOXEL or BS_SIGN_SHARED");
+    BS_REQUIRE(prm.sign_mode == BS_SIGN_PER_VOXEL || prm.sign_mode == BS_SIGN_SHARED ||
+                   prm.sign_mode == BS_SIGN_SURROGATE,
+               BS_ERR_INVALID_ARG, "sign_mode must be BS_SIGN_PER_VOXEL, BS_SIGN_SHARED or BS_SIGN_SURROGATE");
     if (n == 0) throw Error(BS_ERR_EMPTY, "empty mesh: the reference returns None");
     BS_REQUIRE(n < (1ull << 30), BS_ERR_INVALID_ARG, "more than 2^30 triangles");
     hipStream_t st = ctx->stream;
@@ -1424,9 +1426,10 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     exclusive_scan(ctx, rows, row_off, n + 1);
     exclusive_scan(ctx, subs, sub_off, n + 1);
     const bool shared_signs = prm.sign_mode == BS_SIGN_SHARED;
+    const bool surrogate_signs = prm.sign_mode == BS_SIGN_SURROGATE;
     uint32_t max_rows = 1;  // largest subdivision count: bounds the sub-triangle rounding
     uint32_t* d_max = nullptr;
-    if (shared_signs) {
+    if (shared_signs || surrogate_signs) {
         d_max = reinterpret_cast<uint32_t*>(S.get(S_MISC2, 64));
         size_t bytes = 0;
         BS_HIP(hipcub::DeviceReduce::Max(nullptr, bytes, rows, d_max, (int)n, st));
@@ -1559,7 +1562,7 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     uint16_t* roots = nullptr;
     uint32_t* rep_cnt = nullptr;
     float group_thr = 0.0f;
-    if (shared_signs) {
+    if (shared_signs || surrogate_signs) {
         double maxc = 0.0;
         for (int k = 0; k < 3; ++k)
             maxc = std::max({maxc, std::fabs((double)(long long)hc[C_BOX_LO + k]),
@@ -1573,6 +1576,8 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
         roots = S.as<uint16_t>(S_SIGN_ROOTS, n_leaves * LEAF_VOX);
         rep_cnt = S.as<uint32_t>(S_REP_CNT, n_leaves + 1);
         BS_HIP(hipMemsetAsync(rep_cnt + n_leaves, 0, sizeof(uint32_t), st));
+    }
+    if (shared_signs) {
         k_distance<true><<<(unsigned)n_leaves, DIST_BLOCK, 0, st>>>(vol->origins, cand_off, cand_list, recs, sub, h,
                                                                     (uint32_t)n_cov, vol->values, vol->masks,
                                                                     group_thr, roots, rep_cnt);
@@ -1613,11 +1618,16 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     BS_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
     tm.mark();
 
-    // ---- 6. group roots (shared signs only): the winding number is evaluated
+    // ---- 6. group roots (shared and surrogate signs): the winding number is evaluated
     // at one voxel per group
     const uint32_t* wn_list = act;
     uint32_t n_wn = n_act;
-    if (shared_signs && n_act > 0) {
+    if ((shared_signs || surrogate_signs) && n_act > 0) {
+        // surrogate signs: tile gate and star groups on the finished distances
+        // (the same rounding margin as the shared-sign links, beyond h)
+        if (surrogate_signs)
+            surrogate_groups(ctx, st, vol, h, group_thr - (float)((double)h * (1.0 + 0x1p-10)), roots, rep_cnt,
+                             counters + C_SG_TILES);
         uint32_t* rep_off = S.as<uint32_t>(S_REP_OFF, n_leaves + 1);
         exclusive_scan(ctx, rep_cnt, rep_off, n_leaves + 1);
         n_wn = read_scalar(ctx, rep_off + n_leaves);
@@ -1681,7 +1691,11 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     stt.ms_bvh = tm.ms(3, 4);
     stt.ms_winding = tm.ms(5, 6);
     stt.n_sign_evals = n_wn;
-    stt.ms_sign_share = shared_signs ? tm.ms(4, 5) + tm.ms(6, 7) : 0.0;
+    stt.ms_sign_share = (shared_signs || surrogate_signs) ? tm.ms(4, 5) + tm.ms(6, 7) : 0.0;
+    stt.n_sg_tiles = hc[C_SG_TILES];
+    stt.n_sg_culled = hc[C_SG_CULLED];
+    stt.n_sg_misclassified = hc[C_SG_MISS];
+    stt.n_sg_fallback = hc[C_SG_FALLBACK];
     *out = guard.release();
 }
 

@@ -4,27 +4,27 @@
 // The reference visits the leaves and active tiles serially in tree order and
 // runs Custodio et al.'s corrected Marching Cubes 33 on every cube whose eight
 // corners have values, with Lewiner et al.'s case tables
-// (src/voxel/meshing/lookup_table.rs).  Those tables are not retyped here.  At
-// first use the host derives, for every corner-sign configuration and every
-// outcome of its ambiguous-face tests, the surface from the cube itself: the
-// contour segments of the six faces chain into loops; two loops become one
-// tube where the configuration's interior test says so; a loop of eight or
-// more crossings is fanned around the centre vertex; the reference's two
-// table-indexing behaviours (13.5.2 takes the complementary configuration's
-// tiling, 10.1.2 and 12.1.2 after two true face tests take their tube
-// reversed) are kept.  The result matches the reference cube by cube in
-// crossings, topology and orientation; the diagonal that splits a polygon and
-// the order of the triangles inside a cube are this library's own.
+// (src/voxel/meshing/lookup_table.rs; here the data of mc_tables.inc, written
+// from that file by gen_mc_tables.py).  Each cube takes the tiling that
+// handle_cube (marching_cubes.rs:72-243) selects with the same face and
+// interior tests, and emits its triangles in add_faces' vertex order
+// (:245-272).  The centre vertex is mesher state: compute_c_vertex
+// (:919-940) sets it in cases 7.3, 10.2, 12.2, 13.3 and 13.4, and case 6.1.2
+// uses vertex 12 without computing it, reading the centre left by the last
+// earlier cube (in the serial visit order) that computed one, or zero.  When
+// a volume holds a 6.1.2 cube, a summary pass finds each unit's last centre
+// and a scan hands every unit the centre it starts with.
 //
 // Device side: the units (leaves and active tiles) in the reference's visit
 // order; one wavefront per unit.  A leaf stages its values and the +1 halo
-// (9^3) in LDS; lane (x, y) owns the z-row of cubes with that corner.  A tile
-// meets the cubes of its six faces, (i, j) major, faces left, right, top,
-// bottom, front, back (the reference's tile visitor, :970-994, which meets
-// edge cubes more than once); cubes with every corner inside the tile are all
-// one value and emit nothing, so only cubes touching the tile's far faces are
-// evaluated.  A counting launch, a scan over units and a writing launch give
-// the serial order.
+// (9^3) in LDS; lane (x, y) owns the z-row of cubes with that corner, which
+// is also the serial order (x, y, z).  A tile meets the cubes of its six
+// faces, (i, j) major, faces left, right, top, bottom, front, back (the
+// reference's tile visitor, :970-994, which meets edge cubes more than once);
+// cubes with every corner inside the tile are all one value and emit
+// nothing, so only cubes touching the tile's far faces are evaluated.  A
+// counting launch, a scan over units and a writing launch give the serial
+// order.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -44,16 +44,6 @@ Topology& ensure_topology(bs_ctx* ctx, bs_volume* v);
 namespace {
 
 // ------------------------------------------------------------------ cube geometry
-// Edge::from_edge_num (lookup_table.rs:54-69): lower corner first; 12 = centre
-constexpr int MC_EDGE[12][2] = {{0, 1}, {1, 2}, {3, 2}, {0, 3}, {4, 5}, {5, 6},
-                                {7, 6}, {4, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
-// cube faces 1..6 (test_face numbering, marching_cubes.rs:327-334), corners
-// counter-clockwise seen from outside
-constexpr int MC_CCW[7][4] = {{0, 0, 0, 0}, {1, 5, 4, 0}, {2, 6, 5, 1}, {3, 7, 6, 2},
-                              {0, 4, 7, 3}, {0, 3, 2, 1}, {5, 6, 7, 4}};
-// interior_ambiguity (:474-530): face pairs 1|3, 2|4, 5|6 use list 0, 1, 2
-constexpr int MC_FAMILY[7] = {2, 0, 1, 0, 1, 2, 2};
-
 // device copies: CUBE_OFFSETS (utils.rs:86-95), edges and their axes, the
 // test_face corner order a, b, c, d, the interior_ambiguity lists (corner,
 // corner, edge named, in test order), and for interior_ambiguity_verification
@@ -74,240 +64,10 @@ __constant__ int8_t c_ver[12][4][2] = {
     {{0, 3}, {4, 7}, {5, 6}, {1, 2}}, {{1, 2}, {0, 3}, {4, 7}, {5, 6}}, {{4, 0}, {7, 3}, {6, 2}, {5, 1}},
     {{5, 1}, {4, 0}, {7, 3}, {6, 2}}, {{6, 2}, {5, 1}, {4, 0}, {7, 3}}, {{7, 3}, {6, 2}, {5, 1}, {4, 0}}};
 
-// ------------------------------------------------------------------ host tables
-// Per configuration id (bit i: corner i negative): the ambiguous faces, and
-// per pattern of their outcomes (bit j: the negative corners of the j-th
-// ambiguous face are joined across it) the interior test to run and the
-// triangles without and with a tube.
-enum : uint8_t { TEST_NONE = 0, TEST_VERIFY = 1, TEST_CASE13 = 2 };
-struct McTables {
-    std::vector<uint8_t> amb_mask;  // 256: bit f-1 for ambiguous face f
-    std::vector<uint16_t> base;     // 256: first pattern slot
-    std::vector<uint8_t> spec;      // per slot: test | arg negative << 2 | list mask << 3
-    std::vector<uint32_t> entry;    // per slot * 2 + tube: offset | count << 24 | centre << 28
-    std::vector<uint16_t> tris;     // e0 | e1 << 4 | e2 << 8
-};
-
-using HTri = std::array<int, 3>;
-using HLoop = std::vector<int>;
-
-int h_edge_of(int a, int b) {
-    for (int e = 0; e < 12; ++e)
-        if ((MC_EDGE[e][0] == a && MC_EDGE[e][1] == b) || (MC_EDGE[e][0] == b && MC_EDGE[e][1] == a)) return e;
-    return -1;
-}
-
-// contour loops: faces walked counter-clockwise from outside, a segment from a
-// positive-to-negative crossing to a negative-to-positive one
-std::vector<HLoop> h_loops(const bool* neg, const bool* joined) {
-    int next[12];
-    for (int& s : next) s = -1;
-    for (int f = 1; f <= 6; ++f) {
-        const int* q = MC_CCW[f];
-        bool sg[4];
-        int e[4], nx = 0;
-        for (int i = 0; i < 4; ++i) {
-            sg[i] = neg[q[i]];
-            e[i] = h_edge_of(q[i], q[(i + 1) & 3]);
-        }
-        for (int i = 0; i < 4; ++i) nx += sg[i] != sg[(i + 1) & 3];
-        if (nx == 4) {
-            for (int i = 0; i < 4; ++i) {
-                if (sg[i] == joined[f]) continue;
-                if (sg[i])
-                    next[e[(i + 3) & 3]] = e[i];
-                else
-                    next[e[i]] = e[(i + 3) & 3];
-            }
-        } else if (nx == 2) {
-            int pn = -1, np = -1;
-            for (int i = 0; i < 4; ++i) {
-                if (!sg[i] && sg[(i + 1) & 3]) pn = e[i];
-                if (sg[i] && !sg[(i + 1) & 3]) np = e[i];
-            }
-            next[pn] = np;
-        }
-    }
-    std::vector<HLoop> loops;
-    bool seen[12] = {};
-    for (int s = 0; s < 12; ++s) {
-        if (next[s] < 0 || seen[s]) continue;
-        HLoop l;
-        for (int x = s; !seen[x]; x = next[x]) {
-            seen[x] = true;
-            l.push_back(x);
-        }
-        loops.push_back(l);
-    }
-    return loops;
-}
-
-void h_fan(const HLoop& l, std::vector<HTri>& out) {
-    for (size_t i = 1; i + 1 < l.size(); ++i) out.push_back({l[0], l[i], l[i + 1]});
-}
-void h_centre_fan(const HLoop& l, std::vector<HTri>& out) {
-    for (size_t i = 0; i < l.size(); ++i) out.push_back({l[i], l[(i + 1) % l.size()], 12});
-}
-// tube: A forward, B backward, each step closes one loop edge; `centre` splits
-// the rung between the first vertices through the centre vertex
-void h_tube(const HLoop& A, const HLoop& B, bool centre, bool reversed, std::vector<HTri>& out) {
-    const size_t n1 = A.size(), n2 = B.size();
-    std::vector<int> b(B.rbegin(), B.rend());
-    size_t i = 0, k = 0;
-    auto put = [&](HTri t) { out.push_back(reversed ? HTri{t[0], t[2], t[1]} : t); };
-    while (i < n1 || k < n2) {
-        HTri t;
-        if (k >= n2 || (i < n1 && (i + 1) * n2 <= (k + 1) * n1)) {
-            t = {A[i % n1], A[(i + 1) % n1], b[k % n2]};
-            ++i;
-        } else {
-            t = {b[(k + 1) % n2], b[k % n2], A[i % n1]};
-            ++k;
-        }
-        int ia = -1, ib = -1;
-        for (int j = 0; j < 3; ++j) {
-            if (t[j] == A[0]) ia = j;
-            if (t[j] == b[0]) ib = j;
-        }
-        if (centre && ia >= 0 && ib >= 0) {
-            HTri t1 = t, t2 = t;
-            t1[ib] = 12;
-            t2[ia] = 12;
-            put(t1);
-            put(t2);
-        } else {
-            put(t);
-        }
-    }
-}
-
-int h_loop_corner(const HLoop& l) {
-    for (int v = 0; v < 8; ++v) {
-        bool all = true;
-        for (int e : l) all = all && (MC_EDGE[e][0] == v || MC_EDGE[e][1] == v);
-        if (all) return v;
-    }
-    return -1;
-}
-
-// The surface of configuration `id` with face outcomes `joined`: the interior
-// test it needs (spec), and its triangles without (t0) and with (t1) a tube.
-void h_configure(int id, const bool* joined, uint8_t& spec, std::vector<HTri>& t0, std::vector<HTri>& t1) {
-    spec = TEST_NONE;
-    bool neg[8];
-    int pc = 0;
-    for (int i = 0; i < 8; ++i) {
-        neg[i] = (id >> i) & 1;
-        pc += neg[i];
-    }
-    if (pc == 0 || pc == 8) return;
-    int amb_faces[6], n_amb = 0, n_join = 0;
-    for (int f = 1; f <= 6; ++f) {
-        const int* q = MC_CCW[f];
-        if (neg[q[0]] != neg[q[1]] && neg[q[1]] != neg[q[2]] && neg[q[2]] != neg[q[3]]) {
-            amb_faces[n_amb++] = f;
-            n_join += joined[f];
-        }
-    }
-    const std::vector<HLoop> loops = h_loops(neg, joined);
-    int k = 0, edges_in = 0;
-    bool minor[8];
-    for (int i = 0; i < 8; ++i) {
-        minor[i] = neg[i] == (pc <= 4);
-        k += minor[i];
-    }
-    for (int e = 0; e < 12; ++e) edges_in += minor[MC_EDGE[e][0]] && minor[MC_EDGE[e][1]];
-    const bool s0_neg = pc > 4;
-    bool tube = false, centre = false, reversed = false;
-    if (k == 2 && n_amb == 0 && loops.size() == 2) {  // case 4 (test_interior, :405-419)
-        spec = TEST_VERIFY | (s0_neg << 2) | (7 << 3);
-        tube = true;
-    } else if (k == 3 && edges_in == 1 && loops.size() == 2) {  // case 6.1 (:421-427)
-        spec = TEST_VERIFY | (s0_neg << 2) | ((1 << MC_FAMILY[amb_faces[0]]) << 3);
-        tube = centre = true;
-    } else if (k == 3 && edges_in == 0 && n_join == (pc <= 4 ? 3 : 0)) {  // case 7.4 (:428-444)
-        spec = TEST_VERIFY | ((!s0_neg) << 2) | (7 << 3);
-        tube = true;
-    } else if (k == 4 && edges_in == 2 && loops.size() == 2) {  // cases 10.1, 12.1 (:445-463)
-        const bool both = joined[amb_faces[0]];
-        spec = TEST_VERIFY | (both << 2) | (((1 << MC_FAMILY[amb_faces[0]]) | (1 << MC_FAMILY[amb_faces[1]])) << 3);
-        tube = true;
-        reversed = both;
-    } else if (k == 4 && edges_in == 0 && loops.size() == 3 && n_join == 3) {  // case 13.5 (:228-238)
-        spec = TEST_CASE13;
-        // the complementary configuration's tiling: faces 1..4 of the pattern
-        // a quarter turn on, every sign flipped
-        bool rp[7] = {};
-        if (id == 165) {
-            rp[2] = joined[1], rp[3] = joined[2], rp[4] = joined[3], rp[1] = joined[4];
-        } else {
-            rp[1] = joined[2], rp[2] = joined[3], rp[3] = joined[4], rp[4] = joined[1];
-        }
-        rp[5] = joined[5], rp[6] = joined[6];
-        bool np[8];
-        for (int i = 0; i < 8; ++i) np[i] = !neg[i];
-        const std::vector<HLoop> L = h_loops(np, rp);
-        int six = -1, with = -1, alone = -1;
-        for (int j = 0; j < (int)L.size(); ++j) {
-            if (L[j].size() == 6)
-                six = j;
-            else if (np[h_loop_corner(L[j])])
-                with = j;
-            else
-                alone = j;
-        }
-        h_tube(L[six], L[with], false, false, t1);
-        h_fan(L[alone], t1);
-    }
-    for (const HLoop& l : loops) {
-        if (l.size() >= 8)
-            h_centre_fan(l, t0);
-        else
-            h_fan(l, t0);
-    }
-    if (tube) {
-        h_tube(loops[0], loops[1], centre, reversed, t1);
-        for (size_t j = 2; j < loops.size(); ++j) h_fan(loops[j], t1);
-    }
-}
-
-const McTables& mc_tables() {
-    static McTables T;
-    static std::once_flag once;
-    std::call_once(once, [] {
-        T.amb_mask.assign(256, 0);
-        T.base.assign(256, 0);
-        for (int id = 0; id < 256; ++id) {
-            bool neg[8];
-            for (int i = 0; i < 8; ++i) neg[i] = (id >> i) & 1;
-            int amb_faces[6], n_amb = 0;
-            for (int f = 1; f <= 6; ++f) {
-                const int* q = MC_CCW[f];
-                if (neg[q[0]] != neg[q[1]] && neg[q[1]] != neg[q[2]] && neg[q[2]] != neg[q[3]]) {
-                    amb_faces[n_amb++] = f;
-                    T.amb_mask[id] |= (uint8_t)(1 << (f - 1));
-                }
-            }
-            T.base[id] = (uint16_t)T.spec.size();
-            for (int p = 0; p < (1 << n_amb); ++p) {
-                bool joined[7] = {};
-                for (int j = 0; j < n_amb; ++j) joined[amb_faces[j]] = (p >> j) & 1;
-                uint8_t spec;
-                std::vector<HTri> t[2];
-                h_configure(id, joined, spec, t[0], t[1]);
-                T.spec.push_back(spec);
-                for (int tube = 0; tube < 2; ++tube) {
-                    const std::vector<HTri>& tl = (spec == TEST_NONE || !tube) ? t[0] : t[1];
-                    bool centre = false;
-                    for (const HTri& tr : tl) centre = centre || tr[0] == 12 || tr[1] == 12 || tr[2] == 12;
-                    T.entry.push_back((uint32_t)T.tris.size() | ((uint32_t)tl.size() << 24) | ((uint32_t)centre << 28));
-                    for (const HTri& tr : tl) T.tris.push_back((uint16_t)(tr[0] | (tr[1] << 4) | (tr[2] << 8)));
-                }
-            }
-        }
-    });
-    return T;
-}
+// ------------------------------------------------------------------ case tables
+#define MC_TABLE __constant__ static const
+#include "mc_tables.inc"
+#undef MC_TABLE
 
 // ------------------------------------------------------------------ device
 struct McArgs {
@@ -322,11 +82,9 @@ struct McArgs {
     uint64_t n_units;
     const int4* tile_org;  // x, y, z, size per visited tile
     const float* tile_val;
-    const uint8_t* amb_mask;
-    const uint16_t* base;
-    const uint8_t* spec;
-    const uint32_t* entry;
-    const uint16_t* tris;
+    uint32_t* flags;        // bit 0: a cube took case 6.1.2
+    const float4* v12_in;   // per unit: the centre vertex the unit starts with (xyz)
+    float4* unit_cv;        // per unit: its last computed centre (xyz), w = 1 when there is one
     float voxel_size;
 };
 
@@ -431,57 +189,174 @@ __device__ inline void mc_vertex(const float* c, int e, int gx, int gy, int gz, 
         z = z + t;
 }
 
-// One cube with a surface (min corner g; c: its eight clamped corner values,
-// in LDS, configuration id): the number of kept triangles; with `out`, they
-// are written there, 9 floats each.
-__device__ uint32_t mc_cube(const McArgs& a, const float* c, int id, int gx, int gy, int gz, float* out) {
-    const int pc = __popc(id);
-    const uint8_t am = a.amb_mask[id];
-    int p = 0, j = 0;
-    for (int f = 1; f <= 6; ++f) {
-        if (!((am >> (f - 1)) & 1)) continue;
-        const bool r = mc_test_face(c, pc <= 4 ? f : -f);
-        p |= (int)(pc <= 4 ? r : !r) << j;
-        ++j;
+// interior_ambiguity's diagonal list of an ambiguous face (:474-530): faces
+// 1|3, 2|4 and 5|6 (and 0)
+__device__ inline int mc_family(int amb_face) { return (amb_face == 1 || amb_face == 3) ? 0 : (amb_face == 2 || amb_face == 4) ? 1 : 2; }
+
+// test_interior (:404-468)
+__device__ bool mc_test_interior(const float* c, int cas, int cfg, int face) {
+    const float f = (float)face;
+    switch (cas) {
+        case 4:
+            return mc_verify(c, mc_amb_edge(c, 0, f)) + mc_verify(c, mc_amb_edge(c, 1, f)) +
+                       mc_verify(c, mc_amb_edge(c, 2, f)) != 0;
+        case 6: return mc_verify(c, mc_amb_edge(c, mc_family(abs(MC_TEST_6[cfg][0])), f)) != 0;
+        case 7:
+            return mc_verify(c, mc_amb_edge(c, 0, -f)) + mc_verify(c, mc_amb_edge(c, 1, -f)) +
+                       mc_verify(c, mc_amb_edge(c, 2, -f)) != 0;
+        case 10: return mc_verify(c, mc_amb_edge(c, mc_family(abs(MC_TEST_10[cfg][0])), f)) != 0;
+        default:
+            return mc_verify(c, mc_amb_edge(c, mc_family(abs(MC_TEST_12[cfg][0])), f)) +
+                       mc_verify(c, mc_amb_edge(c, mc_family(abs(MC_TEST_12[cfg][1])), f)) != 0;
     }
-    const int slot = a.base[id] + p;
-    const uint8_t sp = a.spec[slot];
-    int tube = 0;
-    if ((sp & 3) == TEST_VERIFY) {
-        const float arg = (sp & 4) ? -7.0f : 7.0f;
-        int sum = 0;
-        for (int l = 0; l < 3; ++l)
-            if ((sp >> (3 + l)) & 1) sum += mc_verify(c, mc_amb_edge(c, l, arg));
-        tube = sum == 0;
-    } else if ((sp & 3) == TEST_CASE13) {
-        tube = !mc_interior13(c);
-    }
-    const uint32_t e = a.entry[slot * 2 + tube];
-    const uint32_t off = e & 0xFFFFFFu, cnt = (e >> 24) & 15u;
-    float mx = 0.0f, my = 0.0f, mz = 0.0f;
-    if ((e >> 28) & 1u) {  // compute_c_vertex (:919-940): the mean of the crossings, summed in edge order
-        int n = 0;
-        for (int k = 0; k < 12; ++k) {
-            if ((c[c_edge[k][0]] < 0.0f) == (c[c_edge[k][1]] < 0.0f)) continue;
-            float x, y, z;
-            mc_vertex(c, k, gx, gy, gz, x, y, z);
-            mx = mx + x, my = my + y, mz = mz + z;
-            ++n;
+}
+
+// The tiling handle_cube selects (:72-243): its entries, their count, and
+// kind 1 where compute_c_vertex runs first (7.3, 10.2, 12.2, 13.3, 13.4) or
+// kind 2 where vertex 12 is read without it (6.1.2).
+struct McSel {
+    const int8_t* t;
+    int n;
+    int kind;
+};
+#define MC_PICK(tab) (r.t = &(tab)[0], r.n = (int)sizeof(tab))
+__device__ McSel mc_select(const float* c, int id) {
+    const int cas = MC_CASES[id][0], cfg = MC_CASES[id][1];
+    McSel r{nullptr, 0, 0};
+    switch (cas) {
+        case 1: MC_PICK(MC_TILING_1[cfg]); break;
+        case 2: MC_PICK(MC_TILING_2[cfg]); break;
+        case 3:
+            if (mc_test_face(c, MC_TEST_3[cfg])) MC_PICK(MC_TILING_3_2[cfg]);
+            else MC_PICK(MC_TILING_3_1[cfg]);
+            break;
+        case 4:
+            if (mc_test_interior(c, 4, cfg, MC_TEST_4[cfg])) MC_PICK(MC_TILING_4_1[cfg]);
+            else MC_PICK(MC_TILING_4_2[cfg]);
+            break;
+        case 5: MC_PICK(MC_TILING_5[cfg]); break;
+        case 6:
+            if (mc_test_face(c, MC_TEST_6[cfg][0])) MC_PICK(MC_TILING_6_2[cfg]);
+            else if (mc_test_interior(c, 6, cfg, MC_TEST_6[cfg][1])) MC_PICK(MC_TILING_6_1_1[cfg]);
+            else MC_PICK(MC_TILING_6_1_2[cfg]), r.kind = 2;
+            break;
+        case 7: {
+            const int sub = (int)mc_test_face(c, MC_TEST_7[cfg][0]) + 2 * (int)mc_test_face(c, MC_TEST_7[cfg][1]) +
+                            4 * (int)mc_test_face(c, MC_TEST_7[cfg][2]);
+            if (sub == 0) MC_PICK(MC_TILING_7_1[cfg]);
+            else if (sub == 1 || sub == 2) MC_PICK(MC_TILING_7_2[cfg][sub - 1]);
+            else if (sub == 4) MC_PICK(MC_TILING_7_2[cfg][2]);
+            else if (sub == 3) MC_PICK(MC_TILING_7_3[cfg][0]), r.kind = 1;
+            else if (sub == 5 || sub == 6) MC_PICK(MC_TILING_7_3[cfg][sub - 4]), r.kind = 1;
+            else if (mc_test_interior(c, 7, cfg, MC_TEST_7[cfg][3])) MC_PICK(MC_TILING_7_4_1[cfg]);
+            else MC_PICK(MC_TILING_7_4_2[cfg]);
+            break;
         }
-        mx = mx / (float)n, my = my / (float)n, mz = mz / (float)n;
+        case 8: MC_PICK(MC_TILING_8[cfg]); break;
+        case 9: MC_PICK(MC_TILING_9[cfg]); break;
+        case 10:
+            if (mc_test_face(c, MC_TEST_10[cfg][0])) {
+                if (mc_test_face(c, MC_TEST_10[cfg][1])) {
+                    if (mc_test_interior(c, 10, cfg, -MC_TEST_10[cfg][2])) MC_PICK(MC_TILING_10_1_1_[cfg]);
+                    else MC_PICK(MC_TILING_10_1_2[5 - cfg]);
+                } else {
+                    MC_PICK(MC_TILING_10_2[cfg]), r.kind = 1;
+                }
+            } else if (mc_test_face(c, MC_TEST_10[cfg][1])) {
+                MC_PICK(MC_TILING_10_2_[cfg]), r.kind = 1;
+            } else if (mc_test_interior(c, 10, cfg, MC_TEST_10[cfg][2])) {
+                MC_PICK(MC_TILING_10_1_1[cfg]);
+            } else {
+                MC_PICK(MC_TILING_10_1_2[cfg]);
+            }
+            break;
+        case 11: MC_PICK(MC_TILING_11[cfg]); break;
+        case 12:
+            if (mc_test_face(c, MC_TEST_12[cfg][0])) {
+                if (mc_test_face(c, MC_TEST_12[cfg][1])) {
+                    if (mc_test_interior(c, 12, cfg, -MC_TEST_12[cfg][2])) MC_PICK(MC_TILING_12_1_1_[cfg]);
+                    else MC_PICK(MC_TILING_12_1_2[23 - cfg]);
+                } else {
+                    MC_PICK(MC_TILING_12_2[cfg]), r.kind = 1;
+                }
+            } else if (mc_test_face(c, MC_TEST_12[cfg][1])) {
+                MC_PICK(MC_TILING_12_2_[cfg]), r.kind = 1;
+            } else if (mc_test_interior(c, 12, cfg, MC_TEST_12[cfg][2])) {
+                MC_PICK(MC_TILING_12_1_1[cfg]);
+            } else {
+                MC_PICK(MC_TILING_12_1_2[cfg]);
+            }
+            break;
+        case 13: {
+            int sub = 0;
+            for (int k = 0; k < 6; ++k) sub |= (int)mc_test_face(c, MC_TEST_13[cfg][k]) << k;
+            const int o = MC_SUB_CONFIG_13[sub];
+            if (o == 0) MC_PICK(MC_TILING_13_1[cfg]);
+            else if (o <= 6) MC_PICK(MC_TILING_13_2[cfg][o - 1]);
+            else if (o <= 18) MC_PICK(MC_TILING_13_3[cfg][o - 7]), r.kind = 1;
+            else if (o <= 22) MC_PICK(MC_TILING_13_4[cfg][o - 19]), r.kind = 1;
+            else if (o <= 26) {
+                if (mc_interior13(c)) MC_PICK(MC_TILING_13_5_1[cfg][o - 23]);
+                else MC_PICK(MC_TILING_13_5_2[1 - cfg][o - 23]);
+            } else if (o <= 38) MC_PICK(MC_TILING_13_3_[cfg][o - 27]), r.kind = 1;
+            else if (o <= 44) MC_PICK(MC_TILING_13_2_[cfg][o - 39]);
+            else if (o == 45) MC_PICK(MC_TILING_13_1_[cfg]);
+            break;
+        }
+        case 14: MC_PICK(MC_TILING_14[cfg]); break;
+        default: break;
     }
+    return r;
+}
+#undef MC_PICK
+
+__device__ inline bool mc_crosses(const float* c, int e) { return (c[c_edge[e][0]] < 0.0f) != (c[c_edge[e][1]] < 0.0f); }
+
+// compute_c_vertex (:919-940): the mean of the crossings, summed in edge order
+__device__ inline float3 mc_centre(const float* c, int gx, int gy, int gz) {
+    float mx = 0.0f, my = 0.0f, mz = 0.0f;
+    int n = 0;
+    for (int k = 0; k < 12; ++k) {
+        if (!mc_crosses(c, k)) continue;
+        float x, y, z;
+        mc_vertex(c, k, gx, gy, gz, x, y, z);
+        mx = mx + x, my = my + y, mz = mz + z;
+        ++n;
+    }
+    return make_float3(mx / (float)n, my / (float)n, mz / (float)n);
+}
+
+// The centre vertex a cube leaves behind: v12 updated where it computes one.
+__device__ inline void mc_track(const float* c, int id, int gx, int gy, int gz, float3& v12, bool& has) {
+    if (mc_select(c, id).kind == 1) {
+        v12 = mc_centre(c, gx, gy, gz);
+        has = true;
+    }
+}
+
+// One cube with a surface (min corner g; c: its eight clamped corner values,
+// configuration id; v12: the mesher's centre vertex, updated here): the number
+// of kept triangles; with `out`, they are written there, 9 floats each.
+__device__ uint32_t mc_cube(const McArgs& a, const float* c, int id, int gx, int gy, int gz, float* out, float3& v12) {
+    const McSel sel = mc_select(c, id);
+    if (sel.kind == 1) v12 = mc_centre(c, gx, gy, gz);
+    if (sel.kind == 2) atomicOr(a.flags, 1u);
     const float h = a.voxel_size;
     uint32_t kept = 0;
-    for (uint32_t t = 0; t < cnt; ++t) {
-        const uint16_t tr = a.tris[off + t];
+    for (int i = 0; i + 2 < sel.n; i += 3) {
+        const int ed[3] = {sel.t[i], sel.t[i + 2], sel.t[i + 1]};  // add_faces: entries i, i + 2, i + 1
         float v[9];
+        bool ok = true;
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            const int ed = (tr >> (4 * q)) & 15;
-            float x = mx, y = my, z = mz;
-            if (ed != 12) mc_vertex(c, ed, gx, gy, gz, x, y, z);
+            float x = v12.x, y = v12.y, z = v12.z;
+            if (ed[q] != 12) {
+                ok = ok && mc_crosses(c, ed[q]);  // no intersection: the reference skips the triangle
+                mc_vertex(c, ed[q], gx, gy, gz, x, y, z);
+            }
             v[3 * q] = x * h, v[3 * q + 1] = y * h, v[3 * q + 2] = z * h;
         }
+        if (!ok) continue;
         // Triangle3::is_degenerate (triangle3.rs:272-275)
         const float ux = v[3] - v[0], uy = v[4] - v[1], uz = v[5] - v[2];
         const float wx = v[6] - v[0], wy = v[7] - v[1], wz = v[8] - v[2];
@@ -495,6 +370,24 @@ __device__ uint32_t mc_cube(const McArgs& a, const float* c, int id, int gx, int
         ++kept;
     }
     return kept;
+}
+
+// The centre each lane starts with, in serial (lane) order: that of the last
+// earlier lane that computed one, else `seed`; `last` = the wave's last one
+// (seed when none did).
+__device__ inline float3 mc_lane_in(bool has, float3 mine, float3 seed, int lane, float3& last) {
+    const unsigned long long m = __ballot(has);
+    const unsigned long long before = m & ((1ull << lane) - 1);
+    const int src = before ? 63 - __clzll((long long)before) : lane;
+    float3 r = make_float3(__shfl(mine.x, src, 64), __shfl(mine.y, src, 64), __shfl(mine.z, src, 64));
+    if (!before) r = seed;
+    if (m) {
+        const int hi = 63 - __clzll((long long)m);
+        last = make_float3(__shfl(mine.x, hi, 64), __shfl(mine.y, hi, 64), __shfl(mine.z, hi, 64));
+    } else {
+        last = seed;
+    }
+    return r;
 }
 
 __device__ inline uint32_t wave_exclusive(uint32_t v, int lane, uint32_t& total) {
@@ -515,92 +408,159 @@ __device__ inline int mc_corner_off(int k) {
     return (((k + 1) >> 1) & 1) * (MB * MB) + ((k >> 1) & 1) * MB + ((k >> 2) & 1);
 }
 
-// WRITE = false: unit_cnt[u] = kept triangles of unit u, lane_cnt[64 u + l] =
-// those of lane l of a leaf unit.  WRITE = true: the triangles at
-// out + 9 * unit_off[u].
-template <bool WRITE>
+// Stages a leaf unit's 9^3 block (its values and the +x/+y/+z neighbour
+// blocks: leaf, 8^3 tile or 128^3 tile) in LDS, clamped like Cube::from_voxel.
+struct McStage {
+    float val[MB * MB * MB];
+    uint8_t act[MB * MB * MB];
+    int32_t nb_leaf[8];
+    float nb_val[8];
+    uint8_t nb_kind[8];  // 0 none, 1 leaf, 2 tile
+};
+__device__ void mc_stage_leaf(const McArgs& a, McStage& S, int64_t unit, int4 org, int lane) {
+    if (lane < 8) {
+        const int dx = lane >> 2, dy = (lane >> 1) & 1, dz = lane & 1;
+        const int bx = (org.x >> 3) + dx, by = (org.y >> 3) + dy, bz = (org.z >> 3) + dz;
+        int32_t s = lane == 0 ? (int32_t)unit : a.leaves.find(bx, by, bz);
+        uint8_t kind = s >= 0 ? 1 : 0;
+        float tv = 0.0f;
+        if (s < 0 && a.n8) {
+            const int32_t t = a.t8.find(bx, by, bz);
+            if (t >= 0) kind = 2, tv = a.t8_val[t];
+        }
+        if (s < 0 && kind == 0 && a.n128) {
+            const int32_t t = a.t128.find(bx >> 4, by >> 4, bz >> 4);
+            if (t >= 0) kind = 2, tv = a.t128_val[t];
+        }
+        S.nb_leaf[lane] = s;
+        S.nb_kind[lane] = kind;
+        S.nb_val[lane] = tv;
+    }
+    __syncthreads();
+    for (int i = lane; i < MB * MB * MB; i += 64) {
+        const int x = i / (MB * MB), y = (i / MB) % MB, z = i % MB;
+        const int nb = ((x >> 3) << 2) | ((y >> 3) << 1) | (z >> 3);
+        float v = 0.0f;
+        bool on = false;
+        if (S.nb_kind[nb] == 1) {
+            const uint64_t s = (uint64_t)S.nb_leaf[nb];
+            const int o = ((x & 7) << 6) | ((y & 7) << 3) | (z & 7);
+            on = (a.masks[s * 8 + (o >> 6)] >> (o & 63)) & 1ull;
+            if (on) v = a.values[s * LEAF_VOX + o];
+        } else if (S.nb_kind[nb] == 2) {
+            on = true;
+            v = S.nb_val[nb];
+        }
+        S.val[i] = mc_clamp(v);
+        S.act[i] = on;
+    }
+    __syncthreads();
+}
+
+// Cube (x, y, z) of a staged leaf: its corners copied to c; false when a
+// corner has no value or the cube has no surface.
+__device__ inline bool mc_leaf_cube(const McStage& S, int x, int y, int z, float* c, int& id) {
+    const int b0 = (x * MB + y) * MB + z;
+    id = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int i = b0 + mc_corner_off(k);
+        if (!S.act[i]) return false;
+        id |= (S.val[i] < 0.0f) << k;
+    }
+    if (id == 0 || id == 255) return false;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c[k] = S.val[b0 + mc_corner_off(k)];
+    return true;
+}
+
+// Face cube f of pair (i, j) of a tile of size s (origin to, value tv): its
+// local min corner in l, corners in c; false when it is interior to the
+// tile, has a corner without a value or has no surface.
+__device__ inline bool mc_tile_cube(const McArgs& a, int4 to, int s, float tv, int f, int i, int j, int* l, float* c,
+                                    int& id) {
+    int lx, ly, lz;
+    switch (f) {
+        case 0: lx = 0, ly = i, lz = j; break;
+        case 1: lx = s - 1, ly = i, lz = j; break;
+        case 2: lx = i, ly = j, lz = s - 1; break;
+        case 3: lx = i, ly = j, lz = 0; break;
+        case 4: lx = i, ly = s - 1, lz = j; break;
+        default: lx = i, ly = 0, lz = j; break;
+    }
+    if (lx < s - 1 && ly < s - 1 && lz < s - 1) return false;  // one value throughout
+    id = 0;
+    for (int k = 0; k < 8; ++k) {
+        const int cx = lx + c_corner[k][0], cy = ly + c_corner[k][1], cz = lz + c_corner[k][2];
+        float v = tv;
+        if (!(cx < s && cy < s && cz < s)) {
+            if (!mc_at(a, to.x + cx, to.y + cy, to.z + cz, v)) return false;
+            v = mc_clamp(v);
+        }
+        c[k] = v;
+        id |= (v < 0.0f) << k;
+    }
+    l[0] = lx, l[1] = ly, l[2] = lz;
+    return id != 0 && id != 255;
+}
+
+__device__ inline float3 f3(float4 v) { return make_float3(v.x, v.y, v.z); }
+
+// MODE 0: summary (unit_cv[u] = the unit's last computed centre, w = 1 when
+// there is one).  MODE 1: count (unit_cnt[u] = kept triangles of unit u,
+// lane_cnt[64 u + l] = those of lane l of a leaf unit).  MODE 2: write (the
+// triangles at out + 9 * unit_off[u]).  STALE: the volume has a 6.1.2 cube,
+// so every cube needs the serial centre: each unit starts from v12_in[u] and
+// each lane from the last centre of the lanes before it.
+template <int MODE, bool STALE>
 __global__ __launch_bounds__(64) void k_mc_units(McArgs a, uint32_t* __restrict__ unit_cnt,
                                                  uint16_t* __restrict__ lane_cnt, const uint64_t* __restrict__ unit_off,
                                                  float* __restrict__ out) {
-    __shared__ float val[MB * MB * MB];
-    __shared__ uint8_t act[MB * MB * MB];
+    constexpr bool WRITE = MODE == 2;
+    __shared__ McStage S;
     __shared__ float cv[64 * CS];
-    __shared__ int32_t nb_leaf[8];
-    __shared__ float nb_val[8];
-    __shared__ uint8_t nb_kind[8];  // 0 none, 1 leaf, 2 tile
     const int lane = threadIdx.x;
     float* mine_c = cv + lane * CS;
     for (uint64_t u = blockIdx.x; u < a.n_units; u += gridDim.x) {
         const int64_t unit = a.units[u];
         uint64_t running = WRITE ? unit_off[u] : 0;
+        const float3 seed = STALE ? f3(a.v12_in[u]) : make_float3(0.0f, 0.0f, 0.0f);
         if (unit >= 0) {
             const int4 org = a.origins[unit];
-            if (lane < 8) {  // the leaf and its +x/+y/+z neighbour blocks
-                const int dx = lane >> 2, dy = (lane >> 1) & 1, dz = lane & 1;
-                const int bx = (org.x >> 3) + dx, by = (org.y >> 3) + dy, bz = (org.z >> 3) + dz;
-                int32_t s = lane == 0 ? (int32_t)unit : a.leaves.find(bx, by, bz);
-                uint8_t kind = s >= 0 ? 1 : 0;
-                float tv = 0.0f;
-                if (s < 0 && a.n8) {
-                    const int32_t t = a.t8.find(bx, by, bz);
-                    if (t >= 0) kind = 2, tv = a.t8_val[t];
-                }
-                if (s < 0 && kind == 0 && a.n128) {
-                    const int32_t t = a.t128.find(bx >> 4, by >> 4, bz >> 4);
-                    if (t >= 0) kind = 2, tv = a.t128_val[t];
-                }
-                nb_leaf[lane] = s;
-                nb_kind[lane] = kind;
-                nb_val[lane] = tv;
-            }
-            __syncthreads();
-            for (int i = lane; i < MB * MB * MB; i += 64) {
-                const int x = i / (MB * MB), y = (i / MB) % MB, z = i % MB;
-                const int nb = ((x >> 3) << 2) | ((y >> 3) << 1) | (z >> 3);
-                float v = 0.0f;
-                bool on = false;
-                if (nb_kind[nb] == 1) {
-                    const uint64_t s = (uint64_t)nb_leaf[nb];
-                    const int o = ((x & 7) << 6) | ((y & 7) << 3) | (z & 7);
-                    on = (a.masks[s * 8 + (o >> 6)] >> (o & 63)) & 1ull;
-                    if (on) v = a.values[s * LEAF_VOX + o];
-                } else if (nb_kind[nb] == 2) {
-                    on = true;
-                    v = nb_val[nb];
-                }
-                val[i] = mc_clamp(v);
-                act[i] = on;
-            }
-            __syncthreads();
+            mc_stage_leaf(a, S, unit, org, lane);
             const int x = lane >> 3, y = lane & 7;
-            uint32_t base = 0;
-            if (WRITE) {
-                uint32_t total;
-                base = wave_exclusive(lane_cnt[u * 64 + lane], lane, total);
+            const int gx = org.x + x, gy = org.y + y;
+            float3 v12 = seed;
+            if (MODE == 0 || STALE) {  // the lane's last centre, then the one it starts with
+                bool has = false;
+                float3 last = make_float3(0.0f, 0.0f, 0.0f);
+                int id;
+                for (int z = 0; z < 8; ++z)
+                    if (mc_leaf_cube(S, x, y, z, mine_c, id)) mc_track(mine_c, id, gx, gy, org.z + z, last, has);
+                float3 ulast;
+                v12 = mc_lane_in(has, last, seed, lane, ulast);
+                const bool any = __ballot(has) != 0;  // every lane takes part in the ballot
+                if (MODE == 0 && lane == 0) a.unit_cv[u] = make_float4(ulast.x, ulast.y, ulast.z, any ? 1.0f : 0.0f);
             }
-            uint32_t mine = 0;
-            for (int z = 0; z < 8; ++z) {
-                const int b0 = (x * MB + y) * MB + z;
-                bool full = true;
-                int id = 0;
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const int i = b0 + mc_corner_off(k);
-                    full = full && act[i];
-                    id |= (val[i] < 0.0f) << k;
+            if (MODE != 0) {
+                uint32_t base = 0;
+                if (WRITE) {
+                    uint32_t total;
+                    base = wave_exclusive(lane_cnt[u * 64 + lane], lane, total);
                 }
-                if (!full || id == 0 || id == 255) continue;
-#pragma unroll
-                for (int k = 0; k < 8; ++k) mine_c[k] = val[b0 + mc_corner_off(k)];
-                const uint32_t kept = mc_cube(a, mine_c, id, org.x + x, org.y + y, org.z + z,
-                                              WRITE ? out + 9 * (running + base + mine) : nullptr);
-                mine += kept;
-            }
-            if (!WRITE) {
-                uint32_t total;
-                wave_exclusive(mine, lane, total);
-                lane_cnt[u * 64 + lane] = (uint16_t)mine;
-                if (lane == 0) unit_cnt[u] = total;
+                uint32_t mine = 0;
+                int id;
+                for (int z = 0; z < 8; ++z) {
+                    if (!mc_leaf_cube(S, x, y, z, mine_c, id)) continue;
+                    mine += mc_cube(a, mine_c, id, gx, gy, org.z + z,
+                                    WRITE ? out + 9 * (running + base + mine) : nullptr, v12);
+                }
+                if (MODE == 1) {
+                    uint32_t total;
+                    wave_exclusive(mine, lane, total);
+                    lane_cnt[u * 64 + lane] = (uint16_t)mine;
+                    if (lane == 0) unit_cnt[u] = total;
+                }
             }
             __syncthreads();
         } else {
@@ -609,53 +569,56 @@ __global__ __launch_bounds__(64) void k_mc_units(McArgs a, uint32_t* __restrict_
             const int s = to.w;
             const uint32_t pairs = (uint32_t)s * (uint32_t)s;
             uint32_t tile_total = 0;
+            float3 carry = seed;  // the centre left by the chunks before
+            bool unit_has = false;
             for (uint32_t q0 = 0; q0 < pairs; q0 += 64) {
                 const uint32_t q = q0 + lane;
                 const int i = (int)(q / (uint32_t)s), j = (int)(q % (uint32_t)s);
-                uint32_t mine = 0, base = 0;
-                for (int pass = 0; pass < (WRITE ? 2 : 1); ++pass) {
-                    if (pass == 1) {
-                        uint32_t total;
-                        base = wave_exclusive(mine, lane, total);
-                        mine = 0;
-                    }
-                    for (int f = 0; f < 6 && q < pairs; ++f) {
-                        int lx, ly, lz;
-                        switch (f) {
-                            case 0: lx = 0, ly = i, lz = j; break;
-                            case 1: lx = s - 1, ly = i, lz = j; break;
-                            case 2: lx = i, ly = j, lz = s - 1; break;
-                            case 3: lx = i, ly = j, lz = 0; break;
-                            case 4: lx = i, ly = s - 1, lz = j; break;
-                            default: lx = i, ly = 0, lz = j; break;
-                        }
-                        if (lx < s - 1 && ly < s - 1 && lz < s - 1) continue;  // one value throughout
-                        bool full = true;
-                        int id = 0;
-                        for (int k = 0; k < 8 && full; ++k) {
-                            const int cx = lx + c_corner[k][0], cy = ly + c_corner[k][1], cz = lz + c_corner[k][2];
-                            float v = tv;
-                            if (!(cx < s && cy < s && cz < s)) {
-                                full = mc_at(a, to.x + cx, to.y + cy, to.z + cz, v);
-                                v = mc_clamp(v);
-                            }
-                            mine_c[k] = v;
-                            id |= (v < 0.0f) << k;
-                        }
-                        if (!full || id == 0 || id == 255) continue;
-                        mine += mc_cube(a, mine_c, id, to.x + lx, to.y + ly, to.z + lz,
-                                        pass == 1 ? out + 9 * (running + base + mine) : nullptr);
-                    }
+                float3 lane_v12 = carry;
+                int l[3], id;
+                if (MODE == 0 || STALE) {
+                    bool has = false;
+                    float3 last = make_float3(0.0f, 0.0f, 0.0f);
+                    for (int f = 0; f < 6 && q < pairs; ++f)
+                        if (mc_tile_cube(a, to, s, tv, f, i, j, l, mine_c, id))
+                            mc_track(mine_c, id, to.x + l[0], to.y + l[1], to.z + l[2], last, has);
+                    float3 clast;
+                    lane_v12 = mc_lane_in(has, last, carry, lane, clast);
+                    const bool any = __ballot(has) != 0;  // every lane takes part in the ballot
+                    unit_has = unit_has || any;
+                    carry = clast;
                 }
-                uint32_t total;
-                wave_exclusive(mine, lane, total);
-                running += total;
-                tile_total += total;
+                if (MODE != 0) {
+                    uint32_t mine = 0, base = 0;
+                    for (int pass = 0; pass < (WRITE ? 2 : 1); ++pass) {
+                        if (pass == 1) {
+                            uint32_t total;
+                            base = wave_exclusive(mine, lane, total);
+                            mine = 0;
+                        }
+                        float3 v12 = lane_v12;
+                        for (int f = 0; f < 6 && q < pairs; ++f) {
+                            if (!mc_tile_cube(a, to, s, tv, f, i, j, l, mine_c, id)) continue;
+                            mine += mc_cube(a, mine_c, id, to.x + l[0], to.y + l[1], to.z + l[2],
+                                            pass == 1 ? out + 9 * (running + base + mine) : nullptr, v12);
+                        }
+                    }
+                    uint32_t total;
+                    wave_exclusive(mine, lane, total);
+                    running += total;
+                    tile_total += total;
+                }
             }
-            if (!WRITE && lane == 0) unit_cnt[u] = tile_total;
+            if (MODE == 0 && lane == 0) a.unit_cv[u] = make_float4(carry.x, carry.y, carry.z, unit_has ? 1.0f : 0.0f);
+            if (MODE == 1 && lane == 0) unit_cnt[u] = tile_total;
         }
     }
 }
+
+// the centre a unit starts with: that of the last earlier unit holding one
+struct LastCentre {
+    __device__ __host__ float4 operator()(const float4& a, const float4& b) const { return b.w != 0.0f ? b : a; }
+};
 
 void mc_scan(bs_ctx* ctx, const uint32_t* in, uint64_t* out, uint64_t n) {
     size_t bytes = 0;
@@ -705,19 +668,6 @@ void marching_cubes_device(bs_ctx* ctx, bs_volume* vol, float voxel_size, float*
         BS_HIP(hipEventCreate(&e1));
         BS_HIP(hipEventRecord(e0, st));
     }
-    // host-built tables, one buffer
-    const McTables& T = mc_tables();
-    const size_t o_base = 256, o_spec = o_base + 2 * 256, o_entry = (o_spec + T.spec.size() + 3) & ~size_t(3),
-                 o_tris = o_entry + 4 * T.entry.size(), tab_bytes = o_tris + 2 * T.tris.size();
-    std::vector<uint8_t> tab(tab_bytes, 0);
-    std::memcpy(tab.data(), T.amb_mask.data(), 256);
-    std::memcpy(tab.data() + o_base, T.base.data(), 2 * 256);
-    std::memcpy(tab.data() + o_spec, T.spec.data(), T.spec.size());
-    std::memcpy(tab.data() + o_entry, T.entry.data(), 4 * T.entry.size());
-    std::memcpy(tab.data() + o_tris, T.tris.data(), 2 * T.tris.size());
-    uint8_t* dtab = ctx->scratch.as<uint8_t>(S_MC_TAB, tab_bytes);
-    BS_HIP(hipMemcpyAsync(dtab, tab.data(), tab_bytes, hipMemcpyHostToDevice, st));
-
     McArgs a{};
     uint32_t* fail = ctx->scratch.as<uint32_t>(S_DC_FAIL, 1);
     BS_HIP(hipMemsetAsync(fail, 0, sizeof(uint32_t), st));
@@ -791,32 +741,57 @@ void marching_cubes_device(bs_ctx* ctx, bs_volume* vol, float voxel_size, float*
     a.origins = reinterpret_cast<const int4*>(vol->origins);
     a.masks = vol->masks;
     a.values = vol->values;
-    a.amb_mask = dtab;
-    a.base = reinterpret_cast<const uint16_t*>(dtab + o_base);
-    a.spec = dtab + o_spec;
-    a.entry = reinterpret_cast<const uint32_t*>(dtab + o_entry);
-    a.tris = reinterpret_cast<const uint16_t*>(dtab + o_tris);
     a.voxel_size = voxel_size;
+    a.flags = ctx->scratch.as<uint32_t>(S_MC_FLAGS, 1);
+    BS_HIP(hipMemsetAsync(a.flags, 0, sizeof(uint32_t), st));
 
     const unsigned grid = (unsigned)std::min<uint64_t>(n_units, 16384);
     uint32_t* cnt = ctx->scratch.as<uint32_t>(S_DC_TRICNT, n_units);
     uint64_t* off = ctx->scratch.as<uint64_t>(S_DC_TRIOFF, n_units + 1);
     uint16_t* lane_cnt = ctx->scratch.as<uint16_t>(S_MC_LANES, n_units * 64);
-    k_mc_units<false><<<grid, 64, 0, st>>>(a, cnt, lane_cnt, nullptr, nullptr);
-    BS_HIP(hipGetLastError());
-    mc_scan(ctx, cnt, off, n_units);
-    uint32_t h_fail = 0, last_cnt = 0;
+    uint32_t h_fail = 0, h_flags = 0, last_cnt = 0;
     uint64_t last_off = 0;
-    BS_HIP(hipMemcpyAsync(&h_fail, fail, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    BS_HIP(hipMemcpyAsync(&last_cnt, cnt + n_units - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    BS_HIP(hipMemcpyAsync(&last_off, off + n_units - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-    BS_HIP(hipStreamSynchronize(st));
-    BS_REQUIRE(!h_fail, BS_ERR_INTERNAL, "marching cubes: a leaf or tile key could not be registered");
+    auto count = [&](bool stale) {
+        if (stale)
+            k_mc_units<1, true><<<grid, 64, 0, st>>>(a, cnt, lane_cnt, nullptr, nullptr);
+        else
+            k_mc_units<1, false><<<grid, 64, 0, st>>>(a, cnt, lane_cnt, nullptr, nullptr);
+        BS_HIP(hipGetLastError());
+        mc_scan(ctx, cnt, off, n_units);
+        BS_HIP(hipMemcpyAsync(&h_fail, fail, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        BS_HIP(hipMemcpyAsync(&h_flags, a.flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        BS_HIP(hipMemcpyAsync(&last_cnt, cnt + n_units - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        BS_HIP(hipMemcpyAsync(&last_off, off + n_units - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+        BS_HIP(hipStreamSynchronize(st));
+        BS_REQUIRE(!h_fail, BS_ERR_INTERNAL, "marching cubes: a leaf or tile key could not be registered");
+    };
+    count(false);
+    // a 6.1.2 cube reads the centre vertex an earlier cube left: find each
+    // unit's last centre, hand every unit the one it starts with, count again
+    const bool stale = (h_flags & 1u) != 0;
+    if (stale) {
+        a.unit_cv = ctx->scratch.as<float4>(S_MC_CV, n_units);
+        float4* v12_in = ctx->scratch.as<float4>(S_MC_V12, n_units);
+        k_mc_units<0, false><<<grid, 64, 0, st>>>(a, nullptr, nullptr, nullptr, nullptr);
+        BS_HIP(hipGetLastError());
+        size_t bytes = 0;
+        const float4 none = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        BS_HIP(hipcub::DeviceScan::ExclusiveScan(nullptr, bytes, a.unit_cv, v12_in, LastCentre{}, none, (int)n_units,
+                                                 st));
+        void* tmp = ctx->scratch.get(S_CUB, bytes);
+        BS_HIP(hipcub::DeviceScan::ExclusiveScan(tmp, bytes, a.unit_cv, v12_in, LastCentre{}, none, (int)n_units,
+                                                 st));
+        a.v12_in = v12_in;
+        count(true);
+    }
     const uint64_t total = last_off + last_cnt;
     if (total) {
         float* out = nullptr;
         BS_HIP(hipMalloc(&out, sizeof(float) * 9 * total));
-        k_mc_units<true><<<grid, 64, 0, st>>>(a, nullptr, lane_cnt, off, out);
+        if (stale)
+            k_mc_units<2, true><<<grid, 64, 0, st>>>(a, nullptr, lane_cnt, off, out);
+        else
+            k_mc_units<2, false><<<grid, 64, 0, st>>>(a, nullptr, lane_cnt, off, out);
         hipError_t le = hipGetLastError();
         if (le == hipSuccess) le = hipStreamSynchronize(st);
         if (le != hipSuccess) {

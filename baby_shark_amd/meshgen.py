@@ -15,6 +15,8 @@ A soup is a ``(n, 3, 3)`` float32 array: triangle, vertex, xyz.
 """
 from __future__ import annotations
 
+import functools
+
 import numpy as np
 
 _MASK = np.uint64(0xFFFFFFFFFFFFFFFF)
@@ -82,8 +84,10 @@ def _icosahedron():
     return v, f
 
 
+@functools.lru_cache(maxsize=2)
 def icosphere(level: int):
-    """Unit icosphere: (vertices float64 (V,3), faces int64 (F,3)), 20*4^level faces."""
+    """Unit icosphere: (vertices float64 (V,3), faces int64 (F,3)), 20*4^level faces.
+    Cached: callers must not modify the arrays."""
     v, f = _icosahedron()
     for _ in range(level):
         e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]], 0)
@@ -184,3 +188,23 @@ def batch_meshes(count: int = 256, seed: int = 5, lo: float = 1e3, hi: float = 2
         ext = float((soup.reshape(-1, 3).max(0) - soup.reshape(-1, 3).min(0)).max())
         out.append((soup, ext / span_voxels))
     return out
+
+
+def _displaced_icosphere_args(args):
+    return displaced_icosphere(*args)
+
+
+def displaced_icospheres(level: int, seeds, radius: float = 1.0, amp: float = 0.05, processes: int = 0):
+    """displaced_icosphere for each seed, built in worker processes (started
+    fresh, not forked, so a caller that holds a GPU context may use it)."""
+    import concurrent.futures
+    import multiprocessing
+    import os
+    seeds = list(seeds)
+    if not processes:
+        processes = max(1, min(len(seeds), 16, os.cpu_count() or 1))
+    if processes == 1:
+        return [displaced_icosphere(level, radius, amp, s) for s in seeds]
+    with concurrent.futures.ProcessPoolExecutor(max_workers=processes,
+                                                mp_context=multiprocessing.get_context("spawn")) as ex:
+        return list(ex.map(_displaced_icosphere_args, [(level, radius, amp, s) for s in seeds]))

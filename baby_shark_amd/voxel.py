@@ -49,6 +49,15 @@ class Context:
         """Force alternate code paths (BS_PATH_* flags of bs_voxel.h); 0 = by size."""
         check(load().bs_ctx_set_paths(self._h, int(flags)))
 
+    def set_surrogate(self, classifier, threshold_vox: float) -> None:
+        """The tile classifier (TileClassifier, or None for the perfect one
+        that reads each tile's true smallest |SDF|) and its cull threshold in
+        voxels, used by the surrogate sign mode on this context.  The
+        classifier must outlive its use."""
+        self._surrogate = classifier
+        check(load().bs_ctx_set_surrogate(self._h, classifier.handle if classifier is not None else None,
+                                          float(threshold_vox)))
+
     def last_offset_stats(self) -> dict:
         s = _lib.OffsetStats()
         check(load().bs_ctx_last_offset_stats(self.handle, C.byref(s)))
@@ -205,6 +214,32 @@ class Volume:
         check(load().bs_volume_signs_at(self._ctx.handle, self._h, a.ctypes.data_as(C.POINTER(C.c_int32)),
                                         a.shape[0], out.ctypes.data_as(C.POINTER(C.c_uint8))))
         return out.astype(bool)
+
+    def device_arrays(self, copy: bool = True):
+        """The leaf arrays as device tensors (origins (L, 4) int32, masks (L, 8)
+        int64 holding the uint64 mask words' bits, values (L, 512) float32);
+        copies unless copy=False, whose views live only as long as this volume."""
+        import torch
+        L = load()
+        o, m, v = C.POINTER(C.c_int32)(), C.POINTER(C.c_uint64)(), C.POINTER(C.c_float)()
+        n = C.c_uint64(0)
+        check(L.bs_volume_device_leaves(self._h, C.byref(o), C.byref(m), C.byref(v), C.byref(n)))
+        nl = n.value
+
+        class _View:
+            def __init__(self, ptr, shape, typestr):
+                self.__cuda_array_interface__ = {"shape": shape, "typestr": typestr, "version": 2, "strides": None,
+                                                 "data": (C.cast(ptr, C.c_void_p).value or 0, False)}
+        dev = torch.device("cuda", self._ctx.device)
+        out = []
+        for ptr, shape, ts, dt in ((o, (nl, 4), "<i4", torch.int32), (m, (nl, 8), "<i8", torch.int64),
+                                   (v, (nl, 512), "<f4", torch.float32)):
+            if nl == 0:
+                out.append(torch.zeros(shape, dtype=dt, device=dev))
+                continue
+            t = torch.as_tensor(_View(ptr, shape, ts), device=dev)
+            out.append(t.clone() if copy else t)
+        return tuple(out)
 
     def export(self) -> VolumeExport:
         L = load()
@@ -473,6 +508,14 @@ class MeshToVolume:
         self.params.voxel_size = float(size)
         return self
 
+    def with_surrogate_signs(self) -> "MeshToVolume":
+        """The tile-gated sign mode (BS_SIGN_SURROGATE, closed meshes only):
+        the context's classifier (Context.set_surrogate) picks the tiles whose
+        voxels share one winding-number evaluation where the exact distances
+        prove it; the others are evaluated voxel by voxel."""
+        self.params.sign_mode = _lib.BS_SIGN_SURROGATE
+        return self
+
     def with_shared_signs(self, on: bool = True) -> "MeshToVolume":
         """One winding-number evaluation per group of voxels that provably
         share a sign (closed meshes only); off = the reference's per-voxel
@@ -578,3 +621,108 @@ class VoxelRemesher:
             return np.ctypeslib.as_array(out, shape=(n.value * 9,)).copy().reshape(-1, 3, 3)
         finally:
             L.bs_free(C.cast(out, C.c_void_p))
+
+
+class TileClassifier:
+    """The tile classifier of BS_SIGN_SURROGATE (``bs_mlp``): 3 -> 64 -> 64 ->
+    64 -> 1 ReLU network, BF16 weights and activations, FP32 accumulation,
+    trained on the device.  Predicts a 4^3 tile's smallest |SDF| in voxels
+    from its centre.  No counterpart in the reference."""
+
+    N_PARAMS = _lib.BS_MLP_PARAMS
+
+    def __init__(self, ctx: Optional[Context] = None, seed: int = 0):
+        self.ctx = ctx or default_context()
+        h = C.c_void_p()
+        check(load().bs_mlp_create(self.ctx.handle, int(seed), C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            load().bs_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.N_PARAMS, np.float32)
+        check(load().bs_mlp_get_params(self.ctx.handle, self._h, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    @params.setter
+    def params(self, p) -> None:
+        a = np.ascontiguousarray(np.asarray(p, np.float32).reshape(-1))
+        assert a.size == self.N_PARAMS
+        check(load().bs_mlp_set_params(self.ctx.handle, self._h, a.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def set_input_map(self, shift, scale: float) -> "TileClassifier":
+        check(load().bs_mlp_set_input_map(self._h, float(shift[0]), float(shift[1]), float(shift[2]), float(scale)))
+        return self
+
+    @staticmethod
+    def _dev(t):
+        import torch
+        t = t.detach().to(dtype=torch.float32).contiguous()
+        assert t.is_cuda
+        return t
+
+    def forward(self, x):
+        """Predictions for the (n, 3) device tensor x (a device tensor back)."""
+        import torch
+        x = self._dev(x).reshape(-1, 3)
+        out = torch.empty(x.shape[0], dtype=torch.float32, device=x.device)
+        torch.cuda.current_stream(x.device).synchronize()
+        check(load().bs_mlp_forward_device(self.ctx.handle, self._h, C.c_void_p(x.data_ptr()), x.shape[0],
+                                           C.c_void_p(out.data_ptr())))
+        return out
+
+    def gradient(self, x, y):
+        """(gradient of mean((net(x) - y)^2) as BS_MLP_PARAMS floats, that loss)."""
+        import torch
+        x = self._dev(x).reshape(-1, 3)
+        y = self._dev(y).reshape(-1)
+        g = np.zeros(self.N_PARAMS, np.float32)
+        loss = C.c_double(0.0)
+        torch.cuda.current_stream(x.device).synchronize()
+        check(load().bs_mlp_gradient_device(self.ctx.handle, self._h, C.c_void_p(x.data_ptr()),
+                                            C.c_void_p(y.data_ptr()), x.shape[0],
+                                            g.ctypes.data_as(C.POINTER(C.c_float)), C.byref(loss)))
+        return g, loss.value
+
+    def train(self, x, y, epochs: int = 20, batch: int = 8192, lr: float = 3e-3, lr_final: float = 1e-4,
+              seed: int = 0) -> np.ndarray:
+        """Mini-batch Adam on the squared error; returns each epoch's mean loss."""
+        import torch
+        x = self._dev(x).reshape(-1, 3)
+        y = self._dev(y).reshape(-1)
+        tp = _lib.MlpTrainParams(epochs, batch, lr, lr_final, 0.9, 0.999, 1e-8, seed)
+        losses = np.zeros(epochs, np.float32)
+        torch.cuda.current_stream(x.device).synchronize()
+        check(load().bs_mlp_train_device(self.ctx.handle, self._h, C.c_void_p(x.data_ptr()),
+                                         C.c_void_p(y.data_ptr()), x.shape[0], C.byref(tp),
+                                         losses.ctypes.data_as(C.POINTER(C.c_float))))
+        return losses
+
+
+def tile_samples(volume: "Volume"):
+    """(centres (n, 3), smallest |value| / voxel size (n,)) of the tiles of a
+    volume that hold an active voxel, as device tensors."""
+    import torch
+    n_cap = 8 * volume.info()["n_leaves"]
+    x = torch.empty((max(n_cap, 1), 3), dtype=torch.float32, device="cuda")
+    y = torch.empty(max(n_cap, 1), dtype=torch.float32, device="cuda")
+    n = C.c_size_t(0)
+    torch.cuda.synchronize()
+    check(load().bs_volume_tile_samples(volume._ctx.handle, volume._h, C.c_void_p(x.data_ptr()),
+                                        C.c_void_p(y.data_ptr()), n_cap, C.byref(n)))
+    keep = y[:n.value] >= 0
+    return x[:n.value][keep], y[:n.value][keep]

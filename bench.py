@@ -61,6 +61,44 @@ def winding_flop(stats):
         77.0 * stats["wn_leaf_tris"]
 
 
+def surrogate_leg(ctx, n_train=32, n_test=8, h=0.002, band=3):
+    """The learned tile gate on the headline family: a 3-64-64-64-1 network
+    (BF16 weights, FP32 accumulation, trained on the GPU by the library's own
+    kernels) learns a 4^3 tile's smallest |SDF| from 2M tile samples of
+    n_train seeded displaced icospheres (8 subdivisions), then gates the sign
+    work on n_test unseen seeds.  Reported beside the per-voxel and shared-sign
+    paths and the same gate fed each tile's true smallest |SDF| (a perfect
+    classifier); every volume is compared with the per-voxel one."""
+    import numpy as np
+    from baby_shark_amd import meshgen as mg
+    from baby_shark_amd.surrogate import compare_sign_modes, train_tile_classifier
+    t0 = time.perf_counter()
+    train = mg.displaced_icospheres(8, list(range(100, 100 + n_train)))
+    tests = mg.displaced_icospheres(8, list(range(200, 200 + n_test)))
+    gen_s = time.perf_counter() - t0
+    net, rep = train_tile_classifier(ctx, train, h, band, samples=2_000_000, epochs=12, seed=3)
+    del train
+    thr = 2.0 + max(rep["over_prediction_q999_vox"], 0.0)
+    rows = compare_sign_modes(ctx, tests, h, band, net, thr)
+    net.close()
+    out = {"workload": f"displaced icospheres L8, voxel {h}, band {band}: trained on {n_train} seeds, "
+                       f"tested on {n_test} unseen seeds", "mesh_generation_s": round(gen_s, 1),
+           "classifier": rep, "threshold_vox": round(thr, 4)}
+    for name in ("per_voxel", "shared", "surrogate", "exact_gate"):
+        r = {"ms_mean": round(float(np.mean([x[name]["ms"] for x in rows])), 3),
+             "evaluated_fraction": round(float(np.mean([x[name]["evals"] / x[name]["active"] for x in rows])), 5),
+             "speedup_vs_per_voxel": round(float(np.mean([x["per_voxel"]["ms"] / x[name]["ms"] for x in rows])), 3),
+             "speedup_vs_shared": round(float(np.mean([x["shared"]["ms"] / x[name]["ms"] for x in rows])), 3)}
+        if name != "per_voxel":
+            r["values_differing"] = int(sum(x[name]["values_differing"] for x in rows))
+            r["sign_mismatches"] = int(sum(x[name]["sign_mismatches"] for x in rows))
+        if name in ("surrogate", "exact_gate"):
+            for k in ("tiles", "culled", "misclassified", "fallback_voxels"):
+                r[k] = int(sum(x[name][k] for x in rows))
+        out[name] = r
+    return out
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -400,6 +438,7 @@ def main():
     ap.add_argument("--no-pipeline", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-shared", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--traffic-probe", action="store_true")
     args = ap.parse_args()
     if args.traffic_probe:
@@ -501,6 +540,8 @@ def main():
         out["remesh"] = remesh(ctx)
     if not args.no_batch:
         out["batch"] = batch(ctx)
+    if not args.no_surrogate:
+        out["surrogate"] = surrogate_leg(ctx)
     if not args.no_cpu:
         v = m2v.convert(dev)
         ge = v.export()

@@ -61,6 +61,15 @@ typedef struct {
  * surface) the group shares one sign.  Not for open meshes. */
 #define BS_SIGN_PER_VOXEL 0
 #define BS_SIGN_SHARED 1
+/* SURROGATE (no counterpart in the reference; closed meshes only): each leaf's
+ * eight 4^3 tiles are gated by the tile classifier set with
+ * bs_ctx_set_surrogate.  A tile it culls (predicted smallest |SDF| at or above
+ * the threshold) takes one winding-number evaluation at its voxel of largest
+ * |d|, r; each of its voxels v with |d_v| > |v - r| (plus a rounding margin)
+ * shares r's sign -- no surface point lies in the ball of radius |d_v| about
+ * v, which holds r -- and every other voxel, culled tile or not, is
+ * evaluated itself.  The exact distances thus judge every culled tile. */
+#define BS_SIGN_SURROGATE 2
 
 typedef struct {
     float voxel_size;
@@ -80,6 +89,10 @@ typedef struct {
     double ms_total, ms_subdivide, ms_leaves, ms_distance, ms_bvh, ms_winding;
     uint64_t n_sign_evals; /* voxels whose winding number was evaluated          */
     double ms_sign_share;  /* sign groups, representatives and sign copy (SHARED) */
+    /* SURROGATE: 4^3 tiles holding an active voxel, tiles culled, culled tiles
+     * in which some voxel failed the exact check (misclassified), and the
+     * voxels that failed it (evaluated themselves) */
+    uint64_t n_sg_tiles, n_sg_culled, n_sg_misclassified, n_sg_fallback;
 } bs_m2v_stats;
 
 /* Work counters of the last bs_volume_offset call. */
@@ -305,6 +318,49 @@ bs_status bs_ctx_last_mc_stats(const bs_ctx* ctx, bs_dc_stats* out);
 #define BS_MESHING_MANIFOLD 1
 bs_status bs_voxel_remesh(bs_ctx* ctx, const float* tri_xyz, size_t n, int device_input, float voxel_size,
                           int method, float** out_xyz, size_t* n_tris);
+
+/* Tile classifier of BS_SIGN_SURROGATE (no counterpart in the reference): a
+ * 3 -> 64 -> 64 -> 64 -> 1 ReLU network predicting a 4^3 tile's smallest
+ * |SDF| in voxels from the tile centre.  BF16 weights and activations, FP32
+ * accumulation, FP32 master weights trained with Adam on the device.
+ * Parameters (FP32, this order): W1[64][3], b1[64], W2[64][64], b2[64],
+ * W3[64][64], b3[64], W4[64], b4; W[out][in].  The network input is
+ * (point - shift) * scale. */
+typedef struct bs_mlp bs_mlp;
+#define BS_MLP_PARAMS 8641
+typedef struct {
+    uint32_t epochs, batch;
+    float lr, lr_final;     /* cosine decay from lr to lr_final over all steps */
+    float beta1, beta2, eps;
+    uint64_t seed;          /* sample order of each epoch                      */
+} bs_mlp_train_params;
+bs_status bs_mlp_create(bs_ctx* ctx, uint64_t seed, bs_mlp** out); /* He-uniform weights, zero biases */
+void bs_mlp_destroy(bs_mlp* m);
+bs_status bs_mlp_set_params(bs_ctx* ctx, bs_mlp* m, const float* params); /* host, resets Adam */
+bs_status bs_mlp_get_params(bs_ctx* ctx, const bs_mlp* m, float* params);
+bs_status bs_mlp_set_input_map(bs_mlp* m, float shift_x, float shift_y, float shift_z, float scale);
+/* out_dev[i] = network(x_dev[3 i .. 3 i + 2]), device arrays */
+bs_status bs_mlp_forward_device(bs_ctx* ctx, const bs_mlp* m, const float* x_dev, size_t n, float* out_dev);
+/* gradient of mean((network(x) - y)^2) over the n samples (grad: host,
+ * BS_MLP_PARAMS floats) and that loss */
+bs_status bs_mlp_gradient_device(bs_ctx* ctx, bs_mlp* m, const float* x_dev, const float* y_dev, size_t n,
+                                 float* grad, double* loss);
+/* mini-batch Adam on the squared error; epoch_loss (host, epochs floats, may
+ * be NULL) receives each epoch's mean loss */
+bs_status bs_mlp_train_device(bs_ctx* ctx, bs_mlp* m, const float* x_dev, const float* y_dev, size_t n,
+                              const bs_mlp_train_params* params, float* epoch_loss);
+/* Training samples from a volume: for the eight 4^3 tiles of each leaf
+ * (leaf-major, tile = x-half << 2 | y-half << 1 | z-half) the tile centre
+ * (x_dev, 3 floats) and the smallest |value| / voxel size over its active
+ * voxels (y_dev; -1 for a tile without one).  *n_out = 8 * leaves; capacity
+ * below that gives BS_ERR_CAPACITY. */
+bs_status bs_volume_tile_samples(bs_ctx* ctx, const bs_volume* v, float* x_dev, float* y_dev, size_t capacity,
+                                 size_t* n_out);
+/* The classifier and cull threshold (voxels) of BS_SIGN_SURROGATE on this
+ * context.  The network is not owned and must outlive its use.  With m NULL
+ * a tile is culled when its true smallest |SDF| is at or above the threshold
+ * (a perfect classifier, for measurement). */
+bs_status bs_ctx_set_surrogate(bs_ctx* ctx, const bs_mlp* m, float threshold_vox);
 
 /* Binary STL (io/stl.rs), host only, no device needed.  bs_stl_read is
  * StlReader (:64-92): *out_xyz receives 9 floats per face (malloc'd, release

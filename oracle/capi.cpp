@@ -579,11 +579,15 @@ int or_marching_cubes(const or_volume* a, float h, float** out, size_t* n_tris) 
     return 0;
 }
 
-// The triangles of one cube as crossing-edge triples (12 = centre), for the
-// per-cube checks: c = 8 clamped corner values; returns the triangle count.
+// The triangles of one cube as crossing-edge triples (12 = centre) in the
+// emitted vertex order, for the per-cube checks: c = 8 clamped corner values;
+// returns the triangle count.
 int or_mc_cube_tiling(const float* c, int* out_edges) {
     std::vector<mc::Tri> t;
-    mc::cube_tiling(c, t);
+    int id = 0;
+    for (int i = 0; i < 8; ++i)
+        if (c[i] < 0.0f) id |= 1 << i;
+    mc::handle_cube(c, id, t);
     for (size_t i = 0; i < t.size(); ++i)
         for (int k = 0; k < 3; ++k) out_edges[3 * i + k] = t[i][k];
     return (int)t.size();

@@ -3,27 +3,14 @@
 //
 // Restatement of MarchingCubesMesher (src/voxel/meshing/marching_cubes.rs),
 // the corrected Marching Cubes 33 that the reference drives with Lewiner et
-// al.'s case tables (src/voxel/meshing/lookup_table.rs).  The tables are not
-// retyped.  The surface in a cube is derived from the cube itself:
-//   * every face with two diagonal corners of each sign is decided with the
-//     reference's face test (test_face, marching_cubes.rs:323-342), called
-//     with the sign the tables give every tested face (+ when at most four
-//     corners are negative, else -) and read as the tables read it;
-//   * the contour segments on the six faces chain into closed loops;
-//   * the interior tests of cases 4, 6, 7, 10, 12 and 13 (test_interior,
-//     interior_ambiguity, interior_ambiguity_verification and
-//     interior_test_case13, marching_cubes.rs:253-917) decide whether two loops
-//     bound one tube;
-//   * a loop of eight or more crossings is fanned around the centre vertex
-//     (compute_c_vertex, :919-940), as the tables do in cases 7.3, 10.2, 12.2,
-//     13.3 and 13.4; the tube of case 6.1.2 holds the centre vertex too.
-// Two behaviours of the reference's table indexing are kept: case 13.5.2
-// emits the tiling of the complementary configuration (TILING_13_5_2 indexed
-// by the other config, :228-238), and cases 10.1.2 and 12.1.2 reached with both
-// face tests true emit their tube reversed (index 5 - config and 23 - config,
-// :161,183).  The result has the reference's crossings, topology and
-// orientation per cube; which diagonal splits a polygon and the order of the
-// triangles within a cube are this restatement's own.
+// al.'s case tables (src/voxel/meshing/lookup_table.rs, here as the data of
+// mc_tables.inc, generated from that file): handle_cube's case selection
+// (:72-243) with test_face (:323-342), test_interior and its helpers
+// (:404-917) and interior_test_case13 (:269-321); add_faces' vertex order
+// (:245-272, entries i, i + 2, i + 1; a triangle with an edge that has no
+// crossing is skipped, a degenerate one dropped); the centre vertex as mesher
+// state set only by compute_c_vertex (:919-940), which case 6.1.2 does not
+// call; the serial visit of leaves and tile faces (:957-1000).
 #pragma once
 #include <array>
 #include <cmath>
@@ -48,9 +35,6 @@ constexpr int EDGE_AXIS[12] = {0, 1, 0, 1, 0, 1, 0, 1, 2, 2, 2, 2};
 // test_face corner order a, b, c, d of faces 1..6 (marching_cubes.rs:327-334)
 constexpr int FACE_Q[7][4] = {{0, 0, 0, 0}, {0, 4, 5, 1}, {1, 5, 6, 2}, {2, 6, 7, 3},
                               {3, 7, 4, 0}, {0, 3, 2, 1}, {4, 7, 6, 5}};
-// the same faces, corners counter-clockwise seen from outside the cube
-constexpr int FACE_CCW[7][4] = {{0, 0, 0, 0}, {1, 5, 4, 0}, {2, 6, 5, 1}, {3, 7, 6, 2},
-                                {0, 4, 7, 3}, {0, 3, 2, 1}, {5, 6, 7, 4}};
 // interior_ambiguity (:474-530): faces 1|3, 2|4 and 5|6 share a list of body
 // diagonals (corner, corner, edge named), tested in this order, the last match wins
 constexpr int FAMILY[7] = {2, 0, 1, 0, 1, 2, 2};
@@ -65,12 +49,6 @@ constexpr int VER[12][4][2] = {
     {{1, 0}, {2, 3}, {6, 7}, {5, 4}}, {{2, 1}, {3, 0}, {7, 4}, {6, 5}}, {{3, 0}, {2, 1}, {6, 5}, {7, 4}},
     {{0, 3}, {4, 7}, {5, 6}, {1, 2}}, {{1, 2}, {0, 3}, {4, 7}, {5, 6}}, {{4, 0}, {7, 3}, {6, 2}, {5, 1}},
     {{5, 1}, {4, 0}, {7, 3}, {6, 2}}, {{6, 2}, {5, 1}, {4, 0}, {7, 3}}, {{7, 3}, {6, 2}, {5, 1}, {4, 0}}};
-
-inline int edge_of(int a, int b) {
-    for (int e = 0; e < 12; ++e)
-        if ((EDGE_V[e][0] == a && EDGE_V[e][1] == b) || (EDGE_V[e][0] == b && EDGE_V[e][1] == a)) return e;
-    std::abort();
-}
 
 // marching_cubes.rs:323-342
 inline bool test_face(const float* c, int face) {
@@ -131,198 +109,131 @@ inline bool interior13(const float* c) {
 }
 
 using Tri = std::array<int, 3>;
-using Loop = std::vector<int>;
 
-// Contour loops over the cube faces.  Faces are walked counter-clockwise from
-// outside; a segment runs from a positive-to-negative crossing to a
-// negative-to-positive one, so a loop's direction makes its fan face the
-// positive side, as the reference's triangles do.
-inline std::vector<Loop> face_loops(const bool* neg, const bool* negconn) {
-    int succ[12];
-    for (int& s : succ) s = -1;
-    for (int f = 1; f <= 6; ++f) {
-        const int* q = FACE_CCW[f];
-        bool sg[4];
-        int e[4], nx = 0;
-        for (int i = 0; i < 4; ++i) {
-            sg[i] = neg[q[i]];
-            e[i] = edge_of(q[i], q[(i + 1) & 3]);
-        }
-        for (int i = 0; i < 4; ++i) nx += sg[i] != sg[(i + 1) & 3];
-        if (nx == 4) {
-            for (int i = 0; i < 4; ++i) {
-                if (sg[i] == negconn[f]) continue;  // corners of the joined sign are not cut off
-                const int ein = e[(i + 3) & 3], eout = e[i];
-                if (sg[i])
-                    succ[ein] = eout;
-                else
-                    succ[eout] = ein;
-            }
-        } else if (nx == 2) {
-            int pn = -1, np = -1;
-            for (int i = 0; i < 4; ++i) {
-                if (!sg[i] && sg[(i + 1) & 3]) pn = e[i];
-                if (sg[i] && !sg[(i + 1) & 3]) np = e[i];
-            }
-            succ[pn] = np;
-        }
-    }
-    std::vector<Loop> loops;
-    bool seen[12] = {};
-    for (int s = 0; s < 12; ++s) {
-        if (succ[s] < 0 || seen[s]) continue;
-        Loop l;
-        for (int x = s; !seen[x]; x = succ[x]) {
-            seen[x] = true;
-            l.push_back(x);
-        }
-        loops.push_back(l);
-    }
-    return loops;
-}
+#define MC_TABLE static const
+#include "mc_tables.inc"
+#undef MC_TABLE
 
-inline void fan(const Loop& l, std::vector<Tri>& out) {
-    for (size_t i = 1; i + 1 < l.size(); ++i) out.push_back({l[0], l[i], l[i + 1]});
-}
-inline void centre_fan(const Loop& l, std::vector<Tri>& out) {
-    for (size_t i = 0; i < l.size(); ++i) out.push_back({l[i], l[(i + 1) % l.size()], 12});
-}
-// A tube between two loops, both in their boundary direction: A walked
-// forward, B backward, each step closing one edge of one loop.  With `centre`
-// the rung between the first vertices is split through the centre vertex.
-inline void tube(const Loop& A, const Loop& B, bool centre, bool reversed, std::vector<Tri>& out) {
-    const size_t n1 = A.size(), n2 = B.size();
-    std::vector<int> b(B.rbegin(), B.rend());
-    std::vector<Tri> t;
-    size_t i = 0, k = 0;
-    while (i < n1 || k < n2) {
-        if (k >= n2 || (i < n1 && (i + 1) * n2 <= (k + 1) * n1)) {
-            t.push_back({A[i % n1], A[(i + 1) % n1], b[k % n2]});
-            ++i;
-        } else {
-            t.push_back({b[(k + 1) % n2], b[k % n2], A[i % n1]});
-            ++k;
-        }
-    }
-    for (Tri tr : t) {
-        if (centre) {
-            int ia = -1, ib = -1;
-            for (int j = 0; j < 3; ++j) {
-                if (tr[j] == A[0]) ia = j;
-                if (tr[j] == b[0]) ib = j;
-            }
-            if (ia >= 0 && ib >= 0) {
-                Tri t1 = tr, t2 = tr;
-                t1[ib] = 12;
-                t2[ia] = 12;
-                out.push_back(reversed ? Tri{t1[0], t1[2], t1[1]} : t1);
-                out.push_back(reversed ? Tri{t2[0], t2[2], t2[1]} : t2);
-                continue;
-            }
-        }
-        out.push_back(reversed ? Tri{tr[0], tr[2], tr[1]} : tr);
+// test_interior (marching_cubes.rs:404-468): `face` carries the sign of the
+// corners whose body diagonal is followed
+inline bool test_interior(const float* c, int cas, int config, int face) {
+    auto ver = [&](int amb_face, int f) { return verify(c, amb_edge(c, FAMILY[amb_face], (float)f)); };
+    switch (cas) {
+        case 4: return ver(1, face) + ver(2, face) + ver(5, face) != 0;
+        case 6: return ver(std::abs(MC_TEST_6[config][0]), face) != 0;
+        case 7: return ver(1, -face) + ver(2, -face) + ver(5, -face) != 0;
+        case 10: return ver(std::abs(MC_TEST_10[config][0]), face) != 0;
+        case 12:
+            return ver(std::abs(MC_TEST_12[config][0]), face) + ver(std::abs(MC_TEST_12[config][1]), face) != 0;
+        default: std::abort();
     }
 }
 
-inline int loop_corner(const Loop& l) {
-    for (int v = 0; v < 8; ++v) {
-        bool all = true;
-        for (int e : l) all = all && (EDGE_V[e][0] == v || EDGE_V[e][1] == v);
-        if (all) return v;
-    }
-    return -1;
+// handle_cube (marching_cubes.rs:72-243): the tiling the tables select for a
+// cube of corner values c and configuration id, appended as triangles in the
+// emitted vertex order (add_faces, :245-272, reads entries i, i + 2, i + 1).
+// Returns true where the reference computes the centre vertex first
+// (compute_c_vertex): 7.3, 10.2, 12.2, 13.3 and 13.4.  Case 6.1.2 also uses
+// vertex 12 but does not compute it, so it reads the centre left by an
+// earlier cube.
+template <size_t N>
+inline void add_faces(const int8_t (&t)[N], std::vector<Tri>& out) {
+    for (size_t i = 0; i + 2 < N; i += 3) out.push_back({t[i], t[i + 2], t[i + 1]});
 }
-
-// The triangles of one cube as crossing-edge triples (12 = centre vertex), in
-// the orientation of the reference's emitted vertex order.  c: the clamped
-// corner values (Cube::from_voxel, :1161-1182).
-inline void cube_tiling(const float* c, std::vector<Tri>& out) {
-    bool neg[8];
-    int pc = 0;
-    for (int i = 0; i < 8; ++i) {
-        neg[i] = c[i] < 0.0f;
-        pc += neg[i];
-    }
-    if (pc == 0 || pc == 8) return;
-    const float s0 = pc <= 4 ? 7.0f : -7.0f;
-    bool negconn[7] = {};
-    int amb_faces[6], n_amb = 0, n_conn = 0;
-    for (int f = 1; f <= 6; ++f) {
-        const int* q = FACE_CCW[f];
-        if (neg[q[0]] != neg[q[1]] && neg[q[1]] != neg[q[2]] && neg[q[2]] != neg[q[3]]) {
-            const bool r = test_face(c, s0 > 0.0f ? f : -f);
-            negconn[f] = s0 > 0.0f ? r : !r;
-            amb_faces[n_amb++] = f;
-            n_conn += negconn[f];
-        }
-    }
-    std::vector<Loop> loops = face_loops(neg, negconn);
-    // minority corners (the negative ones when there are at most four)
-    int k = 0, edges_in = 0;
-    bool minor[8];
-    for (int i = 0; i < 8; ++i) {
-        minor[i] = neg[i] == (pc <= 4);
-        k += minor[i];
-    }
-    for (int e = 0; e < 12; ++e) edges_in += minor[EDGE_V[e][0]] && minor[EDGE_V[e][1]];
-
-    int ta = -1, tb = -1;  // loops joined by a tube
-    bool centre = false, reversed = false;
-    if (k == 2 && n_amb == 0 && loops.size() == 2) {  // case 4
-        const int sum = verify(c, amb_edge(c, 0, s0)) + verify(c, amb_edge(c, 1, s0)) + verify(c, amb_edge(c, 2, s0));
-        if (sum == 0) ta = 0, tb = 1;
-    } else if (k == 3 && edges_in == 1 && loops.size() == 2) {  // case 6.1
-        if (verify(c, amb_edge(c, FAMILY[amb_faces[0]], s0)) == 0) ta = 0, tb = 1, centre = true;
-    } else if (k == 3 && edges_in == 0 && n_conn == (pc <= 4 ? 3 : 0)) {  // case 7.4: all three faces join
-        const float s = -s0;
-        const int sum = verify(c, amb_edge(c, 0, s)) + verify(c, amb_edge(c, 1, s)) + verify(c, amb_edge(c, 2, s));
-        if (sum == 0) ta = 0, tb = 1;
-    } else if (k == 4 && edges_in == 2 && loops.size() == 2) {  // cases 10.1 and 12.1
-        const bool both = negconn[amb_faces[0]];
-        const float arg = both ? -7.0f : 7.0f;
-        int sum = verify(c, amb_edge(c, FAMILY[amb_faces[0]], arg));
-        if (FAMILY[amb_faces[0]] != FAMILY[amb_faces[1]]) sum += verify(c, amb_edge(c, FAMILY[amb_faces[1]], arg));
-        if (sum == 0) ta = 0, tb = 1, reversed = both;
-    } else if (k == 4 && edges_in == 0 && loops.size() == 3 && n_conn == 3) {  // case 13.5
-        if (!interior13(c)) {
-            // the complementary configuration's tiling: faces 1..4 of the
-            // pattern turned a quarter, signs flipped (TEST_13 face orders)
-            const int id = [&] {
-                int v = 0;
-                for (int i = 0; i < 8; ++i) v |= neg[i] << i;
-                return v;
-            }();
-            bool rp[7] = {};
-            if (id == 165) {
-                rp[2] = negconn[1], rp[3] = negconn[2], rp[4] = negconn[3], rp[1] = negconn[4];
-            } else {
-                rp[1] = negconn[2], rp[2] = negconn[3], rp[3] = negconn[4], rp[4] = negconn[1];
+inline bool handle_cube(const float* c, int id, std::vector<Tri>& out) {
+    const int cas = MC_CASES[id][0], cfg = MC_CASES[id][1];
+    auto tf = [&](int f) { return test_face(c, f); };
+    auto ti = [&](int f) { return test_interior(c, cas, cfg, f); };
+    switch (cas) {
+        case 0: return false;
+        case 1: add_faces(MC_TILING_1[cfg], out); return false;
+        case 2: add_faces(MC_TILING_2[cfg], out); return false;
+        case 3:
+            if (tf(MC_TEST_3[cfg])) add_faces(MC_TILING_3_2[cfg], out);
+            else add_faces(MC_TILING_3_1[cfg], out);
+            return false;
+        case 4:
+            if (ti(MC_TEST_4[cfg])) add_faces(MC_TILING_4_1[cfg], out);
+            else add_faces(MC_TILING_4_2[cfg], out);
+            return false;
+        case 5: add_faces(MC_TILING_5[cfg], out); return false;
+        case 6:
+            if (tf(MC_TEST_6[cfg][0])) add_faces(MC_TILING_6_2[cfg], out);
+            else if (ti(MC_TEST_6[cfg][1])) add_faces(MC_TILING_6_1_1[cfg], out);
+            else add_faces(MC_TILING_6_1_2[cfg], out);
+            return false;
+        case 7: {
+            const int sub = (int)tf(MC_TEST_7[cfg][0]) + 2 * (int)tf(MC_TEST_7[cfg][1]) + 4 * (int)tf(MC_TEST_7[cfg][2]);
+            switch (sub) {
+                case 0: add_faces(MC_TILING_7_1[cfg], out); return false;
+                case 1: add_faces(MC_TILING_7_2[cfg][0], out); return false;
+                case 2: add_faces(MC_TILING_7_2[cfg][1], out); return false;
+                case 3: add_faces(MC_TILING_7_3[cfg][0], out); return true;
+                case 4: add_faces(MC_TILING_7_2[cfg][2], out); return false;
+                case 5: add_faces(MC_TILING_7_3[cfg][1], out); return true;
+                case 6: add_faces(MC_TILING_7_3[cfg][2], out); return true;
+                default:
+                    if (ti(MC_TEST_7[cfg][3])) add_faces(MC_TILING_7_4_1[cfg], out);
+                    else add_faces(MC_TILING_7_4_2[cfg], out);
+                    return false;
             }
-            rp[5] = negconn[5], rp[6] = negconn[6];
-            bool np[8];
-            for (int i = 0; i < 8; ++i) np[i] = !neg[i];
-            std::vector<Loop> L = face_loops(np, rp);
-            int six = -1, joined = -1, alone = -1;
-            for (int j = 0; j < (int)L.size(); ++j) {
-                if (L[j].size() == 6)
-                    six = j;
-                else if (np[loop_corner(L[j])])
-                    joined = j;
-                else
-                    alone = j;
-            }
-            tube(L[six], L[joined], false, false, out);
-            fan(L[alone], out);
-            return;
         }
-    }
-    if (ta >= 0) tube(loops[ta], loops[tb], centre, reversed, out);
-    for (int j = 0; j < (int)loops.size(); ++j) {
-        if (j == ta || j == tb) continue;
-        if (loops[j].size() >= 8)
-            centre_fan(loops[j], out);
-        else
-            fan(loops[j], out);
+        case 8: add_faces(MC_TILING_8[cfg], out); return false;
+        case 9: add_faces(MC_TILING_9[cfg], out); return false;
+        case 10:
+            if (tf(MC_TEST_10[cfg][0])) {
+                if (tf(MC_TEST_10[cfg][1])) {
+                    if (ti(-MC_TEST_10[cfg][2])) add_faces(MC_TILING_10_1_1_[cfg], out);
+                    else add_faces(MC_TILING_10_1_2[5 - cfg], out);
+                    return false;
+                }
+                add_faces(MC_TILING_10_2[cfg], out);
+                return true;
+            }
+            if (tf(MC_TEST_10[cfg][1])) {
+                add_faces(MC_TILING_10_2_[cfg], out);
+                return true;
+            }
+            if (ti(MC_TEST_10[cfg][2])) add_faces(MC_TILING_10_1_1[cfg], out);
+            else add_faces(MC_TILING_10_1_2[cfg], out);
+            return false;
+        case 11: add_faces(MC_TILING_11[cfg], out); return false;
+        case 12:
+            if (tf(MC_TEST_12[cfg][0])) {
+                if (tf(MC_TEST_12[cfg][1])) {
+                    if (ti(-MC_TEST_12[cfg][2])) add_faces(MC_TILING_12_1_1_[cfg], out);
+                    else add_faces(MC_TILING_12_1_2[23 - cfg], out);
+                    return false;
+                }
+                add_faces(MC_TILING_12_2[cfg], out);
+                return true;
+            }
+            if (tf(MC_TEST_12[cfg][1])) {
+                add_faces(MC_TILING_12_2_[cfg], out);
+                return true;
+            }
+            if (ti(MC_TEST_12[cfg][2])) add_faces(MC_TILING_12_1_1[cfg], out);
+            else add_faces(MC_TILING_12_1_2[cfg], out);
+            return false;
+        case 13: {
+            int sub = 0;
+            for (int k = 0; k < 6; ++k)
+                if (tf(MC_TEST_13[cfg][k])) sub += 1 << k;
+            const int o = MC_SUB_CONFIG_13[sub];
+            if (o == 0) add_faces(MC_TILING_13_1[cfg], out);
+            else if (o <= 6) add_faces(MC_TILING_13_2[cfg][o - 1], out);
+            else if (o <= 18) { add_faces(MC_TILING_13_3[cfg][o - 7], out); return true; }
+            else if (o <= 22) { add_faces(MC_TILING_13_4[cfg][o - 19], out); return true; }
+            else if (o <= 26) {
+                if (interior13(c)) add_faces(MC_TILING_13_5_1[cfg][o - 23], out);
+                else add_faces(MC_TILING_13_5_2[1 - cfg][o - 23], out);
+            } else if (o <= 38) { add_faces(MC_TILING_13_3_[cfg][o - 27], out); return true; }
+            else if (o <= 44) add_faces(MC_TILING_13_2_[cfg][o - 39], out);
+            else if (o == 45) add_faces(MC_TILING_13_1_[cfg], out);
+            return false;
+        }
+        case 14: add_faces(MC_TILING_14[cfg], out); return false;
+        default: std::abort();
     }
 }
 
@@ -334,23 +245,29 @@ template <class Grid>
 std::vector<Vec3f> mesh(const Grid& grid, float voxel_size) {
     std::vector<Vec3f> verts;
     std::vector<Tri> tris;
+    // the centre vertex is mesher state (MarchingCubesMesher::v12, zero in a
+    // new mesher, :941-955) that only compute_c_vertex sets: case 6.1.2 reads
+    // the one an earlier cube left
+    Vec3f v12{0.0f, 0.0f, 0.0f};
     auto handle = [&](int64_t x, int64_t y, int64_t z) {
         float c[8];
         Vec3i idx[8];
-        for (int i = 0; i < 8; ++i) {
+        int id = 0;
+        for (int i = 0; i < 8; ++i) {  // Cube::from_voxel (:1161-1182)
             idx[i] = {x + CORNER[i][0], y + CORNER[i][1], z + CORNER[i][2]};
             const float* p = grid.at(idx[i]);
             if (!p) return;
             float v = *p;
             if (std::fabs(v) < MIN_ABS) v = std::copysign(MIN_ABS, v);
             c[i] = v;
+            if (v < 0.0f) id |= 1 << i;
         }
         tris.clear();
-        cube_tiling(c, tris);
+        const bool need_c = handle_cube(c, id, tris);
         if (tris.empty()) return;
         // ComputeEdgeIntersections (:1070-1098): t = |v1| / (|v1| + |v2|) from the lower corner
         Vec3f p[13];
-        bool has[12];
+        bool has[13];
         for (int e = 0; e < 12; ++e) {
             const int a = EDGE_V[e][0], b = EDGE_V[e][1];
             has[e] = (c[a] < 0.0f) != (c[b] < 0.0f);
@@ -361,16 +278,17 @@ std::vector<Vec3f> mesh(const Grid& grid, float voxel_size) {
             q[EDGE_AXIS[e]] = q[EDGE_AXIS[e]] + t;
             p[e] = q;
         }
-        bool need_c = false;
-        for (const Tri& t : tris) need_c = need_c || t[0] == 12 || t[1] == 12 || t[2] == 12;
         if (need_c) {  // compute_c_vertex (:919-940)
             Vec3f s{0.0f, 0.0f, 0.0f};
             int n = 0;
             for (int e = 0; e < 12; ++e)
                 if (has[e]) s = add(s, p[e]), ++n;
-            p[12] = divs(s, (float)n);
+            v12 = divs(s, (float)n);
         }
+        p[12] = v12;
+        has[12] = true;
         for (const Tri& t : tris) {  // add_faces (:245-272)
+            if (!has[t[0]] || !has[t[1]] || !has[t[2]]) continue;  // no intersection: the triangle is skipped
             const Vec3f v1 = mul(p[t[0]], voxel_size), v2 = mul(p[t[1]], voxel_size), v3 = mul(p[t[2]], voxel_size);
             if (norm_sq(cross(sub(v2, v1), sub(v3, v1))) == 0.0f) continue;
             verts.push_back(v1);

@@ -1,5 +1,6 @@
-"""Writes mc_topology.npz: corner values of single cubes and the topology of
-the surface the reference's MarchingCubesMesher emits for each of them
+"""Writes mc_topology.npz: corner values of single cubes, the triangles the
+reference's MarchingCubesMesher emits for each of them (crossing-edge
+triples, 12 = centre vertex, in the emitted vertex order) and their topology
 (src/voxel/meshing/marching_cubes.rs:72-243 with the tables of
 src/voxel/meshing/lookup_table.rs, read here as data).  Run in a container
 that holds the reference at /root/reference; the output is a fixture of
@@ -218,7 +219,7 @@ def signature(tris):
 def main(out=os.path.join(os.path.dirname(os.path.abspath(__file__)), "mc_topology.npz")):
     T = load_tables()
     rng = np.random.default_rng(20261202)
-    vals, sigs, subs, seen = [], [], [], {}
+    vals, sigs, subs, tri_lists, seen = [], [], [], [], {}
     special = [90, 165, 60, 85, 170, 105, 150, 195, 26, 88, 218, 130, 125]
     k = 0
     while len(vals) < 3000 or min(seen.values()) < 40:
@@ -239,9 +240,15 @@ def main(out=os.path.join(os.path.dirname(os.path.abspath(__file__)), "mc_topolo
         vals.append(v)
         sigs.append(signature(tris))
         subs.append(sub)
+        tri_lists.append(tris)
         if k > 2000000:
             sys.exit("a subcase was not reached")
-    np.savez_compressed(out, values=np.array(vals, np.float32), signature=np.array(sigs), subcase=np.array(subs))
+    tri = np.full((len(vals), 12, 3), -1, np.int8)
+    for i, ts in enumerate(tri_lists):
+        if ts:
+            tri[i, :len(ts)] = ts
+    np.savez_compressed(out, values=np.array(vals, np.float32), signature=np.array(sigs), subcase=np.array(subs),
+                        triangles=tri, n_triangles=np.array([len(t) for t in tri_lists], np.int8))
     print(len(vals), "cubes,", len(seen), "subcases:", dict(sorted(seen.items())))
 
 

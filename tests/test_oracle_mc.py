@@ -137,3 +137,33 @@ def test_torus_is_closed_oriented_genus1(oracle_lib):
     V, E, F = _closed_oriented(t)
     assert V - E + F == 0
     assert abs(_signed_volume(t) - 2 * np.pi ** 2 * R * r ** 2) < 0.02 * (2 * np.pi ** 2 * R * r ** 2)
+
+
+def test_cube_triangles_match_reference_tables(oracle_lib):
+    """Triangle for triangle, in the emitted vertex order: the oracle's tiling
+    of every fixture cube equals the one the reference's tables select."""
+    f = np.load(os.path.join(GOLDEN, "mc_topology.npz"))
+    bad = {}
+    for v, t, n, sub in zip(f["values"], f["triangles"], f["n_triangles"], f["subcase"]):
+        want = [tuple(int(x) for x in r) for r in t[:n]]
+        if oracle_lib.mc_cube_tiling(v) != want:
+            bad[str(sub)] = bad.get(str(sub), 0) + 1
+    assert not bad, bad
+
+
+def test_planted_triangle_order_mistakes_are_caught(oracle_lib):
+    """The exact check fails when a tiling is emitted with its vertices
+    rotated, its triangles reordered or a quad split along the other diagonal."""
+    f = np.load(os.path.join(GOLDEN, "mc_topology.npz"))
+    i = [k for k, s in enumerate(f["subcase"]) if s == "2"][0]
+    want = [tuple(int(x) for x in r) for r in f["triangles"][i][:f["n_triangles"][i]]]
+    got = oracle_lib.mc_cube_tiling(f["values"][i])
+    assert got == want and len(got) == 2
+    assert [(b, c, a) for a, b, c in got] != want
+    assert got[::-1] != want
+    (a, b, c), (d, e, g) = got
+    quad = [a, b, c, d, e, g]
+    shared = [x for x in (a, b, c) if x in (d, e, g)]
+    other = [x for x in quad if x not in shared]
+    assert len(shared) == 2 and len(set(other)) == 2
+    assert [(other[0], shared[0], other[1]), (other[1], shared[1], other[0])] != want
