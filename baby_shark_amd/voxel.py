@@ -407,9 +407,8 @@ class DualContouringMesher:
 
 class MarchingCubesMesher:
     """meshing/marching_cubes.rs:17-63 (Marching Cubes 33).  Default voxel
-    size 1.0.  Cube by cube the reference's crossings, topology and
-    orientation; the diagonal that splits a polygon and the order inside a
-    cube are this library's own."""
+    size 1.0.  Cube by cube the tiling the reference's tables select, in its
+    vertex order."""
 
     def __init__(self, ctx: Optional[Context] = None):
         self._ctx = ctx

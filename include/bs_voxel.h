@@ -298,8 +298,9 @@ void bs_device_free(void* p_dev);
  * cube whose eight corners have values, leaves and active tiles in the tree's
  * visit order (a tile meets the cubes of its six faces, :970-994).  *out_xyz
  * receives 9 floats per triangle (malloc'd, release with bs_free).  Each cube
- * emits the reference's crossings, topology and orientation; the diagonal that
- * splits a polygon and the order inside a cube are this library's own. */
+ * emits the tiling the reference's tables (lookup_table.rs) select, in its
+ * vertex order; case 6.1.2 uses the centre vertex an earlier cube left, as
+ * the reference does. */
 bs_status bs_marching_cubes(bs_ctx* ctx, const bs_volume* v, float voxel_size, float** out_xyz, size_t* n_tris);
 /* Same, with the soup left in device memory (release with bs_device_free). */
 bs_status bs_marching_cubes_device(bs_ctx* ctx, const bs_volume* v, float voxel_size, float** out_xyz_dev,
