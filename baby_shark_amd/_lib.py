@@ -27,6 +27,7 @@ BS_MESHING_FEATURE_PRESERVING = 0
 BS_MESHING_MANIFOLD = 1
 BS_PATH_FORCE_HASH = 1
 BS_PATH_FORCE_LDS_STACK = 2
+BS_PATH_LBVH = 4
 BS_SIGN_PER_VOXEL = 0
 BS_SIGN_SHARED = 1
 BS_SIGN_SURROGATE = 2
@@ -57,7 +58,8 @@ class M2VStats(C.Structure):
               [(k, C.c_double) for k in ("ms_total", "ms_subdivide", "ms_leaves", "ms_distance", "ms_bvh",
                                          "ms_winding")] + \
               [("n_sign_evals", C.c_uint64), ("ms_sign_share", C.c_double)] + \
-              [(k, C.c_uint64) for k in ("n_sg_tiles", "n_sg_culled", "n_sg_misclassified", "n_sg_fallback")]
+              [(k, C.c_uint64) for k in ("n_sg_tiles", "n_sg_culled", "n_sg_misclassified", "n_sg_fallback",
+                                         "n_bvh_iterations")]
 
 
 class MlpTrainParams(C.Structure):

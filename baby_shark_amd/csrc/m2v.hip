@@ -451,44 +451,58 @@ __global__ __launch_bounds__(DIST_BLOCK) void k_distance(const int32_t* __restri
     const int ox = origins[4 * leaf], oy = origins[4 * leaf + 1], oz = origins[4 * leaf + 2];
     const uint32_t c1 = min(cand_off[leaf + 1], n_cand), c0 = min(cand_off[leaf], c1);
     __syncthreads();
-    for (uint32_t c = c0 + wave; c < c1; c += DIST_BLOCK / 64) {
-        const uint32_t t = __builtin_amdgcn_readfirstlane(cand_list[c]);
-        const DistRec& r = recs[t];
-        const int x0 = max(r.box[0], ox), x1 = min(r.box[3], ox + 7);
-        const int y0 = max(r.box[1], oy), y1 = min(r.box[4], oy + 7);
-        const int z0 = max(r.box[2], oz), z1 = min(r.box[5], oz + 7);
-        const int nz = z1 - z0 + 1, ny = y1 - y0 + 1, nyz = ny * nz;
-        if (x1 < x0 || ny <= 0 || nz <= 0) continue;  // box misses the leaf: never for a listed candidate
-        const int spp = 64 / nyz;  // x-slices per pass
-        const int s = (int)(((uint32_t)lane * kMagic[nyz]) >> 16);
-        const int rr = lane - s * nyz;
-        const int yq = (int)(((uint32_t)rr * kMagic[nz]) >> 16);
-        const int y = y0 + yq, z = z0 + (rr - yq * nz);
-        const int loff = ((y - oy) << 3) | (z - oz);
-        const float py = (float)y * h, pz = (float)z * h;
-        if (s < spp) {
-            if (r.flags == 0) {
-                const float dy = py - r.a[1], dz = pz - r.a[2];
-                const float bu = __builtin_fmaf(r.U[1], dy, r.U[2] * dz);
-                const float bv = __builtin_fmaf(r.V[1], dy, r.V[2] * dz);
-                const float bw = __builtin_fmaf(r.N[1], dy, r.N[2] * dz);
-                for (int x = x0 + s; x <= x1; x += spp) {
-                    const float dx = (float)x * h - r.a[0];
-                    const float pu = __builtin_fmaf(r.U[0], dx, bu);
-                    const float pv = __builtin_fmaf(r.V[0], dx, bv);
-                    const float pw = __builtin_fmaf(r.N[0], dx, bw);
-                    const float d2 = __builtin_fmaf(pw, pw, plane_dist2(r, pu, pv));
-                    const unsigned long long key = ((unsigned long long)__float_as_uint(d2) << 32) | t;
-                    atomicMin(&best[((x - ox) << 6) | loff], key);
-                }
-            } else {
-                const float* v = sub + 9 * (uint64_t)t;
-                for (int x = x0 + s; x <= x1; x += spp) {
-                    const float d = ericson_distance({v[0], v[1], v[2]}, {v[3], v[4], v[5]}, {v[6], v[7], v[8]},
-                                                     {(float)x * h, py, pz});
-                    if (d == d) {  // a NaN distance never enters the field (`dist < cur` is false)
-                        const unsigned long long key = ((unsigned long long)__float_as_uint(d * d) << 32) | t;
+    constexpr uint32_t WAVES = DIST_BLOCK / 64;
+    for (uint32_t cb = c0 + wave; cb < c1; cb += 64 * WAVES) {
+        // the wave's next 64 candidates, one index per lane: one vector load
+        // instead of a dependent load per candidate
+        const uint32_t cl = cb + (uint32_t)lane * WAVES;
+        const int tl = cl < c1 ? (int)cand_list[cl] : 0;
+        const int nk = (int)min(64u, (c1 - cb + WAVES - 1) / WAVES);
+        for (int k = 0; k < nk; ++k) {
+            const uint32_t t = (uint32_t)__builtin_amdgcn_readlane(tl, k);
+            const DistRec r = recs[t];
+            pin_dist_rec(r);  // the whole record in one round of scalar loads
+            const int x0 = max(r.box[0], ox), x1 = min(r.box[3], ox + 7);
+            const int y0 = max(r.box[1], oy), y1 = min(r.box[4], oy + 7);
+            const int z0 = max(r.box[2], oz), z1 = min(r.box[5], oz + 7);
+            const int nz = z1 - z0 + 1, ny = y1 - y0 + 1, nyz = ny * nz;
+            if (x1 < x0 || ny <= 0 || nz <= 0) continue;  // box misses the leaf: never for a listed candidate
+            const uint32_t m_yz = kMagic[nyz], m_z = kMagic[nz];
+            asm volatile("" ::"s"(m_yz), "s"(m_z));  // both table reads in one round trip
+            const int spp = (int)((64u * m_yz) >> 16);  // x-slices per pass, 64 / nyz
+            const int s = (int)(((uint32_t)lane * m_yz) >> 16);
+            const int rr = lane - s * nyz;
+            const int yq = (int)(((uint32_t)rr * m_z) >> 16);
+            const int y = y0 + yq, z = z0 + (rr - yq * nz);
+            const int loff = ((y - oy) << 3) | (z - oz);
+            const float py = (float)y * h, pz = (float)z * h;
+            if (s < spp) {
+                if (r.flags == 0) {
+                    const float dy = py - r.a[1], dz = pz - r.a[2];
+                    const float bu = __builtin_fmaf(r.U[1], dy, r.U[2] * dz);
+                    const float bv = __builtin_fmaf(r.V[1], dy, r.V[2] * dz);
+                    const float bw = __builtin_fmaf(r.N[1], dy, r.N[2] * dz);
+                    // the fields the x loop reads, in scalar registers before it
+                    // starts (else their loads sink into the loop body)
+                    const PlaneRec q = plane_rec(r);
+                    for (int x = x0 + s; x <= x1; x += spp) {
+                        const float dx = (float)x * h - q.a0;
+                        const float pu = __builtin_fmaf(q.U0, dx, bu);
+                        const float pv = __builtin_fmaf(q.V0, dx, bv);
+                        const float pw = __builtin_fmaf(q.N0, dx, bw);
+                        const float d2 = __builtin_fmaf(pw, pw, plane_dist2(q, pu, pv));
+                        const unsigned long long key = ((unsigned long long)__float_as_uint(d2) << 32) | t;
                         atomicMin(&best[((x - ox) << 6) | loff], key);
+                    }
+                } else {
+                    const float* v = sub + 9 * (uint64_t)t;
+                    for (int x = x0 + s; x <= x1; x += spp) {
+                        const float d = ericson_distance({v[0], v[1], v[2]}, {v[3], v[4], v[5]}, {v[6], v[7], v[8]},
+                                                         {(float)x * h, py, pz});
+                        if (d == d) {  // a NaN distance never enters the field (`dist < cur` is false)
+                            const unsigned long long key = ((unsigned long long)__float_as_uint(d * d) << 32) | t;
+                            atomicMin(&best[((x - ox) << 6) | loff], key);
+                        }
                     }
                 }
             }
@@ -772,23 +786,16 @@ __device__ inline void acc_finalize(const NodeAcc& a, float acc, NodeRec* rec) {
     rec->D = make_float2(k3 * (o2[2] + o2[6]), k3 * (o2[5] + o2[7]));
 }
 
-// Leaf data, then node heights bottom-up with one 32-bit atomicMax per step:
-// the first child to arrive stores its height + 1 and stops, the second reads
-// it back from the same atomic and carries on.  No data other than the atomic
-// word crosses threads, so no memory fences are needed; node data is then
-// combined level by level (k_lbvh_level), one launch per height.
-__global__ void k_lbvh_leaves(const float* __restrict__ tris, int n, const uint32_t* __restrict__ order, float acc,
-                              const int* __restrict__ parent, unsigned* hword, uint32_t* height, NodeAcc* accs,
-                              NodeRec* nodes, float* btris, unsigned* err) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    F3 a, b, c;
-    load_tri(tris + 9 * (uint64_t)order[k], a, b, c);
-    // traversal copy of the triangle in Morton order, 3 x float4 (48 B) per triangle
-    float4* bt = reinterpret_cast<float4*>(btris) + 3 * (uint64_t)k;
+// Traversal copy of triangle k, 3 x float4 (48 B) per triangle.
+__device__ inline void store_btri(float* btris, uint64_t k, const F3& a, const F3& b, const F3& c) {
+    float4* bt = reinterpret_cast<float4*>(btris) + 3 * k;
     bt[0] = make_float4(a.x, a.y, a.z, b.x);
     bt[1] = make_float4(b.y, b.z, c.x, c.y);
     bt[2] = make_float4(c.z, 0.0f, 0.0f, 0.0f);
+}
+
+// InitData of one triangle (aabb_tree.rs:711-801) and its bbox.
+__device__ inline NodeAcc tri_acc(const F3& a, const F3& b, const F3& c) {
     NodeAcc d;
     F3 e0 = f3_sub(b, a), e1 = f3_sub(c, a);
     F3 cr = {e0.y * e1.z - e0.z * e1.y, e0.z * e1.x - e0.x * e1.z, e0.x * e1.y - e0.y * e1.x};
@@ -812,6 +819,24 @@ __global__ void k_lbvh_leaves(const float* __restrict__ tris, int n, const uint3
         for (int r = 0; r < 3; ++r)
             for (int q = 0; q < 3; ++q) d.o1[3 * r + q] = (area * ce[r]) * nrm[q];
     }
+    return d;
+}
+
+// Leaf data, then node heights bottom-up with one 32-bit atomicMax per step:
+// the first child to arrive stores its height + 1 and stops, the second reads
+// it back from the same atomic and carries on.  No data other than the atomic
+// word crosses threads, so no memory fences are needed; node data is then
+// combined level by level (k_lbvh_level), one launch per height.
+__global__ void k_lbvh_leaves(const float* __restrict__ tris, int n, const uint32_t* __restrict__ order, float acc,
+                              const int* __restrict__ parent, unsigned* hword, uint32_t* height, NodeAcc* accs,
+                              NodeRec* nodes, float* btris, unsigned* err) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    F3 a, b, c;
+    load_tri(tris + 9 * (uint64_t)order[k], a, b, c);
+    // traversal copy of the triangle in Morton order
+    store_btri(btris, k, a, b, c);
+    const NodeAcc d = tri_acc(a, b, c);
     int node = n - 1 + k;
     accs[node] = d;
     acc_finalize(d, acc, nodes + node);
@@ -853,6 +878,199 @@ __global__ void k_level_starts(const uint32_t* __restrict__ hs, uint32_t n, uint
     if (i >= n) return;
     const uint32_t h = hs[i];
     if (h < 256 && (i == 0 || hs[i - 1] != h)) starts[h] = i;
+}
+
+// ---------------------------------------------------------------------------
+// Clustered build (PLOC: Meister and Bittner, "Parallel Locally-Ordered
+// Clustering for Bounding Volume Hierarchy Construction", IEEE TVCG 2018).
+// The clusters start as the triangles in Morton order.  In each iteration
+// every cluster finds, among the clusters within PLOC_R places of it, the one
+// whose union with it has the smallest box surface (ties to the lower place);
+// mutual pairs merge into a new node at the lower place and the array is
+// compacted in order.  Boxes built from nearby clusters this way are tighter
+// than the Morton-split boxes, so fewer voxels find a node near.  The finished
+// tree is laid out depth-first with the two children of every node side by
+// side, 128-byte aligned (one cache line for the sibling pair the traversal
+// reads together), and the triangles in the same depth-first order.
+constexpr int PLOC_R = 32;
+constexpr int PLOC_BLOCK = 256;
+constexpr uint64_t PLOC_MIN_TRIS = 1u << 18;  // smaller meshes take the Morton split
+
+// Per temporary node: triangles below it, child pairs its subtree allocates
+// in the traversal layout (0 when it is evaluated as a leaf), its height in
+// that layout, and (parent << 1 | side), -1 while it has no parent.
+struct PlocInfo {
+    int tris, pairs, height, parent;
+};
+
+// Temporary ids: triangle k of the Morton order is node k, merged nodes are
+// n, n + 1, ... in the order merges take them.  ring[0] = cluster count,
+// ring[2] = next merged id.
+__global__ void k_ploc_leaves(const float* __restrict__ tris, int n, const uint32_t* __restrict__ order,
+                              NodeAcc* __restrict__ accs, float4* __restrict__ box, PlocInfo* __restrict__ info,
+                              int* __restrict__ clus, uint32_t* __restrict__ keep, uint32_t* __restrict__ ring) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) {
+        ring[0] = (uint32_t)n;
+        ring[2] = 0;
+        keep[n] = 0;  // the scan runs over n + 1 entries: the last one gives the count
+    }
+    if (k >= n) return;
+    F3 a, b, c;
+    load_tri(tris + 9 * (uint64_t)order[k], a, b, c);
+    const NodeAcc d = tri_acc(a, b, c);
+    accs[k] = d;
+    box[2 * k] = make_float4(d.bmin[0], d.bmin[1], d.bmin[2], 0.0f);
+    box[2 * k + 1] = make_float4(d.bmax[0], d.bmax[1], d.bmax[2], 0.0f);
+    info[k] = PlocInfo{1, 0, 0, -1};
+    clus[k] = k;
+}
+
+// Nearest neighbour of every cluster within PLOC_R places, by union box
+// surface (a NaN surface counts as infinite, so a neighbour is always found
+// when the cluster has one).
+__global__ __launch_bounds__(PLOC_BLOCK) void k_ploc_nn(const int* __restrict__ clus, const float4* __restrict__ box,
+                                                        const uint32_t* __restrict__ count, int* __restrict__ nn) {
+    __shared__ float4 slo[PLOC_BLOCK + 2 * PLOC_R], shi[PLOC_BLOCK + 2 * PLOC_R];
+    const int N = (int)*count;
+    const int base = blockIdx.x * PLOC_BLOCK;
+    if (base >= N) return;  // the whole block: no barrier is skipped by part of it
+    for (int t = threadIdx.x; t < PLOC_BLOCK + 2 * PLOC_R; t += PLOC_BLOCK) {
+        const int j = base - PLOC_R + t;
+        if (j >= 0 && j < N) {
+            const int c = clus[j];
+            slo[t] = box[2 * c];
+            shi[t] = box[2 * c + 1];
+        }
+    }
+    __syncthreads();
+    const int i = base + (int)threadIdx.x;
+    if (i >= N) return;
+    const float4 a0 = slo[threadIdx.x + PLOC_R], a1 = shi[threadIdx.x + PLOC_R];
+    float best = INFINITY;
+    int bj = -1;
+    const int jlo = max(i - PLOC_R, 0), jhi = min(i + PLOC_R, N - 1);
+    for (int j = jlo; j <= jhi; ++j) {
+        if (j == i) continue;
+        const float4 b0 = slo[j - base + PLOC_R], b1 = shi[j - base + PLOC_R];
+        const float dx = fmaxf(a1.x, b1.x) - fminf(a0.x, b0.x);
+        const float dy = fmaxf(a1.y, b1.y) - fminf(a0.y, b0.y);
+        const float dz = fmaxf(a1.z, b1.z) - fminf(a0.z, b0.z);
+        float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        d2 = (d2 == d2) ? d2 : INFINITY;
+        if (bj < 0 || d2 < best) {
+            best = d2;
+            bj = j;
+        }
+    }
+    nn[i] = bj;
+}
+
+// Mutual nearest neighbours i < j merge into a new node at place i; place j
+// is dropped.  keep[i] = 1 for every place that stays (0 past the count).
+__global__ void k_ploc_merge(int* __restrict__ clus, const int* __restrict__ nn, const uint32_t* __restrict__ count,
+                             int n, uint32_t* __restrict__ keep, uint32_t* __restrict__ next_id,
+                             NodeAcc* __restrict__ accs, float4* __restrict__ box, PlocInfo* __restrict__ info,
+                             int2* __restrict__ kids) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int N = (int)*count;
+    if (i >= N) {
+        keep[i] = 0;
+        return;
+    }
+    const int j = nn[i];
+    const bool mutual = j >= 0 && nn[j] == i;
+    keep[i] = (mutual && j < i) ? 0u : 1u;
+    if (!mutual || j < i) return;
+    const int a = clus[i], b = clus[j];
+    const int id = n + (int)atomicAdd(next_id, 1u);
+    NodeAcc s;
+    acc_add(s, accs[a], accs[b]);
+    accs[id] = s;
+    box[2 * id] = make_float4(s.bmin[0], s.bmin[1], s.bmin[2], 0.0f);
+    box[2 * id + 1] = make_float4(s.bmax[0], s.bmax[1], s.bmax[2], 0.0f);
+    const PlocInfo ia = info[a], ib = info[b];
+    const int t = ia.tris + ib.tris;
+    const bool leaf = t <= BVH_LEAF_SIZE;
+    info[id] = PlocInfo{t, leaf ? 0 : 1 + ia.pairs + ib.pairs, leaf ? 0 : 1 + max(ia.height, ib.height), -1};
+    info[a].parent = 2 * id;
+    info[b].parent = 2 * id + 1;
+    kids[id - n] = make_int2(a, b);
+    clus[i] = id;
+}
+
+// In-order compaction of the kept places; thread 0 writes the new count.
+__global__ void k_ploc_compact(const int* __restrict__ clus, const uint32_t* __restrict__ keep,
+                               const uint32_t* __restrict__ pos, const uint32_t* __restrict__ count, int n,
+                               uint32_t* __restrict__ count_next, int* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *count_next = pos[n];
+    if (i >= (int)*count) return;
+    if (keep[i]) out[pos[i]] = clus[i];
+}
+
+// The root (place 0 once one cluster is left) and the tree height.
+__global__ void k_ploc_root(const int* __restrict__ clus, const PlocInfo* __restrict__ info, uint32_t* out) {
+    const int r = clus[0];
+    out[0] = (uint32_t)r;
+    out[1] = (uint32_t)info[r].height;
+}
+
+// Depth-first layout.  The child pair of node X goes to records (2 P(X),
+// 2 P(X) + 1), P(root) = 1, and P(child) = P(X) + 1 (+ the pairs of the left
+// sibling's subtree for a right child); the triangles of X start at the
+// number of triangles in the left subtrees along its path.  Each node walks
+// its path to the root to sum these.  Record 0 is the root, record 1 unused.
+__global__ void k_ploc_place(const uint32_t* __restrict__ root_buf, const PlocInfo* __restrict__ info,
+                             const int2* __restrict__ kids, const NodeAcc* __restrict__ accs,
+                             const float* __restrict__ tris, const uint32_t* __restrict__ order, int n, float acc,
+                             NodeRec* __restrict__ nodes, float* __restrict__ btris, unsigned* err) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= 2 * n - 1) return;
+    const PlocInfo me = info[x];
+    bool rec = true;  // x has a record of its own: its parent is not evaluated as a leaf
+    if (me.parent >= 0)
+        rec = info[me.parent >> 1].tris > BVH_LEAF_SIZE;
+    else if (x != (int)root_buf[0])
+        return;  // never merged: the clustering ran to one cluster, so only the root
+    if (!rec && x >= n) return;
+    int first = 0, q = 1, side0 = 0, left_pairs0 = 0, depth = 0;
+    for (int pw = me.parent; pw >= 0; ++depth) {
+        if (depth > 1024) {  // longer than any path of a tree over 2^30 triangles built by merges
+            atomicOr(err, E_TREE);
+            return;
+        }
+        const int p = pw >> 1, side = pw & 1;
+        int lt = 0, lp = 0;
+        if (side) {
+            const PlocInfo l = info[kids[p - n].x];
+            lt = l.tris;
+            lp = l.pairs;
+        }
+        first += lt;
+        if (depth == 0) {
+            side0 = side;
+            left_pairs0 = lp;
+        } else {
+            q += 1 + lp;
+        }
+        pw = info[p].parent;
+    }
+    if (x < n) {
+        F3 a, b, c;
+        load_tri(tris + 9 * (uint64_t)order[x], a, b, c);
+        store_btri(btris, (uint64_t)first, a, b, c);
+    }
+    if (!rec) return;
+    NodeRec* r = nodes + (me.parent < 0 ? 0 : 2 * q + side0);
+    acc_finalize(accs[x], acc, r);
+    if (me.tris <= BVH_LEAF_SIZE) {
+        r->L = make_int2(first, -me.tris);
+    } else {
+        const int P = me.parent < 0 ? 1 : q + 1 + left_pairs0;
+        r->L = make_int2(2 * P, 2 * P + 1);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1372,6 +1590,12 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     int H = 0;  // tree height (root level)
     bool use_lds = false;
     const int n_nodes = 2 * nt - 1;
+    // clustered build for large meshes unless asked for the Morton split: its
+    // ~3 log2(n) small launches pay off where the winding pass is long (2 * id
+    // must fit an int)
+    const bool ploc = !(ctx->paths & BS_PATH_LBVH) && n >= PLOC_MIN_TRIS && n < (1ull << 28);
+    const int n_rec = ploc ? 2 * nt : n_nodes;  // records of the traversal layout
+    int ploc_iters = 0;
     unsigned* bounds = reinterpret_cast<unsigned*>(S.get(S_BVH_MISC, 64));
     const unsigned nb = div_up(n, 256);
     uint4* bparts = S.as<uint4>(S_BVH_BOX, 2 * (uint64_t)nb);
@@ -1379,14 +1603,16 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     uint64_t* codes_s = S.as<uint64_t>(S_MORTON_SORTED, n);
     uint32_t* idx = S.as<uint32_t>(S_IDX, n);
     uint32_t* idx_s = S.as<uint32_t>(S_IDX_SORTED, n);
-    int* parent = S.as<int>(S_BVH_PARENT, n_nodes);
-    unsigned* hword = S.as<unsigned>(S_BVH_FLAGS, std::max(n_int, 1));
-    uint32_t* height = S.as<uint32_t>(S_BVH_HEIGHT, std::max(n_int, 1));
-    uint32_t* height_s = S.as<uint32_t>(S_BVH_HEIGHT_S, std::max(n_int, 1));
-    uint32_t* inode = S.as<uint32_t>(S_BVH_INODE, std::max(n_int, 1));
-    uint32_t* inode_s = S.as<uint32_t>(S_BVH_INODE_S, std::max(n_int, 1));
+    // Morton-split build only: parent links, height words and the height sort
+    const int n_lb = ploc ? 1 : std::max(n_int, 1);
+    int* parent = S.as<int>(S_BVH_PARENT, ploc ? 1 : n_nodes);
+    unsigned* hword = S.as<unsigned>(S_BVH_FLAGS, n_lb);
+    uint32_t* height = S.as<uint32_t>(S_BVH_HEIGHT, n_lb);
+    uint32_t* height_s = S.as<uint32_t>(S_BVH_HEIGHT_S, n_lb);
+    uint32_t* inode = S.as<uint32_t>(S_BVH_INODE, n_lb);
+    uint32_t* inode_s = S.as<uint32_t>(S_BVH_INODE_S, n_lb);
     NodeAcc* accs = S.as<NodeAcc>(S_BVH_INIT, n_nodes);
-    NodeRec* nodes = S.as<NodeRec>(S_BVH_NODES, (uint64_t)n_nodes);  // internal nodes, then leaves
+    NodeRec* nodes = S.as<NodeRec>(S_BVH_NODES, (uint64_t)n_rec);
     float* btris = S.as<float>(S_BVH_TRIS, 12 * (n + BVH_LEAF_SIZE));  // padded: leaves read BVH_LEAF_SIZE triangles
     int2* kids = S.as<int2>(S_BVH_KIDS, std::max(n_int, 1));  // binary children of internal nodes
     uint32_t* starts = reinterpret_cast<uint32_t*>(S.get(S_BVH_STARTS, sizeof(uint32_t) * 258));
@@ -1399,13 +1625,51 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     k_bounds_final<<<1, 1024, 0, sb>>>(bparts, nb, bounds);
     k_morton<<<div_up(n, TB), TB, 0, sb>>>(d_tris, n, bounds, codes, idx);
     sort_pairs_on(ctx, sb, S_CUB_B, codes, codes_s, idx, idx_s, n, 63);
-    BS_HIP(hipMemsetAsync(parent, 0xFF, sizeof(int) * n_nodes, sb));
-    BS_HIP(hipMemsetAsync(hword, 0, sizeof(unsigned) * std::max(n_int, 1), sb));
-    if (n_int > 0) k_lbvh_build<<<div_up(n_int, TB), TB, 0, sb>>>(codes_s, nt, nodes, kids, parent);
-    k_lbvh_leaves<<<div_up(n, TB), TB, 0, sb>>>(d_tris, nt, idx_s, prm.wn_accuracy, parent, hword, height, accs, nodes,
-                                                btris, err);
-    uint32_t* hs = ctx->host_pinned;  // level starts [0, 258), then the error word
-    if (n_int > 0) {
+    uint32_t* hs = ctx->host_pinned;  // level starts [0, 258) (clustered: count ring, root, height), error word
+    // clustered build: the iterations the common case needs run here without
+    // a host round trip; the count is read back after the distance launch
+    float4* pbox = nullptr;
+    PlocInfo* pinfo = nullptr;
+    int *cl0 = nullptr, *cl1 = nullptr, *pnn = nullptr;
+    uint32_t *keep = nullptr, *kpos = nullptr, *ring = nullptr;
+    void* scan_tmp = nullptr;
+    size_t scan_bytes = 0;
+    auto ploc_iteration = [&](int it) {
+        const uint32_t* cnt = ring + (it & 1);
+        int* cin = (it & 1) ? cl1 : cl0;
+        int* cout = (it & 1) ? cl0 : cl1;
+        k_ploc_nn<<<div_up(n, PLOC_BLOCK), PLOC_BLOCK, 0, sb>>>(cin, pbox, cnt, pnn);
+        k_ploc_merge<<<div_up(n, TB), TB, 0, sb>>>(cin, pnn, cnt, nt, keep, ring + 2, accs, pbox,
+                                                    pinfo, kids);
+        BS_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, keep, kpos, nt + 1, sb));
+        k_ploc_compact<<<div_up(n, TB), TB, 0, sb>>>(cin, keep, kpos, cnt, nt, ring + ((it + 1) & 1), cout);
+    };
+    auto ploc_result = [&](int iters) {
+        k_ploc_root<<<1, 1, 0, sb>>>((iters & 1) ? cl1 : cl0, pinfo, ring + 3);
+        BS_HIP(hipMemcpyAsync(hs, ring, sizeof(uint32_t) * 5, hipMemcpyDeviceToHost, sb));
+        BS_HIP(hipMemcpyAsync(hs + 258, err, sizeof(uint32_t), hipMemcpyDeviceToHost, sb));
+    };
+    if (ploc) {
+        // buffers now; the launches follow the distance launch (below), so the
+        // host enqueues the subdivision and leaf passes without waiting behind them
+        pbox = S.as<float4>(S_PLOC_BOX, 2 * (uint64_t)n_nodes);
+        pinfo = S.as<PlocInfo>(S_PLOC_INFO, n_nodes);
+        cl0 = S.as<int>(S_PLOC_C0, n);
+        cl1 = S.as<int>(S_PLOC_C1, n);
+        pnn = S.as<int>(S_PLOC_NN, n);
+        keep = S.as<uint32_t>(S_PLOC_KEEP, n + 1);
+        kpos = S.as<uint32_t>(S_PLOC_POS, n + 1);
+        ring = S.as<uint32_t>(S_PLOC_RING, 8);
+        BS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, keep, kpos, nt + 1, sb));
+        scan_tmp = S.get(S_CUB_B, scan_bytes);
+    } else {
+        BS_HIP(hipMemsetAsync(parent, 0xFF, sizeof(int) * n_nodes, sb));
+        BS_HIP(hipMemsetAsync(hword, 0, sizeof(unsigned) * std::max(n_int, 1), sb));
+        if (n_int > 0) k_lbvh_build<<<div_up(n_int, TB), TB, 0, sb>>>(codes_s, nt, nodes, kids, parent);
+        k_lbvh_leaves<<<div_up(n, TB), TB, 0, sb>>>(d_tris, nt, idx_s, prm.wn_accuracy, parent, hword, height, accs,
+                                                    nodes, btris, err);
+    }
+    if (!ploc && n_int > 0) {
         // internal nodes grouped by height; combined level by level further down
         k_iota<<<div_up(n_int, TB), TB, 0, sb>>>(inode, (uint32_t)n_int);
         sort_pairs_on(ctx, sb, S_CUB_B, height, height_s, inode, inode_s, (uint64_t)n_int, 8);
@@ -1588,8 +1852,33 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     }
     tm.mark();
 
-    // ---- BVH levels on the second stream, beside the distance pass
-    if (n_int > 0) {
+    // ---- BVH levels (clustered build: its layout) on the second stream, beside the distance pass
+    if (ploc) {
+        k_ploc_leaves<<<div_up(n, TB), TB, 0, sb>>>(d_tris, nt, idx_s, accs, pbox, pinfo, cl0, keep, ring);
+        // a fifth to a third of the clusters merge per iteration: 3 log2(n)
+        // iterations cover it; those past one cluster cost a few launches each
+        ploc_iters = (int)std::ceil(3.0 * std::log2((double)n + 1.0));
+        for (int it = 0; it < ploc_iters; ++it) ploc_iteration(it);
+        ploc_result(ploc_iters);
+        BS_HIP(hipStreamSynchronize(sb));
+        uint32_t N = hs[ploc_iters & 1];
+        // rare: more iterations than estimated, one at a time (each merges at
+        // least the closest pair, so the count falls every time)
+        while (N > 1) {
+            ploc_iteration(ploc_iters++);
+            BS_HIP(hipMemcpyAsync(hs, ring, sizeof(uint32_t) * 2, hipMemcpyDeviceToHost, sb));
+            BS_HIP(hipStreamSynchronize(sb));
+            const uint32_t N2 = hs[ploc_iters & 1];
+            if (N2 >= N) throw Error(BS_ERR_INTERNAL, "BVH clustering made no progress");
+            N = N2;
+        }
+        ploc_result(ploc_iters);
+        BS_HIP(hipStreamSynchronize(sb));
+        H = (int)hs[4];
+        BS_REQUIRE(H < 250, BS_ERR_INTERNAL, "BVH deeper than the traversal stack");
+        k_ploc_place<<<div_up(n_nodes, TB), TB, 0, sb>>>(ring + 3, pinfo, kids, accs, d_tris, idx_s, nt,
+                                                        prm.wn_accuracy, nodes, btris, err);
+    } else if (n_int > 0) {
         BS_HIP(hipStreamSynchronize(sb));
         if (hs[258] & E_TREE) throw Error(BS_ERR_INTERNAL, "BVH parent links are corrupt");
         // heights run 1..H contiguously in the sorted order
@@ -1646,7 +1935,7 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
         // the visit counters feed the work accounting of profiled runs only
         auto kern = use_lds ? (ctx->counting ? k_winding<LdsStack, true> : k_winding<LdsStack, false>)
                             : (ctx->counting ? k_winding<LaneStack, true> : k_winding<LaneStack, false>);
-        kern<<<div_up(n_wn, WN_BLOCK), WN_BLOCK, 0, st>>>(wn_list, n_wn, vol->origins, h, nodes, n_nodes, btris,
+        kern<<<div_up(n_wn, WN_BLOCK), WN_BLOCK, 0, st>>>(wn_list, n_wn, vol->origins, h, nodes, n_rec, btris,
                                                           prm.sign_threshold, prm.fp64_band, vol->values, recheck,
                                                           counters, err);
     }
@@ -1657,7 +1946,7 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     if (h_err & E_HASH) throw Error(BS_ERR_INTERNAL, "leaf lookup failed");
     const uint32_t n_re = (uint32_t)hc[C_RECHECK];
     if (n_re > 0)
-        k_winding_fp64<<<div_up(n_re, 64), 64, 0, st>>>(recheck, n_re, vol->origins, h, nodes, n_nodes, btris,
+        k_winding_fp64<<<div_up(n_re, 64), 64, 0, st>>>(recheck, n_re, vol->origins, h, nodes, n_rec, btris,
                                                         prm.sign_threshold, vol->values, err);
     if (roots) k_sign_broadcast<<<(unsigned)n_leaves, 512, 0, st>>>(roots, vol->values);
     tm.mark();
@@ -1696,6 +1985,7 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     stt.n_sg_culled = hc[C_SG_CULLED];
     stt.n_sg_misclassified = hc[C_SG_MISS];
     stt.n_sg_fallback = hc[C_SG_FALLBACK];
+    stt.n_bvh_iterations = (uint64_t)ploc_iters;
     *out = guard.release();
 }
 

@@ -101,9 +101,34 @@ struct alignas(16) DistRec {
 };
 static_assert(sizeof(DistRec) == 112, "DistRec layout");
 
+// Every field of a DistRec as a scalar-register operand of an empty asm: all
+// of the record's loads are issued before it, one wait for the lot.
+__device__ inline void pin_dist_rec(const DistRec& r) {
+    asm volatile("" ::"s"(r.a[0]), "s"(r.a[1]), "s"(r.a[2]), "s"(r.U[0]), "s"(r.U[1]), "s"(r.U[2]), "s"(r.V[0]),
+                 "s"(r.V[1]), "s"(r.V[2]), "s"(r.N[0]), "s"(r.N[1]), "s"(r.N[2]), "s"(r.L0), "s"(r.inv0), "s"(r.cu),
+                 "s"(r.cv), "s"(r.e1x), "s"(r.e1y), "s"(r.inv1), "s"(r.inv2), "s"(r.box[0]), "s"(r.box[1]),
+                 "s"(r.box[2]), "s"(r.box[3]), "s"(r.box[4]), "s"(r.box[5]), "s"(r.flags));
+}
+
+// The fields of a DistRec that the per-voxel step reads, held in scalar
+// registers: plane_rec's empty asm takes each as an SGPR operand, so their
+// loads are all issued (and waited for) once per candidate.
+struct PlaneRec {
+    float a0, U0, V0, N0, L0, inv0, cu, cv, e1x, e1y, inv1, inv2;
+};
+
+__device__ inline PlaneRec plane_rec(const DistRec& r) {
+    PlaneRec q{r.a[0], r.U[0], r.V[0], r.N[0], r.L0, r.inv0, r.cu, r.cv, r.e1x, r.e1y, r.inv1, r.inv2};
+    asm volatile("" ::"s"(q.a0), "s"(q.U0), "s"(q.V0), "s"(q.N0), "s"(q.L0), "s"(q.inv0), "s"(q.cu), "s"(q.cv),
+                 "s"(q.e1x), "s"(q.e1y), "s"(q.inv1), "s"(q.inv2));
+    return q;
+}
+
 // In-plane squared distance from (pu, pv) to the frame triangle (0,0), (L0,0),
-// (cu,cv): 0 inside, else the nearest clamped edge projection.
-__device__ inline float plane_dist2(const DistRec& r, float pu, float pv) {
+// (cu,cv): 0 inside, else the nearest clamped edge projection.  The three edge
+// terms are computed for every lane (the empty asm keeps them out of a branch
+// on `inside`, which would cost more in mask handling than it saves).
+__device__ inline float plane_dist2(const PlaneRec& r, float pu, float pv) {
     const float t0 = __builtin_amdgcn_fmed3f(pu * r.inv0, 0.0f, 1.0f);
     const float qx = __builtin_fmaf(-t0, r.L0, pu);
     const float d0 = __builtin_fmaf(qx, qx, pv * pv);
@@ -118,7 +143,9 @@ __device__ inline float plane_dist2(const DistRec& r, float pu, float pv) {
     const float f1 = __builtin_fmaf(r.e1x, pv, -r.e1y * rx);   // cross(c-b, q-b)
     const float f2 = __builtin_fmaf(r.cv, wx, -r.cu * wy);     // cross(a-c, q-c)
     const bool inside = fminf(fminf(pv, f1), f2) >= 0.0f;
-    return inside ? 0.0f : fminf(fminf(d0, d1), d2);
+    float m = fminf(fminf(d0, d1), d2);
+    asm volatile("" : "+v"(m));
+    return inside ? 0.0f : m;
 }
 
 // ---------------------------------------------------------------- winding numbers
