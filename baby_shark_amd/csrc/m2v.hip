@@ -1159,8 +1159,8 @@ struct LdsStack {
 
 
 // Empty asm that consumes every word of a node record as a scalar register:
-// the record's loads must all be issued before it, so the two siblings of a
-// pair cost one memory round trip instead of one per use site.
+// the record's loads must all be issued before it (one memory round trip
+// instead of one per use site).
 __device__ inline void pin_record(const NodeRec& r) {
     asm volatile("" ::"s"(r.A.x), "s"(r.A.y), "s"(r.A.z), "s"(r.A.w), "s"(r.B.x), "s"(r.B.y), "s"(r.B.z),
                  "s"(r.B.w), "s"(r.C.x), "s"(r.C.y), "s"(r.C.z), "s"(r.C.w), "s"(r.D.x), "s"(r.D.y), "s"(r.L.x),
