@@ -28,6 +28,8 @@ BS_MESHING_MANIFOLD = 1
 BS_PATH_FORCE_HASH = 1
 BS_PATH_FORCE_LDS_STACK = 2
 BS_PATH_LBVH = 4
+BS_DC_VERTEX_PARTICLE = 0
+BS_DC_VERTEX_QEF_F64 = 1
 BS_SIGN_PER_VOXEL = 0
 BS_SIGN_SHARED = 1
 BS_SIGN_SURROGATE = 2
@@ -123,6 +125,7 @@ def load():
             "bs_ctx_set_profiling": (C.c_int, [vp, C.c_int]),
             "bs_ctx_last_m2v_stats": (C.c_int, [vp, C.POINTER(M2VStats)]),
             "bs_ctx_set_paths": (C.c_int, [vp, C.c_int]),
+            "bs_ctx_set_dc_vertex": (C.c_int, [vp, C.c_int]),
             "bs_m2v_params_default": (None, [C.POINTER(M2VParams)]),
             "bs_mesh_to_volume": (C.c_int, [vp, f32p, C.c_size_t, C.POINTER(M2VParams), pp]),
             "bs_mesh_to_volume_device": (C.c_int, [vp, vp, C.c_size_t, C.POINTER(M2VParams), pp]),

@@ -148,6 +148,15 @@ bs_status bs_ctx_set_paths(bs_ctx* ctx, int flags) {
     return BS_OK;
 }
 
+bs_status bs_ctx_set_dc_vertex(bs_ctx* ctx, int mode) {
+    if (!ctx || (mode != BS_DC_VERTEX_PARTICLE && mode != BS_DC_VERTEX_QEF_F64)) {
+        g_last_error = "unknown BS_DC_VERTEX_* mode";
+        return BS_ERR_INVALID_ARG;
+    }
+    ctx->dc_vertex = mode;
+    return BS_OK;
+}
+
 bs_status bs_ctx_last_m2v_stats(const bs_ctx* ctx, bs_m2v_stats* out) {
     if (!ctx || !out) return BS_ERR_INVALID_ARG;
     *out = ctx->last_m2v;

@@ -85,6 +85,7 @@ struct bs_ctx {
     bool profiling = false;  // HIP events around the mesh-to-volume phases
     bool counting = false;   // winding-traversal visit counters (a slower kernel variant)
     int paths = 0;  // BS_PATH_* flags
+    int dc_vertex = BS_DC_VERTEX_PARTICLE;  // dual-contouring vertex placement
     bs::Scratch scratch;
     bs_m2v_stats last_m2v = {};
     bs_offset_stats last_offset = {};

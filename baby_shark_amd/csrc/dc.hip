@@ -135,6 +135,86 @@ __constant__ int8_t kCell[3][4][3] = {{{0, 0, 0}, {0, 0, -1}, {0, -1, -1}, {0, -
                                       {{0, 0, 0}, {-1, 0, 0}, {-1, 0, -1}, {0, 0, -1}},
                                       {{0, -1, 0}, {-1, -1, 0}, {-1, 0, 0}, {0, 0, 0}}};
 
+// Opt-in vertex placement (BS_DC_VERTEX_QEF_F64; the reference has only the
+// particle iteration): the point minimising sum_i (n_i . (x - p_i))^2 over the
+// cell's edge crossings p_i with unit normals n_i, in FP64.  Relative to the
+// mass point m (the crossings' mean), A = sum n n^T and b = sum n (n . (p - m));
+// A's eigenvectors come from cyclic Jacobi rotations and x = m + sum_k
+// (v_k . b / l_k) v_k over the eigenvalues l_k > QEF_TRUNC * l_max (a plane
+// fixes one direction, an edge two, a corner three; the rest stays at m).
+// The point is clamped to the cell.
+constexpr double QEF_TRUNC = 0.1;
+
+// One Jacobi rotation zeroing A[p][q] of the symmetric A (Numerical Recipes'
+// form: theta = (A_qq - A_pp) / (2 A_pq), t = sgn(theta) / (|theta| +
+// sqrt(theta^2 + 1))), accumulated into the eigenvector columns of V.
+template <int P, int Q>
+__device__ inline void jacobi_rotate(double (&A)[3][3], double (&V)[3][3]) {
+    const double apq = A[P][Q];
+    if (apq == 0.0) return;
+    const double theta = (A[Q][Q] - A[P][P]) / (2.0 * apq);
+    const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+    const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+    constexpr int R = 3 - P - Q;  // the third index
+    const double arp = A[R][P], arq = A[R][Q];
+    A[P][P] -= t * apq;
+    A[Q][Q] += t * apq;
+    A[P][Q] = A[Q][P] = 0.0;
+    A[R][P] = A[P][R] = c * arp - s * arq;
+    A[R][Q] = A[Q][R] = s * arp + c * arq;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const double vp = V[r][P], vq = V[r][Q];
+        V[r][P] = c * vp - s * vq;
+        V[r][Q] = s * vp + c * vq;
+    }
+}
+
+__device__ inline void qef_point(const float (&P)[12][3], const float (&N)[12][3], uint32_t valid, float cx0,
+                                 float cy0, float cz0, float out[3]) {
+    double m[3] = {0.0, 0.0, 0.0};
+    const double cnt = (double)__popc(valid);
+#pragma unroll
+    for (int e = 0; e < 12; ++e)
+        if (valid & (1u << e))
+#pragma unroll
+            for (int k = 0; k < 3; ++k) m[k] += (double)P[e][k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) m[k] /= cnt;
+    double A[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}}, b[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int e = 0; e < 12; ++e) {
+        if (!(valid & (1u << e))) continue;
+        const double n0 = N[e][0], n1 = N[e][1], n2 = N[e][2];
+        const double d = n0 * ((double)P[e][0] - m[0]) + n1 * ((double)P[e][1] - m[1]) + n2 * ((double)P[e][2] - m[2]);
+        A[0][0] += n0 * n0, A[0][1] += n0 * n1, A[0][2] += n0 * n2;
+        A[1][1] += n1 * n1, A[1][2] += n1 * n2, A[2][2] += n2 * n2;
+        b[0] += n0 * d, b[1] += n1 * d, b[2] += n2 * d;
+    }
+    A[1][0] = A[0][1], A[2][0] = A[0][2], A[2][1] = A[1][2];
+    double V[3][3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
+    for (int sweep = 0; sweep < 8; ++sweep) {
+        jacobi_rotate<0, 1>(A, V);
+        jacobi_rotate<0, 2>(A, V);
+        jacobi_rotate<1, 2>(A, V);
+    }
+    const double lmax = fmax(fmax(A[0][0], A[1][1]), A[2][2]);
+    double x[3] = {m[0], m[1], m[2]};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double l = A[k][k];
+        if (!(l > QEF_TRUNC * lmax)) continue;
+        const double w = (V[0][k] * b[0] + V[1][k] * b[1] + V[2][k] * b[2]) / l;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) x[r] += w * V[r][k];
+    }
+    // the cell spans [c0, c0 + 1] on each axis (voxel units)
+    out[0] = (float)fmin(fmax(x[0], (double)cx0), (double)cx0 + 1.0);
+    out[1] = (float)fmin(fmax(x[1], (double)cy0), (double)cy0 + 1.0);
+    out[2] = (float)fmin(fmax(x[2], (double)cz0), (double)cz0 + 1.0);
+}
+
+template <bool QEF>
 __global__ __launch_bounds__(64) void k_dc_cells(uint64_t n_leaves, DcArgs a) {
     __shared__ Block B;
     __shared__ uint16_t cell_list[512];  // offset x*64 + y*8 + z of each listed cell
@@ -201,6 +281,11 @@ __global__ __launch_bounds__(64) void k_dc_cells(uint64_t n_leaves, DcArgs a) {
                 crossing(B, org, ex, ey, ez, d, P[e], N[e]);
                 valid |= 1u << e;
             }
+            float* out = a.cell_pts + (leaf * 512 + (uint64_t)cell) * 3;
+            if constexpr (QEF) {
+                qef_point(P, N, valid, (float)(org.x + x), (float)(org.y + y), (float)(org.z + z), out);
+                continue;
+            }
             // find_feature_point (dual_contouring.rs:433-458)
             const float fc = (float)__popc(valid);
             float cx = 0.0f, cy = 0.0f, cz = 0.0f;
@@ -224,7 +309,6 @@ __global__ __launch_bounds__(64) void k_dc_cells(uint64_t n_leaves, DcArgs a) {
                 cz = cz + (fz * damping) / fc;
                 if ((fx * fx + fy * fy) + fz * fz < 1e-6f) break;
             }
-            float* out = a.cell_pts + (leaf * 512 + (uint64_t)cell) * 3;
             out[0] = cx;
             out[1] = cy;
             out[2] = cz;
@@ -395,7 +479,7 @@ uint64_t hash_capacity(uint64_t n) {
 
 // Everything up to the triangle count: leaf map, cell points, per-leaf
 // triangle counts and their scan.  Returns the soup's triangle count.
-uint64_t dc_count(hipStream_t st, const bs_volume* vol, float voxel_size, DcWork& w, DcArgs& a) {
+uint64_t dc_count(hipStream_t st, const bs_volume* vol, float voxel_size, bool qef, DcWork& w, DcArgs& a) {
     const uint64_t n = vol->n_leaves;
     if (w.map.dense) {
         fill_u32(st, w.map.cells, (uint64_t)w.map.nx * w.map.ny * w.map.nz, 0xFFFFFFFFu);
@@ -418,7 +502,10 @@ uint64_t dc_count(hipStream_t st, const bs_volume* vol, float voxel_size, DcWork
     a.voxel_size = voxel_size;
     a.fail = w.fail;
     const unsigned grid = (unsigned)std::min<uint64_t>(n, 8192);
-    k_dc_cells<<<grid, 64, 0, st>>>(n, a);
+    if (qef)
+        k_dc_cells<true><<<grid, 64, 0, st>>>(n, a);
+    else
+        k_dc_cells<false><<<grid, 64, 0, st>>>(n, a);
     BS_HIP(hipGetLastError());
     k_dc_triangles<false><<<grid, 64, 0, st>>>(n, a, w.tri_count, nullptr, nullptr);
     BS_HIP(hipGetLastError());
@@ -545,7 +632,7 @@ void dual_contour_device(bs_ctx* ctx, bs_volume* vol, float voxel_size, float** 
     w.scan_tmp = ctx->scratch.get(S_CUB, w.scan_bytes);
 
     DcArgs a{};
-    const uint64_t total = dc_count(st, vol, voxel_size, w, a);
+    const uint64_t total = dc_count(st, vol, voxel_size, ctx->dc_vertex == BS_DC_VERTEX_QEF_F64, w, a);
     if (total) {
         float* out = nullptr;
         BS_HIP(hipMalloc(&out, sizeof(float) * 9 * total));
@@ -613,7 +700,7 @@ void dual_contour_into(bs_ctx* ctx, const bs_volume* vol, float voxel_size, floa
     w.scan_bytes = need;
 
     DcArgs a{};
-    const uint64_t total = dc_count(st, vol, voxel_size, w, a);
+    const uint64_t total = dc_count(st, vol, voxel_size, ctx->dc_vertex == BS_DC_VERTEX_QEF_F64, w, a);
     *n_tris = total;
     BS_REQUIRE(total <= capacity_tris, BS_ERR_CAPACITY,
                "bs_dual_contour_into: the soup has " + std::to_string(total) +

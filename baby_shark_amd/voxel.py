@@ -20,7 +20,7 @@ from typing import Optional
 import numpy as np
 
 from . import _lib
-from ._lib import BSError, check, load
+from ._lib import BS_DC_VERTEX_PARTICLE, BS_DC_VERTEX_QEF_F64, BSError, check, load
 
 
 class Context:
@@ -373,6 +373,7 @@ class DualContouringMesher:
     def __init__(self, ctx: Optional[Context] = None):
         self._ctx = ctx
         self.voxel_size = 1.0
+        self.vertex_mode = BS_DC_VERTEX_PARTICLE
 
     @property
     def ctx(self) -> Context:
@@ -384,12 +385,28 @@ class DualContouringMesher:
         self.voxel_size = float(size)
         return self
 
+    def with_qef_vertices(self) -> "DualContouringMesher":
+        """Vertices at each cell's FP64 quadric-error minimiser (sharp
+        features kept) instead of the reference's particle iteration; no
+        counterpart in the reference (BS_DC_VERTEX_QEF_F64)."""
+        self.vertex_mode = BS_DC_VERTEX_QEF_F64
+        return self
+
+    def _call(self, fn, *args) -> None:
+        # the mode is a context setting: set for this call, the default after
+        L = load()
+        check(L.bs_ctx_set_dc_vertex(self.ctx.handle, self.vertex_mode))
+        try:
+            check(fn(*args))
+        finally:
+            L.bs_ctx_set_dc_vertex(self.ctx.handle, BS_DC_VERTEX_PARTICLE)
+
     def mesh(self, volume: "Volume") -> np.ndarray:
         """Triangle soup (n, 3, 3) float32 on the host."""
         L = load()
         out = C.POINTER(C.c_float)()
         n = C.c_size_t(0)
-        check(L.bs_dual_contour(self.ctx.handle, volume._h, self.voxel_size, C.byref(out), C.byref(n)))
+        self._call(L.bs_dual_contour, self.ctx.handle, volume._h, self.voxel_size, C.byref(out), C.byref(n))
         if n.value == 0:
             return np.zeros((0, 3, 3), np.float32)
         try:
@@ -401,7 +418,8 @@ class DualContouringMesher:
         """Triangle soup left in device memory (see DeviceTriangles)."""
         ptr = C.c_void_p()
         n = C.c_size_t(0)
-        check(load().bs_dual_contour_device(self.ctx.handle, volume._h, self.voxel_size, C.byref(ptr), C.byref(n)))
+        self._call(load().bs_dual_contour_device, self.ctx.handle, volume._h, self.voxel_size, C.byref(ptr),
+                   C.byref(n))
         return DeviceTriangles(ptr.value or 0, n.value)
 
 

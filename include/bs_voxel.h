@@ -299,6 +299,19 @@ bs_status bs_dual_contour_device(bs_ctx* ctx, const bs_volume* v, float voxel_si
 bs_status bs_dual_contour_into(bs_ctx* ctx, const bs_volume* v, float voxel_size, float* out_xyz_dev,
                                size_t capacity_tris, size_t* n_tris);
 bs_status bs_ctx_last_dc_stats(const bs_ctx* ctx, bs_dc_stats* out);
+/* Vertex placement of the bs_dual_contour* calls on this context.
+ * BS_DC_VERTEX_PARTICLE (the default) is the reference's damped particle
+ * iteration (find_feature_point, dual_contouring.rs:429-458).  No counterpart
+ * in the reference, opt-in: BS_DC_VERTEX_QEF_F64 puts each cell's vertex at
+ * the minimiser of the quadric error sum_i (n_i . (x - p_i))^2 over the cell's
+ * edge crossings, solved in FP64 (eigen-decomposition of the 3x3 normal
+ * matrix, eigenvalues below 0.1 of the largest dropped, so planes, edges and
+ * corners keep their sharp features), clamped to the cell.  The triangles
+ * (which cells they join, their order) are the same in both modes; only the
+ * vertex positions differ. */
+#define BS_DC_VERTEX_PARTICLE 0
+#define BS_DC_VERTEX_QEF_F64 1
+bs_status bs_ctx_set_dc_vertex(bs_ctx* ctx, int mode);
 void bs_free(void* p);
 void bs_device_free(void* p_dev);
 
