@@ -58,6 +58,7 @@ constexpr VoxelPlanes make_voxel_planes() {
     return t;
 }
 __constant__ VoxelPlanes kVoxelPlanes = make_voxel_planes();
+constexpr VoxelPlanes kVoxelPlanesHost = make_voxel_planes();  // plane sizes folded into unrolled loops
 
 struct OffsetState {
     uint32_t pool_count;  // leaves in the pool before the sweeps
@@ -147,13 +148,19 @@ __global__ void k_plane_hist(uint32_t n, const int4* __restrict__ origins, int s
     atomicAdd(&hist[k], 1u);
 }
 
+// order[] = the pool's leaves bucketed by plane; org_order[] their origins in
+// the same order (so a sweep reads a leaf's slot and origin side by side).
 __global__ void k_plane_scatter(uint32_t n, const int4* __restrict__ origins, int sx, int sy, int sz, int32_t kmin,
-                                uint32_t n_planes, uint32_t* __restrict__ cursor, uint32_t* __restrict__ order) {
+                                uint32_t n_planes, uint32_t* __restrict__ cursor, uint32_t* __restrict__ order,
+                                int4* __restrict__ org_order) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int32_t k = leaf_plane(origins[i], sx, sy, sz) - kmin;
+    const int4 o = origins[i];
+    const int32_t k = leaf_plane(o, sx, sy, sz) - kmin;
     if (k < 0 || (uint32_t)k >= n_planes) return;
-    order[atomicAdd(&cursor[k], 1u)] = i;
+    const uint32_t q = atomicAdd(&cursor[k], 1u);
+    order[q] = i;
+    org_order[q] = o;
 }
 
 // ---------------------------------------------------------------- sweep
@@ -221,31 +228,86 @@ __device__ inline void sweep_voxel(LeafLds& L, const SweepArgs& a, int o) {
     }
 }
 
-// One leaf plane of one sweep: the leaves of bucket [b0, b1) of `order` plus
-// the leaves created while the previous plane was swept.  starts[p + 1] is
-// the pool size when plane p begins and created[p + 1] the leaves plane p
-// creates (all on plane p + 1), so this plane sweeps the pool range
-// [starts[plane], starts[plane] + created[plane]) after its bucket and
-// allocates its own new leaves from starts[plane] + created[plane] on.
-// One 64-lane block per leaf, looping over the plane's leaves.
-__global__ __launch_bounds__(64) void k_sweep_plane(SweepArgs a, const uint32_t* __restrict__ order,
-                                                     const uint32_t* __restrict__ bucket_off, uint32_t plane,
-                                                     uint32_t n_planes, uint32_t* starts, uint32_t* created,
-                                                     OffsetState* st) {
-    __shared__ LeafLds L;
-    const uint32_t b0 = bucket_off[plane], b1 = bucket_off[plane + 1];
-    const uint32_t n_old = b1 - b0;
-    const uint32_t new_lo = starts[plane];
-    const uint32_t start = min(new_lo + created[plane], a.capacity);
-    const uint32_t n_new = start - new_lo;
-    const int lane = threadIdx.x;
-    const int lx = lane >> 3, ly = lane & 7;  // this lane's z-row of the leaf
-    if (blockIdx.x == 0 && lane == 0) starts[plane + 1] = start;
-    // each lane's voxel on each of the 22 voxel planes (padded index, -1 for
-    // none), resolved once for all the leaves the block sweeps
-    __shared__ int16_t plane_vox[22][64];
-    for (int p = 0; p < 22; ++p) {
-        const int s0 = kVoxelPlanes.start[p], cnt = kVoxelPlanes.start[p + 1] - s0;
+// The leaf's own values and state (this lane's z-row; lane = (lx, ly)) and,
+// in lanes 0..5, its six face-neighbour slots, as registers.  stage_issue
+// only issues the loads (dense map: the raw cell word, resolved when stored),
+// so the leaf's data and its neighbour lookups travel in one round trip.
+struct LeafStage {
+    float4 v0, v1;
+    uint64_t mw, fw;
+    int32_t nb;  // lanes 0..5: neighbour slot or map cell word
+};
+
+__device__ inline void stage_issue(const SweepArgs& a, uint32_t slot, int4 org, int lane, LeafStage& s) {
+    const int lx = lane >> 3;
+    const float4* vp = reinterpret_cast<const float4*>(a.values + (uint64_t)slot * 512 + lane * 8);
+    s.v0 = vp[0];
+    s.v1 = vp[1];
+    s.mw = a.masks[(uint64_t)slot * 8 + lx];
+    s.fw = a.frozen[(uint64_t)slot * 8 + lx];
+    s.nb = -1;
+    if (lane < 6) {
+        const int ax = lane >> 1, dir = (lane & 1) ? 1 : -1;
+        const int32_t nx = (org.x >> 3) + (ax == 0 ? dir : 0), ny = (org.y >> 3) + (ax == 1 ? dir : 0),
+                      nz = (org.z >> 3) + (ax == 2 ? dir : 0);
+        if (a.map.dense) {
+            const uint32_t ix = (uint32_t)(nx - a.map.x0), iy = (uint32_t)(ny - a.map.y0), iz = (uint32_t)(nz - a.map.z0);
+            if (ix < a.map.nx && iy < a.map.ny && iz < a.map.nz)
+                s.nb = a.map.cells[((uint64_t)ix * a.map.ny + iy) * a.map.nz + iz];
+        } else {
+            s.nb = a.map.find(nx, ny, nz);
+        }
+    }
+}
+
+__device__ inline void stage_store(LeafLds& L, const LeafStage& s, int lane) {
+    const int lx = lane >> 3, ly = lane & 7;
+    const uint64_t mw = s.mw >> (ly * 8), fw = s.fw >> (ly * 8);
+    const float v[8] = {s.v0.x, s.v0.y, s.v0.z, s.v0.w, s.v1.x, s.v1.y, s.v1.z, s.v1.w};
+    const int base = pidx(lx, ly, 0);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        L.val[base + k] = v[k];
+        L.st[base + k] = (uint8_t)(((mw >> k) & 1) | (((fw >> k) & 1) << 1));
+    }
+    if (lane < 6) L.nb[lane] = s.nb >= 0 ? s.nb : -1;  // a cell being created (-2) is no leaf yet
+}
+
+// The six neighbour layers next to the leaf, into the padding (L.nb visible).
+__device__ inline void stage_layers(LeafLds& L, const SweepArgs& a, int lane) {
+    const int i = lane >> 3, j = lane & 7;
+    uint8_t on[6];
+    float val[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int32_t n = L.nb[k];
+        const int f = (k & 1) ? 0 : 7;  // -side neighbour: its far layer 7; +side: layer 0
+        const int o = k < 2 ? ((f << 6) | (i << 3) | j) : (k < 4 ? ((i << 6) | (f << 3) | j) : ((i << 6) | (j << 3) | f));
+        on[k] = 0;
+        val[k] = 0.0f;
+        if (n >= 0) {
+            // all twelve loads in flight together
+            const uint64_t mw = a.masks[(uint64_t)n * 8 + (o >> 6)];
+            const float nv = a.values[(uint64_t)n * 512 + o];
+            on[k] = (mw >> (o & 63)) & 1ull;
+            val[k] = on[k] ? nv : 0.0f;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int g = (k & 1) ? 8 : -1;  // the padding layer this maps to
+        const int po = k < 2 ? pidx(g, i, j) : (k < 4 ? pidx(i, g, j) : pidx(i, j, g));
+        L.st[po] = on[k];
+        L.val[po] = val[k];
+    }
+}
+
+// Each lane's voxel on each of the 22 voxel planes of a leaf in sweep order
+// (padded index, -1 for none).
+__device__ inline void init_plane_vox(int16_t (*plane_vox)[64], const SweepArgs& a, int lane) {
+#pragma unroll
+    for (int p = 0; p < 22; ++p) {  // unrolled: the 22 table reads are in flight together
+        const int s0 = kVoxelPlanesHost.start[p], cnt = kVoxelPlanesHost.start[p + 1] - s0;
         int16_t q = -1;
         if (lane < cnt) {
             const int e = kVoxelPlanes.entry[s0 + lane];
@@ -254,100 +316,97 @@ __global__ __launch_bounds__(64) void k_sweep_plane(SweepArgs a, const uint32_t*
         }
         plane_vox[p][lane] = q;
     }
+}
+
+// The staged leaf swept (the wavefront over its 22 voxel planes), written
+// back, and its pushes (fast_sweep.rs:186-273): the +direction face of each
+// axis; lanes 0..2 create the pushed neighbours of the three axes at once.
+// A created slot is published without a fence: in this plane a lookup of the
+// new leaf reads only its masks and values, which hold the empty leaf
+// already, and its origin is read only in a later plane.
+__device__ inline void sweep_and_finish(LeafLds& L, const int16_t (*plane_vox)[64], const SweepArgs& a,
+                                        uint32_t slot, int4 org, uint32_t plane, uint32_t n_planes, uint32_t start,
+                                        uint32_t* created, OffsetState* st, int lane) {
+    const int lx = lane >> 3, ly = lane & 7;
+    for (int p = 0; p < 22; ++p) {
+        const int q = plane_vox[p][lane];
+        if (q >= 0) sweep_voxel(L, a, q);
+        __syncthreads();
+    }
+    {
+        const int base = pidx(lx, ly, 0);
+        float4* vp = reinterpret_cast<float4*>(a.values + (uint64_t)slot * 512 + lane * 8);
+        vp[0] = make_float4(L.val[base + 0], L.val[base + 1], L.val[base + 2], L.val[base + 3]);
+        vp[1] = make_float4(L.val[base + 4], L.val[base + 5], L.val[base + 6], L.val[base + 7]);
+        uint32_t row = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) row |= (uint32_t)(L.st[base + k] & 1) << k;
+        const uint64_t w = gather_word(row, ly);
+        if (ly == 0) a.masks[(uint64_t)slot * 8 + lx] = w;
+    }
+    bool push[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const int dir = ax == 0 ? a.sx : (ax == 1 ? a.sy : a.sz);
+        const int f = dir > 0 ? 7 : 0;
+        const int i = lx, j = ly;
+        const int o = ax == 0 ? pidx(f, i, j) : (ax == 1 ? pidx(i, f, j) : pidx(i, j, f));
+        const float v = L.val[o];
+        const bool q = (L.st[o] & 1) && (int)(__float_as_uint(v) >> 31) == a.neg && fabsf(v) < a.limit_abs;
+        push[ax] = __ballot(q) != 0;
+    }
+    if (lane < 3 && (lane == 0 ? push[0] : (lane == 1 ? push[1] : push[2]))) {
+        const int ax = lane;
+        const int dir = ax == 0 ? a.sx : (ax == 1 ? a.sy : a.sz);
+        if (plane + 1 >= n_planes) atomicOr(&st->fail, 1u | 4u);
+        const int32_t nx = (org.x >> 3) + (ax == 0 ? dir : 0), ny = (org.y >> 3) + (ax == 1 ? dir : 0),
+                      nz = (org.z >> 3) + (ax == 2 ? dir : 0);
+        int32_t* cell = a.map.claim(nx, ny, nz, &st->fail);
+        if (cell) {
+            const uint32_t s = start + atomicAdd(&created[plane + 1], 1u);
+            if (s < a.capacity) {
+                a.origins[s] = make_int4(nx * 8, ny * 8, nz * 8, 0);
+                atomicExch(cell, (int32_t)s);
+            } else {
+                atomicOr(&st->fail, 2u);  // pool overflow: retried with a larger pool
+            }
+        }
+    }
+}
+
+// One leaf plane of one sweep: the leaves of bucket [b0, b1) of `order` plus
+// the leaves created while the previous plane was swept.  starts[p + 1] is
+// the pool size when plane p begins and created[p + 1] the leaves plane p
+// creates (all on plane p + 1), so this plane sweeps the pool range
+// [starts[plane], starts[plane] + created[plane]) after its bucket and
+// allocates its own new leaves from starts[plane] + created[plane] on.
+// One 64-lane block per leaf, looping over the plane's leaves.
+__global__ __launch_bounds__(64) void k_sweep_plane(SweepArgs a, const uint32_t* __restrict__ order,
+                                                     const int4* __restrict__ org_order,
+                                                     const uint32_t* __restrict__ bucket_off, uint32_t plane,
+                                                     uint32_t n_planes, uint32_t* starts, uint32_t* created,
+                                                     OffsetState* st) {
+    __shared__ LeafLds L;
+    __shared__ int16_t plane_vox[22][64];
+    const uint32_t b0 = bucket_off[plane], b1 = bucket_off[plane + 1];
+    const uint32_t n_old = b1 - b0;
+    const uint32_t new_lo = starts[plane];
+    const uint32_t start = min(new_lo + created[plane], a.capacity);
+    const uint32_t n_new = start - new_lo;
+    const int lane = threadIdx.x;
+    if (blockIdx.x == 0 && lane == 0) starts[plane + 1] = start;
+    init_plane_vox(plane_vox, a, lane);
     for (uint32_t item = blockIdx.x; item < n_old + n_new; item += gridDim.x) {
+        // a bucket leaf's slot and origin in one round trip, a pushed one's origin
         const uint32_t slot = item < n_old ? order[b0 + item] : new_lo + (item - n_old);
-        const int4 org = a.origins[slot];
-        {
-            const float4* vp = reinterpret_cast<const float4*>(a.values + (uint64_t)slot * 512 + lane * 8);
-            const float4 q0 = vp[0], q1 = vp[1];
-            const float v[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
-            const uint64_t mw = a.masks[(uint64_t)slot * 8 + lx] >> (ly * 8);
-            const uint64_t fw = a.frozen[(uint64_t)slot * 8 + lx] >> (ly * 8);
-            const int base = pidx(lx, ly, 0);
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                L.val[base + k] = v[k];
-                L.st[base + k] = (uint8_t)(((mw >> k) & 1) | (((fw >> k) & 1) << 1));
-            }
-            if (lane < 6) {
-                const int ax = lane >> 1, dir = (lane & 1) ? 1 : -1;
-                L.nb[lane] = a.map.find((org.x >> 3) + (ax == 0 ? dir : 0), (org.y >> 3) + (ax == 1 ? dir : 0),
-                                        (org.z >> 3) + (ax == 2 ? dir : 0));
-            }
-        }
+        const int4 org = item < n_old ? org_order[b0 + item] : a.origins[slot];
+        LeafStage s;
+        stage_issue(a, slot, org, lane, s);
+        stage_store(L, s, lane);
         __syncthreads();
-        // the six neighbour layers next to this leaf, into the padding
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            const int32_t n = L.nb[k];
-            const int i = lx, j = ly;
-            const int f = (k & 1) ? 0 : 7;  // -side neighbour: its far layer 7; +side: layer 0
-            const int o = k < 2 ? ((f << 6) | (i << 3) | j) : (k < 4 ? ((i << 6) | (f << 3) | j) : ((i << 6) | (j << 3) | f));
-            const int g = (k & 1) ? 8 : -1;  // the padding layer this maps to
-            const int po = k < 2 ? pidx(g, i, j) : (k < 4 ? pidx(i, g, j) : pidx(i, j, g));
-            uint8_t on = 0;
-            float v = 0.0f;
-            if (n >= 0) {
-                // mask word and value in flight together
-                const uint64_t mw = a.masks[(uint64_t)n * 8 + (o >> 6)];
-                const float nv = a.values[(uint64_t)n * 512 + o];
-                on = (mw >> (o & 63)) & 1ull;
-                v = on ? nv : 0.0f;
-            }
-            L.st[po] = on;
-            L.val[po] = v;
-        }
+        stage_layers(L, a, lane);
         __syncthreads();
-        for (int p = 0; p < 22; ++p) {
-            const int q = plane_vox[p][lane];
-            if (q >= 0) sweep_voxel(L, a, q);
-            __syncthreads();
-        }
-        // write back
-        {
-            const int base = pidx(lx, ly, 0);
-            float4* vp = reinterpret_cast<float4*>(a.values + (uint64_t)slot * 512 + lane * 8);
-            vp[0] = make_float4(L.val[base + 0], L.val[base + 1], L.val[base + 2], L.val[base + 3]);
-            vp[1] = make_float4(L.val[base + 4], L.val[base + 5], L.val[base + 6], L.val[base + 7]);
-            uint32_t row = 0;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) row |= (uint32_t)(L.st[base + k] & 1) << k;
-            const uint64_t w = gather_word(row, ly);
-            if (ly == 0) a.masks[(uint64_t)slot * 8 + lx] = w;
-        }
-        // fast_sweep.rs:186-273: the +direction face of each axis; lanes 0..2
-        // create the pushed neighbours of the three axes at the same time.
-        // The slot is published without a fence before it: in this launch a
-        // lookup of the new leaf reads only its masks and values, which hold
-        // the empty leaf already, and the origin is read from the next launch.
-        bool push[3];
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax) {
-            const int dir = ax == 0 ? a.sx : (ax == 1 ? a.sy : a.sz);
-            const int f = dir > 0 ? 7 : 0;
-            const int i = lx, j = ly;
-            const int o = ax == 0 ? pidx(f, i, j) : (ax == 1 ? pidx(i, f, j) : pidx(i, j, f));
-            const float v = L.val[o];
-            const bool q = (L.st[o] & 1) && (int)(__float_as_uint(v) >> 31) == a.neg && fabsf(v) < a.limit_abs;
-            push[ax] = __ballot(q) != 0;
-        }
-        if (lane < 3 && (lane == 0 ? push[0] : (lane == 1 ? push[1] : push[2]))) {
-            const int ax = lane;
-            const int dir = ax == 0 ? a.sx : (ax == 1 ? a.sy : a.sz);
-            if (plane + 1 >= n_planes) atomicOr(&st->fail, 1u | 4u);
-            const int32_t nx = (org.x >> 3) + (ax == 0 ? dir : 0), ny = (org.y >> 3) + (ax == 1 ? dir : 0),
-                          nz = (org.z >> 3) + (ax == 2 ? dir : 0);
-            int32_t* cell = a.map.claim(nx, ny, nz, &st->fail);
-            if (cell) {
-                const uint32_t s = start + atomicAdd(&created[plane + 1], 1u);
-                if (s < a.capacity) {
-                    a.origins[s] = make_int4(nx * 8, ny * 8, nz * 8, 0);
-                    atomicExch(cell, (int32_t)s);
-                } else {
-                    atomicOr(&st->fail, 2u);  // pool overflow: retried with a larger pool
-                }
-            }
-        }
+        sweep_and_finish(L, plane_vox, a, slot, org, plane, n_planes, start, created, st, lane);
         __syncthreads();
     }
 }
@@ -546,6 +605,8 @@ OffsetResult offset_attempt(bs_ctx* ctx, bs_volume* vol, float distance, uint64_
 
     // plane range over the whole map box
     uint32_t* order = B.alloc<uint32_t>(capacity);
+    int4* org_order = B.alloc<int4>(capacity);
+
     uint64_t planes_total = 0, launches = 0, swept = 0;
     uint32_t pool = (uint32_t)n_keep;  // pool size at the start of the sweep
     for (int s = 0; s < 8; ++s) {
@@ -578,7 +639,7 @@ OffsetResult offset_attempt(bs_ctx* ctx, bs_volume* vol, float distance, uint64_
         exclusive_scan_u32(ctx, hist, off, n_planes + 1);
         BS_HIP(hipMemcpyAsync(hist, off, sizeof(uint32_t) * (n_planes + 1), hipMemcpyDeviceToDevice, st));
         k_plane_scatter<<<div_up(n_pool, 256), 256, 0, st>>>(n_pool, a.origins, a.sx, a.sy, a.sz, kmin, n_planes, hist,
-                                                            order);
+                                                            order, org_order);
         BS_HIP(hipGetLastError());
         // host view of the bucket sizes: the first and last plane holding a leaf
         std::vector<uint32_t> hoff(n_planes + 1);
@@ -595,7 +656,6 @@ OffsetResult offset_attempt(bs_ctx* ctx, bs_volume* vol, float distance, uint64_
             max_bucket = std::max(max_bucket, c);
         }
         if (!any) break;
-        (void)max_bucket;
         // starts[p + 1], created[p + 1]: pool size when plane p begins, leaves it creates
         uint32_t* starts = B.alloc<uint32_t>(n_planes + 2);
         uint32_t* created = B.alloc<uint32_t>(n_planes + 2);
@@ -614,7 +674,7 @@ OffsetResult offset_attempt(bs_ctx* ctx, bs_volume* vol, float distance, uint64_
             // enough blocks for the bucket and the leaves the previous plane may have created
             const uint32_t prev = p > first ? hoff[p] - hoff[p - 1] : 0;
             const unsigned grid = (unsigned)std::min<uint64_t>(std::max<uint64_t>((uint64_t)bucket + prev + 64, 64), 4096);
-            k_sweep_plane<<<grid, 64, 0, st>>>(a, order, off, p, n_planes, starts, created, dstate);
+            k_sweep_plane<<<grid, 64, 0, st>>>(a, order, org_order, off, p, n_planes, starts, created, dstate);
             BS_HIP(hipGetLastError());
             ++planes_total;
             ++launches;
