@@ -37,29 +37,6 @@ namespace {
 
 constexpr float FS_FAR = 3.40282347e+38f;
 
-// Voxels of an 8^3 leaf grouped by diagonal plane u + v + w = p (p = 0..21),
-// entry = u << 6 | v << 3 | w.
-struct VoxelPlanes {
-    uint16_t start[23];
-    uint16_t entry[512];
-};
-constexpr VoxelPlanes make_voxel_planes() {
-    VoxelPlanes t{};
-    int n = 0;
-    for (int p = 0; p < 22; ++p) {
-        t.start[p] = (uint16_t)n;
-        for (int u = 0; u < 8; ++u)
-            for (int v = 0; v < 8; ++v) {
-                const int w = p - u - v;
-                if (w >= 0 && w < 8) t.entry[n++] = (uint16_t)((u << 6) | (v << 3) | w);
-            }
-    }
-    t.start[22] = (uint16_t)n;
-    return t;
-}
-__constant__ VoxelPlanes kVoxelPlanes = make_voxel_planes();
-constexpr VoxelPlanes kVoxelPlanesHost = make_voxel_planes();  // plane sizes folded into unrolled loops
-
 struct OffsetState {
     uint32_t pool_count;  // leaves in the pool before the sweeps
     uint32_t fail;        // bit 0: a leaf outside the reachable box (bit 2: past the last plane,
@@ -173,7 +150,6 @@ __device__ inline int pidx(int x, int y, int z) { return ((x + 1) * PB + (y + 1)
 struct LeafLds {
     float val[PB * PB * PB];
     uint8_t st[PB * PB * PB];
-    int32_t nb[6];  // -x, +x, -y, +y, -z, +z neighbour slots
 };
 
 // option_min_by(at(+), at(-), cmp_abs) (utils.rs:64-73, fast_sweep.rs:137-140):
@@ -220,9 +196,10 @@ __device__ inline void sweep_voxel(LeafLds& L, const SweepArgs& a, int o) {
     const float dz = pick(szp & 1, vzp, szm & 1, vzm);
     const float first = hx ? dx : (hy ? dy : dz);
     const bool go = !(so & 2) && (hx || hy || hz) && (int)(__float_as_uint(first) >> 31) == a.neg;
+    if (!go) return;  // a wave with no such voxel skips the solve
     const float d_abs = compute_distance(fabsf(hx ? dx : FS_FAR), fabsf(hy ? dy : FS_FAR), fabsf(hz ? dz : FS_FAR), a.h);
     const float d_old = (so & 1) ? vo : FS_FAR;
-    if (go && !(d_abs > a.limit_abs) && d_abs < fabsf(d_old)) {
+    if (!(d_abs > a.limit_abs) && d_abs < fabsf(d_old)) {
         L.val[o] = a.neg ? -fabsf(d_abs) : fabsf(d_abs);  // copysign(d_abs, sweep sign)
         L.st[o] = so | 1;
     }
@@ -270,28 +247,35 @@ __device__ inline void stage_store(LeafLds& L, const LeafStage& s, int lane) {
         L.val[base + k] = v[k];
         L.st[base + k] = (uint8_t)(((mw >> k) & 1) | (((fw >> k) & 1) << 1));
     }
-    if (lane < 6) L.nb[lane] = s.nb >= 0 ? s.nb : -1;  // a cell being created (-2) is no leaf yet
 }
 
-// The six neighbour layers next to the leaf, into the padding (L.nb visible).
-__device__ inline void stage_layers(LeafLds& L, const SweepArgs& a, int lane) {
+// The six neighbour layers next to the leaf, into the padding.  nb: lanes
+// 0..5 hold the neighbour slots (-1 none; a cell being created, -2, is no
+// leaf yet).  The slots are read from those lanes and every one of the twelve
+// loads is issued unconditionally (slot 0 in place of a missing neighbour,
+// its result discarded), so they travel in one round trip.
+__device__ inline void stage_layers(LeafLds& L, const SweepArgs& a, int lane, int32_t nb) {
     const int i = lane >> 3, j = lane & 7;
     uint8_t on[6];
     float val[6];
+    uint64_t mw[6];
+    float nv[6];
+    int32_t n[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-        const int32_t n = L.nb[k];
+        n[k] = __builtin_amdgcn_readlane(nb, k);
         const int f = (k & 1) ? 0 : 7;  // -side neighbour: its far layer 7; +side: layer 0
         const int o = k < 2 ? ((f << 6) | (i << 3) | j) : (k < 4 ? ((i << 6) | (f << 3) | j) : ((i << 6) | (j << 3) | f));
-        on[k] = 0;
-        val[k] = 0.0f;
-        if (n >= 0) {
-            // all twelve loads in flight together
-            const uint64_t mw = a.masks[(uint64_t)n * 8 + (o >> 6)];
-            const float nv = a.values[(uint64_t)n * 512 + o];
-            on[k] = (mw >> (o & 63)) & 1ull;
-            val[k] = on[k] ? nv : 0.0f;
-        }
+        const uint64_t ns = n[k] >= 0 ? (uint64_t)n[k] : 0;
+        mw[k] = a.masks[ns * 8 + (o >> 6)];
+        nv[k] = a.values[ns * 512 + o];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int f = (k & 1) ? 0 : 7;
+        const int o = k < 2 ? ((f << 6) | (i << 3) | j) : (k < 4 ? ((i << 6) | (f << 3) | j) : ((i << 6) | (j << 3) | f));
+        on[k] = n[k] >= 0 ? (uint8_t)((mw[k] >> (o & 63)) & 1ull) : (uint8_t)0;
+        val[k] = on[k] ? nv[k] : 0.0f;
     }
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
@@ -302,35 +286,23 @@ __device__ inline void stage_layers(LeafLds& L, const SweepArgs& a, int lane) {
     }
 }
 
-// Each lane's voxel on each of the 22 voxel planes of a leaf in sweep order
-// (padded index, -1 for none).
-__device__ inline void init_plane_vox(int16_t (*plane_vox)[64], const SweepArgs& a, int lane) {
-#pragma unroll
-    for (int p = 0; p < 22; ++p) {  // unrolled: the 22 table reads are in flight together
-        const int s0 = kVoxelPlanesHost.start[p], cnt = kVoxelPlanesHost.start[p + 1] - s0;
-        int16_t q = -1;
-        if (lane < cnt) {
-            const int e = kVoxelPlanes.entry[s0 + lane];
-            const int u = e >> 6, v = (e >> 3) & 7, w = e & 7;
-            q = (int16_t)pidx(a.sx > 0 ? u : 7 - u, a.sy > 0 ? v : 7 - v, a.sz > 0 ? w : 7 - w);
-        }
-        plane_vox[p][lane] = q;
-    }
-}
-
 // The staged leaf swept (the wavefront over its 22 voxel planes), written
 // back, and its pushes (fast_sweep.rs:186-273): the +direction face of each
 // axis; lanes 0..2 create the pushed neighbours of the three axes at once.
 // A created slot is published without a fence: in this plane a lookup of the
 // new leaf reads only its masks and values, which hold the empty leaf
 // already, and its origin is read only in a later plane.
-__device__ inline void sweep_and_finish(LeafLds& L, const int16_t (*plane_vox)[64], const SweepArgs& a,
-                                        uint32_t slot, int4 org, uint32_t plane, uint32_t n_planes, uint32_t start,
-                                        uint32_t* created, OffsetState* st, int lane) {
+__device__ inline void sweep_and_finish(LeafLds& L, const SweepArgs& a, uint32_t slot, int4 org, uint32_t plane,
+                                        uint32_t n_planes, uint32_t start, uint32_t* created, OffsetState* st,
+                                        int lane) {
     const int lx = lane >> 3, ly = lane & 7;
+    // voxel plane p holds the voxels u + v + w = p (u, v, w counted in the
+    // sweep direction); they are independent of each other, so lane (u, v)
+    // = (lx, ly) takes w = p - u - v when that is a voxel
+    const int pu = a.sx > 0 ? lx : 7 - lx, pv = a.sy > 0 ? ly : 7 - ly;
     for (int p = 0; p < 22; ++p) {
-        const int q = plane_vox[p][lane];
-        if (q >= 0) sweep_voxel(L, a, q);
+        const int w = p - lx - ly;
+        if (w >= 0 && w < 8) sweep_voxel(L, a, pidx(pu, pv, a.sz > 0 ? w : 7 - w));
         __syncthreads();
     }
     {
@@ -387,7 +359,6 @@ __global__ __launch_bounds__(64) void k_sweep_plane(SweepArgs a, const uint32_t*
                                                      uint32_t n_planes, uint32_t* starts, uint32_t* created,
                                                      OffsetState* st) {
     __shared__ LeafLds L;
-    __shared__ int16_t plane_vox[22][64];
     const uint32_t b0 = bucket_off[plane], b1 = bucket_off[plane + 1];
     const uint32_t n_old = b1 - b0;
     const uint32_t new_lo = starts[plane];
@@ -395,7 +366,6 @@ __global__ __launch_bounds__(64) void k_sweep_plane(SweepArgs a, const uint32_t*
     const uint32_t n_new = start - new_lo;
     const int lane = threadIdx.x;
     if (blockIdx.x == 0 && lane == 0) starts[plane + 1] = start;
-    init_plane_vox(plane_vox, a, lane);
     for (uint32_t item = blockIdx.x; item < n_old + n_new; item += gridDim.x) {
         // a bucket leaf's slot and origin in one round trip, a pushed one's origin
         const uint32_t slot = item < n_old ? order[b0 + item] : new_lo + (item - n_old);
@@ -403,10 +373,9 @@ __global__ __launch_bounds__(64) void k_sweep_plane(SweepArgs a, const uint32_t*
         LeafStage s;
         stage_issue(a, slot, org, lane, s);
         stage_store(L, s, lane);
+        stage_layers(L, a, lane, s.nb >= 0 ? s.nb : -1);  // the padding: no overlap with stage_store
         __syncthreads();
-        stage_layers(L, a, lane);
-        __syncthreads();
-        sweep_and_finish(L, plane_vox, a, slot, org, plane, n_planes, start, created, st, lane);
+        sweep_and_finish(L, a, slot, org, plane, n_planes, start, created, st, lane);
         __syncthreads();
     }
 }
