@@ -220,14 +220,34 @@ std::unique_ptr<Topology> Topology::from_leaves(const std::vector<std::array<int
         return t->n2[a.child[o]];
     };
     t->leaves.resize(origins.size());
+    LeafVec& recs = t->leaves.seg[0];
     const size_t n = origins.size();
     bool sorted = n >= 65536;
     for (size_t i = 1; sorted && i < n; ++i) sorted = origins[i - 1] < origins[i];
     if (sorted) {
         // Leaves sorted by origin: those of one 128-voxel x-slab are contiguous
         // and own the 32^3-node slots of that slab, so slabs are built in
-        // parallel.  Nodes are numbered by first appearance, as serially.
-        for (size_t i = 0; i < n; ++i) node1_for(origins[i][0], origins[i][1], origins[i][2]);
+        // parallel.  Nodes are numbered by first appearance, as serially: the
+        // few distinct 4096^3 keys are listed per chunk in order of first
+        // appearance, in parallel, and created chunk by chunk.
+        {
+            constexpr size_t CH = 32768;
+            const size_t n_ch = (n + CH - 1) / CH;
+            std::vector<std::vector<Key>> seen(n_ch);
+            parallel_for(n_ch, [&](size_t c) {
+                std::vector<Key>& ks = seen[c];
+                Key last{INT32_MIN, 0, 0};
+                for (size_t i = c * CH, e = std::min(n, (c + 1) * CH); i < e; ++i) {
+                    const Key k{origins[i][0] & ~(L1_SPAN - 1), origins[i][1] & ~(L1_SPAN - 1),
+                                origins[i][2] & ~(L1_SPAN - 1)};
+                    if (k == last) continue;
+                    last = k;
+                    if (std::find(ks.begin(), ks.end(), k) == ks.end()) ks.push_back(k);
+                }
+            }, 1);
+            for (const auto& ks : seen)
+                for (const Key& k : ks) node1_for(k[0], k[1], k[2]);
+        }
         std::vector<size_t> cut{0};
         for (size_t i = 1; i < n; ++i)
             if ((origins[i][0] >> 7) != (origins[i - 1][0] >> 7)) cut.push_back(i);
@@ -259,12 +279,7 @@ std::unique_ptr<Topology> Topology::from_leaves(const std::vector<std::array<int
                 const int o = l2_slot(g[0], g[1], g[2]);
                 b.kind[o] = SLOT_BRANCH;
                 b.child[o] = (int32_t)i;
-                LeafRef& r = t->leaves[i];
-                r.x = g[0];
-                r.y = g[1];
-                r.z = g[2];
-                r.a = (int32_t)i;
-                r.op = LEAF_COPY_A;
+                recs[i] = leaf_ref(g[0], g[1], g[2], (int32_t)i);
             }
         }, 1);
         size_t total = 0;
@@ -282,12 +297,7 @@ std::unique_ptr<Topology> Topology::from_leaves(const std::vector<std::array<int
         const int o = l2_slot(g[0], g[1], g[2]);
         b.kind[o] = SLOT_BRANCH;
         b.child[o] = (int32_t)i;
-        LeafRef& r = t->leaves[i];
-        r.x = g[0];
-        r.y = g[1];
-        r.z = g[2];
-        r.a = (int32_t)i;
-        r.op = LEAF_COPY_A;
+        recs[i] = leaf_ref(g[0], g[1], g[2], (int32_t)i);
     }
     for (const auto& tl : tiles) {
         if (tl.size == L2_SPAN) {
@@ -450,12 +460,16 @@ struct Csg {
     // emptied pair is applied afterwards with the same effect.
     void node1_pair(int32_t a, int32_t b) {
         std::vector<std::array<int32_t, 3>> pairs;  // (slot, node of self, node of other)
+        std::vector<int32_t> flips;                 // 16^3 nodes moved in and negated
         slots(t.n1[a], t.n1[b],
               [&](int32_t ka, int32_t kb, int o) {
                   pairs.push_back({o, ka, kb});
                   return false;
               },
-              [&](int32_t k) { flip_node2(k); });
+              [&](int32_t k) { flips.push_back(k); });
+        // a negation touches only its own node and leaves, and no later step
+        // of the slot pass reads them: applied afterwards, in parallel
+        parallel_for(flips.size(), [&](size_t i) { flip_node2(flips[i]); }, 1);
         Node1& s = t.n1[a];
         std::vector<uint8_t> empty(pairs.size(), 0);
         parallel_for(pairs.size(), [&](size_t i) { empty[i] = node2_pair(pairs[i][1], pairs[i][2]); });
@@ -741,7 +755,7 @@ std::unique_ptr<Topology> Topology::after_offset(const std::vector<uint32_t>& ke
             t->n1[ai].child[o1] = (int32_t)t->n2.size() - 1;
         }
     }
-    std::vector<LeafRef>& recs = t->leaves.seg[0];
+    LeafVec& recs = t->leaves.seg[0];
     host_parallel_for(n_chunks, [&](size_t c) {
         for (size_t i = c * CHUNK, e = std::min(n_out, i + CHUNK); i < e; ++i) {
             const int32_t x = out_origins[i][0], y = out_origins[i][1], z = out_origins[i][2];
@@ -750,11 +764,7 @@ std::unique_ptr<Topology> Topology::after_offset(const std::vector<uint32_t>& ke
             const int o2 = l2_slot(x, y, z);
             b.kind[o2] = SLOT_BRANCH;
             b.child[o2] = (int32_t)i;
-            LeafRef& r = recs[i];
-            r.x = x;
-            r.y = y;
-            r.z = z;
-            r.a = (int32_t)i;
+            recs[i] = leaf_ref(x, y, z, (int32_t)i);
         }
     });
     prune();
@@ -768,13 +778,7 @@ std::vector<TileRec> Topology::densify_tiles() {
     std::vector<TileRec> made;
     auto new_leaf = [&](int32_t x, int32_t y, int32_t z, float v) {
         const int32_t idx = (int32_t)leaves.size();
-        LeafRef r;
-        r.x = x;
-        r.y = y;
-        r.z = z;
-        r.a = idx;
-        r.op = LEAF_COPY_A;
-        leaves.push_back(r);
+        leaves.push_back(leaf_ref(x, y, z, idx));
         made.push_back({x, y, z, 8, v});
         return idx;
     };

@@ -13,6 +13,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <utility>
 #include <vector>
 
 namespace bs {
@@ -29,20 +30,48 @@ struct TileRec {
     float value;
 };
 
-// A leaf of the volume being built: where its 512 values come from.
+// A leaf of the volume being built: where its 512 values come from.  Plain
+// data (no default member values), so a large table is allocated without a
+// serial initialising pass; leaf_ref gives a fresh record.
 struct LeafRef {
-    int32_t x, y, z;       // origin
-    int32_t a = -1, b = -1;  // source leaf in operand A / operand B
-    uint8_t op = LEAF_COPY_A;
-    uint8_t neg = 0;        // negate every value (flip_signs)
-    uint8_t first_neg = 0, last_neg = 0, empty = 0;  // after the leaf flood fill
+    int32_t x, y, z;  // origin
+    int32_t a, b;     // source leaf in operand A / operand B (-1: none)
+    uint8_t op;       // LEAF_*
+    uint8_t neg;      // negate every value (flip_signs)
+    uint8_t first_neg, last_neg, empty;  // after the leaf flood fill
 };
+inline LeafRef leaf_ref(int32_t x, int32_t y, int32_t z, int32_t a) {
+    return LeafRef{x, y, z, a, -1, LEAF_COPY_A, 0, 0, 0, 0};
+}
+
+// Allocator whose value-less construct leaves a trivially constructible
+// element uninitialised: resizing a table of n records touches no memory, and
+// the threads that fill it are the first to touch its pages.
+template <class T>
+struct DefaultInitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = DefaultInitAlloc<U>;
+    };
+    DefaultInitAlloc() = default;
+    template <class U>
+    DefaultInitAlloc(const DefaultInitAlloc<U>&) noexcept {}
+    template <class U>
+    void construct(U* p) noexcept {
+        ::new (static_cast<void*>(p)) U;
+    }
+    template <class U, class... A>
+    void construct(U* p, A&&... args) {
+        ::new (static_cast<void*>(p)) U(std::forward<A>(args)...);
+    }
+};
+using LeafVec = std::vector<LeafRef, DefaultInitAlloc<LeafRef>>;
 
 // Leaf references in segments, index i running over them in order: a CSG
 // operand's references join the result by moving its segments in, without
 // copying them.
 struct LeafStore {
-    std::vector<std::vector<LeafRef>> seg;
+    std::vector<LeafVec> seg;
     size_t size() const {
         size_t n = 0;
         for (const auto& s : seg) n += s.size();
@@ -56,7 +85,8 @@ struct LeafStore {
         std::abort();  // an index past the end is a bug of the caller
     }
     const LeafRef& operator[](size_t i) const { return const_cast<LeafStore&>(*this)[i]; }
-    // one segment of n default references (the store must be empty)
+    // one segment of n uninitialised references, every one of which the
+    // caller writes (the store must be empty)
     void resize(size_t n) {
         seg.clear();
         seg.emplace_back(n);
