@@ -892,7 +892,7 @@ __global__ void k_level_starts(const uint32_t* __restrict__ hs, uint32_t n, uint
 // tree is laid out depth-first with the two children of every node side by
 // side, 128-byte aligned (one cache line for the sibling pair the traversal
 // reads together), and the triangles in the same depth-first order.
-constexpr int PLOC_R = 32;
+constexpr int PLOC_R = 16;
 constexpr int PLOC_BLOCK = 256;
 constexpr uint64_t PLOC_MIN_TRIS = 1u << 18;  // smaller meshes take the Morton split
 
