@@ -470,10 +470,11 @@ __global__ __launch_bounds__(DIST_BLOCK) void k_distance(const int32_t* __restri
             const uint32_t m_yz = kMagic[nyz], m_z = kMagic[nz];
             asm volatile("" ::"s"(m_yz), "s"(m_z));  // both table reads in one round trip
             const int spp = (int)((64u * m_yz) >> 16);  // x-slices per pass, 64 / nyz
-            const int s = (int)(((uint32_t)lane * m_yz) >> 16);
-            const int rr = lane - s * nyz;
-            const int yq = (int)(((uint32_t)rr * m_z) >> 16);
-            const int y = y0 + yq, z = z0 + (rr - yq * nz);
+            // 24-bit multiplies (full rate; every operand is below 2^17)
+            const int s = (int)(__umul24((uint32_t)lane, m_yz) >> 16);
+            const int rr = lane - (int)__umul24((uint32_t)s, (uint32_t)nyz);
+            const int yq = (int)(__umul24((uint32_t)rr, m_z) >> 16);
+            const int y = y0 + yq, z = z0 + (rr - (int)__umul24((uint32_t)yq, (uint32_t)nz));
             const int loff = ((y - oy) << 3) | (z - oz);
             const float py = (float)y * h, pz = (float)z * h;
             if (s < spp) {

@@ -326,14 +326,19 @@ def volume_from_voxels(idx, values, voxel_size: float, ctx: Optional[Context] = 
     return Volume(h, ctx)
 
 
-def batch_mesh_to_volume(meshes, voxel_sizes, band: int, ctx: Optional[Context] = None, workers: int = 0,
+def batch_mesh_to_volume(meshes, voxel_sizes, band: int, ctx: Optional[Context] = None, workers=0,
                          shared_signs: bool = False):
     """MeshToVolume::convert over many meshes in one library call.
 
     `meshes`: CUDA tensors (device input) or arrays (host input), all of one
-    kind; `voxel_sizes`: one per mesh.  Returns a list of Volume or None
-    (empty mesh), in input order."""
+    kind; `voxel_sizes`: one per mesh; `workers`: worker streams (0: the
+    library's default of 4; "auto": the count tune.batch_workers measured or
+    cached for this device and batch shape).  Returns a list of Volume or
+    None (empty mesh), in input order."""
     ctx = ctx or default_context()
+    if workers == "auto":
+        from .tune import batch_workers
+        workers = batch_workers(meshes, voxel_sizes, band, ctx=ctx, shared_signs=shared_signs)
     L = load()
     n = len(meshes)
     dev = n > 0 and hasattr(meshes[0], "is_cuda") and meshes[0].is_cuda

@@ -410,7 +410,14 @@ def batch(ctx, count=256, seed=5):
     res = {"workload": f"{count} meshes (tori / displaced icospheres), {sum(s.shape[0] for s, _ in ms)} triangles "
                        f"in all, {min(s.shape[0] for s, _ in ms)}-{max(s.shape[0] for s, _ in ms)} each, "
                        "bounding box 256 voxels, band 3"}
-    for name, workers in (("streams_8", 8), ("streams_4", 4), ("streams_1", 1)):
+    # the autotuned worker count (timed once per device and batch shape, then cached)
+    from baby_shark_amd import tune
+    res["autotuned_workers"] = tune.batch_workers(soups, hs, 3, ctx=ctx)
+    res["autotune_cache"] = tune.load_cache().get(
+        tune.batch_key(torch.cuda.get_device_name(0), [s.shape[0] for s in soups], 3, False))
+    runs = [("streams_8", 8), ("streams_4", 4), ("streams_1", 1)]
+    runs.insert(0, ("autotuned", res["autotuned_workers"]))
+    for name, workers in runs:
         best = None
         for _ in range(2):
             t0 = time.perf_counter()

@@ -32,6 +32,30 @@ def test_batch_matches_single_calls(gpu_ctx, device_input):
         assert_same_volume(v.export(), single.export())
 
 
+def test_autotuned_workers(gpu_ctx, tmp_path, monkeypatch):
+    """workers="auto": the tuner times the batch at each candidate count,
+    caches the fastest for this device and batch shape, and reads it back the
+    next time; the volumes equal the one-stream call's."""
+    import json
+    import torch
+    from baby_shark_amd import tune
+    monkeypatch.setenv("BS_TUNE_CACHE", str(tmp_path / "tune.json"))
+    ms = [m for m in _meshes() if m[0].shape[0] > 0]
+    soups = [torch.from_numpy(s).cuda() for s, _ in ms]
+    hs = [h for _, h in ms]
+    w = tune.batch_workers(soups, hs, 3, ctx=gpu_ctx, candidates=(1, 2, 4))
+    cache = json.loads((tmp_path / "tune.json").read_text())
+    key = tune.batch_key(torch.cuda.get_device_name(0), [s.shape[0] for s in soups], 3, False)
+    print(f"autotuned workers {w}: {cache[key]}", flush=True)
+    assert cache[key]["workers"] == w and w in (1, 2, 4)
+    assert set(cache[key]["ms"]) == {"1", "2", "4"}
+    assert tune.batch_workers(soups, hs, 3, ctx=gpu_ctx, candidates=(1, 2, 4)) == w  # from the cache
+    a = bs.batch_mesh_to_volume(soups, hs, 3, ctx=gpu_ctx, workers="auto")
+    b = bs.batch_mesh_to_volume(soups, hs, 3, ctx=gpu_ctx, workers=1)
+    for va, vb in zip(a, b):
+        assert_same_volume(va.export(), vb.export())
+
+
 def test_batch_reports_per_mesh_errors(gpu_ctx):
     L = _lib.load()
     import ctypes as C
