@@ -1082,14 +1082,21 @@ __global__ void k_ploc_place(const uint32_t* __restrict__ root_buf, const PlocIn
 // form: tan(W/2) = det[a b c] / (|a||b||c| + (a.b)|c| + (a.c)|b| + (b.c)|a|),
 // which equals the reference's normalised expression.  Zero when a vertex
 // coincides with the query or the triple product vanishes.
+// The triple product keeps the reference's unfused f32 arithmetic: for a
+// point a few ulps off a triangle's plane, whether it rounds to zero decides
+// between a zero solid angle and +-2 pi, as in the reference.  The lengths and
+// the denominator only scale the angle and are written with explicit FMAs
+// (the library is built without contraction, for the distance arithmetic).
 __device__ inline float solid_angle_rel(float ax, float ay, float az, float bx, float by, float bz, float cx, float cy,
                                         float cz) {
-    const float la = fast_sqrt(ax * ax + ay * ay + az * az);
-    const float lb = fast_sqrt(bx * bx + by * by + bz * bz);
-    const float lc = fast_sqrt(cx * cx + cy * cy + cz * cz);
+    const float la = fast_sqrt(__builtin_fmaf(ax, ax, __builtin_fmaf(ay, ay, az * az)));
+    const float lb = fast_sqrt(__builtin_fmaf(bx, bx, __builtin_fmaf(by, by, bz * bz)));
+    const float lc = fast_sqrt(__builtin_fmaf(cx, cx, __builtin_fmaf(cy, cy, cz * cz)));
     const float det = ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx);
-    const float den = la * lb * lc + (ax * bx + ay * by + az * bz) * lc + (ax * cx + ay * cy + az * cz) * lb +
-                      (bx * cx + by * cy + bz * cz) * la;
+    const float ab = __builtin_fmaf(ax, bx, __builtin_fmaf(ay, by, az * bz));
+    const float ac = __builtin_fmaf(ax, cx, __builtin_fmaf(ay, cy, az * cz));
+    const float bc = __builtin_fmaf(bx, cx, __builtin_fmaf(by, cy, bz * cz));
+    const float den = __builtin_fmaf(la * lb, lc, __builtin_fmaf(ab, lc, __builtin_fmaf(ac, lb, bc * la)));
     const float w = 2.0f * fast_atan2(det, den);
     return (la == 0.0f || lb == 0.0f || lc == 0.0f || det == 0.0f) ? 0.0f : w;
 }
