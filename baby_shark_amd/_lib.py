@@ -28,6 +28,7 @@ BS_MESHING_MANIFOLD = 1
 BS_PATH_FORCE_HASH = 1
 BS_PATH_FORCE_LDS_STACK = 2
 BS_PATH_LBVH = 4
+BS_PATH_PLOC = 8
 BS_DC_VERTEX_PARTICLE = 0
 BS_DC_VERTEX_QEF_F64 = 1
 BS_SIGN_PER_VOXEL = 0

@@ -140,7 +140,7 @@ bs_status bs_ctx_set_profiling(bs_ctx* ctx, int enabled) {  // flags
 }
 
 bs_status bs_ctx_set_paths(bs_ctx* ctx, int flags) {
-    if (!ctx || (flags & ~(BS_PATH_FORCE_HASH | BS_PATH_FORCE_LDS_STACK | BS_PATH_LBVH))) {
+    if (!ctx || (flags & ~(BS_PATH_FORCE_HASH | BS_PATH_FORCE_LDS_STACK | BS_PATH_LBVH | BS_PATH_PLOC))) {
         g_last_error = "unknown BS_PATH_* flag";
         return BS_ERR_INVALID_ARG;
     }

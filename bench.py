@@ -537,9 +537,9 @@ def main():
                         "frac": round(v["tflops"] / PEAK_FP32_TFLOPS, 4)} for k, v in kernels.items()},
         "phases_ms": {k: round(float(np.mean([p[k] for p in prof])), 4)
                       for k in ("ms_subdivide", "ms_leaves", "ms_distance", "ms_bvh", "ms_winding")},
-        # the winding-number tree (clustered build) and its visits per active voxel
-        "winding_tree": {"build": "ploc" if st["n_bvh_iterations"] else "lbvh",
-                         "iterations": int(st["n_bvh_iterations"]), "height": int(st["bvh_height"]),
+        # the winding-number tree (top-down binned build) and its visits per active voxel
+        "winding_tree": {"build": "topdown" if st["n_bvh_iterations"] else "lbvh",
+                         "level_passes": int(st["n_bvh_iterations"]), "height": int(st["bvh_height"]),
                          "near_visits_per_voxel": round(st["wn_node_visits"] / n_active, 2),
                          "far_evals_per_voxel": round(st["wn_far_evals"] / n_active, 2),
                          "leaf_visits_per_voxel": round(st["wn_leaf_visits"] / n_active, 2),

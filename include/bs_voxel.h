@@ -93,7 +93,8 @@ typedef struct {
      * in which some voxel failed the exact check (misclassified), and the
      * voxels that failed it (evaluated themselves) */
     uint64_t n_sg_tiles, n_sg_culled, n_sg_misclassified, n_sg_fallback;
-    /* clustering iterations of the winding-number tree (0 for BS_PATH_LBVH) */
+    /* passes of the winding-number tree build: level passes of the top-down
+     * build, or clustering iterations of BS_PATH_PLOC (0 for the Morton split) */
     uint64_t n_bvh_iterations;
 } bs_m2v_stats;
 
@@ -157,13 +158,17 @@ bs_status bs_ctx_last_m2v_stats(const bs_ctx* ctx, bs_m2v_stats* out);
  *                         trees low enough for the register stack (height < 62);
  * BS_PATH_LBVH            winding-number tree split by Morton order (Karras 2012)
  *                         also for meshes of 2^18 triangles or more, which
- *                         otherwise take the clustered build (PLOC).  Distances are
- *                         identical; the winding numbers differ by the far-field
- *                         error of the two trees, so a sign can differ only where
- *                         the winding number is that close to the threshold. */
+ *                         otherwise take the top-down binned build;
+ * BS_PATH_PLOC            the clustered build (PLOC, Meister and Bittner 2018)
+ *                         for those meshes instead (BS_PATH_LBVH wins if both
+ *                         are set).  With any tree the distances are identical;
+ *                         the winding numbers differ by the far-field error of
+ *                         the trees, so a sign can differ only where the winding
+ *                         number is that close to the threshold. */
 #define BS_PATH_FORCE_HASH 1
 #define BS_PATH_FORCE_LDS_STACK 2
 #define BS_PATH_LBVH 4
+#define BS_PATH_PLOC 8
 bs_status bs_ctx_set_paths(bs_ctx* ctx, int flags);
 
 void bs_m2v_params_default(bs_m2v_params* p);

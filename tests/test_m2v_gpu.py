@@ -240,15 +240,9 @@ def _visits(st):
     return st["wn_node_visits"] + st["wn_far_evals"] + st["wn_leaf_visits"]
 
 
-def test_clustered_tree_against_morton_tree(gpu_ctx, oracle_lib):
-    """The clustered BVH (default from 2^18 triangles) and the Morton-split
-    LBVH (BS_PATH_LBVH) give identical distances and masks and the same signs
-    off the surface; the clustered tree needs fewer node visits per voxel.
-    Both match the oracle (the reference's own SAH tree)."""
-    soup = mg.displaced_icosphere(7, radius=0.8, seed=12)
-    h, band = 0.005, 3
+def _tree_runs(gpu_ctx, soup, h, band, trees):
     runs = {}
-    for name, flags in (("ploc", 0), ("lbvh", bs._lib.BS_PATH_LBVH)):
+    for name, flags in trees:
         gpu_ctx.set_paths(flags)
         gpu_ctx.set_profiling(True, counters=True)
         try:
@@ -258,22 +252,59 @@ def test_clustered_tree_against_morton_tree(gpu_ctx, oracle_lib):
         finally:
             gpu_ctx.set_profiling(False)
             gpu_ctx.set_paths(0)
-    (a, sa), (b, sb) = runs["ploc"], runs["lbvh"]
-    n_act = sa["n_active"]
-    print(f"clustered: {sa['n_bvh_iterations']} iterations, height {sa['bvh_height']}, "
-          f"{_visits(sa) / n_act:.1f} visits/voxel (near {sa['wn_node_visits'] / n_act:.1f}, "
-          f"far {sa['wn_far_evals'] / n_act:.1f}); Morton split: height {sb['bvh_height']}, "
-          f"{_visits(sb) / n_act:.1f} (near {sb['wn_node_visits'] / n_act:.1f}, far {sb['wn_far_evals'] / n_act:.1f})",
-          flush=True)
-    assert sa["n_bvh_iterations"] > 0 and sb["n_bvh_iterations"] == 0
+    return runs
+
+
+def _same_but_signs_near_surface(a, b, h):
     assert np.array_equal(a.origins, b.origins) and np.array_equal(a.masks, b.masks)
     assert np.array_equal(np.abs(a.values), np.abs(b.values))
-    act = a.active()
-    li, off = np.nonzero(act)
+    li, off = np.nonzero(a.active())
     off_surface = ~on_surface(a.values[li, off], voxel_points(a.origins, li, off, h), h)
     assert np.array_equal(np.signbit(a.values[li, off])[off_surface], np.signbit(b.values[li, off])[off_surface])
-    assert _visits(sa) < _visits(sb)
+
+
+def test_tree_builds_agree(gpu_ctx, oracle_lib):
+    """The three winding-number trees of a mesh of >= 2^18 triangles -- the
+    top-down binned tree (default), the clustered one (BS_PATH_PLOC) and the
+    Morton split (BS_PATH_LBVH) -- give identical distances and masks and the
+    same signs off the surface; the top-down tree needs the fewest node
+    evaluations, the Morton split the most.  The default matches the oracle
+    (the reference's own SAH tree)."""
+    soup = mg.displaced_icosphere(7, radius=0.8, seed=12)
+    h, band = 0.005, 3
+    runs = _tree_runs(gpu_ctx, soup, h, band, (("topdown", 0), ("ploc", bs._lib.BS_PATH_PLOC),
+                                               ("lbvh", bs._lib.BS_PATH_LBVH)))
+    n_act = runs["topdown"][1]["n_active"]
+    for name, (_, st) in runs.items():
+        print(f"{name}: {st['n_bvh_iterations']} passes, height {st['bvh_height']}, "
+              f"{_visits(st) / n_act:.1f} visits/voxel (near {st['wn_node_visits'] / n_act:.1f}, "
+              f"far {st['wn_far_evals'] / n_act:.1f}, leaves {st['wn_leaf_visits'] / n_act:.2f})", flush=True)
+    st_td, st_pl, st_lb = runs["topdown"][1], runs["ploc"][1], runs["lbvh"][1]
+    assert st_td["n_bvh_iterations"] > 0 and st_pl["n_bvh_iterations"] > 0 and st_lb["n_bvh_iterations"] == 0
+    _same_but_signs_near_surface(runs["topdown"][0], runs["lbvh"][0], h)
+    _same_but_signs_near_surface(runs["ploc"][0], runs["lbvh"][0], h)
+    assert _visits(st_td) < _visits(st_pl) < _visits(st_lb)
     check_case(gpu_ctx, oracle_lib, soup, h, band)
+
+
+def test_topdown_tree_coincident_centroids(gpu_ctx):
+    """300 copies of each triangle of a closed 1280-triangle mesh (384,000
+    triangles): nodes whose triangles share one centre cannot be cut by bins
+    and are cut at the middle of their range, both in the level passes and in
+    the per-wave subtrees.  The build must finish with a full tree; distances
+    and masks do not depend on the tree and equal the Morton split's (the
+    signs are not compared: the far-field error adds up over the copies)."""
+    base = mg.displaced_icosphere(3, radius=0.8, seed=4)
+    soup = np.ascontiguousarray(np.repeat(base, 300, axis=0))
+    assert soup.shape[0] >= 1 << 18
+    h, band = 0.02, 2
+    runs = _tree_runs(gpu_ctx, soup, h, band, (("topdown", 0), ("lbvh", bs._lib.BS_PATH_LBVH)))
+    st = runs["topdown"][1]
+    print(f"topdown: height {st['bvh_height']}, {st['n_bvh_iterations']} passes", flush=True)
+    assert 0 < st["bvh_height"] < 100
+    a, b = runs["topdown"][0], runs["lbvh"][0]
+    assert np.array_equal(a.origins, b.origins) and np.array_equal(a.masks, b.masks)
+    assert np.array_equal(np.abs(a.values), np.abs(b.values))
 
 
 @pytest.mark.parametrize("n", [2, 3, 4, 5, 9])
