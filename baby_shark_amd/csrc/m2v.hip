@@ -1085,35 +1085,41 @@ __global__ void k_ploc_place(const uint32_t* __restrict__ root_buf, const PlocIn
 // far visits as the reference's 12-bucket SAH tree (aabb_tree.rs:419-514) and
 // fewer triangle evaluations.
 //
-// Nodes of more than TD_SMALL triangles are cut level by level: the level's
-// nodes are split into tasks of TD_CHUNK triangles (k_td_tasks), each task
-// bins its triangles in LDS and merges them into its node's bins (k_td_bin),
-// one wave per node picks the cut and creates the children (k_td_select), and
-// each task moves its triangles, in order, to their side in the other buffer
-// (k_td_partition).  A node of at most TD_SMALL triangles is built down to
-// single triangles by one wave (k_td_small).  Node ids follow the clustered
-// build (the triangle at position k is node k, internal nodes are n, n+1,
-// ...), so the node data per height and the depth-first layout
-// (k_ploc_place) are shared with it.  Ids are taken with atomics, so they vary
-// from run to run, but the tree, the triangle order and the layout do not.
+// The tree is cut level by level; each level moves the triangles of the nodes
+// it cuts, in order, to their side in the other of two buffers.  A node of
+// more than TD_SMALL triangles is split into tasks of TD_CHUNK triangles
+// (k_td_tasks); each task bins its triangles in LDS and merges them into the
+// node's bins (k_td_bin), one wave per node picks the cut and creates the
+// children (k_td_select), and each task moves its triangles (k_td_partition).
+// A node of 2..TD_SMALL triangles is built down to single triangles by one
+// wave (k_td_small), four independent waves per block.  The level kernels
+// loop over their work with grids sized to the level.
+// Node ids follow the clustered build (the triangle at position k is node k,
+// internal nodes are n, n+1, ...), so the depth-first layout (k_ploc_place)
+// is shared with it.  A node of at most TD_SMALL triangles takes the ids of
+// its whole subtree when it is created (its left child's subtree first), so
+// only large nodes take ids with atomics: ids vary from run to run, the tree,
+// the triangle order and the layout do not.
 constexpr int TD_BINS = 16;
-constexpr int TD_SMALL = 64;       // subtrees of at most this many triangles: one wave
+constexpr int TD_SMALL = 64;       // nodes of at most this many triangles: one wave, one triangle per lane
 constexpr int TD_CHUNK = 2048;     // triangles per task
 constexpr int TD_BLOCK = 256;
+constexpr int TD_GRID_MAX = 4096;  // blocks of the looping level kernels, at most
 constexpr int TD_DEPTH_CAP = 64;   // deeper nodes are cut at the middle of their range
 constexpr int TD_BW = 13;          // words per bin: count, box lo xyz, box hi xyz, centroid lo xyz, hi xyz
 constexpr int TD_NODE_WORDS = 3 * TD_BINS * TD_BW;
 constexpr int TD_SW = 7;           // words per bin in k_td_small: count, box lo xyz, box hi xyz
-// counter words
-constexpr int TD_C_FR = 0;         // [0], [1]: frontier sizes of the two level buffers
-constexpr int TD_C_TASKS = 2, TD_C_SMALL = 3, TD_C_NEXT = 4, TD_C_ROOT = 5, TD_C_OUT = 6;  // TD_C_OUT: root, height
+// counter words: [0], [1] frontier sizes of the two level buffers, then
+// tasks, small-node list size, next id, root, (root, height)
+constexpr int TD_C_FR = 0, TD_C_TASKS = 2, TD_C_SMALL = 3, TD_C_NEXT = 5, TD_C_ROOT = 6, TD_C_OUT = 7;
+constexpr int TD_C_WORDS = 16;
 
 struct TdNode {  // a node of more than TD_SMALL triangles at the current level
     int begin, end, id, depth;
     float clo[3], chi[3];  // bounds of its triangles' box centres
     int task0, pad;        // its first task
 };
-struct TdSmall {
+struct TdSmall {  // a node of 2..TD_SMALL triangles; its subtree's ids are id .. id + (end - begin) - 2
     int begin, end, id, depth_buf;  // depth << 1 | the triangle buffer that holds its range
 };
 struct TdSplit {
@@ -1159,6 +1165,27 @@ __device__ inline uint64_t td_best(bool valid, float cost, int lane) {
     return (uint64_t)hi << 32 | lo;
 }
 
+// Prefix (left) and suffix (right) of the 16 bins of each axis held by lanes
+// 0..47 (lane = axis * 16 + bin), W words per bin merged as td_merge does;
+// Rn[] is the right side of a cut after the lane's bin.  Returns the
+// cheapest cut as td_best does.
+template <int W>
+__device__ inline uint64_t td_choose(uint32_t (&L)[W], uint32_t (&Rn)[W], int lane, bool allowed) {
+    const int bin = lane & (TD_BINS - 1);
+    uint32_t R[W];
+    for (int f = 0; f < W; ++f) R[f] = L[f];
+    for (int s = 1; s < TD_BINS; s <<= 1)
+        for (int f = 0; f < W; ++f) {
+            const uint32_t up = (uint32_t)__shfl_up((int)L[f], s), dn = (uint32_t)__shfl_down((int)R[f], s);
+            if (bin >= s) L[f] = td_merge(f, L[f], up);
+            if (bin + s < TD_BINS) R[f] = td_merge(f, R[f], dn);
+        }
+    for (int f = 0; f < W; ++f) Rn[f] = (uint32_t)__shfl_down((int)R[f], 1);
+    const bool valid = allowed && lane < 3 * TD_BINS && bin < TD_BINS - 1 && L[0] > 0 && Rn[0] > 0;
+    const float cost = valid ? td_diag2(L) * (float)L[0] + td_diag2(Rn) * (float)Rn[0] : 0.0f;
+    return td_best(valid, cost, lane);
+}
+
 // Exclusive prefix of x over the block (TD_BLOCK threads); *total gets the sum.
 __device__ inline uint32_t td_block_scan(uint32_t x, uint32_t* wsum, uint32_t* total) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1196,6 +1223,7 @@ __global__ void k_td_init(const float* __restrict__ tris, int n, const unsigned*
     for (int w = threadIdx.x; w < TD_NODE_WORDS; w += blockDim.x) bins[w] = td_identity(w);
     for (int w = threadIdx.x; w < 256; w += blockDim.x) hist[w] = 0;
     if (threadIdx.x == 0) {
+        for (int k = 0; k < TD_C_WORDS; ++k) ctr[k] = 0;
         TdNode r;
         r.begin = 0;
         r.end = n;
@@ -1208,10 +1236,7 @@ __global__ void k_td_init(const float* __restrict__ tris, int n, const unsigned*
         r.task0 = 0;
         r.pad = 0;
         fr[0] = r;
-        ctr[TD_C_FR] = 1;
-        ctr[TD_C_FR + 1] = 0;
-        ctr[TD_C_TASKS] = 0;
-        ctr[TD_C_SMALL] = 0;
+        ctr[TD_C_FR] = 1;  // the root is large: n >= 2^18
         ctr[TD_C_NEXT] = 1;  // the root is internal node n
         ctr[TD_C_ROOT] = (uint32_t)n;
         info[n] = PlocInfo{n, 0, 0, -1};
@@ -1220,7 +1245,7 @@ __global__ void k_td_init(const float* __restrict__ tris, int n, const unsigned*
 }
 
 // One block: the level's tasks (node, first triangle), in node order; clears
-// the next level's frontier size.
+// the next level's list sizes.
 __global__ __launch_bounds__(1024) void k_td_tasks(TdNode* __restrict__ fr, uint32_t* __restrict__ ctr, int in,
                                                    int fr_cap, int2* __restrict__ tasks) {
     __shared__ uint32_t wsum[16];
@@ -1263,336 +1288,368 @@ __global__ __launch_bounds__(1024) void k_td_tasks(TdNode* __restrict__ fr, uint
 __global__ __launch_bounds__(TD_BLOCK) void k_td_bin(const float4* __restrict__ prims, const TdNode* __restrict__ fr,
                                                      const int2* __restrict__ tasks, const uint32_t* __restrict__ ctr,
                                                      uint32_t* __restrict__ gbins, uint32_t* __restrict__ tcnt) {
-    const uint32_t t = blockIdx.x;
-    if (t >= ctr[TD_C_TASKS]) return;  // the whole block
     __shared__ uint32_t sb[TD_NODE_WORDS];
-    const int2 task = tasks[t];
-    const TdNode nd = fr[task.x];
-    for (int w = threadIdx.x; w < TD_NODE_WORDS; w += TD_BLOCK) sb[w] = td_identity(w);
-    float sc[3];
-    for (int k = 0; k < 3; ++k) sc[k] = td_scale(nd.clo[k], nd.chi[k]);
-    __syncthreads();
-    const int e = min(task.y + TD_CHUNK, nd.end);
-    for (int p = task.y + (int)threadIdx.x; p < e; p += TD_BLOCK) {
-        const float4 lo = prims[2 * (uint64_t)p], hi = prims[2 * (uint64_t)p + 1];
-        const float c[3] = {td_centre(lo.x, hi.x), td_centre(lo.y, hi.y), td_centre(lo.z, hi.z)};
-        const unsigned bl[3] = {ordered_bits(lo.x), ordered_bits(lo.y), ordered_bits(lo.z)};
-        const unsigned bh[3] = {ordered_bits(hi.x), ordered_bits(hi.y), ordered_bits(hi.z)};
-        const unsigned bc[3] = {ordered_bits(c[0]), ordered_bits(c[1]), ordered_bits(c[2])};
-        for (int k = 0; k < 3; ++k) {
-            uint32_t* b = sb + (k * TD_BINS + td_bin(c[k], nd.clo[k], sc[k])) * TD_BW;
-            atomicAdd(b, 1u);
-            for (int q = 0; q < 3; ++q) {
-                atomicMin(b + 1 + q, bl[q]);
-                atomicMax(b + 4 + q, bh[q]);
-                atomicMin(b + 7 + q, bc[q]);
-                atomicMax(b + 10 + q, bc[q]);
+    const uint32_t n_tasks = ctr[TD_C_TASKS];
+    for (uint32_t t = blockIdx.x; t < n_tasks; t += gridDim.x) {  // uniform over the block
+        const int2 task = tasks[t];
+        const TdNode nd = fr[task.x];
+        for (int w = threadIdx.x; w < TD_NODE_WORDS; w += TD_BLOCK) sb[w] = td_identity(w);
+        float sc[3];
+        for (int k = 0; k < 3; ++k) sc[k] = td_scale(nd.clo[k], nd.chi[k]);
+        __syncthreads();
+        const int e = min(task.y + TD_CHUNK, nd.end);
+        for (int p = task.y + (int)threadIdx.x; p < e; p += TD_BLOCK) {
+            const float4 lo = prims[2 * (uint64_t)p], hi = prims[2 * (uint64_t)p + 1];
+            const float c[3] = {td_centre(lo.x, hi.x), td_centre(lo.y, hi.y), td_centre(lo.z, hi.z)};
+            const unsigned bl[3] = {ordered_bits(lo.x), ordered_bits(lo.y), ordered_bits(lo.z)};
+            const unsigned bh[3] = {ordered_bits(hi.x), ordered_bits(hi.y), ordered_bits(hi.z)};
+            const unsigned bc[3] = {ordered_bits(c[0]), ordered_bits(c[1]), ordered_bits(c[2])};
+            for (int k = 0; k < 3; ++k) {
+                uint32_t* b = sb + (k * TD_BINS + td_bin(c[k], nd.clo[k], sc[k])) * TD_BW;
+                atomicAdd(b, 1u);
+                for (int q = 0; q < 3; ++q) {
+                    atomicMin(b + 1 + q, bl[q]);
+                    atomicMax(b + 4 + q, bh[q]);
+                    atomicMin(b + 7 + q, bc[q]);
+                    atomicMax(b + 10 + q, bc[q]);
+                }
             }
         }
-    }
-    __syncthreads();
-    if (threadIdx.x < 3 * TD_BINS) tcnt[(uint64_t)t * 3 * TD_BINS + threadIdx.x] = sb[threadIdx.x * TD_BW];
-    uint32_t* g = gbins + (uint64_t)task.x * TD_NODE_WORDS;
-    for (int w = threadIdx.x; w < TD_NODE_WORDS; w += TD_BLOCK) {
-        const int f = w % TD_BW;
-        if (sb[w - f] == 0) continue;  // empty bin
-        const uint32_t v = sb[w];
-        if (f == 0)
-            atomicAdd(g + w, v);
-        else if (td_is_min(f))
-            atomicMin(g + w, v);
-        else
-            atomicMax(g + w, v);
+        __syncthreads();
+        if (threadIdx.x < 3 * TD_BINS) tcnt[(uint64_t)t * 3 * TD_BINS + threadIdx.x] = sb[threadIdx.x * TD_BW];
+        uint32_t* g = gbins + (uint64_t)task.x * TD_NODE_WORDS;
+        for (int w = threadIdx.x; w < TD_NODE_WORDS; w += TD_BLOCK) {
+            const int f = w % TD_BW;
+            if (sb[w - f] == 0) continue;  // empty bin
+            const uint32_t v = sb[w];
+            if (f == 0)
+                atomicAdd(g + w, v);
+            else if (td_is_min(f))
+                atomicMin(g + w, v);
+            else
+                atomicMax(g + w, v);
+        }
+        __syncthreads();  // sb is reset for the next task
     }
 }
 
-// One wave per node of the level: the cheapest cut (prefix and suffix of the
-// bins on each axis, lanes 0..47 = axis * 16 + bin), its children (single
-// triangles become leaves at their final position, nodes of at most TD_SMALL
-// triangles go to the small list with the ids of their whole subtree, larger
-// ones to the next frontier with their centroid bounds from the bins).  Ids and
-// list slots are counted per block in LDS and taken with one global atomic per
-// counter and block.
+// Children of a cut node: a single triangle becomes a leaf at its final
+// position; a node of 2..TD_SMALL triangles goes to the small list with the
+// ids of its subtree (`ids` .. `ids` + size - 2); a larger one takes one id
+// and goes to the frontier (the caller fills its record).  *x: the next id.
+__device__ inline int td_child(int cb, int ce, int parent, int depth, int n, uint32_t id, uint8_t* ndepth,
+                               PlocInfo* info) {
+    const int cs = ce - cb;
+    if (cs == 1) {
+        info[cb] = PlocInfo{1, 0, 0, parent};
+        return cb;
+    }
+    info[n + (int)id] = PlocInfo{cs, 0, 0, parent};
+    ndepth[id] = (uint8_t)min(depth, 255);
+    return n + (int)id;
+}
+
+// One wave per large node of the level: the cheapest cut, its children
+// (small ones to the small list, larger ones to the next frontier with
+// their centroid bounds from the bins).  Ids and list slots are counted per
+// block in LDS and taken with one global atomic per counter and block.
 __global__ __launch_bounds__(TD_BLOCK) void k_td_select(
     const TdNode* __restrict__ fr, uint32_t* __restrict__ ctr, int in, const uint32_t* __restrict__ gbins, int n,
     int fr_cap, int small_cap, TdSplit* __restrict__ split, TdNode* __restrict__ fr_out,
-    uint32_t* __restrict__ gbins_out, TdSmall* __restrict__ small, PlocInfo* __restrict__ info,
+    uint32_t* __restrict__ gbins_out, TdSmall* __restrict__ small_out, PlocInfo* __restrict__ info,
     int2* __restrict__ kids, uint8_t* __restrict__ ndepth, unsigned* err) {
     __shared__ uint32_t cnt[3], base[3];  // ids, small-list slots, frontier slots of the block
-    const int i = blockIdx.x * (TD_BLOCK / 64) + (int)(threadIdx.x >> 6);
-    const bool active = i < (int)min(ctr[TD_C_FR + in], (uint32_t)fr_cap);  // whole waves
+    const uint32_t count = min(ctr[TD_C_FR + in], (uint32_t)fr_cap);
     const int lane = threadIdx.x & 63;
-    if (threadIdx.x < 3) cnt[threadIdx.x] = 0;
-    __syncthreads();
-    TdNode nd{};
-    int axis = 0, sbin = -1, nleft = 0;
-    float cl[2][3], ch[2][3];
-    uint32_t off[3] = {0, 0, 0};
-    if (active) {
-        nd = fr[i];
-        const uint32_t* g = gbins + (uint64_t)i * TD_NODE_WORDS;
-        const int bin = lane & (TD_BINS - 1);
-        uint32_t L[TD_BW], R[TD_BW];
-        for (int f = 0; f < TD_BW; ++f) L[f] = R[f] = lane < 3 * TD_BINS ? g[lane * TD_BW + f] : td_identity(f);
-        for (int s = 1; s < TD_BINS; s <<= 1)
-            for (int f = 0; f < TD_BW; ++f) {
-                const uint32_t up = (uint32_t)__shfl_up((int)L[f], s), dn = (uint32_t)__shfl_down((int)R[f], s);
-                if (bin >= s) L[f] = td_merge(f, L[f], up);
-                if (bin + s < TD_BINS) R[f] = td_merge(f, R[f], dn);
-            }
-        uint32_t Rn[TD_BW];  // the right side of a cut after this lane's bin
-        for (int f = 0; f < TD_BW; ++f) Rn[f] = (uint32_t)__shfl_down((int)R[f], 1);
-        const bool valid =
-            lane < 3 * TD_BINS && bin < TD_BINS - 1 && L[0] > 0 && Rn[0] > 0 && nd.depth < TD_DEPTH_CAP;
-        const float cost = valid ? td_diag2(L) * (float)L[0] + td_diag2(Rn) * (float)Rn[0] : 0.0f;
-        const uint64_t best = td_best(valid, cost, lane);
-        nleft = (nd.end - nd.begin) >> 1;
-        if (best != ~0ull) {
-            const int wl = (int)(best & 63);
-            axis = wl / TD_BINS;
-            sbin = wl % TD_BINS;
-            nleft = (int)__shfl((int)L[0], wl);
-            for (int q = 0; q < 3; ++q) {
-                cl[0][q] = from_ordered_bits((uint32_t)__shfl((int)L[7 + q], wl));
-                ch[0][q] = from_ordered_bits((uint32_t)__shfl((int)L[10 + q], wl));
-                cl[1][q] = from_ordered_bits((uint32_t)__shfl((int)Rn[7 + q], wl));
-                ch[1][q] = from_ordered_bits((uint32_t)__shfl((int)Rn[10 + q], wl));
-            }
-        } else {
-            for (int s = 0; s < 2; ++s)
+    for (uint32_t r0 = blockIdx.x * (TD_BLOCK / 64); r0 < count; r0 += gridDim.x * (TD_BLOCK / 64)) {
+        const int i = (int)r0 + (int)(threadIdx.x >> 6);
+        const bool active = i < (int)count;
+        if (threadIdx.x < 3) cnt[threadIdx.x] = 0;
+        __syncthreads();
+        TdNode nd{};
+        int axis = 0, sbin = -1, nleft = 0;
+        float cl[2][3], ch[2][3];
+        uint32_t off[3] = {0, 0, 0};
+        if (active) {
+            nd = fr[i];
+            const uint32_t* g = gbins + (uint64_t)i * TD_NODE_WORDS;
+            uint32_t L[TD_BW], Rn[TD_BW];
+            for (int f = 0; f < TD_BW; ++f) L[f] = lane < 3 * TD_BINS ? g[lane * TD_BW + f] : td_identity(f);
+            const uint64_t best = td_choose<TD_BW>(L, Rn, lane, nd.depth < TD_DEPTH_CAP);
+            nleft = (nd.end - nd.begin) >> 1;
+            if (best != ~0ull) {
+                const int wl = (int)(best & 63);
+                axis = wl / TD_BINS;
+                sbin = wl % TD_BINS;
+                nleft = (int)__shfl((int)L[0], wl);
                 for (int q = 0; q < 3; ++q) {
-                    cl[s][q] = nd.clo[q];
-                    ch[s][q] = nd.chi[q];
+                    cl[0][q] = from_ordered_bits((uint32_t)__shfl((int)L[7 + q], wl));
+                    ch[0][q] = from_ordered_bits((uint32_t)__shfl((int)L[10 + q], wl));
+                    cl[1][q] = from_ordered_bits((uint32_t)__shfl((int)Rn[7 + q], wl));
+                    ch[1][q] = from_ordered_bits((uint32_t)__shfl((int)Rn[10 + q], wl));
                 }
-        }
-        if (lane == 0) {
-            uint32_t need[3] = {0, 0, 0};
-            for (int side = 0; side < 2; ++side) {
-                const int cs = side ? nd.end - nd.begin - nleft : nleft;
-                if (cs >= 2 && cs <= TD_SMALL) {
-                    need[0] += (uint32_t)(cs - 1);  // the small node and the internal nodes below it
-                    need[1] += 1;
-                } else if (cs > TD_SMALL) {
-                    need[0] += 1;
-                    need[2] += 1;
-                }
-            }
-            for (int c = 0; c < 3; ++c) off[c] = need[c] ? atomicAdd(&cnt[c], need[c]) : 0u;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        base[0] = cnt[0] ? atomicAdd(ctr + TD_C_NEXT, cnt[0]) : 0u;
-        base[1] = cnt[1] ? atomicAdd(ctr + TD_C_SMALL, cnt[1]) : 0u;
-        base[2] = cnt[2] ? atomicAdd(ctr + TD_C_FR + (in ^ 1), cnt[2]) : 0u;
-    }
-    __syncthreads();
-    if (!active) return;
-    int slot[2] = {-1, -1};
-    if (lane == 0) {
-        uint32_t nid = base[0] + off[0], ns = base[1] + off[1], nf = base[2] + off[2];
-        split[i] = TdSplit{axis, sbin, nleft, 0};
-        int kid[2];
-        for (int side = 0; side < 2; ++side) {
-            const int cb = side ? nd.begin + nleft : nd.begin, ce = side ? nd.end : nd.begin + nleft;
-            const int cs = ce - cb, parent = 2 * nd.id + side;
-            if (cs == 1) {
-                kid[side] = cb;
-                info[cb] = PlocInfo{1, 0, 0, parent};
-                continue;
-            }
-            const uint32_t x = nid, take = cs <= TD_SMALL ? (uint32_t)(cs - 1) : 1u;
-            nid += take;
-            if (x + take > (uint32_t)(n - 1)) {  // a full binary tree over n triangles has n - 1 internal nodes
-                atomicOr(err, E_TREE);
-                kid[side] = n;
-                continue;
-            }
-            kid[side] = n + (int)x;
-            info[kid[side]] = PlocInfo{cs, 0, 0, parent};
-            ndepth[x] = (uint8_t)min(nd.depth + 1, 255);
-            if (cs <= TD_SMALL) {
-                const uint32_t j = ns++;
-                if (j >= (uint32_t)small_cap) {
-                    atomicOr(err, E_TREE);
-                    continue;
-                }
-                small[j] = TdSmall{cb, ce, kid[side], (nd.depth + 1) << 1 | (in ^ 1)};
             } else {
-                const uint32_t j = nf++;
-                if (j >= (uint32_t)fr_cap) {
-                    atomicOr(err, E_TREE);
-                    continue;
+                for (int sd = 0; sd < 2; ++sd)
+                    for (int q = 0; q < 3; ++q) {
+                        cl[sd][q] = nd.clo[q];
+                        ch[sd][q] = nd.chi[q];
+                    }
+            }
+            if (lane == 0) {
+                uint32_t need[3] = {0, 0, 0};
+                for (int side = 0; side < 2; ++side) {
+                    const int cs = side ? nd.end - nd.begin - nleft : nleft;
+                    if (cs >= 2 && cs <= TD_SMALL) {
+                        need[0] += (uint32_t)(cs - 1);  // the small node and its subtree
+                        need[1] += 1;
+                    } else if (cs > TD_SMALL) {
+                        need[0] += 1;
+                        need[2] += 1;
+                    }
                 }
-                TdNode c;
-                c.begin = cb;
-                c.end = ce;
-                c.id = kid[side];
-                c.depth = nd.depth + 1;
-                for (int q = 0; q < 3; ++q) {
-                    c.clo[q] = cl[side][q];
-                    c.chi[q] = ch[side][q];
-                }
-                c.task0 = 0;
-                c.pad = 0;
-                fr_out[j] = c;
-                slot[side] = (int)j;
+                for (int c = 0; c < 3; ++c) off[c] = need[c] ? atomicAdd(&cnt[c], need[c]) : 0u;
             }
         }
-        kids[nd.id - n] = make_int2(kid[0], kid[1]);
-    }
-    for (int side = 0; side < 2; ++side) {
-        const int j = __shfl(slot[side], 0);
-        if (j >= 0)
-            for (int w = lane; w < TD_NODE_WORDS; w += 64) gbins_out[(uint64_t)j * TD_NODE_WORDS + w] = td_identity(w);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            base[0] = cnt[0] ? atomicAdd(ctr + TD_C_NEXT, cnt[0]) : 0u;
+            base[1] = cnt[1] ? atomicAdd(ctr + TD_C_SMALL, cnt[1]) : 0u;
+            base[2] = cnt[2] ? atomicAdd(ctr + TD_C_FR + (in ^ 1), cnt[2]) : 0u;
+        }
+        __syncthreads();
+        int slot[2] = {-1, -1};
+        if (active && lane == 0) {
+            uint32_t nid = base[0] + off[0], ns = base[1] + off[1], nf = base[2] + off[2];
+            split[i] = TdSplit{axis, sbin, nleft, 0};
+            int kid[2];
+            for (int side = 0; side < 2; ++side) {
+                const int cb = side ? nd.begin + nleft : nd.begin, ce = side ? nd.end : nd.begin + nleft;
+                const int cs = ce - cb, parent = 2 * nd.id + side;
+                const uint32_t take = cs == 1 ? 0u : (cs <= TD_SMALL ? (uint32_t)(cs - 1) : 1u);
+                if (nid + take > (uint32_t)(n - 1)) {  // a binary tree over n triangles has n - 1 internal nodes
+                    atomicOr(err, E_TREE);
+                    kid[side] = n;
+                    continue;
+                }
+                kid[side] = td_child(cb, ce, parent, nd.depth + 1, n, nid, ndepth, info);
+                const uint32_t id = nid;
+                nid += take;
+                if (cs == 1) continue;
+                if (cs <= TD_SMALL) {
+                    const uint32_t j = ns++;
+                    if (j >= (uint32_t)small_cap) {
+                        atomicOr(err, E_TREE);
+                        continue;
+                    }
+                    small_out[j] = TdSmall{cb, ce, n + (int)id, (nd.depth + 1) << 1 | (in ^ 1)};
+                } else {
+                    const uint32_t j = nf++;
+                    if (j >= (uint32_t)fr_cap) {
+                        atomicOr(err, E_TREE);
+                        continue;
+                    }
+                    TdNode c;
+                    c.begin = cb;
+                    c.end = ce;
+                    c.id = n + (int)id;
+                    c.depth = nd.depth + 1;
+                    for (int q = 0; q < 3; ++q) {
+                        c.clo[q] = cl[side][q];
+                        c.chi[q] = ch[side][q];
+                    }
+                    c.task0 = 0;
+                    c.pad = 0;
+                    fr_out[j] = c;
+                    slot[side] = (int)j;
+                }
+            }
+            kids[nd.id - n] = make_int2(kid[0], kid[1]);
+        }
+        for (int side = 0; side < 2; ++side) {
+            const int j = __shfl(slot[side], 0);
+            if (j >= 0)
+                for (int w = lane; w < TD_NODE_WORDS; w += 64)
+                    gbins_out[(uint64_t)j * TD_NODE_WORDS + w] = td_identity(w);
+        }
+        __syncthreads();  // cnt and base are reset for the next round
     }
 }
 
 // Per task: its triangles to their side of the cut in the other buffer, in
-// order (each thread takes a run of TD_CHUNK / TD_BLOCK, a block scan places
-// the runs).  A triangle that becomes a leaf writes its index to `order`.
+// order: rounds of TD_BLOCK consecutive triangles, placed by a block scan of
+// the side flags.  A triangle that becomes a leaf writes its index to `order`.
 __global__ __launch_bounds__(TD_BLOCK) void k_td_partition(const float4* __restrict__ pin, float4* __restrict__ pout,
                                                            const TdNode* __restrict__ fr, const int2* __restrict__ tasks,
                                                            const uint32_t* __restrict__ ctr,
                                                            const TdSplit* __restrict__ split,
                                                            const uint32_t* __restrict__ tcnt,
                                                            uint32_t* __restrict__ order, unsigned* err) {
-    constexpr int PER = TD_CHUNK / TD_BLOCK;
-    const uint32_t t = blockIdx.x;
-    if (t >= ctr[TD_C_TASKS]) return;  // the whole block
-    __shared__ uint32_t wsum[TD_BLOCK / 64];
-    const int2 task = tasks[t];
-    const TdNode nd = fr[task.x];
-    const TdSplit sp = split[task.x];
-    const int size = nd.end - nd.begin;
-    const int cb = task.y, ce = min(cb + TD_CHUNK, nd.end);
-    uint32_t before, tot;
-    if (sp.bin >= 0) {  // left triangles of the node's earlier tasks
-        uint32_t part = 0;
-        for (int u = nd.task0 + (int)threadIdx.x; u < (int)t; u += TD_BLOCK) {
-            const uint32_t* c = tcnt + (uint64_t)u * 3 * TD_BINS + sp.axis * TD_BINS;
-            for (int b = 0; b <= sp.bin; ++b) part += c[b];
-        }
-        (void)td_block_scan(part, wsum, &before);
-    } else {
-        before = (uint32_t)min(cb - nd.begin, sp.nleft);
-    }
-    const float lo = nd.clo[sp.axis], sc = td_scale(nd.clo[sp.axis], nd.chi[sp.axis]);
-    const int r0 = cb + (int)threadIdx.x * PER;
-    float4 a[PER], b[PER];
-    uint32_t lmask = 0, nl = 0;
-    for (int k = 0; k < PER; ++k) {
-        const int p = r0 + k;
-        if (p >= ce) break;
-        a[k] = pin[2 * (uint64_t)p];
-        b[k] = pin[2 * (uint64_t)p + 1];
-        bool left;
-        if (sp.bin >= 0) {
-            const float c = sp.axis == 0 ? td_centre(a[k].x, b[k].x)
-                                         : (sp.axis == 1 ? td_centre(a[k].y, b[k].y) : td_centre(a[k].z, b[k].z));
-            left = td_bin(c, lo, sc) <= sp.bin;
+    __shared__ uint32_t wsum[TD_BLOCK / 64], wl[TD_BLOCK / 64], wr[TD_BLOCK / 64];
+    const uint32_t n_tasks = ctr[TD_C_TASKS];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (uint32_t t = blockIdx.x; t < n_tasks; t += gridDim.x) {  // uniform over the block
+        const int2 task = tasks[t];
+        const TdNode nd = fr[task.x];
+        const TdSplit sp = split[task.x];
+        const int size = nd.end - nd.begin;
+        const int cb = task.y, ce = min(cb + TD_CHUNK, nd.end);
+        uint32_t before;
+        if (sp.bin >= 0) {  // left triangles of the node's earlier tasks
+            uint32_t part = 0;
+            for (int u = nd.task0 + (int)threadIdx.x; u < (int)t; u += TD_BLOCK) {
+                const uint32_t* c = tcnt + (uint64_t)u * 3 * TD_BINS + sp.axis * TD_BINS;
+                for (int b = 0; b <= sp.bin; ++b) part += c[b];
+            }
+            (void)td_block_scan(part, wsum, &before);
         } else {
-            left = p - nd.begin < sp.nleft;
+            before = (uint32_t)min(cb - nd.begin, sp.nleft);
         }
-        lmask |= (uint32_t)left << k;
-        nl += left;
-    }
-    const uint32_t offl = td_block_scan(nl, wsum, &tot);
-    if (sp.bin >= 0) {  // the cut must agree with the bins it was chosen from
-        uint32_t want = 0;
-        const uint32_t* c = tcnt + (uint64_t)t * 3 * TD_BINS + sp.axis * TD_BINS;
-        for (int q = 0; q <= sp.bin; ++q) want += c[q];
-        if (want != tot) {
-            if (threadIdx.x == 0) atomicOr(err, E_TREE);
-            return;
+        const float lo = nd.clo[sp.axis], sc = td_scale(nd.clo[sp.axis], nd.chi[sp.axis]);
+        const uint32_t l0 = (uint32_t)nd.begin + before;
+        const uint32_t r0 = (uint32_t)(nd.begin + sp.nleft) + ((uint32_t)(cb - nd.begin) - before);
+        const uint32_t lend = (uint32_t)(nd.begin + sp.nleft), rend = (uint32_t)nd.end;
+        const bool leaf_l = sp.nleft == 1, leaf_r = size - sp.nleft == 1;
+        uint32_t run_l = 0, run_r = 0;
+        bool bad = false;
+        for (int base = cb; base < ce; base += TD_BLOCK) {
+            const int p = base + (int)threadIdx.x;
+            const bool act = p < ce;
+            float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), b = a;
+            bool left = false;
+            if (act) {
+                a = pin[2 * (uint64_t)p];
+                b = pin[2 * (uint64_t)p + 1];
+                if (sp.bin >= 0) {
+                    const float c = sp.axis == 0 ? td_centre(a.x, b.x)
+                                                 : (sp.axis == 1 ? td_centre(a.y, b.y) : td_centre(a.z, b.z));
+                    left = td_bin(c, lo, sc) <= sp.bin;
+                } else {
+                    left = p - nd.begin < sp.nleft;
+                }
+            }
+            const uint64_t bl = __ballot(act && left), br = __ballot(act && !left);
+            if (lane == 0) {
+                wl[w] = (uint32_t)__popcll(bl);
+                wr[w] = (uint32_t)__popcll(br);
+            }
+            __syncthreads();
+            uint32_t pl = 0, pr = 0, tl = 0, tr = 0;
+            for (int q = 0; q < TD_BLOCK / 64; ++q) {
+                if (q < w) {
+                    pl += wl[q];
+                    pr += wr[q];
+                }
+                tl += wl[q];
+                tr += wr[q];
+            }
+            __syncthreads();  // wl and wr are rewritten next round
+            if (act) {
+                const uint32_t d = left ? l0 + run_l + pl + (uint32_t)__popcll(bl & below)
+                                        : r0 + run_r + pr + (uint32_t)__popcll(br & below);
+                if (d >= (left ? lend : rend)) {
+                    bad = true;
+                } else {
+                    pout[2 * (uint64_t)d] = a;
+                    pout[2 * (uint64_t)d + 1] = b;
+                    if (left ? leaf_l : leaf_r) order[d] = __float_as_uint(a.w);
+                }
+            }
+            run_l += tl;
+            run_r += tr;
         }
-    }
-    const uint32_t run0 = (uint32_t)min((int)threadIdx.x * PER, ce - cb);
-    uint32_t dl = (uint32_t)nd.begin + before + offl;
-    uint32_t dr = (uint32_t)nd.begin + (uint32_t)sp.nleft + ((uint32_t)(cb - nd.begin) - before) + (run0 - offl);
-    const uint32_t lend = (uint32_t)(nd.begin + sp.nleft), rend = (uint32_t)nd.end;
-    for (int k = 0; k < PER; ++k) {
-        if (r0 + k >= ce) break;
-        const bool left = (lmask >> k) & 1u;
-        const uint32_t d = left ? dl++ : dr++;
-        if (d >= (left ? lend : rend)) {
-            atomicOr(err, E_TREE);
-            return;
+        if (sp.bin >= 0) {  // the cut must agree with the bins it was chosen from
+            uint32_t want = 0;
+            const uint32_t* c = tcnt + (uint64_t)t * 3 * TD_BINS + sp.axis * TD_BINS;
+            for (int q = 0; q <= sp.bin; ++q) want += c[q];
+            bad = bad || want != run_l;
         }
-        pout[2 * (uint64_t)d] = a[k];
-        pout[2 * (uint64_t)d + 1] = b[k];
-        if ((left ? sp.nleft : size - sp.nleft) == 1) order[d] = __float_as_uint(a[k].w);
+        if (bad) atomicOr(err, E_TREE);
     }
 }
 
-// One wave per node of at most TD_SMALL triangles: the whole subtree, cut by
-// the same rule, with the triangles' boxes in LDS and their order permuted
-// between two index buffers.
-__global__ __launch_bounds__(64) void k_td_small(const float4* __restrict__ prims, int n,
-                                                 const TdSmall* __restrict__ small, PlocInfo* __restrict__ info,
-                                                 int2* __restrict__ kids, uint32_t* __restrict__ order,
-                                                 uint8_t* __restrict__ ndepth, unsigned* err) {
-    __shared__ float4 rl[TD_SMALL], rh[TD_SMALL];
-    __shared__ uint16_t perm[2][TD_SMALL];
-    __shared__ uint32_t sb[3 * TD_BINS * TD_SW];
-    __shared__ int4 stk[TD_SMALL / 2];  // pending ranges are disjoint and hold >= 2 triangles
-    const TdSmall sm = small[blockIdx.x];
-    const int lane = threadIdx.x;
+// One wave per node of 2..TD_SMALL triangles, four independent waves per
+// block: the whole subtree, cut by the same rule over each range's own
+// centroid bounds (the middle of the range when bins cannot cut it), with the
+// triangles' boxes in LDS and their order permuted between two index buffers.
+// The node took the ids of its subtree when it was created; its internal
+// nodes take them in the order they are created.
+__device__ inline void td_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(TD_BLOCK) void k_td_small(const float4* __restrict__ prims, int n,
+                                                       const TdSmall* __restrict__ small, const uint32_t* __restrict__ ctr,
+                                                       int small_cap, PlocInfo* __restrict__ info,
+                                                       int2* __restrict__ kids, uint32_t* __restrict__ order,
+                                                       uint8_t* __restrict__ ndepth, unsigned* err) {
+    constexpr int WAVES = TD_BLOCK / 64;
+    __shared__ float4 srl[WAVES][TD_SMALL], srh[WAVES][TD_SMALL];
+    __shared__ uint8_t sperm[WAVES][2][TD_SMALL];
+    __shared__ uint32_t ssb[WAVES][3 * TD_BINS * TD_SW];
+    __shared__ int4 sstk[WAVES][TD_SMALL / 2];  // pending ranges are disjoint and hold >= 2 triangles
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t i = blockIdx.x * WAVES + w;
+    if (i >= min(ctr[TD_C_SMALL], (uint32_t)small_cap)) return;  // whole waves; no block barrier follows
+    float4* rl = srl[w];
+    float4* rh = srh[w];
+    uint32_t* sb = ssb[w];
+    int4* stk = sstk[w];
+    const TdSmall sm = small[i];
     const int s = sm.end - sm.begin;
-    if (s < 2 || s > TD_SMALL) {
+    if (s < 2 || s > TD_SMALL || sm.id < n || sm.id - n + (s - 1) > n - 1) {
         if (lane == 0) atomicOr(err, E_TREE);
         return;
     }
     const float4* src = prims + (uint64_t)(sm.depth_buf & 1) * 2 * (uint64_t)n;
-    for (int i = lane; i < s; i += 64) {
-        rl[i] = src[2 * (uint64_t)(sm.begin + i)];
-        rh[i] = src[2 * (uint64_t)(sm.begin + i) + 1];
-        perm[0][i] = (uint16_t)i;
-    }
-    // the ids below the root, sm.id + 1 .. sm.id + s - 2, were taken with it
-    if (sm.id < n || sm.id - n + (s - 1) > n - 1) {
-        if (lane == 0) atomicOr(err, E_TREE);
-        return;
+    if (lane < s) {
+        rl[lane] = src[2 * (uint64_t)(sm.begin + lane)];
+        rh[lane] = src[2 * (uint64_t)(sm.begin + lane) + 1];
+        sperm[w][0][lane] = (uint8_t)lane;
     }
     int id_next = sm.id + 1;  // used by lane 0
     if (lane == 0) stk[0] = make_int4(0, s, sm.id, sm.depth_buf & ~1);
     int sp = 1;
-    __syncthreads();
+    td_wave_sync();
+    const uint64_t below = (1ull << lane) - 1ull;
     while (sp > 0) {
         --sp;
         const int4 e = stk[sp];
         const int b = e.x, en = e.y, id = e.z, depth = e.w >> 1;
         int buf = e.w & 1;
         const int sz = en - b;
-        int axis = 0, sbin = -1, nleft = sz >> 1;
+        int nleft = sz >> 1;
         if (sz > 2) {
-            float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-            for (int i = b + lane; i < en; i += 64) {
-                const int q = perm[buf][i];
-                const float4 l = rl[q], h = rh[q];
-                const float c[3] = {td_centre(l.x, h.x), td_centre(l.y, h.y), td_centre(l.z, h.z)};
-                for (int k = 0; k < 3; ++k) {
-                    mn[k] = fminf(mn[k], c[k]);
-                    mx[k] = fmaxf(mx[k], c[k]);
-                }
+            const bool mine = lane < sz;
+            float4 l = make_float4(0.0f, 0.0f, 0.0f, 0.0f), h = l;
+            int q = 0;
+            if (mine) {
+                q = sperm[w][buf][b + lane];
+                l = rl[q];
+                h = rh[q];
+            }
+            const float c[3] = {td_centre(l.x, h.x), td_centre(l.y, h.y), td_centre(l.z, h.z)};
+            float mn[3], mx[3], sc[3];
+            for (int k = 0; k < 3; ++k) {
+                mn[k] = mine ? c[k] : INFINITY;
+                mx[k] = mine ? c[k] : -INFINITY;
             }
             for (int o = 32; o >= 1; o >>= 1)
                 for (int k = 0; k < 3; ++k) {
                     mn[k] = fminf(mn[k], __shfl_xor(mn[k], o));
                     mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], o));
                 }
-            float sc[3];
             for (int k = 0; k < 3; ++k) sc[k] = td_scale(mn[k], mx[k]);
+            int axis = 0, sbin = -1;
             if (depth < TD_DEPTH_CAP && (sc[0] > 0.0f || sc[1] > 0.0f || sc[2] > 0.0f)) {
-                for (int w = lane; w < 3 * TD_BINS * TD_SW; w += 64) {
-                    const int f = w % TD_SW;
-                    sb[w] = f == 0 ? 0u : (f <= 3 ? 0xFFFFFFFFu : 0u);
+                for (int x = lane; x < 3 * TD_BINS * TD_SW; x += 64) {
+                    const int f = x % TD_SW;
+                    sb[x] = f == 0 ? 0u : (f <= 3 ? 0xFFFFFFFFu : 0u);
                 }
-                __syncthreads();
-                for (int i = b + lane; i < en; i += 64) {
-                    const int q = perm[buf][i];
-                    const float4 l = rl[q], h = rh[q];
-                    const float c[3] = {td_centre(l.x, h.x), td_centre(l.y, h.y), td_centre(l.z, h.z)};
+                td_wave_sync();
+                if (mine) {
                     const unsigned bl[3] = {ordered_bits(l.x), ordered_bits(l.y), ordered_bits(l.z)};
                     const unsigned bh[3] = {ordered_bits(h.x), ordered_bits(h.y), ordered_bits(h.z)};
                     for (int k = 0; k < 3; ++k) {
@@ -1604,63 +1661,40 @@ __global__ __launch_bounds__(64) void k_td_small(const float4* __restrict__ prim
                         }
                     }
                 }
-                __syncthreads();
-                const int bin = lane & (TD_BINS - 1);
-                uint32_t L[TD_SW], R[TD_SW];
+                td_wave_sync();
+                uint32_t L[TD_SW], Rn[TD_SW];
                 for (int f = 0; f < TD_SW; ++f) {
                     const uint32_t id0 = f == 0 ? 0u : (f <= 3 ? 0xFFFFFFFFu : 0u);
-                    L[f] = R[f] = lane < 3 * TD_BINS ? sb[lane * TD_SW + f] : id0;
+                    L[f] = lane < 3 * TD_BINS ? sb[lane * TD_SW + f] : id0;
                 }
-                for (int st = 1; st < TD_BINS; st <<= 1)
-                    for (int f = 0; f < TD_SW; ++f) {  // words 0..6 merge like the first seven of td_merge
-                        const uint32_t up = (uint32_t)__shfl_up((int)L[f], st), dn = (uint32_t)__shfl_down((int)R[f], st);
-                        if (bin >= st) L[f] = td_merge(f, L[f], up);
-                        if (bin + st < TD_BINS) R[f] = td_merge(f, R[f], dn);
-                    }
-                uint32_t Rn[TD_SW];
-                for (int f = 0; f < TD_SW; ++f) Rn[f] = (uint32_t)__shfl_down((int)R[f], 1);
-                const bool valid = lane < 3 * TD_BINS && bin < TD_BINS - 1 && L[0] > 0 && Rn[0] > 0;
-                const float cost = valid ? td_diag2(L) * (float)L[0] + td_diag2(Rn) * (float)Rn[0] : 0.0f;
-                const uint64_t best = td_best(valid, cost, lane);
+                const uint64_t best = td_choose<TD_SW>(L, Rn, lane, true);
                 if (best != ~0ull) {
                     const int wl = (int)(best & 63);
                     axis = wl / TD_BINS;
                     sbin = wl % TD_BINS;
                     nleft = (int)__shfl((int)L[0], wl);
                 }
+                td_wave_sync();  // sb is read before the next range clears it
             }
             // stable partition of the range into the other index buffer
-            const float lo_a = mn[axis], sc_a = sc[axis];
-            int nl = 0, nr = 0;
-            for (int base = b; base < en; base += 64) {
-                const int i = base + lane;
-                const bool act = i < en;
-                const int q = act ? perm[buf][i] : 0;
-                bool lf = false;
-                if (act) {
-                    if (sbin >= 0) {
-                        const float4 l = rl[q], h = rh[q];
-                        const float c = axis == 0 ? td_centre(l.x, h.x) : (axis == 1 ? td_centre(l.y, h.y) : td_centre(l.z, h.z));
-                        lf = td_bin(c, lo_a, sc_a) <= sbin;
-                    } else {
-                        lf = i - b < nleft;
-                    }
-                }
-                const uint64_t BL = __ballot(act && lf), BR = __ballot(act && !lf);
-                const uint64_t below = (1ull << lane) - 1ull;
-                if (act) {
-                    const int d = lf ? b + nl + __popcll(BL & below) : b + nleft + nr + __popcll(BR & below);
-                    if (d >= b && d < en) perm[buf ^ 1][d] = (uint16_t)q;
-                }
-                nl += __popcll(BL);
-                nr += __popcll(BR);
+            bool lf;
+            if (sbin >= 0) {
+                const float ca = axis == 0 ? c[0] : (axis == 1 ? c[1] : c[2]);
+                lf = mine && td_bin(ca, mn[axis], sc[axis]) <= sbin;
+            } else {
+                lf = lane < nleft;
             }
-            if (nl != nleft) {  // the bins and the partition disagree: never expected
+            const uint64_t BL = __ballot(mine && lf), BR = __ballot(mine && !lf);
+            if (__popcll(BL) != nleft) {  // the bins and the partition disagree: never expected
                 if (lane == 0) atomicOr(err, E_TREE);
-                return;  // the whole wave, which is the whole block
+                return;  // the whole wave
+            }
+            if (mine) {
+                const int d = lf ? b + __popcll(BL & below) : b + nleft + __popcll(BR & below);
+                sperm[w][buf ^ 1][d] = (uint8_t)q;
             }
             buf ^= 1;
-            __syncthreads();
+            td_wave_sync();
         }
         if (lane == 0) {
             int kid[2];
@@ -1671,18 +1705,18 @@ __global__ __launch_bounds__(64) void k_td_small(const float4* __restrict__ prim
                     const int pos = sm.begin + cb;
                     kid[side] = pos;
                     info[pos] = PlocInfo{1, 0, 0, parent};
-                    order[pos] = __float_as_uint(rl[perm[buf][cb]].w);
+                    order[pos] = __float_as_uint(rl[sperm[w][buf][cb]].w);
                 } else {
-                    kid[side] = id_next++;
-                    info[kid[side]] = PlocInfo{cs, 0, 0, parent};
-                    ndepth[kid[side] - n] = (uint8_t)min(depth + 1, 255);
+                    kid[side] = td_child(sm.begin + cb, sm.begin + ce, parent, depth + 1, n, (uint32_t)(id_next - n),
+                                         ndepth, info);
+                    ++id_next;
                     stk[sp++] = make_int4(cb, ce, kid[side], (depth + 1) << 1 | buf);
                 }
             }
             kids[id - n] = make_int2(kid[0], kid[1]);
         }
         sp = __shfl(sp, 0);
-        __syncthreads();
+        td_wave_sync();
     }
 }
 
@@ -2357,19 +2391,25 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     uint32_t* td_hist = nullptr;
     uint8_t* td_depth = nullptr;
     int td_fr_cap = 0, td_task_cap = 0, td_small_cap = 0;
-    // one level of the top-down build: triangles in buffer l & 1, frontier l & 1
+    // one level of the top-down build: triangles in buffer l & 1, lists l & 1
     auto td_level = [&](int l) {
         const int in = l & 1, out = in ^ 1;
         const uint64_t pw = 2 * (uint64_t)n, bw = (uint64_t)td_fr_cap * TD_NODE_WORDS;
+        // level l holds at most 2^l nodes: grids sized to that, looping past TD_GRID_MAX blocks
+        const uint64_t reach = l < 40 ? (1ull << l) : ~0ull;
+        const uint64_t nfr = std::min<uint64_t>(reach, (uint64_t)td_fr_cap);
+        const unsigned g_task = (unsigned)std::min<uint64_t>(
+            {(uint64_t)td_task_cap, n / TD_CHUNK + nfr + 1, (uint64_t)TD_GRID_MAX});
+        const unsigned g_sel = (unsigned)std::min<uint64_t>(div_up(nfr, TD_BLOCK / 64), TD_GRID_MAX);
         k_td_tasks<<<1, 1024, 0, sb>>>(td_fr + (uint64_t)in * td_fr_cap, td_ctr, in, td_fr_cap, td_tasks);
-        k_td_bin<<<td_task_cap, TD_BLOCK, 0, sb>>>(td_prims + in * pw, td_fr + (uint64_t)in * td_fr_cap, td_tasks,
-                                                   td_ctr, td_bins + in * bw, td_tcnt);
-        k_td_select<<<div_up(td_fr_cap, TD_BLOCK / 64), TD_BLOCK, 0, sb>>>(
-            td_fr + (uint64_t)in * td_fr_cap, td_ctr, in, td_bins + in * bw, nt, td_fr_cap, td_small_cap, td_split,
-            td_fr + (uint64_t)out * td_fr_cap, td_bins + out * bw, td_small, pinfo, kids, td_depth, err);
-        k_td_partition<<<td_task_cap, TD_BLOCK, 0, sb>>>(td_prims + in * pw, td_prims + out * pw,
-                                                         td_fr + (uint64_t)in * td_fr_cap, td_tasks, td_ctr, td_split,
-                                                         td_tcnt, idx_s, err);
+        k_td_bin<<<g_task, TD_BLOCK, 0, sb>>>(td_prims + in * pw, td_fr + (uint64_t)in * td_fr_cap, td_tasks, td_ctr,
+                                               td_bins + in * bw, td_tcnt);
+        k_td_select<<<g_sel, TD_BLOCK, 0, sb>>>(td_fr + (uint64_t)in * td_fr_cap, td_ctr, in, td_bins + in * bw, nt,
+                                                td_fr_cap, td_small_cap, td_split, td_fr + (uint64_t)out * td_fr_cap,
+                                                td_bins + out * bw, td_small, pinfo, kids, td_depth, err);
+        k_td_partition<<<g_task, TD_BLOCK, 0, sb>>>(td_prims + in * pw, td_prims + out * pw,
+                                                    td_fr + (uint64_t)in * td_fr_cap, td_tasks, td_ctr, td_split,
+                                                    td_tcnt, idx_s, err);
     };
     // the level passes are enqueued a few at a time just before the host
     // waits on a readback below, so their launches cost the host no time
@@ -2416,7 +2456,7 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
         td_tcnt = S.as<uint32_t>(S_TD_TCNT, (uint64_t)td_task_cap * 3 * TD_BINS);
         td_split = S.as<TdSplit>(S_TD_SPLIT, (uint64_t)td_fr_cap);
         td_small = S.as<TdSmall>(S_TD_SMALL, (uint64_t)td_small_cap);
-        td_ctr = S.as<uint32_t>(S_PLOC_RING, 8);
+        td_ctr = S.as<uint32_t>(S_PLOC_RING, TD_C_WORDS);
         td_hist = S.as<uint32_t>(S_TD_HIST, 1024);
         td_depth = reinterpret_cast<uint8_t*>(height);  // one byte per internal node
         pinfo = S.as<PlocInfo>(S_PLOC_INFO, n_nodes);
@@ -2651,7 +2691,7 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     } else if (topdown) {
         td_pump(ploc_iters);
         auto td_read = [&]() {
-            BS_HIP(hipMemcpyAsync(hs, td_ctr, sizeof(uint32_t) * 8, hipMemcpyDeviceToHost, sb));
+            BS_HIP(hipMemcpyAsync(hs, td_ctr, sizeof(uint32_t) * TD_C_WORDS, hipMemcpyDeviceToHost, sb));
             BS_HIP(hipMemcpyAsync(hs + 258, err, sizeof(uint32_t), hipMemcpyDeviceToHost, sb));
             BS_HIP(hipStreamSynchronize(sb));
             if (hs[258] & E_TREE) throw Error(BS_ERR_INTERNAL, "BVH build failed");
@@ -2665,7 +2705,10 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
             td_read();
         }
         const uint32_t n_small = hs[TD_C_SMALL];
-        if (n_small > 0) k_td_small<<<n_small, 64, 0, sb>>>(td_prims, nt, td_small, pinfo, kids, idx_s, td_depth, err);
+        if (n_small > 0)
+            k_td_small<<<div_up(n_small, TD_BLOCK / 64), TD_BLOCK, 0, sb>>>(td_prims, nt, td_small, td_ctr,
+                                                                           td_small_cap, pinfo, kids, idx_s, td_depth,
+                                                                           err);
         k_td_leaves<<<div_up(n, TB), TB, 0, sb>>>(d_tris, nt, idx_s, accs, err);
         k_td_depth_hist<<<div_up(n_int, 256), 256, 0, sb>>>(td_depth, n_int, td_hist);
         k_td_depth_scan<<<1, 256, 0, sb>>>(td_hist);
