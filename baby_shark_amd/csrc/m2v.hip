@@ -1152,38 +1152,58 @@ __device__ inline float td_diag2(const uint32_t* w) {  // w[1..3] box lo, w[4..6
     return d2;
 }
 
-// Lowest (cost, lane) of the wave; ~0 when no lane offers a cut.
-__device__ inline uint64_t td_best(bool valid, float cost, int lane) {
-    uint32_t hi = valid ? ordered_bits(cost) : 0xFFFFFFFFu, lo = valid ? (uint32_t)lane : 0xFFFFFFFFu;
-    for (int s = 32; s >= 1; s >>= 1) {
-        const uint32_t h2 = (uint32_t)__shfl_xor((int)hi, s), l2 = (uint32_t)__shfl_xor((int)lo, s);
-        if (h2 < hi || (h2 == hi && l2 < lo)) {
-            hi = h2;
-            lo = l2;
-        }
-    }
-    return (uint64_t)hi << 32 | lo;
+// Lane moves within rows of 16 lanes through DPP (no LDS round trip): a lane
+// whose source lies outside its row keeps `old`.  Controls: quad_perm [1,0,3,2]
+// and [2,3,0,1], row_half_mirror, row_mirror, row_shl:S (0x100 + S) and
+// row_shr:S (0x110 + S).
+template <int CTRL>
+__device__ inline uint32_t td_dpp(uint32_t old, uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, 0xF, 0xF, false);
+}
+__device__ inline uint32_t td_readlane(uint32_t v, int l) { return (uint32_t)__builtin_amdgcn_readlane((int)v, l); }
+
+// Minimum (MAX: maximum) of an ordered-bits word over the wave, in every lane.
+template <bool MAX>
+__device__ inline uint32_t td_wave_ext(uint32_t x) {
+    auto f = [](uint32_t a, uint32_t b) { return MAX ? max(a, b) : min(a, b); };
+    x = f(x, td_dpp<0xB1>(x, x));
+    x = f(x, td_dpp<0x4E>(x, x));
+    x = f(x, td_dpp<0x141>(x, x));
+    x = f(x, td_dpp<0x140>(x, x));
+    return f(f(td_readlane(x, 0), td_readlane(x, 16)), f(td_readlane(x, 32), td_readlane(x, 48)));
+}
+
+// The lowest lane of those with the smallest cost; -1 when no lane offers a cut.
+__device__ inline int td_best(bool valid, float cost) {
+    const uint32_t key = valid ? ordered_bits(cost) : 0xFFFFFFFFu;
+    const uint32_t m = td_wave_ext<false>(key);
+    const uint64_t hit = __ballot(valid && key == m);
+    return hit ? __ffsll((unsigned long long)hit) - 1 : -1;
 }
 
 // Prefix (left) and suffix (right) of the 16 bins of each axis held by lanes
-// 0..47 (lane = axis * 16 + bin), W words per bin merged as td_merge does;
-// Rn[] is the right side of a cut after the lane's bin.  Returns the
-// cheapest cut as td_best does.
+// 0..47 (lane = axis * 16 + bin, one DPP row per axis), W words per bin merged
+// as td_merge does; Rn[] is the right side of a cut after the lane's bin.
+// Returns the lane of the cheapest cut as td_best does.
 template <int W>
-__device__ inline uint64_t td_choose(uint32_t (&L)[W], uint32_t (&Rn)[W], int lane, bool allowed) {
+__device__ inline int td_choose(uint32_t (&L)[W], uint32_t (&Rn)[W], int lane, bool allowed) {
     const int bin = lane & (TD_BINS - 1);
     uint32_t R[W];
     for (int f = 0; f < W; ++f) R[f] = L[f];
-    for (int s = 1; s < TD_BINS; s <<= 1)
-        for (int f = 0; f < W; ++f) {
-            const uint32_t up = (uint32_t)__shfl_up((int)L[f], s), dn = (uint32_t)__shfl_down((int)R[f], s);
-            if (bin >= s) L[f] = td_merge(f, L[f], up);
-            if (bin + s < TD_BINS) R[f] = td_merge(f, R[f], dn);
-        }
-    for (int f = 0; f < W; ++f) Rn[f] = (uint32_t)__shfl_down((int)R[f], 1);
+#define TD_SCAN_STEP(S)                                                        \
+    for (int f = 0; f < W; ++f) {                                              \
+        L[f] = td_merge(f, L[f], td_dpp<0x110 + (S)>(td_identity(f), L[f]));   \
+        R[f] = td_merge(f, R[f], td_dpp<0x100 + (S)>(td_identity(f), R[f]));   \
+    }
+    TD_SCAN_STEP(1)
+    TD_SCAN_STEP(2)
+    TD_SCAN_STEP(4)
+    TD_SCAN_STEP(8)
+#undef TD_SCAN_STEP
+    for (int f = 0; f < W; ++f) Rn[f] = td_dpp<0x101>(td_identity(f), R[f]);
     const bool valid = allowed && lane < 3 * TD_BINS && bin < TD_BINS - 1 && L[0] > 0 && Rn[0] > 0;
     const float cost = valid ? td_diag2(L) * (float)L[0] + td_diag2(Rn) * (float)Rn[0] : 0.0f;
-    return td_best(valid, cost, lane);
+    return td_best(valid, cost);
 }
 
 // Exclusive prefix of x over the block (TD_BLOCK threads); *total gets the sum.
@@ -1375,18 +1395,17 @@ __global__ __launch_bounds__(TD_BLOCK) void k_td_select(
             const uint32_t* g = gbins + (uint64_t)i * TD_NODE_WORDS;
             uint32_t L[TD_BW], Rn[TD_BW];
             for (int f = 0; f < TD_BW; ++f) L[f] = lane < 3 * TD_BINS ? g[lane * TD_BW + f] : td_identity(f);
-            const uint64_t best = td_choose<TD_BW>(L, Rn, lane, nd.depth < TD_DEPTH_CAP);
+            const int wl = td_choose<TD_BW>(L, Rn, lane, nd.depth < TD_DEPTH_CAP);
             nleft = (nd.end - nd.begin) >> 1;
-            if (best != ~0ull) {
-                const int wl = (int)(best & 63);
+            if (wl >= 0) {
                 axis = wl / TD_BINS;
                 sbin = wl % TD_BINS;
-                nleft = (int)__shfl((int)L[0], wl);
+                nleft = (int)td_readlane(L[0], wl);
                 for (int q = 0; q < 3; ++q) {
-                    cl[0][q] = from_ordered_bits((uint32_t)__shfl((int)L[7 + q], wl));
-                    ch[0][q] = from_ordered_bits((uint32_t)__shfl((int)L[10 + q], wl));
-                    cl[1][q] = from_ordered_bits((uint32_t)__shfl((int)Rn[7 + q], wl));
-                    ch[1][q] = from_ordered_bits((uint32_t)__shfl((int)Rn[10 + q], wl));
+                    cl[0][q] = from_ordered_bits(td_readlane(L[7 + q], wl));
+                    ch[0][q] = from_ordered_bits(td_readlane(L[10 + q], wl));
+                    cl[1][q] = from_ordered_bits(td_readlane(Rn[7 + q], wl));
+                    ch[1][q] = from_ordered_bits(td_readlane(Rn[10 + q], wl));
                 }
             } else {
                 for (int sd = 0; sd < 2; ++sd)
@@ -1636,12 +1655,11 @@ __global__ __launch_bounds__(TD_BLOCK) void k_td_small(const float4* __restrict_
                 mn[k] = mine ? c[k] : INFINITY;
                 mx[k] = mine ? c[k] : -INFINITY;
             }
-            for (int o = 32; o >= 1; o >>= 1)
-                for (int k = 0; k < 3; ++k) {
-                    mn[k] = fminf(mn[k], __shfl_xor(mn[k], o));
-                    mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], o));
-                }
-            for (int k = 0; k < 3; ++k) sc[k] = td_scale(mn[k], mx[k]);
+            for (int k = 0; k < 3; ++k) {
+                mn[k] = from_ordered_bits(td_wave_ext<false>(ordered_bits(mn[k])));
+                mx[k] = from_ordered_bits(td_wave_ext<true>(ordered_bits(mx[k])));
+                sc[k] = td_scale(mn[k], mx[k]);
+            }
             int axis = 0, sbin = -1;
             if (depth < TD_DEPTH_CAP && (sc[0] > 0.0f || sc[1] > 0.0f || sc[2] > 0.0f)) {
                 for (int x = lane; x < 3 * TD_BINS * TD_SW; x += 64) {
@@ -1667,12 +1685,11 @@ __global__ __launch_bounds__(TD_BLOCK) void k_td_small(const float4* __restrict_
                     const uint32_t id0 = f == 0 ? 0u : (f <= 3 ? 0xFFFFFFFFu : 0u);
                     L[f] = lane < 3 * TD_BINS ? sb[lane * TD_SW + f] : id0;
                 }
-                const uint64_t best = td_choose<TD_SW>(L, Rn, lane, true);
-                if (best != ~0ull) {
-                    const int wl = (int)(best & 63);
+                const int wl = td_choose<TD_SW>(L, Rn, lane, true);
+                if (wl >= 0) {
                     axis = wl / TD_BINS;
                     sbin = wl % TD_BINS;
-                    nleft = (int)__shfl((int)L[0], wl);
+                    nleft = (int)td_readlane(L[0], wl);
                 }
                 td_wave_sync();  // sb is read before the next range clears it
             }
