@@ -1087,10 +1087,11 @@ __global__ void k_ploc_place(const uint32_t* __restrict__ root_buf, const PlocIn
 //
 // The tree is cut level by level; each level moves the triangles of the nodes
 // it cuts, in order, to their side in the other of two buffers.  A node of
-// more than TD_SMALL triangles is split into tasks of TD_CHUNK triangles
-// (k_td_tasks); each task bins its triangles in LDS and merges them into the
-// node's bins (k_td_bin), one wave per node picks the cut and creates the
-// children (k_td_select), and each task moves its triangles (k_td_partition).
+// more than TD_SMALL triangles is split into tasks of TD_CHUNK triangles; each
+// task bins its triangles in LDS and merges them into the node's bins
+// (k_td_bin), one wave per node picks the cut and creates the children with
+// their tasks (k_td_select), and each task moves its triangles and gathers
+// the children's centroid bounds (k_td_partition).
 // A node of 2..TD_SMALL triangles is built down to single triangles by one
 // wave (k_td_small), four independent waves per block.  The level kernels
 // loop over their work with grids sized to the level.
@@ -1106,24 +1107,25 @@ constexpr int TD_CHUNK = 2048;     // triangles per task
 constexpr int TD_BLOCK = 256;
 constexpr int TD_GRID_MAX = 4096;  // blocks of the looping level kernels, at most
 constexpr int TD_DEPTH_CAP = 64;   // deeper nodes are cut at the middle of their range
-constexpr int TD_BW = 13;          // words per bin: count, box lo xyz, box hi xyz, centroid lo xyz, hi xyz
+constexpr int TD_BW = 7;           // words per bin: count, box lo xyz, box hi xyz (ordered bits)
 constexpr int TD_NODE_WORDS = 3 * TD_BINS * TD_BW;
-constexpr int TD_SW = 7;           // words per bin in k_td_small: count, box lo xyz, box hi xyz
-// counter words: [0], [1] frontier sizes of the two level buffers, then
-// tasks, small-node list size, next id, root, (root, height)
-constexpr int TD_C_FR = 0, TD_C_TASKS = 2, TD_C_SMALL = 3, TD_C_NEXT = 5, TD_C_ROOT = 6, TD_C_OUT = 7;
+// counter words: [0], [1] frontier sizes and [2], [3] task counts of the two
+// level buffers, then the small-node list size, next id, root, (root, height)
+constexpr int TD_C_FR = 0, TD_C_TASKS = 2, TD_C_SMALL = 4, TD_C_NEXT = 5, TD_C_ROOT = 6, TD_C_OUT = 7;
 constexpr int TD_C_WORDS = 16;
 
 struct TdNode {  // a node of more than TD_SMALL triangles at the current level
     int begin, end, id, depth;
-    float clo[3], chi[3];  // bounds of its triangles' box centres
-    int task0, pad;        // its first task
+    uint32_t clo[3], chi[3];  // bounds of its triangles' box centres (ordered bits)
+    int task0, pad;           // its first task
 };
 struct TdSmall {  // a node of 2..TD_SMALL triangles; its subtree's ids are id .. id + (end - begin) - 2
     int begin, end, id, depth_buf;  // depth << 1 | the triangle buffer that holds its range
 };
 struct TdSplit {
-    int axis, bin, nleft, pad;  // bin < 0: cut at the middle of the range
+    int axis, bin, nleft;  // bin < 0: cut at the middle of the range
+    int slot[2];           // the children's frontier slots (-1: not a large node)
+    int pad[3];
 };
 
 // Triangle records in the level buffers: (box lo, triangle index bits), (box hi, 0).
@@ -1135,7 +1137,7 @@ __device__ inline float td_scale(float lo, float hi) {
 __device__ inline int td_bin(float c, float lo, float scale) {
     return (int)fminf(fmaxf((c - lo) * scale, 0.0f), (float)(TD_BINS - 1));
 }
-__device__ inline bool td_is_min(int f) { return (f >= 1 && f <= 3) || (f >= 7 && f <= 9); }
+__device__ inline bool td_is_min(int f) { return f >= 1 && f <= 3; }
 __device__ inline uint32_t td_identity(int w) {
     const int f = w % TD_BW;
     return f == 0 ? 0u : (td_is_min(f) ? 0xFFFFFFFFu : 0u);
@@ -1226,12 +1228,14 @@ __device__ inline uint32_t td_block_scan(uint32_t x, uint32_t* wsum, uint32_t* t
     return before + inc - x;
 }
 
-// The root: triangles as records in buffer 0, the level-0 frontier, counters.
+// The root: triangles as records in buffer 0, the level-0 frontier and its
+// tasks, counters.
 __global__ void k_td_init(const float* __restrict__ tris, int n, const unsigned* __restrict__ bounds,
-                          float4* __restrict__ prims, TdNode* __restrict__ fr, uint32_t* __restrict__ bins,
-                          uint32_t* __restrict__ ctr, PlocInfo* __restrict__ info, uint8_t* __restrict__ ndepth,
-                          uint32_t* __restrict__ hist) {
+                          float4* __restrict__ prims, TdNode* __restrict__ fr, int2* __restrict__ tasks,
+                          uint32_t* __restrict__ bins, uint32_t* __restrict__ ctr, PlocInfo* __restrict__ info,
+                          uint8_t* __restrict__ ndepth, uint32_t* __restrict__ hist) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n_tasks = (n + TD_CHUNK - 1) / TD_CHUNK;
     if (i < n) {
         F3 a, b, c;
         load_tri(tris + 9 * (uint64_t)i, a, b, c);
@@ -1239,6 +1243,7 @@ __global__ void k_td_init(const float* __restrict__ tris, int n, const unsigned*
         prims[2 * (uint64_t)i] = make_float4(mn.x, mn.y, mn.z, __uint_as_float((uint32_t)i));
         prims[2 * (uint64_t)i + 1] = make_float4(mx.x, mx.y, mx.z, 0.0f);
     }
+    if (i < n_tasks) tasks[i] = make_int2(0, i * TD_CHUNK);
     if (blockIdx.x != 0) return;
     for (int w = threadIdx.x; w < TD_NODE_WORDS; w += blockDim.x) bins[w] = td_identity(w);
     for (int w = threadIdx.x; w < 256; w += blockDim.x) hist[w] = 0;
@@ -1250,13 +1255,14 @@ __global__ void k_td_init(const float* __restrict__ tris, int n, const unsigned*
         r.id = n;
         r.depth = 0;
         for (int k = 0; k < 3; ++k) {
-            r.clo[k] = from_ordered_bits(bounds[k]);
-            r.chi[k] = from_ordered_bits(bounds[3 + k]);
+            r.clo[k] = bounds[k];
+            r.chi[k] = bounds[3 + k];
         }
         r.task0 = 0;
         r.pad = 0;
         fr[0] = r;
         ctr[TD_C_FR] = 1;  // the root is large: n >= 2^18
+        ctr[TD_C_TASKS] = (uint32_t)n_tasks;
         ctr[TD_C_NEXT] = 1;  // the root is internal node n
         ctr[TD_C_ROOT] = (uint32_t)n;
         info[n] = PlocInfo{n, 0, 0, -1};
@@ -1264,74 +1270,40 @@ __global__ void k_td_init(const float* __restrict__ tris, int n, const unsigned*
     }
 }
 
-// One block: the level's tasks (node, first triangle), in node order; clears
-// the next level's list sizes.
-__global__ __launch_bounds__(1024) void k_td_tasks(TdNode* __restrict__ fr, uint32_t* __restrict__ ctr, int in,
-                                                   int fr_cap, int2* __restrict__ tasks) {
-    __shared__ uint32_t wsum[16];
-    __shared__ uint32_t carry;
-    const uint32_t N = min(ctr[TD_C_FR + in], (uint32_t)fr_cap);  // more is an error k_td_select reported
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (uint32_t base = 0; base < N; base += 1024) {
-        const uint32_t i = base + threadIdx.x;
-        const uint32_t c = i < N ? (uint32_t)((fr[i].end - fr[i].begin + TD_CHUNK - 1) / TD_CHUNK) : 0u;
-        uint32_t inc = c;
-        for (int s = 1; s < 64; s <<= 1) {
-            const uint32_t y = (uint32_t)__shfl_up((int)inc, s);
-            if (lane >= s) inc += y;
-        }
-        if (lane == 63) wsum[w] = inc;
-        __syncthreads();
-        uint32_t before = carry;
-        for (int q = 0; q < w; ++q) before += wsum[q];
-        const uint32_t t0 = before + inc - c;
-        if (i < N) {
-            fr[i].task0 = (int)t0;
-            const int b = fr[i].begin;
-            for (uint32_t t = 0; t < c; ++t) tasks[t0 + t] = make_int2((int)i, b + (int)t * TD_CHUNK);
-        }
-        __syncthreads();
-        if (threadIdx.x == 1023) carry = t0 + c;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        ctr[TD_C_TASKS] = carry;
-        ctr[TD_C_FR + (in ^ 1)] = 0;
-    }
-}
-
-// Per task: its triangles' centroid bins on the three axes, in LDS, then
-// merged into the node's bins; the task's bin counts are kept for the
-// partition.
+// Per task: its triangles' centroid bins on the three axes (count and box),
+// in LDS, then merged into the node's bins; the task's bin counts are kept
+// for the partition.  Block 0 clears the next level's frontier and task counts.
 __global__ __launch_bounds__(TD_BLOCK) void k_td_bin(const float4* __restrict__ prims, const TdNode* __restrict__ fr,
-                                                     const int2* __restrict__ tasks, const uint32_t* __restrict__ ctr,
+                                                     const int2* __restrict__ tasks, uint32_t* __restrict__ ctr, int in,
                                                      uint32_t* __restrict__ gbins, uint32_t* __restrict__ tcnt) {
     __shared__ uint32_t sb[TD_NODE_WORDS];
-    const uint32_t n_tasks = ctr[TD_C_TASKS];
+    const uint32_t n_tasks = ctr[TD_C_TASKS + in];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctr[TD_C_FR + (in ^ 1)] = 0;
+        ctr[TD_C_TASKS + (in ^ 1)] = 0;
+    }
     for (uint32_t t = blockIdx.x; t < n_tasks; t += gridDim.x) {  // uniform over the block
         const int2 task = tasks[t];
         const TdNode nd = fr[task.x];
         for (int w = threadIdx.x; w < TD_NODE_WORDS; w += TD_BLOCK) sb[w] = td_identity(w);
-        float sc[3];
-        for (int k = 0; k < 3; ++k) sc[k] = td_scale(nd.clo[k], nd.chi[k]);
+        float lo[3], sc[3];
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = from_ordered_bits(nd.clo[k]);
+            sc[k] = td_scale(lo[k], from_ordered_bits(nd.chi[k]));
+        }
         __syncthreads();
         const int e = min(task.y + TD_CHUNK, nd.end);
         for (int p = task.y + (int)threadIdx.x; p < e; p += TD_BLOCK) {
-            const float4 lo = prims[2 * (uint64_t)p], hi = prims[2 * (uint64_t)p + 1];
-            const float c[3] = {td_centre(lo.x, hi.x), td_centre(lo.y, hi.y), td_centre(lo.z, hi.z)};
-            const unsigned bl[3] = {ordered_bits(lo.x), ordered_bits(lo.y), ordered_bits(lo.z)};
-            const unsigned bh[3] = {ordered_bits(hi.x), ordered_bits(hi.y), ordered_bits(hi.z)};
-            const unsigned bc[3] = {ordered_bits(c[0]), ordered_bits(c[1]), ordered_bits(c[2])};
+            const float4 a = prims[2 * (uint64_t)p], b = prims[2 * (uint64_t)p + 1];
+            const float c[3] = {td_centre(a.x, b.x), td_centre(a.y, b.y), td_centre(a.z, b.z)};
+            const unsigned bl[3] = {ordered_bits(a.x), ordered_bits(a.y), ordered_bits(a.z)};
+            const unsigned bh[3] = {ordered_bits(b.x), ordered_bits(b.y), ordered_bits(b.z)};
             for (int k = 0; k < 3; ++k) {
-                uint32_t* b = sb + (k * TD_BINS + td_bin(c[k], nd.clo[k], sc[k])) * TD_BW;
-                atomicAdd(b, 1u);
+                uint32_t* bb = sb + (k * TD_BINS + td_bin(c[k], lo[k], sc[k])) * TD_BW;
+                atomicAdd(bb, 1u);
                 for (int q = 0; q < 3; ++q) {
-                    atomicMin(b + 1 + q, bl[q]);
-                    atomicMax(b + 4 + q, bh[q]);
-                    atomicMin(b + 7 + q, bc[q]);
-                    atomicMax(b + 10 + q, bc[q]);
+                    atomicMin(bb + 1 + q, bl[q]);
+                    atomicMax(bb + 4 + q, bh[q]);
                 }
             }
         }
@@ -1370,26 +1342,26 @@ __device__ inline int td_child(int cb, int ce, int parent, int depth, int n, uin
 }
 
 // One wave per large node of the level: the cheapest cut, its children
-// (small ones to the small list, larger ones to the next frontier with
-// their centroid bounds from the bins).  Ids and list slots are counted per
-// block in LDS and taken with one global atomic per counter and block.
+// (small ones to the small list, larger ones to the next frontier with their
+// tasks; their centroid bounds are gathered by k_td_partition).  Ids, list
+// slots and tasks are counted per block in LDS and taken with one global
+// atomic per counter and block.
 __global__ __launch_bounds__(TD_BLOCK) void k_td_select(
     const TdNode* __restrict__ fr, uint32_t* __restrict__ ctr, int in, const uint32_t* __restrict__ gbins, int n,
-    int fr_cap, int small_cap, TdSplit* __restrict__ split, TdNode* __restrict__ fr_out,
-    uint32_t* __restrict__ gbins_out, TdSmall* __restrict__ small_out, PlocInfo* __restrict__ info,
-    int2* __restrict__ kids, uint8_t* __restrict__ ndepth, unsigned* err) {
-    __shared__ uint32_t cnt[3], base[3];  // ids, small-list slots, frontier slots of the block
+    int fr_cap, int small_cap, int task_cap, TdSplit* __restrict__ split, TdNode* __restrict__ fr_out,
+    uint32_t* __restrict__ gbins_out, int2* __restrict__ tasks_out, TdSmall* __restrict__ small_out,
+    PlocInfo* __restrict__ info, int2* __restrict__ kids, uint8_t* __restrict__ ndepth, unsigned* err) {
+    __shared__ uint32_t cnt[4], base[4];  // ids, small-list slots, frontier slots, tasks of the block
     const uint32_t count = min(ctr[TD_C_FR + in], (uint32_t)fr_cap);
     const int lane = threadIdx.x & 63;
     for (uint32_t r0 = blockIdx.x * (TD_BLOCK / 64); r0 < count; r0 += gridDim.x * (TD_BLOCK / 64)) {
         const int i = (int)r0 + (int)(threadIdx.x >> 6);
         const bool active = i < (int)count;
-        if (threadIdx.x < 3) cnt[threadIdx.x] = 0;
+        if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
         __syncthreads();
         TdNode nd{};
         int axis = 0, sbin = -1, nleft = 0;
-        float cl[2][3], ch[2][3];
-        uint32_t off[3] = {0, 0, 0};
+        uint32_t off[4] = {0, 0, 0, 0};
         if (active) {
             nd = fr[i];
             const uint32_t* g = gbins + (uint64_t)i * TD_NODE_WORDS;
@@ -1401,21 +1373,9 @@ __global__ __launch_bounds__(TD_BLOCK) void k_td_select(
                 axis = wl / TD_BINS;
                 sbin = wl % TD_BINS;
                 nleft = (int)td_readlane(L[0], wl);
-                for (int q = 0; q < 3; ++q) {
-                    cl[0][q] = from_ordered_bits(td_readlane(L[7 + q], wl));
-                    ch[0][q] = from_ordered_bits(td_readlane(L[10 + q], wl));
-                    cl[1][q] = from_ordered_bits(td_readlane(Rn[7 + q], wl));
-                    ch[1][q] = from_ordered_bits(td_readlane(Rn[10 + q], wl));
-                }
-            } else {
-                for (int sd = 0; sd < 2; ++sd)
-                    for (int q = 0; q < 3; ++q) {
-                        cl[sd][q] = nd.clo[q];
-                        ch[sd][q] = nd.chi[q];
-                    }
             }
             if (lane == 0) {
-                uint32_t need[3] = {0, 0, 0};
+                uint32_t need[4] = {0, 0, 0, 0};
                 for (int side = 0; side < 2; ++side) {
                     const int cs = side ? nd.end - nd.begin - nleft : nleft;
                     if (cs >= 2 && cs <= TD_SMALL) {
@@ -1424,9 +1384,10 @@ __global__ __launch_bounds__(TD_BLOCK) void k_td_select(
                     } else if (cs > TD_SMALL) {
                         need[0] += 1;
                         need[2] += 1;
+                        need[3] += (uint32_t)((cs + TD_CHUNK - 1) / TD_CHUNK);
                     }
                 }
-                for (int c = 0; c < 3; ++c) off[c] = need[c] ? atomicAdd(&cnt[c], need[c]) : 0u;
+                for (int c = 0; c < 4; ++c) off[c] = need[c] ? atomicAdd(&cnt[c], need[c]) : 0u;
             }
         }
         __syncthreads();
@@ -1434,12 +1395,12 @@ __global__ __launch_bounds__(TD_BLOCK) void k_td_select(
             base[0] = cnt[0] ? atomicAdd(ctr + TD_C_NEXT, cnt[0]) : 0u;
             base[1] = cnt[1] ? atomicAdd(ctr + TD_C_SMALL, cnt[1]) : 0u;
             base[2] = cnt[2] ? atomicAdd(ctr + TD_C_FR + (in ^ 1), cnt[2]) : 0u;
+            base[3] = cnt[3] ? atomicAdd(ctr + TD_C_TASKS + (in ^ 1), cnt[3]) : 0u;
         }
         __syncthreads();
-        int slot[2] = {-1, -1};
+        int slot[2] = {-1, -1}, t0[2] = {0, 0}, nt[2] = {0, 0}, cb0[2] = {0, 0};
         if (active && lane == 0) {
-            uint32_t nid = base[0] + off[0], ns = base[1] + off[1], nf = base[2] + off[2];
-            split[i] = TdSplit{axis, sbin, nleft, 0};
+            uint32_t nid = base[0] + off[0], ns = base[1] + off[1], nf = base[2] + off[2], ntk = base[3] + off[3];
             int kid[2];
             for (int side = 0; side < 2; ++side) {
                 const int cb = side ? nd.begin + nleft : nd.begin, ce = side ? nd.end : nd.begin + nleft;
@@ -1462,8 +1423,9 @@ __global__ __launch_bounds__(TD_BLOCK) void k_td_select(
                     }
                     small_out[j] = TdSmall{cb, ce, n + (int)id, (nd.depth + 1) << 1 | (in ^ 1)};
                 } else {
-                    const uint32_t j = nf++;
-                    if (j >= (uint32_t)fr_cap) {
+                    const uint32_t j = nf++, tn = (uint32_t)((cs + TD_CHUNK - 1) / TD_CHUNK), tb = ntk;
+                    ntk += tn;
+                    if (j >= (uint32_t)fr_cap || tb + tn > (uint32_t)task_cap) {
                         atomicOr(err, E_TREE);
                         continue;
                     }
@@ -1473,22 +1435,27 @@ __global__ __launch_bounds__(TD_BLOCK) void k_td_select(
                     c.id = n + (int)id;
                     c.depth = nd.depth + 1;
                     for (int q = 0; q < 3; ++q) {
-                        c.clo[q] = cl[side][q];
-                        c.chi[q] = ch[side][q];
+                        c.clo[q] = 0xFFFFFFFFu;  // k_td_partition gathers them
+                        c.chi[q] = 0u;
                     }
-                    c.task0 = 0;
+                    c.task0 = (int)tb;
                     c.pad = 0;
                     fr_out[j] = c;
                     slot[side] = (int)j;
+                    t0[side] = (int)tb;
+                    nt[side] = (int)tn;
+                    cb0[side] = cb;
                 }
             }
             kids[nd.id - n] = make_int2(kid[0], kid[1]);
+            split[i] = TdSplit{axis, sbin, nleft, {slot[0], slot[1]}, {0, 0, 0}};
         }
         for (int side = 0; side < 2; ++side) {
             const int j = __shfl(slot[side], 0);
-            if (j >= 0)
-                for (int w = lane; w < TD_NODE_WORDS; w += 64)
-                    gbins_out[(uint64_t)j * TD_NODE_WORDS + w] = td_identity(w);
+            if (j < 0) continue;
+            for (int w = lane; w < TD_NODE_WORDS; w += 64) gbins_out[(uint64_t)j * TD_NODE_WORDS + w] = td_identity(w);
+            const int tb = __shfl(t0[side], 0), tn = __shfl(nt[side], 0), cb = __shfl(cb0[side], 0);
+            for (int u = lane; u < tn; u += 64) tasks_out[tb + u] = make_int2(j, cb + u * TD_CHUNK);
         }
         __syncthreads();  // cnt and base are reset for the next round
     }
@@ -1496,15 +1463,19 @@ __global__ __launch_bounds__(TD_BLOCK) void k_td_select(
 
 // Per task: its triangles to their side of the cut in the other buffer, in
 // order: rounds of TD_BLOCK consecutive triangles, placed by a block scan of
-// the side flags.  A triangle that becomes a leaf writes its index to `order`.
+// the side flags.  A triangle that becomes a leaf writes its index to `order`;
+// the centroid bounds of each side that is a large node are reduced over the
+// task and merged into its frontier record.
 __global__ __launch_bounds__(TD_BLOCK) void k_td_partition(const float4* __restrict__ pin, float4* __restrict__ pout,
                                                            const TdNode* __restrict__ fr, const int2* __restrict__ tasks,
-                                                           const uint32_t* __restrict__ ctr,
+                                                           const uint32_t* __restrict__ ctr, int in,
                                                            const TdSplit* __restrict__ split,
                                                            const uint32_t* __restrict__ tcnt,
-                                                           uint32_t* __restrict__ order, unsigned* err) {
+                                                           TdNode* __restrict__ fr_out, uint32_t* __restrict__ order,
+                                                           unsigned* err) {
     __shared__ uint32_t wsum[TD_BLOCK / 64], wl[TD_BLOCK / 64], wr[TD_BLOCK / 64];
-    const uint32_t n_tasks = ctr[TD_C_TASKS];
+    __shared__ uint32_t wb[TD_BLOCK / 64][12];
+    const uint32_t n_tasks = ctr[TD_C_TASKS + in];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint64_t below = (1ull << lane) - 1ull;
     for (uint32_t t = blockIdx.x; t < n_tasks; t += gridDim.x) {  // uniform over the block
@@ -1524,12 +1495,20 @@ __global__ __launch_bounds__(TD_BLOCK) void k_td_partition(const float4* __restr
         } else {
             before = (uint32_t)min(cb - nd.begin, sp.nleft);
         }
-        const float lo = nd.clo[sp.axis], sc = td_scale(nd.clo[sp.axis], nd.chi[sp.axis]);
+        const float lo = from_ordered_bits(nd.clo[sp.axis]);
+        const float sc = td_scale(lo, from_ordered_bits(nd.chi[sp.axis]));
         const uint32_t l0 = (uint32_t)nd.begin + before;
         const uint32_t r0 = (uint32_t)(nd.begin + sp.nleft) + ((uint32_t)(cb - nd.begin) - before);
         const uint32_t lend = (uint32_t)(nd.begin + sp.nleft), rend = (uint32_t)nd.end;
         const bool leaf_l = sp.nleft == 1, leaf_r = size - sp.nleft == 1;
         uint32_t run_l = 0, run_r = 0;
+        // centroid bounds of the sides (ordered bits): [side][lo xyz, hi xyz]
+        uint32_t cbd[2][6];
+        for (int q = 0; q < 3; ++q)
+            for (int sd = 0; sd < 2; ++sd) {
+                cbd[sd][q] = 0xFFFFFFFFu;
+                cbd[sd][3 + q] = 0u;
+            }
         bool bad = false;
         for (int base = cb; base < ce; base += TD_BLOCK) {
             const int p = base + (int)threadIdx.x;
@@ -1539,12 +1518,18 @@ __global__ __launch_bounds__(TD_BLOCK) void k_td_partition(const float4* __restr
             if (act) {
                 a = pin[2 * (uint64_t)p];
                 b = pin[2 * (uint64_t)p + 1];
+                const float c[3] = {td_centre(a.x, b.x), td_centre(a.y, b.y), td_centre(a.z, b.z)};
                 if (sp.bin >= 0) {
-                    const float c = sp.axis == 0 ? td_centre(a.x, b.x)
-                                                 : (sp.axis == 1 ? td_centre(a.y, b.y) : td_centre(a.z, b.z));
-                    left = td_bin(c, lo, sc) <= sp.bin;
+                    const float ca = sp.axis == 0 ? c[0] : (sp.axis == 1 ? c[1] : c[2]);
+                    left = td_bin(ca, lo, sc) <= sp.bin;
                 } else {
                     left = p - nd.begin < sp.nleft;
+                }
+                const int sd = left ? 0 : 1;
+                for (int q = 0; q < 3; ++q) {
+                    const uint32_t o = ordered_bits(c[q]);
+                    cbd[sd][q] = min(cbd[sd][q], o);
+                    cbd[sd][3 + q] = max(cbd[sd][3 + q], o);
                 }
             }
             const uint64_t bl = __ballot(act && left), br = __ballot(act && !left);
@@ -1577,6 +1562,28 @@ __global__ __launch_bounds__(TD_BLOCK) void k_td_partition(const float4* __restr
             run_l += tl;
             run_r += tr;
         }
+        for (int sd = 0; sd < 2; ++sd) {
+            if (sp.slot[sd] < 0) continue;  // uniform over the block
+            for (int q = 0; q < 3; ++q) {
+                const uint32_t mn = td_wave_ext<false>(cbd[sd][q]), mx = td_wave_ext<true>(cbd[sd][3 + q]);
+                if (lane == 0) {
+                    wb[w][q] = mn;
+                    wb[w][3 + q] = mx;
+                }
+            }
+            __syncthreads();
+            if (threadIdx.x < 6) {
+                const int q = threadIdx.x;
+                uint32_t v = wb[0][q];
+                for (int x = 1; x < TD_BLOCK / 64; ++x) v = q < 3 ? min(v, wb[x][q]) : max(v, wb[x][q]);
+                TdNode* c = fr_out + sp.slot[sd];
+                if (q < 3)
+                    atomicMin(&c->clo[q], v);
+                else
+                    atomicMax(&c->chi[q - 3], v);
+            }
+            __syncthreads();  // wb is rewritten for the other side
+        }
         if (sp.bin >= 0) {  // the cut must agree with the bins it was chosen from
             uint32_t want = 0;
             const uint32_t* c = tcnt + (uint64_t)t * 3 * TD_BINS + sp.axis * TD_BINS;
@@ -1607,7 +1614,7 @@ __global__ __launch_bounds__(TD_BLOCK) void k_td_small(const float4* __restrict_
     constexpr int WAVES = TD_BLOCK / 64;
     __shared__ float4 srl[WAVES][TD_SMALL], srh[WAVES][TD_SMALL];
     __shared__ uint8_t sperm[WAVES][2][TD_SMALL];
-    __shared__ uint32_t ssb[WAVES][3 * TD_BINS * TD_SW];
+    __shared__ uint32_t ssb[WAVES][3 * TD_BINS * TD_BW];
     __shared__ int4 sstk[WAVES][TD_SMALL / 2];  // pending ranges are disjoint and hold >= 2 triangles
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t i = blockIdx.x * WAVES + w;
@@ -1662,8 +1669,8 @@ __global__ __launch_bounds__(TD_BLOCK) void k_td_small(const float4* __restrict_
             }
             int axis = 0, sbin = -1;
             if (depth < TD_DEPTH_CAP && (sc[0] > 0.0f || sc[1] > 0.0f || sc[2] > 0.0f)) {
-                for (int x = lane; x < 3 * TD_BINS * TD_SW; x += 64) {
-                    const int f = x % TD_SW;
+                for (int x = lane; x < 3 * TD_BINS * TD_BW; x += 64) {
+                    const int f = x % TD_BW;
                     sb[x] = f == 0 ? 0u : (f <= 3 ? 0xFFFFFFFFu : 0u);
                 }
                 td_wave_sync();
@@ -1671,7 +1678,7 @@ __global__ __launch_bounds__(TD_BLOCK) void k_td_small(const float4* __restrict_
                     const unsigned bl[3] = {ordered_bits(l.x), ordered_bits(l.y), ordered_bits(l.z)};
                     const unsigned bh[3] = {ordered_bits(h.x), ordered_bits(h.y), ordered_bits(h.z)};
                     for (int k = 0; k < 3; ++k) {
-                        uint32_t* bb = sb + (k * TD_BINS + td_bin(c[k], mn[k], sc[k])) * TD_SW;
+                        uint32_t* bb = sb + (k * TD_BINS + td_bin(c[k], mn[k], sc[k])) * TD_BW;
                         atomicAdd(bb, 1u);
                         for (int q3 = 0; q3 < 3; ++q3) {
                             atomicMin(bb + 1 + q3, bl[q3]);
@@ -1680,12 +1687,12 @@ __global__ __launch_bounds__(TD_BLOCK) void k_td_small(const float4* __restrict_
                     }
                 }
                 td_wave_sync();
-                uint32_t L[TD_SW], Rn[TD_SW];
-                for (int f = 0; f < TD_SW; ++f) {
+                uint32_t L[TD_BW], Rn[TD_BW];
+                for (int f = 0; f < TD_BW; ++f) {
                     const uint32_t id0 = f == 0 ? 0u : (f <= 3 ? 0xFFFFFFFFu : 0u);
-                    L[f] = lane < 3 * TD_BINS ? sb[lane * TD_SW + f] : id0;
+                    L[f] = lane < 3 * TD_BINS ? sb[lane * TD_BW + f] : id0;
                 }
-                const int wl = td_choose<TD_SW>(L, Rn, lane, true);
+                const int wl = td_choose<TD_BW>(L, Rn, lane, true);
                 if (wl >= 0) {
                     axis = wl / TD_BINS;
                     sbin = wl % TD_BINS;
@@ -2418,15 +2425,17 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
         const unsigned g_task = (unsigned)std::min<uint64_t>(
             {(uint64_t)td_task_cap, n / TD_CHUNK + nfr + 1, (uint64_t)TD_GRID_MAX});
         const unsigned g_sel = (unsigned)std::min<uint64_t>(div_up(nfr, TD_BLOCK / 64), TD_GRID_MAX);
-        k_td_tasks<<<1, 1024, 0, sb>>>(td_fr + (uint64_t)in * td_fr_cap, td_ctr, in, td_fr_cap, td_tasks);
-        k_td_bin<<<g_task, TD_BLOCK, 0, sb>>>(td_prims + in * pw, td_fr + (uint64_t)in * td_fr_cap, td_tasks, td_ctr,
-                                               td_bins + in * bw, td_tcnt);
+        const int2* tin = td_tasks + (uint64_t)in * td_task_cap;
+        int2* tout = td_tasks + (uint64_t)out * td_task_cap;
+        k_td_bin<<<g_task, TD_BLOCK, 0, sb>>>(td_prims + in * pw, td_fr + (uint64_t)in * td_fr_cap, tin, td_ctr, in,
+                                              td_bins + in * bw, td_tcnt);
         k_td_select<<<g_sel, TD_BLOCK, 0, sb>>>(td_fr + (uint64_t)in * td_fr_cap, td_ctr, in, td_bins + in * bw, nt,
-                                                td_fr_cap, td_small_cap, td_split, td_fr + (uint64_t)out * td_fr_cap,
-                                                td_bins + out * bw, td_small, pinfo, kids, td_depth, err);
+                                                td_fr_cap, td_small_cap, td_task_cap, td_split,
+                                                td_fr + (uint64_t)out * td_fr_cap, td_bins + out * bw, tout, td_small,
+                                                pinfo, kids, td_depth, err);
         k_td_partition<<<g_task, TD_BLOCK, 0, sb>>>(td_prims + in * pw, td_prims + out * pw,
-                                                    td_fr + (uint64_t)in * td_fr_cap, td_tasks, td_ctr, td_split,
-                                                    td_tcnt, idx_s, err);
+                                                    td_fr + (uint64_t)in * td_fr_cap, tin, td_ctr, in, td_split,
+                                                    td_tcnt, td_fr + (uint64_t)out * td_fr_cap, idx_s, err);
     };
     // the level passes are enqueued a few at a time just before the host
     // waits on a readback below, so their launches cost the host no time
@@ -2469,7 +2478,7 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
         td_prims = S.as<float4>(S_TD_PRIMS, 4 * (uint64_t)n);  // two buffers of two float4 per triangle
         td_fr = S.as<TdNode>(S_TD_FR, 2 * (uint64_t)td_fr_cap);
         td_bins = S.as<uint32_t>(S_TD_BINS, 2 * (uint64_t)td_fr_cap * TD_NODE_WORDS);
-        td_tasks = S.as<int2>(S_TD_TASKS, (uint64_t)td_task_cap);
+        td_tasks = S.as<int2>(S_TD_TASKS, 2 * (uint64_t)td_task_cap);  // the task lists of two levels
         td_tcnt = S.as<uint32_t>(S_TD_TCNT, (uint64_t)td_task_cap * 3 * TD_BINS);
         td_split = S.as<TdSplit>(S_TD_SPLIT, (uint64_t)td_fr_cap);
         td_small = S.as<TdSmall>(S_TD_SMALL, (uint64_t)td_small_cap);
@@ -2482,8 +2491,8 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
         // between the host's readbacks; the rest follows the distance launch.
         // A balanced cut halves a node: log2(n / TD_SMALL) levels, a few more
         // for uneven cuts; levels past the last large node cost a few empty launches
-        k_td_init<<<div_up(n, TB), TB, 0, sb>>>(d_tris, nt, bounds, td_prims, td_fr, td_bins, td_ctr, pinfo, td_depth,
-                                                td_hist);
+        k_td_init<<<div_up(n, TB), TB, 0, sb>>>(d_tris, nt, bounds, td_prims, td_fr, td_tasks, td_bins, td_ctr, pinfo,
+                                                td_depth, td_hist);
         ploc_iters = (int)std::ceil(std::log2((double)n / TD_SMALL)) + 4;
         td_pump(2);
     } else {
