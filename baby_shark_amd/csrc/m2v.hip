@@ -2437,12 +2437,6 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
                                                     td_fr + (uint64_t)in * td_fr_cap, tin, td_ctr, in, td_split,
                                                     td_tcnt, td_fr + (uint64_t)out * td_fr_cap, idx_s, err);
     };
-    // the level passes are enqueued a few at a time just before the host
-    // waits on a readback below, so their launches cost the host no time
-    int td_next = 0;
-    auto td_pump = [&](int upto) {
-        for (; td_next < std::min(upto, ploc_iters); ++td_next) td_level(td_next);
-    };
     auto ploc_iteration = [&](int it) {
         const uint32_t* cnt = ring + (it & 1);
         int* cin = (it & 1) ? cl1 : cl0;
@@ -2486,15 +2480,15 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
         td_hist = S.as<uint32_t>(S_TD_HIST, 1024);
         td_depth = reinterpret_cast<uint8_t*>(height);  // one byte per internal node
         pinfo = S.as<PlocInfo>(S_PLOC_INFO, n_nodes);
-        // the level passes need no host round trip: they run beside the
-        // subdivision and leaf passes below, where the device is mostly idle
-        // between the host's readbacks; the rest follows the distance launch.
-        // A balanced cut halves a node: log2(n / TD_SMALL) levels, a few more
-        // for uneven cuts; levels past the last large node cost a few empty launches
+        // the level passes follow the distance launch (below) and run beside
+        // the candidate sort and the distance pass: run in the subdivision and
+        // leaf passes instead, between the host's readbacks, they lengthened
+        // that part by more than they slow the distance pass.  A balanced cut
+        // halves a node: log2(n / TD_SMALL) levels, a few more for uneven cuts;
+        // levels past the last large node cost a few empty launches
         k_td_init<<<div_up(n, TB), TB, 0, sb>>>(d_tris, nt, bounds, td_prims, td_fr, td_tasks, td_bins, td_ctr, pinfo,
                                                 td_depth, td_hist);
         ploc_iters = (int)std::ceil(std::log2((double)n / TD_SMALL)) + 4;
-        td_pump(2);
     } else {
         BS_HIP(hipMemsetAsync(parent, 0xFF, sizeof(int) * n_nodes, sb));
         BS_HIP(hipMemsetAsync(hword, 0, sizeof(unsigned) * std::max(n_int, 1), sb));
@@ -2540,7 +2534,6 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     rb.add(row_off + n, &h_rows);
     rb.add(sub_off + n, &n_sub);
     rb.add(err, &h_err);
-    td_pump(8);
     rb.wait();
     max_rows = std::max<uint32_t>(1u, h_max_rows);
     const uint64_t total_rows = h_rows;
@@ -2567,7 +2560,6 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     rb.add(counters, hc, C_COUNT);
     rb.add(cov_off + n_sub, &h_cov);
     rb.add(err, &h_err);
-    td_pump(14);
     rb.wait();
     const uint64_t n_cov = h_cov;
     if (h_err & E_RANGE)
@@ -2619,7 +2611,6 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     uint32_t h_leaves = 0;
     rb.add(d_count, &h_leaves);
     rb.add(err, &h_err);
-    td_pump(ploc_iters);
     rb.wait();
     const uint64_t n_leaves = h_leaves;
     if (h_err & E_HASH) throw Error(BS_ERR_INTERNAL, "leaf hash table overflow");
@@ -2715,7 +2706,7 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
         k_ploc_place<<<div_up(n_nodes, TB), TB, 0, sb>>>(ring + 3, pinfo, kids, accs, d_tris, idx_s, nt,
                                                         prm.wn_accuracy, nodes, btris, err);
     } else if (topdown) {
-        td_pump(ploc_iters);
+        for (int l = 0; l < ploc_iters; ++l) td_level(l);
         auto td_read = [&]() {
             BS_HIP(hipMemcpyAsync(hs, td_ctr, sizeof(uint32_t) * TD_C_WORDS, hipMemcpyDeviceToHost, sb));
             BS_HIP(hipMemcpyAsync(hs + 258, err, sizeof(uint32_t), hipMemcpyDeviceToHost, sb));
