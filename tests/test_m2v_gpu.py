@@ -287,6 +287,22 @@ def test_tree_builds_agree(gpu_ctx, oracle_lib):
     check_case(gpu_ctx, oracle_lib, soup, h, band)
 
 
+def test_topdown_tree_is_deterministic(gpu_ctx):
+    """The top-down build takes node ids with atomics, so ids differ from run
+    to run; the tree, the triangle order and the layout must not: two
+    conversions of the same mesh give the same volume bit for bit and the
+    same visit counts."""
+    soup = mg.displaced_icosphere(7, radius=0.8, seed=3)
+    h, band = 0.004, 3
+    runs = _tree_runs(gpu_ctx, soup, h, band, (("a", 0), ("b", 0)))
+    (a, sa), (b, sb) = runs["a"], runs["b"]
+    assert sa["n_bvh_iterations"] > 0
+    assert np.array_equal(a.origins, b.origins) and np.array_equal(a.masks, b.masks)
+    assert np.array_equal(a.values.view(np.uint32), b.values.view(np.uint32))
+    for k in ("wn_node_visits", "wn_far_evals", "wn_leaf_visits", "wn_leaf_tris", "bvh_height"):
+        assert sa[k] == sb[k], k
+
+
 def test_topdown_tree_coincident_centroids(gpu_ctx):
     """300 copies of each triangle of a closed 1280-triangle mesh (384,000
     triangles): nodes whose triangles share one centre cannot be cut by bins
