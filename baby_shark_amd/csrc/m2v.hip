@@ -1495,20 +1495,24 @@ __global__ __launch_bounds__(TD_BLOCK) void k_td_partition(const float4* __restr
         } else {
             before = (uint32_t)min(cb - nd.begin, sp.nleft);
         }
-        const float lo = from_ordered_bits(nd.clo[sp.axis]);
-        const float sc = td_scale(lo, from_ordered_bits(nd.chi[sp.axis]));
+        // the cut axis's bounds by selects (a run-time index would send nd to LDS)
+        const uint32_t clo_a = sp.axis == 0 ? nd.clo[0] : (sp.axis == 1 ? nd.clo[1] : nd.clo[2]);
+        const uint32_t chi_a = sp.axis == 0 ? nd.chi[0] : (sp.axis == 1 ? nd.chi[1] : nd.chi[2]);
+        const float lo = from_ordered_bits(clo_a);
+        const float sc = td_scale(lo, from_ordered_bits(chi_a));
         const uint32_t l0 = (uint32_t)nd.begin + before;
         const uint32_t r0 = (uint32_t)(nd.begin + sp.nleft) + ((uint32_t)(cb - nd.begin) - before);
         const uint32_t lend = (uint32_t)(nd.begin + sp.nleft), rend = (uint32_t)nd.end;
         const bool leaf_l = sp.nleft == 1, leaf_r = size - sp.nleft == 1;
         uint32_t run_l = 0, run_r = 0;
-        // centroid bounds of the sides (ordered bits): [side][lo xyz, hi xyz]
-        uint32_t cbd[2][6];
-        for (int q = 0; q < 3; ++q)
-            for (int sd = 0; sd < 2; ++sd) {
-                cbd[sd][q] = 0xFFFFFFFFu;
-                cbd[sd][3 + q] = 0u;
-            }
+        // centroid bounds of the sides (ordered bits), lo xyz then hi xyz; the
+        // side is picked by selects (an array indexed by it would go to LDS)
+        uint32_t cl0[3], ch0[3], cl1[3], ch1[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            cl0[q] = cl1[q] = 0xFFFFFFFFu;
+            ch0[q] = ch1[q] = 0u;
+        }
         bool bad = false;
         for (int base = cb; base < ce; base += TD_BLOCK) {
             const int p = base + (int)threadIdx.x;
@@ -1525,11 +1529,13 @@ __global__ __launch_bounds__(TD_BLOCK) void k_td_partition(const float4* __restr
                 } else {
                     left = p - nd.begin < sp.nleft;
                 }
-                const int sd = left ? 0 : 1;
+#pragma unroll
                 for (int q = 0; q < 3; ++q) {
                     const uint32_t o = ordered_bits(c[q]);
-                    cbd[sd][q] = min(cbd[sd][q], o);
-                    cbd[sd][3 + q] = max(cbd[sd][3 + q], o);
+                    cl0[q] = left ? min(cl0[q], o) : cl0[q];
+                    ch0[q] = left ? max(ch0[q], o) : ch0[q];
+                    cl1[q] = left ? cl1[q] : min(cl1[q], o);
+                    ch1[q] = left ? ch1[q] : max(ch1[q], o);
                 }
             }
             const uint64_t bl = __ballot(act && left), br = __ballot(act && !left);
@@ -1562,10 +1568,13 @@ __global__ __launch_bounds__(TD_BLOCK) void k_td_partition(const float4* __restr
             run_l += tl;
             run_r += tr;
         }
+#pragma unroll
         for (int sd = 0; sd < 2; ++sd) {
             if (sp.slot[sd] < 0) continue;  // uniform over the block
+#pragma unroll
             for (int q = 0; q < 3; ++q) {
-                const uint32_t mn = td_wave_ext<false>(cbd[sd][q]), mx = td_wave_ext<true>(cbd[sd][3 + q]);
+                const uint32_t mn = td_wave_ext<false>(sd ? cl1[q] : cl0[q]);
+                const uint32_t mx = td_wave_ext<true>(sd ? ch1[q] : ch0[q]);
                 if (lane == 0) {
                     wb[w][q] = mn;
                     wb[w][3 + q] = mx;
