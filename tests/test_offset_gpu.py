@@ -91,3 +91,35 @@ def test_offset_of_subtracted_icospheres(gpu_ctx, oracle_lib):
     I = _import(gpu_ctx, oa).intersect(_import(gpu_ctx, ob)).offset(-0.01)
     rI = oa.intersect(ob).offset(-0.01)
     assert_same_volume(I.export(), rI.export())
+
+
+@pytest.mark.parametrize("r,h,c,D,bx", [
+    (0.3, 0.02, (2.2, 0.013, -0.011), 0.1, 128),      # grows into a fresh 128^3 internal node
+    (3.0, 0.25, (1020.5, 0.1, -0.2), 1.0, 4096),      # grows into a fresh 4096^3 internal node
+    (0.44, 0.02, (-0.5, 0.2, 0.1), 0.1, 0),           # grows across the origin plane
+])
+def test_offset_grows_into_new_blocks(gpu_ctx, oracle_lib, r, h, c, D, bx):
+    """A sphere whose surface sits a few voxels below a block boundary on the
+    x axis, offset outward across it: the output tree gains nodes the input did
+    not have.  The grown volume, its marching-cubes soup, and signs after a
+    flood fill over a grid covering the new block all equal the oracle's."""
+    from test_dc_gpu import _same
+    src = oracle_lib.sphere(r, c, h).export()
+    assert src.origins[:, 0].max() < bx
+    g = bs.VolumeBuilder(gpu_ctx, h).sphere(r, c).offset(D)
+    ref = oracle_lib.sphere(r, c, h).offset(D)
+    ge = g.export()
+    assert ge.origins[:, 0].max() >= bx
+    assert_same_volume(ge, ref.export())
+    _same(bs.MarchingCubesMesher(gpu_ctx).with_voxel_size(h).mesh(g), ref.marching_cubes(h))
+    g.flood_fill()
+    ref.flood_fill()
+    rv = int(round((r + abs(D)) / h)) + 6
+    cx, cy, cz = (int(round(v / h)) for v in c)
+    ax = np.arange(bx - 20, bx + 12, dtype=np.int32)
+    ay = np.arange(cy - rv, cy + rv, 2, dtype=np.int32)
+    az = np.arange(cz - rv, cz + rv, 2, dtype=np.int32)
+    q = np.stack(np.meshgrid(ax, ay, az, indexing="ij"), -1).reshape(-1, 3)
+    gs, rs = g.signs_at(q), ref.signs_at(q)
+    assert np.array_equal(gs, rs)
+    assert gs.min() != gs.max()      # the grid holds inside and outside voxels
