@@ -104,8 +104,9 @@ static void release_side(bs_ctx* c) {
     }
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->ev_leaves) (void)hipEventDestroy(c->ev_leaves);
     if (c->host_pinned) (void)hipHostFree(c->host_pinned);
-    c->ev_fork = c->ev_join = nullptr;
+    c->ev_fork = c->ev_join = c->ev_leaves = nullptr;
     c->host_pinned = nullptr;
 }
 

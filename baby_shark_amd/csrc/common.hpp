@@ -49,7 +49,8 @@ enum ScratchSlot : int {
     S_DC_CELLMASK, S_DC_TRICNT, S_DC_TRIOFF, S_MC_TAB, S_MC_UNITS, S_MC_TILES, S_MC_T8, S_MC_T8K, S_MC_T128,
     S_MC_T128K, S_MC_TORG, S_MC_LANES, S_LEAF_BOX, S_SG_PRED, S_MLP_PART, S_MLP_GRAD, S_MLP_PERM, S_MC_FLAGS, S_MC_CV, S_MC_V12,
     S_PLOC_BOX, S_PLOC_INFO, S_PLOC_C0, S_PLOC_C1, S_PLOC_NN, S_PLOC_KEEP, S_PLOC_POS, S_PLOC_RING,
-    S_TD_PRIMS, S_TD_FR, S_TD_BINS, S_TD_TASKS, S_TD_TCNT, S_TD_SPLIT, S_TD_SMALL, S_TD_HIST, S_COUNT
+    S_TD_PRIMS, S_TD_FR, S_TD_BINS, S_TD_TASKS, S_TD_TCNT, S_TD_SPLIT, S_TD_SMALL, S_TD_HIST,
+    S_LEAF_MKEY, S_LEAF_MKEY_S, S_LEAF_IDS, S_LEAF_PERM, S_COUNT
 };
 
 struct Scratch {
@@ -97,7 +98,7 @@ struct bs_ctx {
     // second stream for work that overlaps the main one (the BVH build), with
     // its events and a pinned readback buffer; created on first use
     hipStream_t stream2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_leaves = nullptr;
     uint32_t* host_pinned = nullptr;  // 512 words
     // tile classifier of BS_SIGN_SURROGATE (not owned), and its cull threshold
     // in voxels; without a network the gate uses each tile's true smallest |d|

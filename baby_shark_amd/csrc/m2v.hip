@@ -570,22 +570,53 @@ __global__ __launch_bounds__(DIST_BLOCK) void k_distance(const int32_t* __restri
     }
 }
 
-// active voxel list: slot = leaf * 512 + offset, leaves in order
-__global__ void k_leaf_popcount(const uint64_t* __restrict__ masks, uint64_t n_leaves, uint32_t* cnt) {
+// 21 bits spread to every third bit (Morton codes)
+__device__ inline uint64_t spread21(uint64_t v) {
+    v &= 0x1FFFFF;
+    v = (v | v << 32) & 0x1F00000000FFFFull;
+    v = (v | v << 16) & 0x1F0000FF0000FFull;
+    v = (v | v << 8) & 0x100F00F00F00F00Full;
+    v = (v | v << 4) & 0x10C30C30C30C30C3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+}
+
+// Active voxel list, slot = leaf * 512 + offset.  The leaves are listed in the
+// order perm gives (identity when perm is null): count per listed leaf here.
+__global__ void k_leaf_popcount(const uint64_t* __restrict__ masks, uint64_t n_leaves,
+                                const uint32_t* __restrict__ perm, uint32_t* cnt) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_leaves) return;
+    const uint64_t leaf = perm ? perm[i] : i;
     uint32_t c = 0;
-    for (int w = 0; w < 8; ++w) c += __popcll(masks[8 * i + w]);
+    for (int w = 0; w < 8; ++w) c += __popcll(masks[8 * leaf + w]);
     cnt[i] = c;
 }
 
-// Active voxel list, slot = leaf * 512 + offset.  Within a leaf the voxels are
-// listed in Morton order, so 64 consecutive entries (one winding-number wave)
-// form compact 4x4x4 blocks rather than 8x8 slabs.
+// Z-order key of each leaf's coordinates (in leaves, relative to the lowest
+// leaf of the grid, 21 bits per axis) with the leaf id as its value: sorted,
+// consecutive leaves are neighbours in space far more often than in the
+// lexicographic order, where the end of one row of leaves jumps to the other
+// side of the surface.
+__global__ void k_leaf_morton(const int32_t* __restrict__ origins, uint64_t n_leaves, int lx0, int ly0, int lz0,
+                              uint64_t* keys, uint32_t* ids) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_leaves) return;
+    const uint64_t x = (uint64_t)(uint32_t)((origins[4 * i] >> 3) - lx0);
+    const uint64_t y = (uint64_t)(uint32_t)((origins[4 * i + 1] >> 3) - ly0);
+    const uint64_t z = (uint64_t)(uint32_t)((origins[4 * i + 2] >> 3) - lz0);
+    keys[i] = (spread21(x) << 2) | (spread21(y) << 1) | spread21(z);
+    ids[i] = (uint32_t)i;
+}
+
+// Within a leaf the voxels are listed in Morton order, so 64 consecutive
+// entries (one winding-number wave) form compact 4x4x4 blocks rather than 8x8
+// slabs.  Block b lists leaf perm[b] (b when perm is null) at act_off[b].
 __global__ __launch_bounds__(512) void k_active_list(const uint64_t* __restrict__ masks,
+                                                     const uint32_t* __restrict__ perm,
                                                      const uint32_t* __restrict__ act_off, uint32_t* out) {
     __shared__ uint32_t wcnt[8];
-    const int leaf = blockIdx.x;
+    const int leaf = perm ? (int)perm[blockIdx.x] : (int)blockIdx.x;
     const int m = threadIdx.x;  // Morton index of the voxel in the leaf
     const int w = m >> 6, lane = m & 63;
     const int x = ((m >> 2) & 1) | ((m >> 4) & 2) | ((m >> 6) & 4);
@@ -598,7 +629,7 @@ __global__ __launch_bounds__(512) void k_active_list(const uint64_t* __restrict_
     __syncthreads();
     uint32_t before = 0;
     for (int k = 0; k < w; ++k) before += wcnt[k];
-    if (act) out[act_off[leaf] + before + __popcll(b & ((1ull << lane) - 1))] = (uint32_t)leaf * 512u + off;
+    if (act) out[act_off[blockIdx.x] + before + __popcll(b & ((1ull << lane) - 1))] = (uint32_t)leaf * 512u + off;
 }
 
 // ---------------------------------------------------------------------------
@@ -663,16 +694,6 @@ __global__ __launch_bounds__(1024) void k_bounds_final(const uint4* __restrict__
             for (int k = 0; k < 3; ++k) { lo[k] = min(lo[k], sl[q][k]); hi[k] = max(hi[k], sh[q][k]); }
         for (int k = 0; k < 3; ++k) { bounds[k] = lo[k]; bounds[3 + k] = hi[k]; }
     }
-}
-
-__device__ inline uint64_t spread21(uint64_t v) {
-    v &= 0x1FFFFF;
-    v = (v | v << 32) & 0x1F00000000FFFFull;
-    v = (v | v << 16) & 0x1F0000FF0000FFull;
-    v = (v | v << 8) & 0x100F00F00F00F00Full;
-    v = (v | v << 4) & 0x10C30C30C30C30C3ull;
-    v = (v | v << 2) & 0x1249249249249249ull;
-    return v;
 }
 
 __global__ void k_morton(const float* __restrict__ tris, uint64_t n, const unsigned* __restrict__ bounds,
@@ -2165,10 +2186,12 @@ __global__ void k_winding_fp64(const uint32_t* __restrict__ list, uint32_t n, co
 // group.  Groups never span leaves.
 
 // the group roots of every leaf, slot = leaf * 512 + offset, in offset order
+// Block b lists the group roots of leaf perm[b] (b when perm is null) at rep_off[b].
 __global__ __launch_bounds__(512) void k_sign_reps(const uint16_t* __restrict__ roots,
+                                                   const uint32_t* __restrict__ perm,
                                                    const uint32_t* __restrict__ rep_off, uint32_t* __restrict__ reps) {
     __shared__ uint32_t wcnt[8];
-    const uint64_t leaf = blockIdx.x;
+    const uint64_t leaf = perm ? perm[blockIdx.x] : blockIdx.x;
     const int v = threadIdx.x;
     const int w = v >> 6, lane = v & 63;
     const bool rep = roots[leaf * LEAF_VOX + v] == (uint16_t)v;
@@ -2177,7 +2200,14 @@ __global__ __launch_bounds__(512) void k_sign_reps(const uint16_t* __restrict__ 
     __syncthreads();
     uint32_t before = 0;
     for (int k = 0; k < w; ++k) before += wcnt[k];
-    if (rep) reps[rep_off[leaf] + before + __popcll(b & ((1ull << lane) - 1))] = (uint32_t)leaf * 512u + (uint32_t)v;
+    if (rep) reps[rep_off[blockIdx.x] + before + __popcll(b & ((1ull << lane) - 1))] = (uint32_t)leaf * 512u + (uint32_t)v;
+}
+
+// out[i] = in[perm[i]]
+__global__ void k_gather_u32(const uint32_t* __restrict__ in, const uint32_t* __restrict__ perm, uint64_t n,
+                             uint32_t* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[perm[i]];
 }
 
 // every non-root voxel takes the sign of its (already signed) root
@@ -2264,6 +2294,7 @@ void ensure_side_stream(bs_ctx* ctx) {
     if (!ctx->stream2) BS_HIP(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
     if (!ctx->ev_fork) BS_HIP(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
     if (!ctx->ev_join) BS_HIP(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+    if (!ctx->ev_leaves) BS_HIP(hipEventCreateWithFlags(&ctx->ev_leaves, hipEventDisableTiming));
     if (!ctx->host_pinned) BS_HIP(hipHostMalloc((void**)&ctx->host_pinned, sizeof(uint32_t) * 512, hipHostMallocDefault));
 }
 
@@ -2324,6 +2355,9 @@ uint32_t next_pow2(uint64_t v) {
 
 // Dense leaf grids up to this many cells (4 B each); larger extents hash.
 constexpr uint64_t DENSE_LEAF_GRID_MAX = 1ull << 28;
+// volumes of at least this many leaves list their active voxels in the leaves'
+// Z order for the winding-number pass (smaller ones keep the leaf order)
+constexpr uint64_t LEAF_Z_MIN = 4096;
 
 void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const bs_m2v_params& prm, bs_volume** out) {
     *out = nullptr;
@@ -2643,6 +2677,10 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
         k_leaf_assign_dense<<<div_up(n_leaves, TB), TB, 0, st>>>(leaf_idx, n_leaves, ix, vol->origins);
     else
         k_leaf_assign_hash<<<div_up(n_leaves, TB), TB, 0, st>>>(hkeys_sorted, n_leaves, ix, vol->origins, err);
+    // the winding-number list takes the leaves in Z order (sorted on the
+    // second stream once the origins are written)
+    const bool leaf_z = n_leaves >= LEAF_Z_MIN;
+    if (leaf_z) BS_HIP(hipEventRecord(ctx->ev_leaves, st));
     // candidate lists
     uint32_t* ckeys = S.as<uint32_t>(S_CAND_LEAF, n_cov);
     uint32_t* cvals = S.as<uint32_t>(S_CAND_CNT, n_cov);
@@ -2772,20 +2810,33 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
             k_lbvh_level<<<div_up(e - b, TB), TB, 0, sb>>>(inode_s + b, e - b, prm.wn_accuracy, kids, accs, nodes);
         }
     }
+    // the leaves' Z order, after the BVH on the second stream (still beside the distance pass)
+    uint32_t* leaf_perm = nullptr;
+    if (leaf_z) {
+        uint64_t* mk = S.as<uint64_t>(S_LEAF_MKEY, n_leaves);
+        uint64_t* mk_s = S.as<uint64_t>(S_LEAF_MKEY_S, n_leaves);
+        uint32_t* ids = S.as<uint32_t>(S_LEAF_IDS, n_leaves);
+        leaf_perm = S.as<uint32_t>(S_LEAF_PERM, n_leaves);
+        int bits = 1;  // key bits per axis (at most 21; wider grids only sort coarser)
+        for (int k = 0; k < 3; ++k)
+            while (bits < 21 && (1ll << bits) < ghi[k] - glo[k] + 1) ++bits;
+        BS_HIP(hipStreamWaitEvent(sb, ctx->ev_leaves, 0));
+        k_leaf_morton<<<div_up(n_leaves, TB), TB, 0, sb>>>(vol->origins, n_leaves, (int)glo[0], (int)glo[1],
+                                                           (int)glo[2], mk, ids);
+        sort_pairs_on(ctx, sb, S_CUB_B, mk, mk_s, ids, leaf_perm, n_leaves, 3 * bits);
+    }
     BS_HIP(hipEventRecord(ctx->ev_join, sb));
 
-    // ---- 4. active list
+    // ---- 4. the BVH and the leaves' Z order (second stream) must be complete; active list
+    BS_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
     uint32_t* act_cnt = S.as<uint32_t>(S_ACT_CNT, n_leaves + 1);
     uint32_t* act_off = S.as<uint32_t>(S_ACT_OFF, n_leaves + 1);
     BS_HIP(hipMemsetAsync(act_cnt + n_leaves, 0, sizeof(uint32_t), st));
-    k_leaf_popcount<<<div_up(n_leaves, TB), TB, 0, st>>>(vol->masks, n_leaves, act_cnt);
+    k_leaf_popcount<<<div_up(n_leaves, TB), TB, 0, st>>>(vol->masks, n_leaves, leaf_perm, act_cnt);
     exclusive_scan(ctx, act_cnt, act_off, n_leaves + 1);
     const uint32_t n_act = read_scalar(ctx, act_off + n_leaves);
     uint32_t* act = S.as<uint32_t>(S_ACT_LIST, n_act);
-    k_active_list<<<(unsigned)n_leaves, 512, 0, st>>>(vol->masks, act_off, act);
-
-    // ---- 5. the BVH (second stream) must be complete
-    BS_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
+    k_active_list<<<(unsigned)n_leaves, 512, 0, st>>>(vol->masks, leaf_perm, act_off, act);
     tm.mark();
 
     // ---- 6. group roots (shared and surrogate signs): the winding number is evaluated
@@ -2798,11 +2849,18 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
         if (surrogate_signs)
             surrogate_groups(ctx, st, vol, h, group_thr - (float)((double)h * (1.0 + 0x1p-10)), roots, rep_cnt,
                              counters + C_SG_TILES);
+        // the roots in the leaves' Z order too (the counts gathered into the
+        // active-list count buffer, whose last word is still 0)
+        const uint32_t* cnt = rep_cnt;
+        if (leaf_perm) {
+            k_gather_u32<<<div_up(n_leaves, TB), TB, 0, st>>>(rep_cnt, leaf_perm, n_leaves, act_cnt);
+            cnt = act_cnt;
+        }
         uint32_t* rep_off = S.as<uint32_t>(S_REP_OFF, n_leaves + 1);
-        exclusive_scan(ctx, rep_cnt, rep_off, n_leaves + 1);
+        exclusive_scan(ctx, cnt, rep_off, n_leaves + 1);
         n_wn = read_scalar(ctx, rep_off + n_leaves);
         uint32_t* reps = S.as<uint32_t>(S_REPS, std::max<uint32_t>(n_wn, 1));
-        k_sign_reps<<<(unsigned)n_leaves, 512, 0, st>>>(roots, rep_off, reps);
+        k_sign_reps<<<(unsigned)n_leaves, 512, 0, st>>>(roots, leaf_perm, rep_off, reps);
         wn_list = reps;
     }
     tm.mark();
