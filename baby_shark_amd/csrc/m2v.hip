@@ -2033,7 +2033,11 @@ __global__ __launch_bounds__(WN_BLOCK) void k_winding(const uint32_t* __restrict
                                                       float* __restrict__ values, uint32_t* __restrict__ recheck,
                                                       unsigned long long* counters, unsigned* err) {
     const int lane = threadIdx.x & 63;
-    const uint32_t gid = blockIdx.x * WN_BLOCK + threadIdx.x;
+    // workgroups go to the 8 XCDs in turn: XCD x takes the x-th contiguous
+    // eighth of the list, so each XCD's L2 serves one compact part of the shell
+    const uint32_t nb = gridDim.x, q = nb >> 3, rem = nb & 7, xcd = blockIdx.x & 7;
+    const uint32_t wg = xcd * q + min(xcd, rem) + (blockIdx.x >> 3);
+    const uint32_t gid = wg * WN_BLOCK + threadIdx.x;
     const bool valid = gid < n_act;
     const uint32_t slot = valid ? act[gid] : 0u;
     const uint32_t leaf = slot >> 9, off = slot & 511u;
