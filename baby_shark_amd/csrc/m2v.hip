@@ -2025,6 +2025,9 @@ __device__ inline unsigned long long wn_visit(const NodeRec& r, unsigned long lo
     return nm;
 }
 
+// consecutive workgroups of the winding-number list that one XCD takes in a row
+constexpr uint32_t WN_CHUNK = 512;
+
 template <class Stack, bool COUNT>
 __global__ __launch_bounds__(WN_BLOCK) void k_winding(const uint32_t* __restrict__ act, uint32_t n_act,
                                                       const int32_t* __restrict__ origins, float h,
@@ -2033,10 +2036,17 @@ __global__ __launch_bounds__(WN_BLOCK) void k_winding(const uint32_t* __restrict
                                                       float* __restrict__ values, uint32_t* __restrict__ recheck,
                                                       unsigned long long* counters, unsigned* err) {
     const int lane = threadIdx.x & 63;
-    // workgroups go to the 8 XCDs in turn: XCD x takes the x-th contiguous
-    // eighth of the list, so each XCD's L2 serves one compact part of the shell
-    const uint32_t nb = gridDim.x, q = nb >> 3, rem = nb & 7, xcd = blockIdx.x & 7;
-    const uint32_t wg = xcd * q + min(xcd, rem) + (blockIdx.x >> 3);
+    // workgroups go to the 8 XCDs in turn; renumbered, each XCD works through
+    // runs of WN_CHUNK consecutive workgroups (compact parts of the Z-ordered
+    // list) that share node records in its L2, and the runs interleave across
+    // XCDs for balance.  The last partial round keeps its numbering.
+    constexpr uint32_t CH = WN_CHUNK;
+    const uint32_t full = gridDim.x / (8 * CH) * (8 * CH);
+    uint32_t wg = blockIdx.x;
+    if (wg < full) {
+        const uint32_t xcd = wg & 7, i = wg >> 3;
+        wg = ((i / CH) * 8 + xcd) * CH + i % CH;
+    }
     const uint32_t gid = wg * WN_BLOCK + threadIdx.x;
     const bool valid = gid < n_act;
     const uint32_t slot = valid ? act[gid] : 0u;
