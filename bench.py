@@ -26,7 +26,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per k_winding launch at the default workload, from the FETCH_SIZE
 # (x2.0 calibration) and WRITE_SIZE counters of `bench.py --traffic-probe`
 # under rocprofv3 --pmc
-WINDING_TRAFFIC_BYTES = 2.782e9
+WINDING_TRAFFIC_BYTES = 1.015e9
 
 
 def traffic_probe():
