@@ -375,14 +375,18 @@ __global__ void k_cand_pairs(const DistRec* __restrict__ recs, uint64_t n_sub, c
 }
 
 // off[k] = first entry whose key is >= k, for k = 0..n_leaves (keys sorted;
-// keys >= n_leaves are failed lookups and end the lists).  Every slot is
-// written: a leaf without candidates gets an empty list.
+// keys >= n_leaves are failed lookups and end the lists), by a binary search
+// per leaf.  Every slot is written: a leaf without candidates gets an empty list.
 __global__ void k_cand_offsets(const uint32_t* __restrict__ keys, uint32_t n, uint32_t n_leaves, uint32_t* off) {
-    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e > n) return;
-    const uint32_t k = e < n ? min(keys[e], n_leaves) : n_leaves + 1;
-    const uint32_t lo = e == 0 ? 0u : min(keys[e - 1], n_leaves) + 1;
-    for (uint32_t j = lo; j <= min(k, n_leaves); ++j) off[j] = e;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > n_leaves) return;
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if (keys[mid] < k) lo = mid + 1;
+        else hi = mid;
+    }
+    off[k] = lo;
 }
 
 // ---------------------------------------------------------------------------
@@ -2703,7 +2707,7 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     uint32_t* cand_off = S.as<uint32_t>(S_CAND_OFF, n_leaves + 1);
     k_cand_pairs<<<div_up(n_sub, TB), TB, 0, st>>>(recs, n_sub, cov_off, ix, (uint32_t)n_leaves, ckeys, cvals, err);
     sort_pairs(ctx, ckeys, ckeys_s, cvals, cand_list, n_cov, bits_for(n_leaves));
-    k_cand_offsets<<<div_up(n_cov + 1, TB), TB, 0, st>>>(ckeys_s, (uint32_t)n_cov, (uint32_t)n_leaves, cand_off);
+    k_cand_offsets<<<div_up(n_leaves + 1, TB), TB, 0, st>>>(ckeys_s, (uint32_t)n_cov, (uint32_t)n_leaves, cand_off);
     // a failed lookup (E_HASH) is reported with the other error bits at the
     // end: its pairs sort past every list and its leaf's list is empty, so
     // the passes below never read them
