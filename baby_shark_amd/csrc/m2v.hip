@@ -354,24 +354,48 @@ __global__ void k_leaf_assign_hash(const uint64_t* __restrict__ sorted, uint64_t
 }
 
 // candidate lists: (leaf id, sub-triangle) for every box/leaf overlap,
-// radix-sorted by leaf id (stable, so sub-triangle order within a leaf)
-__global__ void k_cand_pairs(const DistRec* __restrict__ recs, uint64_t n_sub, const uint32_t* __restrict__ cov_off,
-                             LeafIndex ix, uint32_t n_leaves, uint32_t* keys, uint32_t* vals, unsigned* err) {
-    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_sub) return;
-    const DistRec& r = recs[t];
-    const uint32_t base = cov_off[t];
-    bool ok = true;
-    for_each_leaf(r, [&](uint32_t j, int x, int y, int z) {
-        uint32_t lid = ix.find(x, y, z);
-        if (lid >= n_leaves) {
-            ok = false;
-            lid = n_leaves;  // sorts past every real leaf; never read
+// radix-sorted by leaf id (stable, so sub-triangle order within a leaf).
+// A block's sub-triangles own one contiguous range of the output: it is
+// assembled in LDS and written out coalesced (directly when it is longer than
+// CP_LDS entries).
+constexpr int CP_BLOCK = 256;
+constexpr uint32_t CP_LDS = 4096;
+__global__ __launch_bounds__(CP_BLOCK) void k_cand_pairs(const DistRec* __restrict__ recs, uint64_t n_sub,
+                                                         const uint32_t* __restrict__ cov_off, LeafIndex ix,
+                                                         uint32_t n_leaves, uint32_t* keys, uint32_t* vals,
+                                                         unsigned* err) {
+    __shared__ uint32_t sk[CP_LDS], sv[CP_LDS];
+    const uint64_t t0 = (uint64_t)blockIdx.x * CP_BLOCK;
+    const uint64_t t = t0 + threadIdx.x;
+    const uint32_t b0 = cov_off[t0], b1 = cov_off[min<uint64_t>(t0 + CP_BLOCK, n_sub)];
+    const bool staged = b1 - b0 <= CP_LDS;  // the same for the whole block
+    if (t < n_sub) {
+        const DistRec& r = recs[t];
+        const uint32_t base = cov_off[t];
+        bool ok = true;
+        for_each_leaf(r, [&](uint32_t j, int x, int y, int z) {
+            uint32_t lid = ix.find(x, y, z);
+            if (lid >= n_leaves) {
+                ok = false;
+                lid = n_leaves;  // sorts past every real leaf; never read
+            }
+            if (staged) {
+                sk[base - b0 + j] = lid;
+                sv[base - b0 + j] = (uint32_t)t;
+            } else {
+                keys[base + j] = lid;
+                vals[base + j] = (uint32_t)t;
+            }
+        });
+        if (!ok) atomicOr(err, E_HASH);
+    }
+    if (staged) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < b1 - b0; i += CP_BLOCK) {
+            keys[b0 + i] = sk[i];
+            vals[b0 + i] = sv[i];
         }
-        keys[base + j] = lid;
-        vals[base + j] = (uint32_t)t;
-    });
-    if (!ok) atomicOr(err, E_HASH);
+    }
 }
 
 // off[k] = first entry whose key is >= k, for k = 0..n_leaves (keys sorted;
@@ -2705,7 +2729,8 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     uint32_t* ckeys_s = S.as<uint32_t>(S_CURSOR, n_cov);
     uint32_t* cand_list = S.as<uint32_t>(S_CAND_LIST, n_cov);
     uint32_t* cand_off = S.as<uint32_t>(S_CAND_OFF, n_leaves + 1);
-    k_cand_pairs<<<div_up(n_sub, TB), TB, 0, st>>>(recs, n_sub, cov_off, ix, (uint32_t)n_leaves, ckeys, cvals, err);
+    k_cand_pairs<<<div_up(n_sub, CP_BLOCK), CP_BLOCK, 0, st>>>(recs, n_sub, cov_off, ix, (uint32_t)n_leaves, ckeys,
+                                                              cvals, err);
     sort_pairs(ctx, ckeys, ckeys_s, cvals, cand_list, n_cov, bits_for(n_leaves));
     k_cand_offsets<<<div_up(n_leaves + 1, TB), TB, 0, st>>>(ckeys_s, (uint32_t)n_cov, (uint32_t)n_leaves, cand_off);
     // a failed lookup (E_HASH) is reported with the other error bits at the
