@@ -83,6 +83,13 @@ def test_small_sphere_bands(gpu_ctx, oracle_lib, band):
     check_case(gpu_ctx, oracle_lib, mg.displaced_icosphere(3, radius=0.7, amp=0.05, seed=9), 0.03, band)
 
 
+def test_wide_band_candidate_lists(gpu_ctx, oracle_lib):
+    """Band 12: each sub-triangle's box covers 64 or more leaves, so a block's
+    candidate pairs overflow the LDS staging of the pair kernel and are written
+    directly; the volume must still equal the oracle's."""
+    check_case(gpu_ctx, oracle_lib, mg.displaced_icosphere(2, radius=0.7, amp=0.05, seed=9), 0.03, 12)
+
+
 def test_single_triangle_and_axis_aligned(gpu_ctx, oracle_lib):
     # one triangle; a triangle flat in z (zero box extent -> +-1 expansion,
     # mesh_to_volume.rs:136-141); negative coordinates across leaf boundaries
