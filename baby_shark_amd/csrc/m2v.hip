@@ -639,25 +639,31 @@ __global__ void k_leaf_morton(const int32_t* __restrict__ origins, uint64_t n_le
 
 // Within a leaf the voxels are listed in Morton order, so 64 consecutive
 // entries (one winding-number wave) form compact 4x4x4 blocks rather than 8x8
-// slabs.  Block b lists leaf perm[b] (b when perm is null) at act_off[b].
-__global__ __launch_bounds__(512) void k_active_list(const uint64_t* __restrict__ masks,
-                                                     const uint32_t* __restrict__ perm,
-                                                     const uint32_t* __restrict__ act_off, uint32_t* out) {
-    __shared__ uint32_t wcnt[8];
-    const int leaf = perm ? (int)perm[blockIdx.x] : (int)blockIdx.x;
-    const int m = threadIdx.x;  // Morton index of the voxel in the leaf
-    const int w = m >> 6, lane = m & 63;
-    const int x = ((m >> 2) & 1) | ((m >> 4) & 2) | ((m >> 6) & 4);
-    const int y = ((m >> 1) & 1) | ((m >> 3) & 2) | ((m >> 5) & 4);
-    const int z = (m & 1) | ((m >> 2) & 2) | ((m >> 4) & 4);
-    const int off = (x << 6) | (y << 3) | z;
-    const bool act = (masks[(uint64_t)leaf * 8 + (off >> 6)] >> (off & 63)) & 1ull;
-    const unsigned long long b = __ballot(act);
-    if (lane == 0) wcnt[w] = (uint32_t)__popcll(b);
-    __syncthreads();
-    uint32_t before = 0;
-    for (int k = 0; k < w; ++k) before += wcnt[k];
-    if (act) out[act_off[blockIdx.x] + before + __popcll(b & ((1ull << lane) - 1))] = (uint32_t)leaf * 512u + off;
+// slabs.  One wave per listed leaf: wave b lists leaf perm[b] (b when perm is
+// null) at act_off[b], 64 Morton indices per step.
+constexpr int AL_BLOCK = 256;
+__global__ __launch_bounds__(AL_BLOCK) void k_active_list(const uint64_t* __restrict__ masks,
+                                                          const uint32_t* __restrict__ perm,
+                                                          const uint32_t* __restrict__ act_off, uint64_t n_leaves,
+                                                          uint32_t* out) {
+    const uint64_t b = (uint64_t)blockIdx.x * (AL_BLOCK / 64) + (threadIdx.x >> 6);
+    if (b >= n_leaves) return;
+    const int lane = threadIdx.x & 63;
+    const uint32_t leaf = perm ? perm[b] : (uint32_t)b;
+    const uint64_t* mk = masks + (uint64_t)leaf * 8;
+    uint32_t pos = act_off[b];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int m = (c << 6) | lane;  // Morton index of the voxel in the leaf
+        const int x = ((m >> 2) & 1) | ((m >> 4) & 2) | ((m >> 6) & 4);
+        const int y = ((m >> 1) & 1) | ((m >> 3) & 2) | ((m >> 5) & 4);
+        const int z = (m & 1) | ((m >> 2) & 2) | ((m >> 4) & 4);
+        const int off = (x << 6) | (y << 3) | z;
+        const bool act = (mk[off >> 6] >> (off & 63)) & 1ull;
+        const unsigned long long bl = __ballot(act);
+        if (act) out[pos + __popcll(bl & ((1ull << lane) - 1))] = leaf * 512u + (uint32_t)off;
+        pos += (uint32_t)__popcll(bl);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2879,7 +2885,8 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
     exclusive_scan(ctx, act_cnt, act_off, n_leaves + 1);
     const uint32_t n_act = read_scalar(ctx, act_off + n_leaves);
     uint32_t* act = S.as<uint32_t>(S_ACT_LIST, n_act);
-    k_active_list<<<(unsigned)n_leaves, 512, 0, st>>>(vol->masks, leaf_perm, act_off, act);
+    k_active_list<<<div_up(n_leaves, AL_BLOCK / 64), AL_BLOCK, 0, st>>>(vol->masks, leaf_perm, act_off, n_leaves,
+                                                                       act);
     tm.mark();
 
     // ---- 6. group roots (shared and surrogate signs): the winding number is evaluated
