@@ -2234,21 +2234,26 @@ __global__ void k_winding_fp64(const uint32_t* __restrict__ list, uint32_t n, co
 // group.  Groups never span leaves.
 
 // the group roots of every leaf, slot = leaf * 512 + offset, in offset order
-// Block b lists the group roots of leaf perm[b] (b when perm is null) at rep_off[b].
-__global__ __launch_bounds__(512) void k_sign_reps(const uint16_t* __restrict__ roots,
-                                                   const uint32_t* __restrict__ perm,
-                                                   const uint32_t* __restrict__ rep_off, uint32_t* __restrict__ reps) {
-    __shared__ uint32_t wcnt[8];
-    const uint64_t leaf = perm ? perm[blockIdx.x] : blockIdx.x;
-    const int v = threadIdx.x;
-    const int w = v >> 6, lane = v & 63;
-    const bool rep = roots[leaf * LEAF_VOX + v] == (uint16_t)v;
-    const unsigned long long b = __ballot(rep);
-    if (lane == 0) wcnt[w] = (uint32_t)__popcll(b);
-    __syncthreads();
-    uint32_t before = 0;
-    for (int k = 0; k < w; ++k) before += wcnt[k];
-    if (rep) reps[rep_off[blockIdx.x] + before + __popcll(b & ((1ull << lane) - 1))] = (uint32_t)leaf * 512u + (uint32_t)v;
+// One wave per listed leaf: wave b lists the group roots of leaf perm[b] (b
+// when perm is null) at rep_off[b], in voxel order.
+__global__ __launch_bounds__(AL_BLOCK) void k_sign_reps(const uint16_t* __restrict__ roots,
+                                                        const uint32_t* __restrict__ perm,
+                                                        const uint32_t* __restrict__ rep_off, uint64_t n_leaves,
+                                                        uint32_t* __restrict__ reps) {
+    const uint64_t b = (uint64_t)blockIdx.x * (AL_BLOCK / 64) + (threadIdx.x >> 6);
+    if (b >= n_leaves) return;
+    const int lane = threadIdx.x & 63;
+    const uint32_t leaf = perm ? perm[b] : (uint32_t)b;
+    const uint16_t* rt = roots + (uint64_t)leaf * LEAF_VOX;
+    uint32_t pos = rep_off[b];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int v = (c << 6) | lane;
+        const bool rep = rt[v] == (uint16_t)v;
+        const unsigned long long bl = __ballot(rep);
+        if (rep) reps[pos + __popcll(bl & ((1ull << lane) - 1))] = leaf * 512u + (uint32_t)v;
+        pos += (uint32_t)__popcll(bl);
+    }
 }
 
 // out[i] = in[perm[i]]
@@ -2910,7 +2915,7 @@ void mesh_to_volume_device(bs_ctx* ctx, const float* d_tris, uint64_t n, const b
         exclusive_scan(ctx, cnt, rep_off, n_leaves + 1);
         n_wn = read_scalar(ctx, rep_off + n_leaves);
         uint32_t* reps = S.as<uint32_t>(S_REPS, std::max<uint32_t>(n_wn, 1));
-        k_sign_reps<<<(unsigned)n_leaves, 512, 0, st>>>(roots, leaf_perm, rep_off, reps);
+        k_sign_reps<<<div_up(n_leaves, AL_BLOCK / 64), AL_BLOCK, 0, st>>>(roots, leaf_perm, rep_off, n_leaves, reps);
         wn_list = reps;
     }
     tm.mark();
